@@ -1,0 +1,126 @@
+"""Recipe: build the reference Fortran program into oracle/_ref/.
+
+Reads the reference source where it lies under /root/reference, applies the
+mechanical patches listed below in memory, compiles the patched text from a
+temporary file under /tmp with AMD flang, and writes only the executables into
+oracle/_ref/.  No reference source is copied into the repository or into
+oracle/_ref/.
+
+Patches (each is a parameter or syntax edit, none changes the algorithm):
+  * IF(0)/IF(1) -> IF(.FALSE.)/IF(.TRUE.): flang rejects integer conditions
+    (F6:223, F6:596, F6:633, F6:690).
+  * F6:1481 is commented out: a constant out-of-bounds subscript inside the
+    never-called WRITE_RESULTSPROFILE that flang rejects at compile time.
+  * PAUSE statements removed (F6:113, F6:197): the program would block on stdin.
+  * PARAMETER values: NX, NY (F6:8-9), NON_NEWTONIAN (F6:4), WRITE_OUTPUT
+    (F6:6, so the final state is dumped as list-directed text, which flang
+    prints in shortest round-trip form), TIME_MAX (F6:41).
+  * Optional: the top-wall velocity UXN (F6:1204) for a moving-lid case.
+  * Optional ("local" rate mode): one line at the top of the Collision_MRT node
+    loop (after F6:877) that loads the node's own previous-step rate,
+    CALL ComputeRelaxation_MRT(OMEGA_BN(I,J)).
+
+Usage: python oracle/build_ref.py [--timing]   (no-op when /root/reference is
+absent, e.g. on a machine that only runs the prebuilt files).
+"""
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+REF_ROOT = "/root/reference"
+F6 = os.path.join(REF_ROOT, "6_FreeFall_YieldStress",
+                  "D2Q9_BinghamSediment_Validated_20191127.f90")
+OUT_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref")
+FLANG = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+
+# name -> patch settings.  Parity variants are small grids run to 1000 steps.
+VARIANTS = {
+    "f6_newt": dict(nx=40, ny=120, non_newtonian=False, time_max=1000),
+    "f6_bing_lagged": dict(nx=40, ny=120, non_newtonian=True, time_max=1000),
+    "f6_bing_local": dict(nx=40, ny=120, non_newtonian=True, time_max=1000,
+                          local_rate=True),
+    "f6_newt_lid": dict(nx=40, ny=60, non_newtonian=False, time_max=300,
+                        uxn="0.01D0"),
+}
+# CPU-baseline build: the reference's Bingham settling disk on the 401x1601
+# node grid (NX=400, NY=1600), timed for a bounded number of steps.
+TIMING_VARIANTS = {
+    "f6_bing_400x1600": dict(nx=400, ny=1600, non_newtonian=True, time_max=40,
+                             write_output=False),
+}
+
+
+def patch_source(text, nx, ny, non_newtonian, time_max, local_rate=False,
+                 uxn=None, write_output=True):
+    lines = text.replace("\r", "").split("\n")
+    out = []
+    in_collide = False
+    for n, line in enumerate(lines, start=1):
+        s = line
+        if n == 1481 and "WRITE(15,*)" in s:
+            s = "!" + s
+        s = re.sub(r"\bIF\(0\)THEN", "IF(.FALSE.)THEN", s)
+        s = re.sub(r"\bIF\(1\)THEN", "IF(.TRUE.)THEN", s)
+        if re.search(r";\s*PAUSE\s*$", s):
+            s = re.sub(r";\s*PAUSE\s*$", "", s)
+        s = re.sub(r"(INTEGER,PARAMETER :: NX=)\d+", r"\g<1>%d" % nx, s)
+        s = re.sub(r"(INTEGER,PARAMETER :: NY=)\d+", r"\g<1>%d" % ny, s)
+        s = re.sub(r"(LOGICAL,PARAMETER :: NON_NEWTONIAN=)\s*\.\w+\.",
+                   r"\g<1> .%s." % ("TRUE" if non_newtonian else "FALSE"), s)
+        s = re.sub(r"(LOGICAL,PARAMETER :: WRITE_OUTPUT=)\s*\.\w+\.",
+                   r"\g<1> .%s." % ("TRUE" if write_output else "FALSE"), s)
+        s = re.sub(r"(INTEGER,PARAMETER :: TIME_MAX=).*$",
+                   r"\g<1>%d" % time_max, s)
+        if uxn is not None and re.match(r"\s*UXN=0\.0;UYN=0D0", s):
+            s = s.replace("UXN=0.0", "UXN=" + uxn, 1)
+        if re.match(r"\s*SUBROUTINE Collision_MRT\(", s):
+            in_collide = True
+        out.append(s)
+        if in_collide and re.match(r"\s*DO I=0,nX\s*$", s):
+            if local_rate:
+                out.append("\t\t\tCALL ComputeRelaxation_MRT(OMEGA_BN(I,J))")
+            in_collide = False
+    return "\n".join(out) + "\n"
+
+
+def build(name, cfg, flags):
+    with open(F6, "rb") as fh:
+        text = fh.read().decode("ascii")
+    src = patch_source(text, **cfg)
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")
+    try:
+        path = os.path.join(tmpdir, name + ".f90")
+        with open(path, "w") as fh:
+            fh.write(src)
+        exe = os.path.join(OUT_DIR, name)
+        cmd = [FLANG] + flags + [path, "-o", exe]
+        subprocess.run(cmd, check=True, cwd=tmpdir,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    return exe
+
+
+def main(argv):
+    if not os.path.exists(F6):
+        print("build_ref: reference source not present; nothing to build")
+        return 0
+    parity_flags = ["-O2", "-ffp-contract=off"]
+    timing_flags = ["-O3", "-march=x86-64-v3"]
+    built = []
+    for name, cfg in VARIANTS.items():
+        built.append(build(name, cfg, parity_flags))
+    if "--timing" in argv or "--all" in argv:
+        for name, cfg in TIMING_VARIANTS.items():
+            built.append(build(name, cfg, timing_flags))
+    for b in built:
+        print("built", os.path.relpath(b))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -1,0 +1,79 @@
+"""Run constants of the reference's single-disk program (F6 module PARAM,
+F6:2-83), restated for a given grid so the oracle and the HIP path can be set
+up exactly like a patched reference build.
+
+Every default-REAL literal in F6 is a float32 value widened to double; f32()
+reproduces that.
+"""
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+PI = f32(3.14159265359)  # F6:3
+
+
+def f6_constants(nx, ny):
+    """Compile-time constants of F6's module PARAM for grid nx, ny."""
+    re_s, bn_s, rho_s, rho_f, nu = f32(0.78), f32(0.033), 2.0, 1.0, f32(0.025)
+    l_c = float(nx // 4)                                       # F6:26
+    a_g = -(re_s * nu * nu) / ((l_c * l_c * l_c) * (rho_s - rho_f) * rho_f)  # F6:28
+    u_c = (l_c * l_c) * (rho_s - rho_f) * (-a_g) / nu          # F6:29
+    tau_y = bn_s * (rho_s - rho_f) * (-a_g) * l_c              # F6:30
+    omega = 1.0 / (3.0 * nu + 0.5)                             # F6:101
+    return dict(re_s=re_s, rho_s=rho_s, rho_f=rho_f, nu=nu, l_c=l_c, a_g=a_g,
+                u_c=u_c, tau_y=tau_y, omega=omega)
+
+
+def f6_params(nx, ny, non_newtonian, local_rate=False, uxn=0.0):
+    c = f6_constants(nx, ny)
+    wall = [0.0] * 8
+    wall[4] = uxn
+    d = c["l_c"]
+    cij = -((((((c["rho_s"] - c["rho_f"]) * c["a_g"]) * PI) * 0.5) * d) * 0.5) * d  # F6:691
+    return dict(
+        nx=nx, ny=ny, rheology=1 if non_newtonian else 0,
+        rate_mode=1 if local_rate else 0, q_mode=1, update_angle=1,
+        omega=c["omega"], s0=0.0, s1=f32(1.1), s2=1.25, s3=0.0, s5=0.0, sq=0.0,  # F6:1009-1011
+        nu=c["nu"], mu0=c["nu"], tau_y=c["tau_y"], m_pap=5.5e6, gamma_min=f32(1e-15),
+        rho0=1.0, gy=c["a_g"], wall=wall, corner=[0.0, 0.0],
+        zeta_c=1.0, cij=cij, eps_p=0.5, eps_w=0.5, e_p=10.0, e_w=10.0,  # F6:691-693
+    )
+
+
+def f6_disk(nx, ny):
+    """Initial disk of F6:161-169 as a particle record (oracle field order)."""
+    c = f6_constants(nx, ny)
+    d = c["l_c"]
+    rho_p = c["rho_s"]
+    m_p = rho_p * PI * (0.5 * d) * (0.5 * d)
+    i_p = 0.5 * m_p * (0.5 * d) * (0.5 * d)
+    rec = np.zeros(22)
+    rec[0] = 0.5 * float(nx)
+    rec[1] = float(ny) - 7.0 * c["l_c"]
+    rec[6] = d
+    rec[12] = rho_p
+    rec[13] = m_p
+    rec[14] = i_p
+    rec[15] = 1.0
+    return rec
+
+
+def load_golden(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    return meta, {k: z[k] for k in z.files if k != "meta"}
+
+
+def golden_params(meta):
+    return f6_params(meta["nx"], meta["ny"], meta["non_newtonian"],
+                     meta.get("local_rate", False),
+                     float(meta["uxn"].replace("D", "e")) if meta.get("uxn") else 0.0)

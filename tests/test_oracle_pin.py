@@ -1,0 +1,81 @@
+"""Pins the CPU oracle to the reference program itself.
+
+The fixtures under tests/golden/ were written by the compiled reference
+(oracle/build_ref.py + tests/golden/make_golden.py): F6 run for 1000 steps
+(300 for the moving-lid case) on small grids.  The oracle is run from the same
+initial state for the same number of steps and every output must agree
+bit-for-bit.
+"""
+import numpy as np
+import pytest
+
+import cases
+
+VARIANTS = ["f6_newt", "f6_bing_lagged", "f6_bing_local", "f6_newt_lid"]
+
+
+def _run_oracle(orc, meta):
+    cfg = cases.golden_params(meta)
+    p = orc.make_params(cfg)
+    st = orc.State(p, 1)
+    st.initialize()
+    st.set_particles(cases.f6_disk(meta["nx"], meta["ny"])[:, None])
+    hist = st.run(0, meta["time_max"])
+    return cfg, st, hist
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+def test_constants_match_reference_printout(name):
+    meta, _ = cases.load_golden(name)
+    c = cases.f6_constants(meta["nx"], meta["ny"])
+    k = meta["consts"]
+    assert c["a_g"] == k["A_G"]
+    assert c["tau_y"] == k["Tau_Y_LB"]
+    assert c["u_c"] == k["U_LB_max"]
+    assert 1.0 / c["omega"] == k["TAU_LB"]
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+def test_oracle_bitwise_vs_reference(orc, name):
+    meta, g = cases.load_golden(name)
+    cfg, st, hist = _run_oracle(orc, meta)
+    assert np.array_equal(st.f, g["f"])
+    assert np.array_equal(st.rho, g["rho"])
+    assert np.array_equal(st.ux, g["ux"])
+    assert np.array_equal(st.uy, g["uy"])
+    assert np.array_equal(st.g, g["g"])
+    assert np.array_equal(st.omega_bn, g["omega_bn"])
+    assert np.array_equal(st.image, g["image"])
+    assert np.array_equal(st.stress, g["stress"])
+    assert np.array_equal(st.s_mrt, g["omega_mrt"])
+    part = st.part[:, 0]
+    ref = g["particle"]  # CENTER(2), VELOCITY(3), FORCE(3)
+    mine = np.array([part[orc.PF_X], part[orc.PF_Y], part[orc.PF_VX], part[orc.PF_VY],
+                     part[orc.PF_W], part[orc.PF_FX], part[orc.PF_FY], part[orc.PF_TQ]])
+    assert np.array_equal(mine, ref)
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+def test_oracle_history_vs_reference(orc, name):
+    """Per-step particle history line (F6:315-318), every step."""
+    meta, g = cases.load_golden(name)
+    cfg, st, hist = _run_oracle(orc, meta)
+    c = cases.f6_constants(meta["nx"], meta["ny"])
+    H = orc.H_PCOUNT
+    ref = g["hist"]
+    n = meta["time_max"]
+    assert ref.shape[0] == n
+    y, x, ang = hist[:, orc.H_Y], hist[:, orc.H_X], hist[:, orc.H_ANG]
+    vx, vy, w = hist[:, orc.H_VX], hist[:, orc.H_VY], hist[:, orc.H_W]
+    assert np.array_equal(ref[:, 1], y)
+    assert np.array_equal(ref[:, 2], x)
+    assert np.array_equal(ref[:, 3], ang)
+    assert np.array_equal(ref[:, 4], vx)
+    assert np.array_equal(ref[:, 5], vy / c["u_c"])
+    assert np.array_equal(ref[:, 6], w)
+    # Fx + F_H_IN_x: the line holds the previous step's hydrodynamic force
+    fx_prev = np.concatenate([[0.0], hist[:-1, orc.H_FX]])
+    assert np.array_equal(ref[:, 7], fx_prev + hist[:, orc.H_FHX])
+    # mean density per step, F6:348
+    nodes = (meta["nx"] + 1) * (meta["ny"] + 1)
+    assert np.array_equal(g["avgden"], hist[:, H] / float(nodes))
