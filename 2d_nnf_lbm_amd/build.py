@@ -1,0 +1,41 @@
+"""Build the HIP library liblbm_amd.so in-tree for gfx950.
+
+The library is compiled with contraction off (-ffp-contract=off) so the FP64
+path keeps the reference's rounding, and with packed FP32 instructions
+disabled.
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "liblbm_amd.so")
+SOURCES = [os.path.join(CSRC, "lbm_amd.hip")]
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("lbm_device.h", "lbm_kernels.h")] + [
+    os.path.join(os.path.dirname(HERE), "include", "lbm_amd.h")]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+         "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-command-line-argument"]
+
+
+def build(force=True, verbose=False):
+    """Compile liblbm_amd.so (always when force, else only when stale)."""
+    if not force and os.path.exists(LIB):
+        t = os.path.getmtime(LIB)
+        if all(os.path.getmtime(d) <= t for d in DEPS):
+            return LIB
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SOURCES
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout)
+    if verbose:
+        print(r.stdout)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(force=True))

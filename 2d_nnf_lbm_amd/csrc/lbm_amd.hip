@@ -1,0 +1,667 @@
+// C ABI of the MI355X D2Q9 MRT / smoothed-profile time step (include/lbm_amd.h).
+// Host side: parameter checks, device allocation, layout transposes and the
+// launch sequence of the fused step.  Kernels are in lbm_kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_amd.h"
+#include "lbm_kernels.h"
+
+using namespace lbm;
+
+static_assert(LBM_PF_COUNT == PCOUNT, "particle record layout");
+static_assert(LBM_H_PCOUNT == HPC, "history record layout");
+
+struct lbm_handle {
+    lbm_params prm;
+    DevP P;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // FP64 state (SoA planes)
+    double *f = nullptr, *fp = nullptr, *rho = nullptr, *ux = nullptr, *uy = nullptr;
+    double *g = nullptr, *obn = nullptr, *stress = nullptr;
+    int *image = nullptr;
+    // reduced-format storage
+    void *sA = nullptr, *sB = nullptr;
+    float *rate32 = nullptr;
+    void *erho[2] = {nullptr, nullptr};
+    // particles
+    double *pbuf[2] = {nullptr, nullptr};
+    int pcur = 0;
+    double *dpw[2] = {nullptr, nullptr};
+    int dcur = 0;
+    int *pairs = nullptr, *paircnt = nullptr;
+    double *phi = nullptr;
+    int *porig = nullptr;
+    int PS = 0;
+    // scratch
+    double *staging = nullptr;
+    double *partial = nullptr;
+    double *scalar = nullptr;
+    double *hist_part = nullptr;
+    unsigned long long *hist_slots = nullptr;
+    size_t hist_rows = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evk;
+};
+
+namespace {
+
+int fail(lbm_handle *h, int code, const std::string &msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HIPCHECK(h, call)                                                               \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail((h), LBM_ERR_HIP,                                               \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+inline unsigned nblk(long long n) { return (unsigned)((n + NT - 1) / NT); }
+
+size_t storage_bytes(int storage) {
+    switch (storage) {
+        case LBM_FP64: return 8;
+        case LBM_FP32: return 4;
+        default: return 2;
+    }
+}
+
+int check_launch(lbm_handle *h, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(h, LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_last_error(const lbm_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int lbm_create(const lbm_params *p, lbm_handle **out) {
+    if (!p || !out) return LBM_ERR_PARAM;
+    *out = nullptr;
+    if (p->nx < 2 || p->ny < 2) return LBM_ERR_PARAM;
+    // F6:115-118: tau = 1/omega must not be below 0.5
+    if (!(p->omega > 0.0) || 1.0 / p->omega < 0.5) return LBM_ERR_PARAM;
+    if (p->rheology != LBM_NEWTONIAN && p->rheology != LBM_BINGHAM) return LBM_ERR_PARAM;
+    if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return LBM_ERR_PARAM;
+    if (p->max_particles < 0 || p->max_particles > 65535) return LBM_ERR_PARAM;
+    lbm_handle *h = new lbm_handle();
+    h->prm = *p;
+    DevP &P = h->P;
+    std::memset(&P, 0, sizeof(P));
+    P.nx = p->nx; P.ny = p->ny; P.W = p->nx + 1; P.H = p->ny + 1;
+    P.nn = (long long)P.W * P.H;
+    P.plane = (P.nn + 255) / 256 * 256;
+    P.rheology = p->rheology; P.q_mode = p->q_mode; P.update_angle = p->update_angle;
+    P.np = 0;
+    P.maxp = std::max(1, p->max_particles);
+    P.nbnd = 2 * P.W + 2 * (P.H - 2);
+    P.omega = p->omega; P.s0 = p->s0; P.s1 = p->s1; P.s2 = p->s2; P.s3 = p->s3;
+    P.s5 = p->s5; P.sq = p->sq; P.nu = p->nu; P.mu0 = p->mu0; P.tau_y = p->tau_y;
+    P.m_pap = p->m_pap; P.gamma_min = p->gamma_min; P.rho0 = p->rho0; P.gy = p->gy;
+    for (int k = 0; k < 8; ++k) P.wall[k] = p->wall[k];
+    P.corner[0] = p->corner[0]; P.corner[1] = p->corner[1];
+    P.zeta_c = p->zeta_c; P.cij = p->cij; P.eps_p = p->eps_p; P.eps_w = p->eps_w;
+    P.e_p = p->e_p; P.e_w = p->e_w;
+
+    auto al = [&](void **ptr, size_t bytes) -> int {
+        hipError_t e = hipMalloc(ptr, bytes);
+        if (e != hipSuccess) return fail(h, LBM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        e = hipMemset(*ptr, 0, bytes);
+        if (e != hipSuccess) return fail(h, LBM_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+        return LBM_OK;
+    };
+    size_t pl = (size_t)P.plane;
+    int rc = LBM_OK;
+    hipError_t es = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (es != hipSuccess) { *out = h; return fail(h, LBM_ERR_HIP, "hipStreamCreate failed"); }
+    if ((rc = al((void **)&h->f, 9 * pl * 8)) || (rc = al((void **)&h->fp, 9 * pl * 8)) ||
+        (rc = al((void **)&h->rho, pl * 8)) || (rc = al((void **)&h->ux, pl * 8)) ||
+        (rc = al((void **)&h->uy, pl * 8)) || (rc = al((void **)&h->g, 2 * pl * 8)) ||
+        (rc = al((void **)&h->obn, pl * 8)) || (rc = al((void **)&h->stress, 3 * pl * 8)) ||
+        (rc = al((void **)&h->image, pl * 4)) || (rc = al((void **)&h->staging, 9 * pl * 8)) ||
+        (rc = al((void **)&h->erho[0], (size_t)P.nbnd * 8)) ||
+        (rc = al((void **)&h->erho[1], (size_t)P.nbnd * 8)) ||
+        (rc = al((void **)&h->pbuf[0], (size_t)PCOUNT * P.maxp * 8)) ||
+        (rc = al((void **)&h->pbuf[1], (size_t)PCOUNT * P.maxp * 8)) ||
+        (rc = al((void **)&h->dpw[0], 8)) || (rc = al((void **)&h->dpw[1], 8)) ||
+        (rc = al((void **)&h->pairs, (size_t)P.maxp * MAXNB * 4)) ||
+        (rc = al((void **)&h->paircnt, (size_t)P.maxp * 4)) ||
+        (rc = al((void **)&h->porig, (size_t)P.maxp * 2 * 4)) ||
+        (rc = al((void **)&h->partial, (size_t)nblk(P.nn) * 8)) ||
+        (rc = al((void **)&h->scalar, 8))) {
+        *out = h;
+        return rc;
+    }
+    if (p->storage != LBM_FP64) {
+        size_t sb = storage_bytes(p->storage);
+        if ((rc = al(&h->sA, 9 * pl * sb)) || (rc = al(&h->sB, 9 * pl * sb)) ||
+            (rc = al((void **)&h->rate32, pl * 4))) {
+            *out = h;
+            return rc;
+        }
+    }
+    double inf = INFINITY;
+    (void)hipMemcpy(h->dpw[0], &inf, 8, hipMemcpyHostToDevice);
+    (void)hipMemcpy(h->dpw[1], &inf, 8, hipMemcpyHostToDevice);
+    (void)hipEventCreate(&h->ev0);
+    (void)hipEventCreate(&h->ev1);
+    *out = h;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_handle *h) {
+    if (!h) return LBM_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void *bufs[] = {h->f, h->fp, h->rho, h->ux, h->uy, h->g, h->obn, h->stress, h->image,
+                    h->sA, h->sB, h->rate32, h->erho[0], h->erho[1], h->pbuf[0], h->pbuf[1],
+                    h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
+                    h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    for (auto e : h->evk) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_initialize(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_init, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->rho,
+                       h->ux, h->uy, h->g, h->obn);
+    return check_launch(h, "k_init");
+}
+
+static int upload_planes(lbm_handle *h, const double *host, int nc, double *dev) {
+    const DevP &P = h->P;
+    HIPCHECK(h, hipMemcpyAsync(h->staging, host, (size_t)P.nn * nc * 8, hipMemcpyHostToDevice,
+                               h->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P.nn, nc,
+                       h->staging, dev, P.plane);
+    int rc = check_launch(h, "k_aos_to_soa");
+    if (rc) return rc;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+static int download_planes(lbm_handle *h, const double *dev, int nc, double *host) {
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P.nn, nc, dev,
+                       h->staging, P.plane);
+    int rc = check_launch(h, "k_soa_to_aos");
+    if (rc) return rc;
+    HIPCHECK(h, hipMemcpyAsync(host, h->staging, (size_t)P.nn * nc * 8, hipMemcpyDeviceToHost,
+                               h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_set_f(lbm_handle *h, const double *f) {
+    if (!h || !f) return LBM_ERR_PARAM;
+    return upload_planes(h, f, 9, h->f);
+}
+int lbm_get_f(lbm_handle *h, double *f) {
+    if (!h || !f) return LBM_ERR_PARAM;
+    return download_planes(h, h->f, 9, f);
+}
+int lbm_set_fpost(lbm_handle *h, const double *f) {
+    if (!h || !f) return LBM_ERR_PARAM;
+    return upload_planes(h, f, 9, h->fp);
+}
+int lbm_get_fpost(lbm_handle *h, double *f) {
+    if (!h || !f) return LBM_ERR_PARAM;
+    return download_planes(h, h->fp, 9, f);
+}
+
+int lbm_set_fields(lbm_handle *h, const double *rho, const double *ux, const double *uy,
+                   const double *g, const double *omega_bn) {
+    if (!h) return LBM_ERR_PARAM;
+    int rc;
+    if (rho && (rc = upload_planes(h, rho, 1, h->rho))) return rc;
+    if (ux && (rc = upload_planes(h, ux, 1, h->ux))) return rc;
+    if (uy && (rc = upload_planes(h, uy, 1, h->uy))) return rc;
+    if (g && (rc = upload_planes(h, g, 2, h->g))) return rc;
+    if (omega_bn && (rc = upload_planes(h, omega_bn, 1, h->obn))) return rc;
+    return LBM_OK;
+}
+
+int lbm_get_fields(lbm_handle *h, double *rho, double *ux, double *uy, double *g,
+                   double *omega_bn) {
+    if (!h) return LBM_ERR_PARAM;
+    int rc;
+    if (rho && (rc = download_planes(h, h->rho, 1, rho))) return rc;
+    if (ux && (rc = download_planes(h, h->ux, 1, ux))) return rc;
+    if (uy && (rc = download_planes(h, h->uy, 1, uy))) return rc;
+    if (g && (rc = download_planes(h, h->g, 2, g))) return rc;
+    if (omega_bn && (rc = download_planes(h, h->obn, 1, omega_bn))) return rc;
+    return LBM_OK;
+}
+
+int lbm_get_stress(lbm_handle *h, double *stress) {
+    if (!h || !stress) return LBM_ERR_PARAM;
+    return download_planes(h, h->stress, 3, stress);
+}
+
+int lbm_get_image(lbm_handle *h, int32_t *image) {
+    if (!h || !image) return LBM_ERR_PARAM;
+    HIPCHECK(h, hipMemcpyAsync(image, h->image, (size_t)h->P.nn * 4, hipMemcpyDeviceToHost,
+                               h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_set_particles(lbm_handle *h, int32_t n, const double *part) {
+    if (!h || n < 0 || (n > 0 && !part)) return LBM_ERR_PARAM;
+    if (n > h->prm.max_particles)
+        return fail(h, LBM_ERR_CAPACITY, "more particles than max_particles");
+    DevP &P = h->P;
+    std::vector<double> buf((size_t)PCOUNT * P.maxp, 0.0);
+    double reach = 0.0;
+    for (int f = 0; f < PCOUNT; ++f)
+        for (int p = 0; p < n; ++p) buf[(size_t)f * P.maxp + p] = part[(size_t)f * n + p];
+    for (int p = 0; p < n; ++p) {
+        double r = 0.5 * part[(size_t)LBM_PF_DIAM * n + p] + 0.5 * part[(size_t)LBM_PF_ZETA * n + p] + 1.0;
+        if (!(r > 0.0) || !std::isfinite(r)) return fail(h, LBM_ERR_PARAM, "bad particle diameter");
+        reach = std::max(reach, r);
+    }
+    P.np = n;
+    HIPCHECK(h, hipMemcpyAsync(h->pbuf[h->pcur], buf.data(), buf.size() * 8,
+                               hipMemcpyHostToDevice, h->stream));
+    int PS = (int)std::ceil(2.0 * reach) + 2;
+    if (n > 0 && PS > h->PS) {
+        if (h->phi) (void)hipFree(h->phi);
+        h->phi = nullptr;
+        HIPCHECK(h, hipMalloc((void **)&h->phi, (size_t)P.maxp * PS * PS * 8));
+        HIPCHECK(h, hipMemsetAsync(h->phi, 0, (size_t)P.maxp * PS * PS * 8, h->stream));
+        h->PS = PS;
+    }
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_get_particles(lbm_handle *h, double *part) {
+    if (!h || !part) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    std::vector<double> buf((size_t)PCOUNT * P.maxp);
+    HIPCHECK(h, hipMemcpyAsync(buf.data(), h->pbuf[h->pcur], buf.size() * 8,
+                               hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    for (int f = 0; f < PCOUNT; ++f)
+        for (int p = 0; p < P.np; ++p) part[(size_t)f * P.np + p] = buf[(size_t)f * P.maxp + p];
+    return LBM_OK;
+}
+
+int lbm_get_phi(lbm_handle *h, int32_t ip, double *phi) {
+    if (!h || !phi || ip < 0 || ip >= h->P.np || !h->phi) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    int PS = h->PS;
+    std::vector<double> patch((size_t)PS * PS);
+    int orig[2];
+    HIPCHECK(h, hipMemcpyAsync(patch.data(), h->phi + (size_t)ip * PS * PS, patch.size() * 8,
+                               hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipMemcpyAsync(orig, h->porig + 2 * ip, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    std::fill(phi, phi + P.nn, 0.0);
+    for (int b = 0; b < PS; ++b)
+        for (int a = 0; a < PS; ++a) {
+            int i = orig[0] + a, j = orig[1] + b;
+            if (i < 0 || i > P.nx || j < 0 || j > P.ny) continue;
+            phi[(size_t)j * P.W + i] = patch[(size_t)b * PS + a];
+        }
+    return LBM_OK;
+}
+
+// --------------------------------------------------------------- drop-in
+int lbm_collide(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
+        return fail(h, LBM_ERR_UNSUPPORTED,
+                    "the lagged module-global rate is a sequential recurrence; the GPU path "
+                    "implements rate_mode LBM_RATE_LOCAL");
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_collide, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->rho,
+                       h->ux, h->uy, h->g, h->obn, h->stress, h->image);
+    return check_launch(h, "k_collide");
+}
+
+int lbm_copy_post(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    HIPCHECK(h, hipMemcpyAsync(h->fp, h->f, (size_t)9 * h->P.plane * 8, hipMemcpyDeviceToDevice,
+                               h->stream));
+    return LBM_OK;
+}
+
+int lbm_stream(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_stream, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->fp);
+    return check_launch(h, "k_stream");
+}
+
+int lbm_boundary(lbm_handle *h, int32_t kk) {
+    if (!h) return LBM_ERR_PARAM;
+    (void)kk;
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_boundary, dim3(nblk(P.nbnd)), dim3(NT), 0, h->stream, P, h->f, h->fp,
+                       h->rho);
+    return check_launch(h, "k_boundary");
+}
+
+int lbm_macros(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_macros, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->rho,
+                       h->ux, h->uy);
+    return check_launch(h, "k_macros");
+}
+
+int lbm_spm_phi(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    double *part = h->pbuf[h->pcur];
+    if (P.np > 0) {
+        hipLaunchKernelGGL(k_phi_patch, dim3(nblk((long long)h->PS * h->PS), P.np), dim3(NT), 0,
+                           h->stream, P, part, h->phi, h->porig, h->PS);
+        int rc = check_launch(h, "k_phi_patch");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_image, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, part, h->phi,
+                       h->porig, h->PS, h->image);
+    return check_launch(h, "k_image");
+}
+
+int lbm_spm_forcing(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_forcing, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P,
+                       h->pbuf[h->pcur], h->phi, h->porig, h->PS, h->ux, h->uy, h->g);
+    return check_launch(h, "k_forcing");
+}
+
+int lbm_hydro_force(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    if (P.np == 0) return LBM_OK;
+    hipLaunchKernelGGL(k_hydro_seq, dim3(nblk(P.np)), dim3(NT), 0, h->stream, P,
+                       h->pbuf[h->pcur], h->phi, h->porig, h->PS, h->rho, h->ux, h->uy);
+    return check_launch(h, "k_hydro_seq");
+}
+
+int lbm_correct(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    hipLaunchKernelGGL(k_correct, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->ux, h->uy,
+                       h->g);
+    return check_launch(h, "k_correct");
+}
+
+static int gather_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t *npairs) {
+    const DevP &P = h->P;
+    std::vector<int> cnt(P.np), nb((size_t)P.np * MAXNB);
+    if (P.np > 0) {
+        HIPCHECK(h, hipMemcpyAsync(cnt.data(), h->paircnt, (size_t)P.np * 4,
+                                   hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(h, hipMemcpyAsync(nb.data(), h->pairs, nb.size() * 4, hipMemcpyDeviceToHost,
+                                   h->stream));
+    }
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    int total = 0;
+    bool over = false;
+    for (int p = 0; p < P.np; ++p) {
+        if (cnt[p] > MAXNB) over = true;
+        int c = std::min(cnt[p], MAXNB);
+        for (int q = 0; q < c; ++q) {
+            if (pairs && total < max_pairs) {
+                pairs[2 * total] = p;
+                pairs[2 * total + 1] = nb[(size_t)p * MAXNB + q];
+            }
+            ++total;
+        }
+    }
+    if (npairs) *npairs = total;
+    if (over) return fail(h, LBM_ERR_CAPACITY, "more contact partners than recorded per particle");
+    if (pairs && total > max_pairs) return fail(h, LBM_ERR_CAPACITY, "pair buffer too small");
+    return LBM_OK;
+}
+
+int lbm_contact_force(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t *npairs) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    if (P.np > 0) {
+        hipLaunchKernelGGL(k_contact, dim3(nblk(P.np)), dim3(NT), 0, h->stream, P,
+                           h->pbuf[h->pcur], h->dpw[h->dcur], h->dpw[h->dcur ^ 1], h->pairs,
+                           h->paircnt);
+        int rc = check_launch(h, "k_contact");
+        if (rc) return rc;
+        h->dcur ^= 1;
+    }
+    if (pairs || npairs) return gather_pairs(h, pairs, max_pairs, npairs);
+    return LBM_OK;
+}
+
+int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t *npairs) {
+    if (!h) return LBM_ERR_PARAM;
+    return gather_pairs(h, pairs, max_pairs, npairs);
+}
+
+int lbm_particle_update(lbm_handle *h) {
+    if (!h) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    if (P.np == 0) return LBM_OK;
+    hipLaunchKernelGGL(k_update, dim3(nblk(P.np)), dim3(NT), 0, h->stream, P,
+                       h->pbuf[h->pcur]);
+    return check_launch(h, "k_update");
+}
+
+int lbm_total_mass(lbm_handle *h, double *mass) {
+    if (!h || !mass) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    unsigned nb = nblk(P.nn);
+    hipLaunchKernelGGL(k_sum_partial, dim3(nb), dim3(NT), 0, h->stream, h->rho, P.nn, h->partial);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(NT), 0, h->stream, h->partial, (int)nb,
+                       h->scalar);
+    int rc = check_launch(h, "k_sum");
+    if (rc) return rc;
+    HIPCHECK(h, hipMemcpyAsync(mass, h->scalar, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- fused
+namespace {
+
+template <class St, bool DEV>
+int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, bool timed,
+              double *ms_total, double *ms_main) {
+    (void)kk0;
+    const DevP &P = h->P;
+    const int np = P.np;
+    long long ntiles = (P.nn + NT - 1) / NT;
+    StepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.P = P;
+    a.cf = h->f; a.cfp = h->fp; a.crho = h->rho; a.cux = h->ux; a.cuy = h->uy; a.cg = h->g;
+    a.cobn = h->obn; a.cstress = h->stress; a.cimage = h->image; a.rate32 = h->rate32;
+    a.pairs = h->pairs; a.paircnt = h->paircnt;
+    a.n_pblocks = np;
+    a.ntiles = ntiles;
+    dim3 grid((unsigned)(np + ntiles)), block(NT);
+    const size_t rowp = (size_t)np * HPC;
+
+    // storage buffers: FP64 runs ping-pong between F and F_POST
+    void *buf[2];
+    if (DEV) { buf[0] = h->sA; buf[1] = h->sB; }
+    else { buf[0] = h->f; buf[1] = h->fp; }
+    int e = 0;        // erho parity
+    int pc = h->pcur; // particle buffer holding the current state
+
+    if (timed) {
+        if ((int)h->evk.size() < 2 * nsteps) {
+            for (auto ev : h->evk) (void)hipEventDestroy(ev);
+            h->evk.assign(2 * nsteps, nullptr);
+            for (auto &ev : h->evk) (void)hipEventCreate(&ev);
+        }
+        (void)hipEventRecord(h->ev0, h->stream);
+    }
+    // prologue: collide step kk0's state into f*_{kk0+1}; particles to kk0+1
+    a.src = DEV ? nullptr : buf[0];
+    a.dst = DEV ? buf[0] : buf[1];
+    a.erho_dst = h->erho[e];
+    a.pin = h->pbuf[pc]; a.pout = h->pbuf[pc ^ 1];
+    a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
+    a.slots_yield = record ? h->hist_slots + NSLOT : nullptr;  // row 0, yield half
+    a.slots_mass = nullptr;
+    a.hist_part = nullptr;
+    a.aux = nsteps == 1;
+    hipLaunchKernelGGL((k_prologue<St, DEV>), grid, block, 0, h->stream, a);
+    int rc = check_launch(h, "k_prologue");
+    if (rc) return rc;
+    if (np > 0) { pc ^= 1; h->dcur ^= 1; }
+    int cur = DEV ? 0 : 1;  // buffer holding f*_{k}
+    for (int s = 1; s <= nsteps; ++s) {
+        bool fin = s == nsteps;
+        a.src = buf[cur];
+        a.dst = buf[cur ^ 1];
+        a.erho_src = h->erho[e];
+        a.erho_dst = h->erho[e ^ 1];
+        a.pin = h->pbuf[pc];
+        a.pout = fin ? h->pbuf[pc] : h->pbuf[pc ^ 1];
+        a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
+        a.hist_part = record ? h->hist_part + (size_t)(s - 1) * rowp : nullptr;
+        a.slots_mass = record ? h->hist_slots + (size_t)(s - 1) * 2 * NSLOT : nullptr;
+        a.slots_yield = (record && !fin) ? h->hist_slots + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
+        a.aux = (s == nsteps - 1) || fin;
+        if (timed) (void)hipEventRecord(h->evk[2 * (s - 1)], h->stream);
+        if (fin)
+            hipLaunchKernelGGL((k_step<St, DEV, true>), grid, block, 0, h->stream, a);
+        else
+            hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
+        if (timed) (void)hipEventRecord(h->evk[2 * (s - 1) + 1], h->stream);
+        rc = check_launch(h, "k_step");
+        if (rc) return rc;
+        if (!fin) {
+            cur ^= 1;
+            e ^= 1;
+            if (np > 0) { pc ^= 1; h->dcur ^= 1; }
+        }
+    }
+    if (timed) (void)hipEventRecord(h->ev1, h->stream);
+    // canonical state: FP64 runs wrote F into buf[cur^1] and F_POST is buf[cur]
+    if (!DEV) {
+        double *fk = (double *)buf[cur ^ 1], *fs = (double *)buf[cur];
+        h->f = fk;
+        h->fp = fs;
+    }
+    h->pcur = pc;
+    if (timed) {
+        HIPCHECK(h, hipEventSynchronize(h->ev1));
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, h->ev0, h->ev1);
+        *ms_total = t;
+        double main = 0.0;
+        for (int s = 0; s < nsteps; ++s) {
+            float tk = 0.0f;
+            (void)hipEventElapsedTime(&tk, h->evk[2 * s], h->evk[2 * s + 1]);
+            main += tk;
+        }
+        *ms_main = main;
+    }
+    return LBM_OK;
+}
+
+int fused_dispatch(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, bool timed,
+                   double *ms_total, double *ms_main) {
+    switch (h->prm.storage) {
+        case LBM_FP64: return run_fused<StF64, false>(h, kk0, nsteps, record, timed, ms_total, ms_main);
+        case LBM_FP32: return run_fused<StF32, true>(h, kk0, nsteps, record, timed, ms_total, ms_main);
+        case LBM_FP16: return run_fused<StF16, true>(h, kk0, nsteps, record, timed, ms_total, ms_main);
+        default: return run_fused<StBF16, true>(h, kk0, nsteps, record, timed, ms_total, ms_main);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_step(lbm_handle *h, int32_t kk0, int32_t nsteps, double *hist) {
+    if (!h || nsteps < 0) return LBM_ERR_PARAM;
+    if (nsteps == 0) return LBM_OK;
+    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
+        return fail(h, LBM_ERR_UNSUPPORTED,
+                    "the lagged module-global rate is a sequential recurrence; the GPU path "
+                    "implements rate_mode LBM_RATE_LOCAL");
+    const DevP &P = h->P;
+    bool record = hist != nullptr;
+    const size_t rowp = (size_t)P.np * HPC;
+    if (record) {
+        if ((size_t)nsteps > h->hist_rows || !h->hist_part) {
+            if (h->hist_part) (void)hipFree(h->hist_part);
+            if (h->hist_slots) (void)hipFree(h->hist_slots);
+            h->hist_part = nullptr;
+            h->hist_slots = nullptr;
+            size_t rows = std::max<size_t>(nsteps, 16);
+            HIPCHECK(h, hipMalloc((void **)&h->hist_part, std::max<size_t>(rows * rowp, 1) * 8));
+            HIPCHECK(h, hipMalloc((void **)&h->hist_slots, rows * 2 * NSLOT * 8));
+            h->hist_rows = rows;
+        }
+        HIPCHECK(h, hipMemsetAsync(h->hist_slots, 0, (size_t)nsteps * 2 * NSLOT * 8, h->stream));
+    }
+    int rc = fused_dispatch(h, kk0, nsteps, record, false, nullptr, nullptr);
+    if (rc) return rc;
+    if (record) {
+        std::vector<double> hp(std::max<size_t>((size_t)nsteps * rowp, 1));
+        std::vector<unsigned long long> hs((size_t)nsteps * 2 * NSLOT);
+        if (rowp)
+            HIPCHECK(h, hipMemcpyAsync(hp.data(), h->hist_part, (size_t)nsteps * rowp * 8,
+                                       hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(h, hipMemcpyAsync(hs.data(), h->hist_slots, hs.size() * 8, hipMemcpyDeviceToHost,
+                                   h->stream));
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        const size_t row = rowp + 2;
+        for (int s = 0; s < nsteps; ++s) {
+            double *out = hist + (size_t)s * row;
+            for (size_t c = 0; c < rowp; ++c) out[c] = hp[(size_t)s * rowp + c];
+            long long msum = 0, ysum = 0;
+            for (int q = 0; q < NSLOT; ++q) {
+                msum += (long long)hs[(size_t)s * 2 * NSLOT + q];
+                ysum += (long long)hs[(size_t)s * 2 * NSLOT + NSLOT + q];
+            }
+            out[rowp] = (double)msum / MASS_SCALE;
+            out[rowp + 1] = (double)ysum;
+        }
+    }
+    return LBM_OK;
+}
+
+int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps, double *ms_total,
+                   double *ms_main) {
+    if (!h || nsteps <= 0 || !ms_total || !ms_main) return LBM_ERR_PARAM;
+    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
+        return fail(h, LBM_ERR_UNSUPPORTED, "rate_mode LBM_RATE_LAGGED is not provided on the GPU");
+    return fused_dispatch(h, kk0, nsteps, false, true, ms_total, ms_main);
+}
+
+}  // extern "C"

@@ -1,0 +1,599 @@
+// Device-side building blocks of the D2Q9 MRT / smoothed-profile time step
+// for gfx950.  Reference routines are cited as F6:line
+// (6_FreeFall_YieldStress/D2Q9_BinghamSediment_Validated_20191127.f90) and
+// F3:line (3_MultiParticle_Sedimentation/D2Q9_RTParticle_20191003.f90).
+//
+// Two per-node pipelines share the index logic:
+//  * FP64, reference-exact: every expression keeps the reference's operation
+//    order and float32-rounded literals (the library is built with
+//    -ffp-contract=off), so it reproduces the reference up to the last bit of
+//    sin/exp;
+//  * FP32 deviation form: populations are handled as h_i = f_i - w_i*rho0 and
+//    every moment is taken relative to the rest state, so FP32 arithmetic
+//    keeps the small quantities at full relative precision.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+constexpr int NT = 256;        // threads per block (4 wavefronts of 64)
+constexpr int NSLOT = 32;      // fixed-point accumulation slots per step
+constexpr double MASS_SCALE = 68719476736.0;  // 2^36: fixed point of mass sums
+constexpr int MAXNB = 32;      // contact partners recorded per particle
+
+// float32-rounded literals of the reference, widened to double
+constexpr double D_MRT = (double)(1.0f / 36.0f);    // F6:873
+constexpr double INV6 = (double)(1.0f / 6.0f);      // F6:1202
+constexpr double PI_F = (double)3.14159265359f;     // F6:3
+constexpr double FEQ_W0 = (double)(2.0f / 9.0f);    // F6:1135
+constexpr double FEQ_W1 = (double)(1.0f / 18.0f);   // F6:1136
+constexpr double FEQ_W2 = (double)(1.0f / 36.0f);   // F6:1137
+
+// Lattice weights of the rest state used by the deviation form.
+constexpr float WF0 = 4.0f / 9.0f, WF1 = 1.0f / 9.0f, WF2 = 1.0f / 36.0f;
+
+// Particle record fields, SoA with stride maxp (LBM_PF_* of lbm_amd.h).
+enum { PX = 0, PY, PANG, PVX, PVY, PW, PDIAM, PFX, PFY, PTQ, PFCX, PFCY,
+       PRHOP, PMP, PIP, PZETA, PVOX, PVOY, PVOW, PFHX, PFHY, PFHW, PCOUNT };
+constexpr int HPC = 12;        // history doubles per particle
+
+struct DevP {
+    int nx, ny, W, H;
+    long long nn;        // nodes (W*H)
+    long long plane;     // SoA plane stride in elements
+    int rheology, q_mode, update_angle, np;
+    int maxp;            // particle SoA stride
+    int nbnd;            // boundary nodes
+    double omega, s0, s1, s2, s3, s5, sq, nu, mu0, tau_y, m_pap, gamma_min, rho0, gy;
+    double wall[8], corner[2];
+    double zeta_c, cij, eps_p, eps_w, e_p, e_w;
+};
+
+__device__ __forceinline__ int cxk(int k) {
+    return (k == 1 || k == 5 || k == 8) ? 1 : ((k == 3 || k == 6 || k == 7) ? -1 : 0);
+}
+__device__ __forceinline__ int cyk(int k) {
+    return (k == 2 || k == 5 || k == 6) ? 1 : ((k == 4 || k == 7 || k == 8) ? -1 : 0);
+}
+
+// Index of boundary node (i, j) in the edge arrays: bottom row, top row,
+// left column (j=1..ny-1), right column (j=1..ny-1); -1 for interior nodes.
+__device__ __forceinline__ int bnd_index(const DevP &P, int i, int j) {
+    if (j == 0) return i;
+    if (j == P.ny) return P.W + i;
+    if (i == 0) return 2 * P.W + (j - 1);
+    if (i == P.nx) return 2 * P.W + (P.H - 2) + (j - 1);
+    return -1;
+}
+
+// ---------------------------------------------------------------- storage
+// Storage formats of the fused path.  FP64 stores f itself; the reduced
+// formats store h = f - w_i*rho0 (times 2^10 for FP16 to stay clear of
+// subnormals), so load/store are exact inverses up to the format's rounding.
+struct StF64 {
+    using T = double;
+    __device__ static double ld(const T *p) { return *p; }
+    __device__ static void st(T *p, double v) { *p = v; }
+};
+struct StF32 {
+    using T = float;
+    __device__ static float ld(const T *p) { return *p; }
+    __device__ static void st(T *p, float v) { *p = v; }
+};
+struct StF16 {
+    using T = uint16_t;
+    __device__ static float ld(const T *p) {
+        return (float)__builtin_bit_cast(_Float16, *p) * (1.0f / 1024.0f);
+    }
+    __device__ static void st(T *p, float v) {
+        *p = __builtin_bit_cast(uint16_t, (_Float16)(v * 1024.0f));
+    }
+};
+struct StBF16 {
+    using T = uint16_t;
+    __device__ static float ld(const T *p) {
+        return __builtin_bit_cast(float, (uint32_t)(*p) << 16);
+    }
+    __device__ static void st(T *p, float v) {
+        uint32_t u = __builtin_bit_cast(uint32_t, v);
+        u += 0x7FFFu + ((u >> 16) & 1u);           // round to nearest even
+        *p = (uint16_t)(u >> 16);
+    }
+};
+
+__device__ __forceinline__ float wdev(int k) {
+    return k == 0 ? WF0 : (k < 5 ? WF1 : WF2);
+}
+
+// --------------------------------------------------------- FP64 pipeline
+// MRT relaxation vector for a node rate (ComputeRelaxation_MRT, F6:1002-1011).
+__device__ __forceinline__ void rates64(const DevP &P, double om, double s[9]) {
+    double q = P.q_mode ? (8.0 * (2.0 - om)) / (8.0 - om) : P.sq;
+    s[0] = P.s0; s[1] = P.s1; s[2] = P.s2; s[3] = P.s3; s[4] = q;
+    s[5] = P.s5; s[6] = q; s[7] = om; s[8] = om;
+}
+
+// Pull-stream with in-domain sources (F6:1164-1188) and the bounce-back
+// rules (F6:1206-1294, F6:1364-1386) for node (i, j): v receives f_k.
+// src: post-collision populations (SoA planes), erho: previous-step density
+// on the boundary nodes.
+template <class St>
+__device__ __forceinline__ void load_fk64(const DevP &P, const typename St::T *src,
+                                          const double *erho, long long node, int i,
+                                          int j, double v[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        int si = i - cxk(k), sj = j - cyk(k);
+        bool in = (unsigned)si <= (unsigned)P.nx && (unsigned)sj <= (unsigned)P.ny;
+        long long s = in ? node - cxk(k) - (long long)cyk(k) * P.W : node;
+        v[k] = St::ld(src + k * P.plane + s);
+    }
+    if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
+        const double clb2 = 1.0;
+        double rw = erho[bnd_index(P, i, j)];
+        auto fo = [&](int k) { return St::ld(src + k * P.plane + node); };
+        const double *wl = P.wall;
+        if (j == P.ny && i >= 1 && i <= P.nx - 1) {
+            double uxN = wl[4], uyN = wl[5];
+            v[8] = fo(6) - INV6 * clb2 * rw * (-uxN + uyN);
+            v[4] = fo(2) - 4.0 * INV6 * clb2 * rw * (uyN);
+            v[7] = fo(5) - INV6 * clb2 * rw * (uxN + uyN);
+        } else if (j == 0 && i >= 1 && i <= P.nx - 1) {
+            double uxS = wl[0], uyS = wl[1];
+            v[5] = fo(7) - INV6 * clb2 * rw * (-uxS - uyS);
+            v[2] = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyS);
+            v[6] = fo(8) - INV6 * clb2 * rw * (uxS - uyS);
+        } else if (i == P.nx && j >= 1 && j <= P.ny - 1) {
+            double uxR = wl[2], uyR = wl[3];
+            v[6] = fo(8) - INV6 * clb2 * rw * (uxR - uyR);
+            v[3] = fo(1) - 4.0 * INV6 * clb2 * rw * (uxR);
+            v[7] = fo(5) - INV6 * clb2 * rw * (uxR + uyR);
+        } else if (i == 0 && j >= 1 && j <= P.ny - 1) {
+            double uxL = wl[6], uyL = wl[7];
+            v[5] = fo(7) - INV6 * clb2 * rw * (-uxL - uyL);
+            v[1] = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxL);
+            v[8] = fo(6) - INV6 * clb2 * rw * (-uxL + uyL);
+        } else {
+            double uxw = P.corner[0], uyw = P.corner[1];
+            if (i == 0 && j == 0) {
+                v[1] = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxw);
+                v[5] = fo(7) - INV6 * clb2 * rw * (-uxw - uyw);
+                v[2] = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyw);
+            } else if (i == 0) {  // (0, ny)
+                v[1] = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxw);
+                v[8] = fo(6) - INV6 * clb2 * rw * (-uxw + uyw);
+                v[4] = fo(2) - 4.0 * INV6 * clb2 * rw * (uyw);
+            } else if (j == P.ny) {  // (nx, ny)
+                v[3] = fo(1) - 4.0 * INV6 * clb2 * rw * (uxw);
+                v[7] = fo(5) - INV6 * clb2 * rw * (uxw + uyw);
+                v[4] = fo(2) - 4.0 * INV6 * clb2 * rw * (uyw);
+            } else {  // (nx, 0)
+                v[2] = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyw);
+                v[6] = fo(8) - INV6 * clb2 * rw * (uxw - uyw);
+                v[3] = fo(1) - 4.0 * INV6 * clb2 * rw * (uxw);
+            }
+        }
+    }
+}
+
+// MACROSCOPICVARIABLES, F6:1405-1408.
+__device__ __forceinline__ void macros64(const double v[9], double &rho, double &ux,
+                                         double &uy) {
+    rho = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7] + v[8];
+    double rinv = 1.0 / rho;
+    ux = rinv * (v[1] + v[5] + v[8] - (v[6] + v[3] + v[7]));
+    uy = rinv * (v[5] + v[2] + v[6] - (v[7] + v[4] + v[8]));
+}
+
+// Solid fraction of one disk at node (i, j), F6:451-459.
+__device__ __forceinline__ double phi_at(double xc, double yc, double diam, double zeta,
+                                         int i, int j) {
+    double rp = 0.5 * diam;
+    double dx = (double)i - xc, dy = (double)j - yc;
+    double di = rp - sqrt(dx * dx + dy * dy);
+    double ph = 0.0;
+    if (di <= -(0.5 * zeta))
+        ph = 0.0;
+    else if (fabs(di) < 0.5 * zeta)
+        ph = 0.5 * (sin(PI_F * di / zeta) + 1.0);
+    else if (di >= 0.5 * zeta)
+        ph = 1.0;
+    return ph;
+}
+
+// Conservative test: can particle p have phi > 0 at node (i, j)?  phi > 0
+// needs distance < r + zeta/2; one lattice unit of margin covers rounding.
+__device__ __forceinline__ bool in_reach(const double *part, int maxp, int p, int i, int j) {
+    double reach = 0.5 * part[PDIAM * maxp + p] + 0.5 * part[PZETA * maxp + p] + 1.0;
+    return fabs((double)i - part[PX * maxp + p]) <= reach &&
+           fabs((double)j - part[PY * maxp + p]) <= reach;
+}
+
+// ComputeForceField_SPM at one node (F6:485-495): the highest particle index
+// with phi > 0 sets G.  Returns true when some particle covers the node.
+__device__ __forceinline__ bool spm_node(const double *part, int maxp, int np, int i, int j,
+                                         double ux, double uy, double &gx, double &gy) {
+    gx = 0.0;
+    gy = 0.0;
+    bool any = false;
+    for (int p = 0; p < np; ++p) {
+        if (!in_reach(part, maxp, p, i, j)) continue;
+        double xc = part[PX * maxp + p], yc = part[PY * maxp + p];
+        double ph = phi_at(xc, yc, part[PDIAM * maxp + p], part[PZETA * maxp + p], i, j);
+        if (ph > 0.0) {
+            double w = part[PW * maxp + p];
+            double uxp = part[PVX * maxp + p] - w * ((double)j - yc);
+            double uyp = part[PVY * maxp + p] + w * ((double)i - xc);
+            gx = ph * (uxp - ux);
+            gy = ph * (uyp - uy);
+            any = true;
+        }
+    }
+    return any;
+}
+
+struct CollideOut {
+    double omega_bn;     // new node rate (Bingham)
+    double stress[3];
+    int yielded;         // tau_mag > tau_y
+};
+
+// Collision_MRT at one node, F6:878-958, with the node's own rate om
+// (Newtonian runs pass OMEGA).  f is overwritten by the post-collision values.
+__device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho, double ux,
+                                          double uy, double gx, double gy, double om,
+                                          CollideOut &o) {
+    double s[9];
+    rates64(P, om, s);
+    double m[9], mf[9], meq[9], mne[9], f1[9];
+    m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    m[1] = -(4.0 * f[0]) - f[1] - f[2] - f[3] - f[4] + 2.0 * (f[5] + f[6] + f[7] + f[8]);
+    m[2] = 4.0 * f[0] - 2.0 * (f[1] + f[2] + f[3] + f[4]) + (f[5] + f[6] + f[7] + f[8]);
+    m[3] = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+    m[4] = -(2.0 * (f[1] - f[3])) + f[5] - f[6] - f[7] + f[8];
+    m[5] = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+    m[6] = -(2.0 * (f[2] - f[4])) + f[5] + f[6] - f[7] - f[8];
+    m[7] = f[1] - f[2] + f[3] - f[4];
+    m[8] = f[5] - f[6] + f[7] - f[8];
+    // ComputeSourceEquilib_MRT, F6:964-989
+    double fx = rho * (gx + 0.0), fy = rho * (gy + 0.0);
+    mf[0] = 0.0;
+    mf[1] = 6.0 * (fx * ux + fy * uy);
+    mf[2] = -(6.0 * (fx * ux + fy * uy));
+    mf[3] = fx; mf[4] = -fx; mf[5] = fy; mf[6] = -fy;
+    mf[7] = 2.0 * (fx * ux - fy * uy);
+    mf[8] = fy * ux + fx * uy;
+    double u2 = ux * ux + uy * uy;
+    meq[0] = rho;
+    meq[1] = rho * (-2.0 + 3.0 * u2);
+    meq[2] = rho * (1.0 - 3.0 * u2);
+    meq[3] = rho * (ux); meq[4] = rho * (-ux);
+    meq[5] = rho * (uy); meq[6] = rho * (-uy);
+    meq[7] = rho * (ux * ux - uy * uy);
+    meq[8] = rho * (ux * uy);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) mne[k] = m[k] - meq[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        m[k] = s[k] * meq[k] + (1.0 - s[k]) * m[k] + (1.0 - 0.5 * s[k]) * mf[k];
+    const double d = D_MRT;
+    f[0] = d * (4.0 * (m[0] - m[1] + m[2]));
+    f[1] = d * (4.0 * m[0] - m[1] - 2.0 * m[2] + 6.0 * m[3] - 6.0 * m[4] + 9.0 * m[7]);
+    f[2] = d * (4.0 * m[0] - m[1] - 2.0 * m[2] + 6.0 * m[5] - 6.0 * m[6] - 9.0 * m[7]);
+    f[3] = d * (4.0 * m[0] - m[1] - 2.0 * m[2] - 6.0 * m[3] + 6.0 * m[4] + 9.0 * m[7]);
+    f[4] = d * (4.0 * m[0] - m[1] - 2.0 * m[2] - 6.0 * m[5] + 6.0 * m[6] - 9.0 * m[7]);
+    f[5] = d * (4.0 * m[0] + 2.0 * m[1] + m[2] + 6.0 * m[3] + 3.0 * m[4] + 6.0 * m[5] + 3.0 * m[6] + 9.0 * m[8]);
+    f[6] = d * (4.0 * m[0] + 2.0 * m[1] + m[2] - 6.0 * m[3] - 3.0 * m[4] + 6.0 * m[5] + 3.0 * m[6] - 9.0 * m[8]);
+    f[7] = d * (4.0 * m[0] + 2.0 * m[1] + m[2] - 6.0 * m[3] - 3.0 * m[4] - 6.0 * m[5] - 3.0 * m[6] + 9.0 * m[8]);
+    f[8] = d * (4.0 * m[0] + 2.0 * m[1] + m[2] + 6.0 * m[3] + 3.0 * m[4] - 6.0 * m[5] - 3.0 * m[6] - 9.0 * m[8]);
+    // Chai strain rate, F6:903-916
+    f1[1] = 4.0 * s[0] * mne[0] - 1.0 * s[1] * mne[1] - 2.0 * s[2] * mne[2] + 6.0 * s[3] * mne[3] - 6.0 * s[4] * mne[4] + 9.0 * s[7] * mne[7];
+    f1[2] = 4.0 * s[0] * mne[0] - 1.0 * s[1] * mne[1] - 2.0 * s[2] * mne[2] + 6.0 * s[5] * mne[5] - 6.0 * s[6] * mne[6] - 9.0 * s[7] * mne[7];
+    f1[3] = 4.0 * s[0] * mne[0] - 1.0 * s[1] * mne[1] - 2.0 * s[2] * mne[2] - 6.0 * s[3] * mne[3] + 6.0 * s[4] * mne[4] + 9.0 * s[7] * mne[7];
+    f1[4] = 4.0 * s[0] * mne[0] - 1.0 * s[1] * mne[1] - 2.0 * s[2] * mne[2] - 6.0 * s[5] * mne[5] + 6.0 * s[6] * mne[6] - 9.0 * s[7] * mne[7];
+    f1[5] = 4.0 * s[0] * mne[0] + 2.0 * s[1] * mne[1] + 1.0 * s[2] * mne[2] + 6.0 * s[3] * mne[3] + 3.0 * s[4] * mne[4] + 6.0 * s[5] * mne[5] + 3.0 * s[6] * mne[6] + 9.0 * s[8] * mne[8];
+    f1[6] = 4.0 * s[0] * mne[0] + 2.0 * s[1] * mne[1] + 1.0 * s[2] * mne[2] - 6.0 * s[3] * mne[3] - 3.0 * s[4] * mne[4] + 6.0 * s[5] * mne[5] + 3.0 * s[6] * mne[6] - 9.0 * s[8] * mne[8];
+    f1[7] = 4.0 * s[0] * mne[0] + 2.0 * s[1] * mne[1] + 1.0 * s[2] * mne[2] - 6.0 * s[3] * mne[3] - 3.0 * s[4] * mne[4] - 6.0 * s[5] * mne[5] - 3.0 * s[6] * mne[6] + 9.0 * s[8] * mne[8];
+    f1[8] = 4.0 * s[0] * mne[0] + 2.0 * s[1] * mne[1] + 1.0 * s[2] * mne[2] + 6.0 * s[3] * mne[3] + 3.0 * s[4] * mne[4] - 6.0 * s[5] * mne[5] - 3.0 * s[6] * mne[6] - 9.0 * s[8] * mne[8];
+    double zarib = -1.5 * d;
+    double d11 = zarib * (f1[1] + f1[5] + f1[6] + f1[3] + f1[7] + f1[8])
+               + 0.75 * ((s[7] - s[1]) * (gx * ux + gy * uy) - s[7] * (2.0 * gx * ux));
+    double d22 = zarib * (f1[5] + f1[2] + f1[6] + f1[7] + f1[4] + f1[8])
+               + 0.75 * ((s[7] - s[1]) * (gx * ux + gy * uy) - s[7] * (2.0 * gy * uy));
+    double d12 = zarib * (f1[5] - f1[6] + f1[7] - f1[8])
+               + 0.75 * (-(s[7] * (gx * uy + gy * ux)));
+    o.stress[0] = P.nu * 2.0 * d11;
+    o.stress[1] = P.nu * 2.0 * d22;
+    o.stress[2] = P.nu * 2.0 * d12;
+    o.omega_bn = om;
+    o.yielded = 0;
+    if (P.rheology == 1) {
+        // Papanastasiou-Bingham viscosity, F6:921-957
+        double inv_rho = 1.0 / rho;
+        d11 = d11 * inv_rho; d22 = d22 * inv_rho; d12 = d12 * inv_rho;
+        double gam = sqrt(2.0 * fabs(d11 * d11 + d22 * d22 + 2.0 * (d12 * d12)));
+        double nu_a;
+        if (gam < P.gamma_min)
+            nu_a = P.mu0 + P.m_pap * P.tau_y;
+        else
+            nu_a = P.mu0 + (P.tau_y / gam) * (1.0 - exp(-(P.m_pap * gam)));
+        double t11 = nu_a * 2.0 * d11, t22 = nu_a * 2.0 * d22, t12 = nu_a * 2.0 * d12;
+        o.stress[0] = t11; o.stress[1] = t22; o.stress[2] = t12;
+        double tmag = sqrt(0.5 * fabs(t11 * t11 + t22 * t22 + 2.0 * (t12 * t12)));
+        o.yielded = (tmag <= P.tau_y) ? 0 : 1;
+        nu_a = nu_a * inv_rho;
+        o.omega_bn = 1.0 / (3.0 * nu_a + 0.5);
+    }
+}
+
+// ------------------------------------------------ FP32 deviation pipeline
+// Pull-stream and bounce-back on deviations h = f - w_i*rho0.  Opposite
+// directions share a weight, so the bounce-back rule keeps its form.
+template <class St>
+__device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T *src,
+                                            const float *erho, long long node, int i, int j,
+                                            float h[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        int si = i - cxk(k), sj = j - cyk(k);
+        bool in = (unsigned)si <= (unsigned)P.nx && (unsigned)sj <= (unsigned)P.ny;
+        long long s = in ? node - cxk(k) - (long long)cyk(k) * P.W : node;
+        h[k] = St::ld(src + k * P.plane + s);
+    }
+    if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
+        float rw = erho[bnd_index(P, i, j)];
+        const float a = (float)INV6, b = (float)(4.0 * INV6);
+        auto fo = [&](int k) { return St::ld(src + k * P.plane + node); };
+        const double *wl = P.wall;
+        if (j == P.ny && i >= 1 && i <= P.nx - 1) {
+            float uxN = (float)wl[4], uyN = (float)wl[5];
+            h[8] = fo(6) - a * rw * (-uxN + uyN);
+            h[4] = fo(2) - b * rw * (uyN);
+            h[7] = fo(5) - a * rw * (uxN + uyN);
+        } else if (j == 0 && i >= 1 && i <= P.nx - 1) {
+            float uxS = (float)wl[0], uyS = (float)wl[1];
+            h[5] = fo(7) - a * rw * (-uxS - uyS);
+            h[2] = fo(4) - b * rw * (-uyS);
+            h[6] = fo(8) - a * rw * (uxS - uyS);
+        } else if (i == P.nx && j >= 1 && j <= P.ny - 1) {
+            float uxR = (float)wl[2], uyR = (float)wl[3];
+            h[6] = fo(8) - a * rw * (uxR - uyR);
+            h[3] = fo(1) - b * rw * (uxR);
+            h[7] = fo(5) - a * rw * (uxR + uyR);
+        } else if (i == 0 && j >= 1 && j <= P.ny - 1) {
+            float uxL = (float)wl[6], uyL = (float)wl[7];
+            h[5] = fo(7) - a * rw * (-uxL - uyL);
+            h[1] = fo(3) - b * rw * (-uxL);
+            h[8] = fo(6) - a * rw * (-uxL + uyL);
+        } else {
+            float uxw = (float)P.corner[0], uyw = (float)P.corner[1];
+            if (i == 0 && j == 0) {
+                h[1] = fo(3) - b * rw * (-uxw);
+                h[5] = fo(7) - a * rw * (-uxw - uyw);
+                h[2] = fo(4) - b * rw * (-uyw);
+            } else if (i == 0) {
+                h[1] = fo(3) - b * rw * (-uxw);
+                h[8] = fo(6) - a * rw * (-uxw + uyw);
+                h[4] = fo(2) - b * rw * (uyw);
+            } else if (j == P.ny) {
+                h[3] = fo(1) - b * rw * (uxw);
+                h[7] = fo(5) - a * rw * (uxw + uyw);
+                h[4] = fo(2) - b * rw * (uyw);
+            } else {
+                h[2] = fo(4) - b * rw * (-uyw);
+                h[6] = fo(8) - a * rw * (uxw - uyw);
+                h[3] = fo(1) - b * rw * (uxw);
+            }
+        }
+    }
+}
+
+// Density deviation and velocity from deviations (sum of c_i w_i is zero).
+__device__ __forceinline__ void macros_dev(const float h[9], float rho0, float &rho_dev,
+                                           float &rho, float &ux, float &uy) {
+    rho_dev = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
+    rho = rho0 + rho_dev;
+    float rinv = 1.0f / rho;
+    ux = rinv * ((h[1] + h[5] + h[8]) - (h[6] + h[3] + h[7]));
+    uy = rinv * ((h[5] + h[2] + h[6]) - (h[7] + h[4] + h[8]));
+}
+
+// Collision_MRT in deviation form and FP32 (same physics as collide64).
+__device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho_dev, float rho,
+                                            float ux, float uy, float gx, float gy, float om,
+                                            float &om_out, float stress[3], int &yielded) {
+    float q = P.q_mode ? (8.0f * (2.0f - om)) / (8.0f - om) : (float)P.sq;
+    float s[9] = {(float)P.s0, (float)P.s1, (float)P.s2, (float)P.s3, q,
+                  (float)P.s5, q, om, om};
+    float m[9], mf[9], meq[9], mne[9];
+    m[0] = rho_dev;
+    m[1] = -4.0f * h[0] - (h[1] + h[2] + h[3] + h[4]) + 2.0f * (h[5] + h[6] + h[7] + h[8]);
+    m[2] = 4.0f * h[0] - 2.0f * (h[1] + h[2] + h[3] + h[4]) + (h[5] + h[6] + h[7] + h[8]);
+    m[3] = (h[1] - h[3]) + (h[5] - h[6]) - (h[7] - h[8]);
+    m[4] = -2.0f * (h[1] - h[3]) + (h[5] - h[6]) - (h[7] - h[8]);
+    m[5] = (h[2] - h[4]) + (h[5] + h[6]) - (h[7] + h[8]);
+    m[6] = -2.0f * (h[2] - h[4]) + (h[5] + h[6]) - (h[7] + h[8]);
+    m[7] = (h[1] - h[2]) + (h[3] - h[4]);
+    m[8] = (h[5] - h[6]) + (h[7] - h[8]);
+    float fx = rho * gx, fy = rho * gy;
+    float fu = fx * ux + fy * uy;
+    mf[0] = 0.0f; mf[1] = 6.0f * fu; mf[2] = -6.0f * fu;
+    mf[3] = fx; mf[4] = -fx; mf[5] = fy; mf[6] = -fy;
+    mf[7] = 2.0f * (fx * ux - fy * uy);
+    mf[8] = fy * ux + fx * uy;
+    float u2 = ux * ux + uy * uy;
+    // equilibrium moments relative to the rest state (rho0, -2 rho0, rho0, 0...)
+    meq[0] = rho_dev;
+    meq[1] = -2.0f * rho_dev + 3.0f * rho * u2;
+    meq[2] = rho_dev - 3.0f * rho * u2;
+    meq[3] = rho * ux; meq[4] = -rho * ux;
+    meq[5] = rho * uy; meq[6] = -rho * uy;
+    meq[7] = rho * (ux * ux - uy * uy);
+    meq[8] = rho * (ux * uy);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) mne[k] = m[k] - meq[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        m[k] = s[k] * meq[k] + (1.0f - s[k]) * m[k] + (1.0f - 0.5f * s[k]) * mf[k];
+    const float d = 1.0f / 36.0f;
+    float a = 4.0f * m[0], b1 = a - m[1] - 2.0f * m[2], b2 = a + 2.0f * m[1] + m[2];
+    h[0] = d * (4.0f * (m[0] - m[1] + m[2]));
+    h[1] = d * (b1 + 6.0f * m[3] - 6.0f * m[4] + 9.0f * m[7]);
+    h[2] = d * (b1 + 6.0f * m[5] - 6.0f * m[6] - 9.0f * m[7]);
+    h[3] = d * (b1 - 6.0f * m[3] + 6.0f * m[4] + 9.0f * m[7]);
+    h[4] = d * (b1 - 6.0f * m[5] + 6.0f * m[6] - 9.0f * m[7]);
+    float c3 = 6.0f * m[3] + 3.0f * m[4], c5 = 6.0f * m[5] + 3.0f * m[6];
+    h[5] = d * (b2 + c3 + c5 + 9.0f * m[8]);
+    h[6] = d * (b2 - c3 + c5 - 9.0f * m[8]);
+    h[7] = d * (b2 - c3 - c5 + 9.0f * m[8]);
+    h[8] = d * (b2 + c3 - c5 - 9.0f * m[8]);
+    // Chai strain rate (F6:903-916); only f1[1..8] enter it
+    float e0 = 4.0f * s[0] * mne[0], e1 = s[1] * mne[1], e2 = s[2] * mne[2];
+    float e3 = 6.0f * s[3] * mne[3], e4 = s[4] * mne[4], e5 = 6.0f * s[5] * mne[5];
+    float e6 = s[6] * mne[6], e7 = 9.0f * s[7] * mne[7], e8 = 9.0f * s[8] * mne[8];
+    float ba = e0 - e1 - 2.0f * e2, bb = e0 + 2.0f * e1 + e2;
+    float f1_1 = ba + e3 - 6.0f * e4 + e7, f1_2 = ba + e5 - 6.0f * e6 - e7;
+    float f1_3 = ba - e3 + 6.0f * e4 + e7, f1_4 = ba - e5 + 6.0f * e6 - e7;
+    float q3 = e3 + 3.0f * e4, q5 = e5 + 3.0f * e6;
+    float f1_5 = bb + q3 + q5 + e8, f1_6 = bb - q3 + q5 - e8;
+    float f1_7 = bb - q3 - q5 + e8, f1_8 = bb + q3 - q5 - e8;
+    float zarib = -1.5f * d;
+    float gu = gx * ux + gy * uy;
+    float d11 = zarib * (f1_1 + f1_5 + f1_6 + f1_3 + f1_7 + f1_8)
+              + 0.75f * ((s[7] - s[1]) * gu - s[7] * (2.0f * gx * ux));
+    float d22 = zarib * (f1_5 + f1_2 + f1_6 + f1_7 + f1_4 + f1_8)
+              + 0.75f * ((s[7] - s[1]) * gu - s[7] * (2.0f * gy * uy));
+    float d12 = zarib * (f1_5 - f1_6 + f1_7 - f1_8) - 0.75f * (s[7] * (gx * uy + gy * ux));
+    float nu = (float)P.nu;
+    stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
+    om_out = om;
+    yielded = 0;
+    if (P.rheology == 1) {
+        float inv_rho = 1.0f / rho;
+        d11 *= inv_rho; d22 *= inv_rho; d12 *= inv_rho;
+        float gam = sqrtf(2.0f * fabsf(d11 * d11 + d22 * d22 + 2.0f * (d12 * d12)));
+        float mu0 = (float)P.mu0, ty = (float)P.tau_y, mp = (float)P.m_pap;
+        float nu_a;
+        if (gam < (float)P.gamma_min)
+            nu_a = mu0 + mp * ty;
+        else
+            nu_a = mu0 + (ty / gam) * (1.0f - expf(-(mp * gam)));
+        float t11 = nu_a * 2.0f * d11, t22 = nu_a * 2.0f * d22, t12 = nu_a * 2.0f * d12;
+        stress[0] = t11; stress[1] = t22; stress[2] = t12;
+        float tmag = sqrtf(0.5f * fabsf(t11 * t11 + t22 * t22 + 2.0f * (t12 * t12)));
+        yielded = (tmag <= ty) ? 0 : 1;
+        om_out = 1.0f / (3.0f * (nu_a * inv_rho) + 0.5f);
+    }
+}
+
+// --------------------------------------------------------- block helpers
+// Fixed-order tree reduction of NV doubles over the block (deterministic).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*NT */) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) lds[a * NT + t] = v[a];
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int a = 0; a < NV; ++a) lds[a * NT + t] += lds[a * NT + t + s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < NV; ++a) v[a] = lds[a * NT];
+    __syncthreads();
+}
+
+// Contact force on particle p (Glowinski model; N-body pair order of
+// F3:746-770, walls of F3:773-804), evaluated by one thread.  The pair branch
+// reads dpw, the D_PW left by the previous call (F3:760).  Writes in-range
+// partners j>p into nb[0..MAXNB) and returns their count; *dpw_last receives
+// this particle's last wall distance.
+__device__ __forceinline__ int contact_one(const DevP &P, const double *part, int maxp, int np,
+                                           int p, double dpw, double &fcx, double &fcy,
+                                           int *nb, double *dpw_last) {
+    const double zeta = P.zeta_c, cij = P.cij;
+    fcx = 0.0; fcy = 0.0;
+    int cnt = 0;
+    auto pair_force = [&](int ip, int jp, double &f0, double &f1) -> bool {
+        double xi = part[PX * maxp + ip], yi = part[PY * maxp + ip];
+        double xj = part[PX * maxp + jp], yj = part[PY * maxp + jp];
+        double di = part[PDIAM * maxp + ip], dj = part[PDIAM * maxp + jp];
+        double dpp = sqrt((xi - xj) * (xi - xj) + (yi - yj) * (yi - yj));
+        double rsum = 0.5 * di + 0.5 * dj;
+        f0 = 0.0; f1 = 0.0;
+        if (dpp > rsum + zeta) return false;
+        double t = (dpp - 0.5 * di - 0.5 * dj - zeta) / zeta;
+        if (dpw > rsum) {
+            f0 = P.eps_p * cij * (t * t) * (xi - xj) / dpp;
+            f1 = P.eps_p * cij * (t * t) * (yi - yj) / dpp;
+        } else {
+            f0 = (P.eps_p * cij * (t * t) + P.e_p * cij * (rsum - dpp) / zeta) * (xi - xj) / dpp;
+            f1 = (P.eps_p * cij * (t * t) + P.e_p * cij * (rsum - dpp) / zeta) * (yi - yj) / dpp;
+        }
+        return true;
+    };
+    // accumulation order seen by particle p in the reference loop: pairs
+    // (j, p) for j < p subtract, then pairs (p, j) for j > p add
+    for (int q = 0; q < np; ++q) {
+        if (q == p) continue;
+        double f0, f1;
+        bool hit = (q < p) ? pair_force(q, p, f0, f1) : pair_force(p, q, f0, f1);
+        if (!hit) continue;
+        if (q < p) { fcx = fcx - f0; fcy = fcy - f1; }
+        else {
+            fcx = fcx + f0; fcy = fcy + f1;
+            if (cnt < MAXNB) nb[cnt] = q;
+            ++cnt;
+        }
+    }
+    double x = part[PX * maxp + p], y = part[PY * maxp + p], dm = part[PDIAM * maxp + p];
+    double d = 0.0;
+    for (int iw = 1; iw <= 4; ++iw) {
+        double rw0, rw1;
+        if (iw == 1) { rw0 = x; rw1 = -(0.5 * dm); }
+        else if (iw == 2) { rw0 = (double)P.nx + 0.5 * dm; rw1 = y; }
+        else if (iw == 3) { rw0 = x; rw1 = (double)P.ny + 0.5 * dm; }
+        else { rw0 = -(0.5 * dm); rw1 = y; }
+        d = sqrt((x - rw0) * (x - rw0) + (y - rw1) * (y - rw1));
+        double f0 = 0.0, f1 = 0.0;
+        if (d > dm + zeta) {
+            f0 = 0.0; f1 = 0.0;
+        } else if (d <= dm + zeta && d > dm) {
+            double t = (d - dm - zeta) / zeta;
+            f0 = P.eps_w * cij * (t * t) * (x - rw0) / d;
+            f1 = P.eps_w * cij * (t * t) * (y - rw1) / d;
+        } else if (d <= dm) {
+            double t = (d - dm - zeta) / zeta;
+            f0 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (x - rw0) / d;
+            f1 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (y - rw1) / d;
+        }
+        fcx = fcx + f0; fcy = fcy + f1;
+    }
+    *dpw_last = d;
+    return cnt;
+}
+
+// Newton-Euler update of one particle record in place (F6:302-312).
+__device__ __forceinline__ void update_one(const DevP &P, double *rec /* PCOUNT, stride 1 */) {
+    const double rho0 = P.rho0;
+    double rhop = rec[PRHOP], mp = rec[PMP], ipm = rec[PIP];
+    double v0 = rec[PVX], v1 = rec[PVY], v2 = rec[PW];
+    double fh0 = (rho0 / rhop) * mp * (v0 - rec[PVOX]);
+    double fh1 = (rho0 / rhop) * mp * (v1 - rec[PVOY]);
+    double fh2 = (rho0 / rhop) * ipm * (v2 - rec[PVOW]);
+    rec[PFHX] = fh0; rec[PFHY] = fh1; rec[PFHW] = fh2;
+    rec[PVOX] = v0; rec[PVOY] = v1; rec[PVOW] = v2;
+    double n0 = v0 + rec[PFX] / mp + fh0 / mp + rec[PFCX] / mp;
+    double n1 = v1 + rec[PFY] / mp + fh1 / mp + (1.0 - rho0 / rhop) * P.gy + rec[PFCY] / mp;
+    double n2 = v2 + rec[PTQ] / ipm + fh2 / ipm;
+    rec[PVX] = n0; rec[PVY] = n1; rec[PW] = n2;
+    rec[PX] = rec[PX] + 0.5 * (n0 + v0);
+    rec[PY] = rec[PY] + 0.5 * (n1 + v1);
+    if (P.update_angle) rec[PANG] = rec[PANG] + 0.5 * (n2 + v2);
+}
+
+}  // namespace lbm

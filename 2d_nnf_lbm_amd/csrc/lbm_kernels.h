@@ -1,0 +1,649 @@
+// Kernels of the D2Q9 MRT / smoothed-profile time step for gfx950.
+//
+// Drop-in kernels (one per reference subroutine) work on the FP64 state in
+// the reference's meaning; the fused kernels run the whole time step in one
+// pass over the grid.  Layout in HBM: structure of arrays, one plane per
+// population (plane stride P.plane elements), node index j*(nx+1)+i.
+#pragma once
+#include "lbm_device.h"
+
+namespace lbm {
+
+// Tiles are handed to the 8 XCDs in contiguous ranges so that rows shared
+// between neighbouring tiles meet in the same L2 (blocks are dispatched to
+// XCDs round-robin by block index).
+__device__ __forceinline__ long long xcd_tile(long long b, long long T) {
+    long long x = b & 7, per = T >> 3, rem = T & 7;
+    long long start = x * per + (x < rem ? x : rem);
+    return start + (b >> 3);
+}
+
+// =============================================================== drop-in
+__global__ void k_init(DevP P, double *f, double *rho, double *ux, double *uy, double *g,
+                       double *obn) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    double r = P.rho0, u = 0.0, v = 0.0;
+    // COMPUTEFEQNEW at rest, F6:1134-1146
+    double u2 = u * u + v * v;
+    double fe[9];
+    fe[0] = FEQ_W0 * r * (2.0 - 3.0 * u2);
+    fe[1] = FEQ_W1 * r * (2.0 + 6.0 * u + 9.0 * u * u - 3.0 * u2);
+    fe[2] = FEQ_W1 * r * (2.0 + 6.0 * v + 9.0 * v * v - 3.0 * u2);
+    fe[3] = FEQ_W1 * r * (2.0 - 6.0 * u + 9.0 * u * u - 3.0 * u2);
+    fe[4] = FEQ_W1 * r * (2.0 - 6.0 * v + 9.0 * v * v - 3.0 * u2);
+    fe[5] = FEQ_W2 * r * (1.0 + 3.0 * (u + v) + 9.0 * u * v + 3.0 * u2);
+    fe[6] = FEQ_W2 * r * (1.0 - 3.0 * (u - v) - 9.0 * u * v + 3.0 * u2);
+    fe[7] = FEQ_W2 * r * (1.0 - 3.0 * (u + v) + 9.0 * u * v + 3.0 * u2);
+    fe[8] = FEQ_W2 * r * (1.0 + 3.0 * (u - v) - 9.0 * u * v + 3.0 * u2);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k * P.plane + n] = fe[k];
+    rho[n] = r; ux[n] = u; uy[n] = v;
+    g[n] = 0.0; g[P.plane + n] = 0.0;
+    obn[n] = P.omega;
+}
+
+// Collision_MRT (F6:867-962) with each node's own rate.
+__global__ void k_collide(DevP P, double *f, const double *rho, const double *ux,
+                          const double *uy, const double *g, double *obn, double *stress,
+                          int *image) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    double v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = f[k * P.plane + n];
+    double om = P.rheology == 1 ? obn[n] : P.omega;
+    CollideOut o;
+    collide64(P, v, rho[n], ux[n], uy[n], g[n], g[P.plane + n], om, o);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k * P.plane + n] = v[k];
+    stress[n] = o.stress[0];
+    stress[P.plane + n] = o.stress[1];
+    stress[2 * P.plane + n] = o.stress[2];
+    if (P.rheology == 1) {
+        image[n] = o.yielded ? 0 : 5;
+        obn[n] = o.omega_bn;
+    }
+}
+
+// STREAMING, F6:1164-1188.
+__global__ void k_stream(DevP P, double *f, const double *fp) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    int j = (int)(n / P.W), i = (int)(n - (long long)j * P.W);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        int si = i - cxk(k), sj = j - cyk(k);
+        if ((unsigned)si <= (unsigned)P.nx && (unsigned)sj <= (unsigned)P.ny)
+            f[k * P.plane + n] = fp[k * P.plane + n - cxk(k) - (long long)cyk(k) * P.W];
+    }
+}
+
+__device__ __forceinline__ void bnd_node(const DevP &P, int b, int &i, int &j) {
+    if (b < P.W) { i = b; j = 0; }
+    else if (b < 2 * P.W) { i = b - P.W; j = P.ny; }
+    else if (b < 2 * P.W + P.H - 2) { i = 0; j = b - 2 * P.W + 1; }
+    else { i = P.nx; j = b - 2 * P.W - (P.H - 2) + 1; }
+}
+
+// BOUNDARYCONDITION, F6:1195-1390: one thread per boundary node.  The rules
+// at each node are those of load_fk64 with the streamed values kept for the
+// populations the rules do not touch.
+__global__ void k_boundary(DevP P, double *f, const double *fp, const double *rho) {
+    int b = blockIdx.x * NT + threadIdx.x;
+    if (b >= P.nbnd) return;
+    int i, j;
+    bnd_node(P, b, i, j);
+    long long n = (long long)j * P.W + i;
+    const double clb2 = 1.0;
+    double rw = rho[n];
+    auto fo = [&](int k) { return fp[k * P.plane + n]; };
+    auto F = [&](int k) -> double & { return f[k * P.plane + n]; };
+    const double *wl = P.wall;
+    if (j == P.ny && i >= 1 && i <= P.nx - 1) {
+        double uxN = wl[4], uyN = wl[5];
+        F(8) = fo(6) - INV6 * clb2 * rw * (-uxN + uyN);
+        F(4) = fo(2) - 4.0 * INV6 * clb2 * rw * (uyN);
+        F(7) = fo(5) - INV6 * clb2 * rw * (uxN + uyN);
+    } else if (j == 0 && i >= 1 && i <= P.nx - 1) {
+        double uxS = wl[0], uyS = wl[1];
+        F(5) = fo(7) - INV6 * clb2 * rw * (-uxS - uyS);
+        F(2) = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyS);
+        F(6) = fo(8) - INV6 * clb2 * rw * (uxS - uyS);
+    } else if (i == P.nx && j >= 1 && j <= P.ny - 1) {
+        double uxR = wl[2], uyR = wl[3];
+        F(6) = fo(8) - INV6 * clb2 * rw * (uxR - uyR);
+        F(3) = fo(1) - 4.0 * INV6 * clb2 * rw * (uxR);
+        F(7) = fo(5) - INV6 * clb2 * rw * (uxR + uyR);
+    } else if (i == 0 && j >= 1 && j <= P.ny - 1) {
+        double uxL = wl[6], uyL = wl[7];
+        F(5) = fo(7) - INV6 * clb2 * rw * (-uxL - uyL);
+        F(1) = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxL);
+        F(8) = fo(6) - INV6 * clb2 * rw * (-uxL + uyL);
+    } else {
+        double uxw = P.corner[0], uyw = P.corner[1];
+        if (i == 0 && j == 0) {
+            F(1) = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxw);
+            F(5) = fo(7) - INV6 * clb2 * rw * (-uxw - uyw);
+            F(2) = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyw);
+        } else if (i == 0) {
+            F(1) = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxw);
+            F(8) = fo(6) - INV6 * clb2 * rw * (-uxw + uyw);
+            F(4) = fo(2) - 4.0 * INV6 * clb2 * rw * (uyw);
+        } else if (j == P.ny) {
+            F(3) = fo(1) - 4.0 * INV6 * clb2 * rw * (uxw);
+            F(7) = fo(5) - INV6 * clb2 * rw * (uxw + uyw);
+            F(4) = fo(2) - 4.0 * INV6 * clb2 * rw * (uyw);
+        } else {
+            F(2) = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyw);
+            F(6) = fo(8) - INV6 * clb2 * rw * (uxw - uyw);
+            F(3) = fo(1) - 4.0 * INV6 * clb2 * rw * (uxw);
+        }
+    }
+}
+
+// MACROSCOPICVARIABLES, F6:1394-1412.
+__global__ void k_macros(DevP P, const double *f, double *rho, double *ux, double *uy) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    double v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = f[k * P.plane + n];
+    double r, u, w;
+    macros64(v, r, u, w);
+    rho[n] = r; ux[n] = u; uy[n] = w;
+}
+
+// Solid-fraction patches (ComputeConcenterationFunction, F6:449-459): phi of
+// particle p on a PS x PS patch whose origin covers every node that can have
+// phi > 0.  Nodes outside the domain hold 0.
+__device__ __forceinline__ void patch_origin(const double *part, int maxp, int p, int &i0,
+                                             int &j0) {
+    double reach = 0.5 * part[PDIAM * maxp + p] + 0.5 * part[PZETA * maxp + p] + 1.0;
+    i0 = (int)floor(part[PX * maxp + p] - reach);
+    j0 = (int)floor(part[PY * maxp + p] - reach);
+}
+
+__global__ void k_phi_patch(DevP P, const double *part, double *phi, int *porig, int PS) {
+    int p = blockIdx.y;
+    int a = blockIdx.x * NT + threadIdx.x;
+    if (a >= PS * PS) return;
+    int i0, j0;
+    patch_origin(part, P.maxp, p, i0, j0);
+    if (a == 0) { porig[2 * p] = i0; porig[2 * p + 1] = j0; }
+    int i = i0 + a % PS, j = j0 + a / PS;
+    double ph = 0.0;
+    if (i >= 0 && i <= P.nx && j >= 0 && j <= P.ny)
+        ph = phi_at(part[PX * P.maxp + p], part[PY * P.maxp + p], part[PDIAM * P.maxp + p],
+                    part[PZETA * P.maxp + p], i, j);
+    phi[(size_t)p * PS * PS + a] = ph;
+}
+
+__device__ __forceinline__ double patch_phi(const double *phi, const int *porig, int PS, int p,
+                                            int i, int j) {
+    int a = i - porig[2 * p], b = j - porig[2 * p + 1];
+    if (a < 0 || b < 0 || a >= PS || b >= PS) return 0.0;
+    return phi[(size_t)p * PS * PS + (size_t)b * PS + a];
+}
+
+// DOMAIN_IMAGE of ComputeConcenterationFunction (F6:438, F6:460-467).
+__global__ void k_image(DevP P, const double *part, const double *phi, const int *porig,
+                        int PS, int *image) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    int j = (int)(n / P.W), i = (int)(n - (long long)j * P.W);
+    int img = 5;
+    for (int p = 0; p < P.np; ++p) {
+        double ph = patch_phi(phi, porig, PS, p, i, j);
+        if (ph >= 0.5) {
+            img = 10;
+            double rp = 0.5 * part[PDIAM * P.maxp + p], tp = part[PANG * P.maxp + p];
+            double xpr = part[PX * P.maxp + p] + 0.75 * rp * cos(tp);
+            double ypr = part[PY * P.maxp + p] + 0.75 * rp * sin(tp);
+            double dp = 25.0 - (((double)i - xpr) * ((double)i - xpr) +
+                                ((double)j - ypr) * ((double)j - ypr));
+            if (dp >= 0) img = 20;
+        }
+    }
+    image[n] = img;
+}
+
+// ComputeForceField_SPM, F6:485-495.
+__global__ void k_forcing(DevP P, const double *part, const double *phi, const int *porig,
+                          int PS, const double *ux, const double *uy, double *g) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    int j = (int)(n / P.W), i = (int)(n - (long long)j * P.W);
+    double gx = 0.0, gy = 0.0;
+    for (int p = 0; p < P.np; ++p) {
+        double ph = patch_phi(phi, porig, PS, p, i, j);
+        if (ph > 0.0) {
+            double w = part[PW * P.maxp + p];
+            double uxp = part[PVX * P.maxp + p] - w * ((double)j - part[PY * P.maxp + p]);
+            double uyp = part[PVY * P.maxp + p] + w * ((double)i - part[PX * P.maxp + p]);
+            gx = ph * (uxp - ux[n]);
+            gy = ph * (uyp - uy[n]);
+        }
+    }
+    g[n] = gx;
+    g[P.plane + n] = gy;
+}
+
+// ComputeForce_HYDRO, F6:542-556: one thread per particle summing its patch
+// in the reference's j-outer / i-inner order (bit-identical to the full-grid
+// sum, since nodes with phi = 0 add nothing).
+__global__ void k_hydro_seq(DevP P, double *part, const double *phi, const int *porig, int PS,
+                            const double *rho, const double *ux, const double *uy) {
+    int p = blockIdx.x * NT + threadIdx.x;
+    if (p >= P.np) return;
+    const int m = P.maxp;
+    double vx = part[PVX * m + p], vy = part[PVY * m + p], w = part[PW * m + p];
+    double xc = part[PX * m + p], yc = part[PY * m + p];
+    int i0 = porig[2 * p], j0 = porig[2 * p + 1];
+    double fp0 = 0.0, fp1 = 0.0, fp2 = 0.0;
+    for (int b = 0; b < PS; ++b) {
+        int j = j0 + b;
+        if (j < 0 || j > P.ny) continue;
+        for (int a = 0; a < PS; ++a) {
+            int i = i0 + a;
+            if (i < 0 || i > P.nx) continue;
+            double ph = phi[(size_t)p * PS * PS + (size_t)b * PS + a];
+            if (ph > 0.0) {
+                long long n = (long long)j * P.W + i;
+                double uxp = vx - w * ((double)j - yc);
+                double uyp = vy + w * ((double)i - xc);
+                double r = rho[n], u = ux[n], v = uy[n];
+                fp0 = fp0 + r * ph * (u - uxp);
+                fp1 = fp1 + r * ph * (v - uyp);
+                fp2 = fp2 + r * ph * (((double)i - xc) * (v - uyp) - ((double)j - yc) * (u - uxp));
+            }
+        }
+    }
+    part[PFX * m + p] = fp0;
+    part[PFY * m + p] = fp1;
+    part[PTQ * m + p] = fp2;
+}
+
+// VelocityCorrector, F6:1118-1124.
+__global__ void k_correct(DevP P, double *ux, double *uy, const double *g) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    ux[n] = ux[n] + 0.5 * 1.0 * (g[n] + 0.0);
+    uy[n] = uy[n] + 0.5 * 1.0 * (g[P.plane + n] + 0.0);
+}
+
+// ComputeCollisionForce: one thread per particle (F3:739-804).
+__global__ void k_contact(DevP P, double *part, const double *dpw_in, double *dpw_out,
+                          int *pairs, int *paircnt) {
+    int p = blockIdx.x * NT + threadIdx.x;
+    if (p >= P.np) return;
+    double fcx, fcy, dl;
+    int cnt = contact_one(P, part, P.maxp, P.np, p, *dpw_in, fcx, fcy, pairs + p * MAXNB, &dl);
+    paircnt[p] = cnt;
+    // all threads have read the centres before any writes the contact force:
+    // the force fields are not read by contact_one
+    part[PFCX * P.maxp + p] = fcx;
+    part[PFCY * P.maxp + p] = fcy;
+    if (p == P.np - 1) *dpw_out = dl;
+}
+
+// Particle update, F6:299-312: one thread per particle.
+__global__ void k_update(DevP P, double *part) {
+    int p = blockIdx.x * NT + threadIdx.x;
+    if (p >= P.np) return;
+    double rec[PCOUNT];
+    for (int f = 0; f < PCOUNT; ++f) rec[f] = part[f * P.maxp + p];
+    update_one(P, rec);
+    for (int f = 0; f < PCOUNT; ++f) part[f * P.maxp + p] = rec[f];
+}
+
+// SUM(RHO): per-block partial sums, then one block sums the partials in order.
+__global__ void k_sum_partial(const double *x, long long n, double *part) {
+    __shared__ double lds[NT];
+    long long i = (long long)blockIdx.x * NT + threadIdx.x;
+    double v[1] = {i < n ? x[i] : 0.0};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+__global__ void k_sum_final(const double *part, int nb, double *out) {
+    __shared__ double lds[NT];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += NT) acc += part[b];
+    double v[1] = {acc};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) *out = v[0];
+}
+
+// AoS (component fastest) <-> SoA planes.
+__global__ void k_aos_to_soa(long long nn, int nc, const double *aos, double *soa,
+                             long long plane) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= nn) return;
+    for (int c = 0; c < nc; ++c) soa[c * plane + n] = aos[n * nc + c];
+}
+__global__ void k_soa_to_aos(long long nn, int nc, const double *soa, double *aos,
+                             long long plane) {
+    long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= nn) return;
+    for (int c = 0; c < nc; ++c) aos[n * nc + c] = soa[c * plane + n];
+}
+
+// ================================================================= fused
+struct StepArgs {
+    DevP P;
+    const void *src;              // post-collision populations f*_k (storage format)
+    void *dst;                    // f*_{k+1} (storage format) or, final step, f_k (FP64)
+    double *cf, *cfp;             // FP64 state arrays F, F_POST
+    double *crho, *cux, *cuy, *cg, *cobn, *cstress;
+    int *cimage;
+    float *rate32;                // node rates of the reduced formats
+    const void *erho_src;         // previous-step density on boundary nodes
+    void *erho_dst;
+    const double *pin;            // particle state k
+    double *pout;                 // particle state k+1 (final step: same as pin)
+    const double *dpw_in;
+    double *dpw_out;
+    int *pairs, *paircnt;
+    double *hist_part;            // history row of step k, or null
+    unsigned long long *slots_mass;   // NSLOT fixed-point mass sums of step k, or null
+    unsigned long long *slots_yield;  // NSLOT yielded-node counts of the collide's step
+    int n_pblocks;
+    long long ntiles;
+    int aux;                      // write STRESS / DOMAIN_IMAGE
+};
+
+// Particle block: thread 0 runs the contact force and the Newton-Euler update
+// of particle p from state k (pin) into state k+1 (pout).
+__device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
+    const DevP &P = a.P;
+    if (threadIdx.x != 0) return;
+    const int m = P.maxp;
+    double rec[PCOUNT];
+    for (int f = 0; f < PCOUNT; ++f) rec[f] = a.pin[f * m + p];
+    double fcx, fcy, dl;
+    int cnt = contact_one(P, a.pin, m, P.np, p, *a.dpw_in, fcx, fcy, a.pairs + p * MAXNB, &dl);
+    a.paircnt[p] = cnt;
+    rec[PFCX] = fcx; rec[PFCY] = fcy;
+    update_one(P, rec);
+    for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
+    if (p == P.np - 1) *a.dpw_out = dl;
+}
+
+// Hydrodynamic force and torque on particle p at step k (ComputeForce_HYDRO,
+// F6:542-556) over the nodes that can have phi > 0, from populations
+// rebuilt by the same pull-stream and bounce-back as the grid pass; a
+// fixed-order block reduction keeps it deterministic.
+template <class St, bool DEV>
+__device__ void particle_force(const StepArgs &a, int p, double (&F)[3]) {
+    __shared__ double lds[3 * NT];
+    const DevP &P = a.P;
+    const int m = P.maxp;
+    const double *pin = a.pin;
+    double xc = pin[PX * m + p], yc = pin[PY * m + p];
+    double vx = pin[PVX * m + p], vy = pin[PVY * m + p], w = pin[PW * m + p];
+    double dm = pin[PDIAM * m + p], zt = pin[PZETA * m + p];
+    double reach = 0.5 * dm + 0.5 * zt + 1.0;
+    int i0 = max(0, (int)ceil(xc - reach)), i1 = min(P.nx, (int)floor(xc + reach));
+    int j0 = max(0, (int)ceil(yc - reach)), j1 = min(P.ny, (int)floor(yc + reach));
+    int bw = i1 - i0 + 1, bh = j1 - j0 + 1;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (bw > 0 && bh > 0) {
+        const typename St::T *src = (const typename St::T *)a.src;
+        for (int t = threadIdx.x; t < bw * bh; t += NT) {
+            int i = i0 + t % bw, j = j0 + t / bw;
+            double ph = phi_at(xc, yc, dm, zt, i, j);
+            if (!(ph > 0.0)) continue;
+            long long node = (long long)j * P.W + i;
+            double r, u, v;
+            if constexpr (DEV) {
+                float h[9], rd, rf, uf, vf;
+                load_fk_dev<St>(P, src, (const float *)a.erho_src, node, i, j, h);
+                macros_dev(h, (float)P.rho0, rd, rf, uf, vf);
+                r = rf; u = uf; v = vf;
+            } else {
+                double f[9];
+                load_fk64<St>(P, src, (const double *)a.erho_src, node, i, j, f);
+                macros64(f, r, u, v);
+            }
+            double uxp = vx - w * ((double)j - yc);
+            double uyp = vy + w * ((double)i - xc);
+            acc[0] = acc[0] + r * ph * (u - uxp);
+            acc[1] = acc[1] + r * ph * (v - uyp);
+            acc[2] = acc[2] + r * ph * (((double)i - xc) * (v - uyp) - ((double)j - yc) * (u - uxp));
+        }
+    }
+    block_sum<3>(acc, lds);
+    F[0] = acc[0]; F[1] = acc[1]; F[2] = acc[2];
+}
+
+__device__ __forceinline__ void hist_write(const StepArgs &a, int p, const double F[3]) {
+    if (!a.hist_part || threadIdx.x != 0) return;
+    const int m = a.P.maxp;
+    double *h = a.hist_part + (size_t)p * HPC;
+    h[0] = a.pin[PX * m + p]; h[1] = a.pin[PY * m + p]; h[2] = a.pin[PANG * m + p];
+    h[3] = a.pin[PVX * m + p]; h[4] = a.pin[PVY * m + p]; h[5] = a.pin[PW * m + p];
+    h[6] = F[0]; h[7] = F[1]; h[8] = F[2];
+    h[9] = a.pin[PFHX * m + p]; h[10] = a.pin[PFHY * m + p]; h[11] = a.pin[PFHW * m + p];
+}
+
+// Block totals of the monitors: FP64 mass as a fixed-point integer (so the
+// per-step total is independent of the order the blocks finish in) and the
+// yielded-node count.
+__device__ __forceinline__ void tile_monitors(const StepArgs &a, long long tile, double mass,
+                                              int yielded, bool want_mass, bool want_yield) {
+    __shared__ double lds[2 * NT];
+    double v[2] = {mass, (double)yielded};
+    block_sum<2>(v, lds);
+    if (threadIdx.x == 0) {
+        int s = (int)(tile % NSLOT);
+        if (want_mass)
+            atomicAdd(a.slots_mass + s, (unsigned long long)(long long)llrint(v[0] * MASS_SCALE));
+        if (want_yield)
+            atomicAdd(a.slots_yield + s, (unsigned long long)(long long)v[1]);
+    }
+}
+
+// Prologue of a fused run: collide the FP64 state of step k (F, RHO, corrected
+// UX/UY, G, OMEGA_BN) into f*_{k+1} in the storage format; particle blocks
+// take the particles from state k to k+1.
+template <class St, bool DEV>
+__global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
+    const DevP &P = a.P;
+    if ((int)blockIdx.x < a.n_pblocks) {
+        particle_advance(a, blockIdx.x);
+        return;
+    }
+    long long tile = xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles);
+    long long n = tile * NT + threadIdx.x;
+    int yielded = 0;
+    if (n < P.nn) {
+        int j = (int)(n / P.W), i = (int)(n - (long long)j * P.W);
+        typename St::T *dst = (typename St::T *)a.dst;
+        double r = a.crho[n], ux = a.cux[n], uy = a.cuy[n];
+        double gx = a.cg[n], gy = a.cg[P.plane + n];
+        int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
+        if constexpr (DEV) {
+            float h[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                double fk = a.cf[k * P.plane + n];
+                h[k] = (float)(fk - (double)wdev(k) * P.rho0);
+            }
+            // density deviation from the same populations
+            float rd = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
+            float om = (float)(P.rheology == 1 ? a.cobn[n] : P.omega), om2, str[3];
+            collide_dev(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy, om, om2,
+                        str, yielded);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k]);
+            if (P.rheology == 1) a.rate32[n] = om2;
+            if (a.aux) {
+                a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
+                a.cstress[2 * P.plane + n] = str[2];
+                if (P.rheology == 1) a.cimage[n] = yielded ? 0 : 5;
+            }
+            if (b >= 0) ((float *)a.erho_dst)[b] = (float)r;
+        } else {
+            double f[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) f[k] = a.cf[k * P.plane + n];
+            CollideOut o;
+            double om = P.rheology == 1 ? a.cobn[n] : P.omega;
+            collide64(P, f, r, ux, uy, gx, gy, om, o);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k]);
+            if (P.rheology == 1) a.cobn[n] = o.omega_bn;
+            yielded = o.yielded;
+            if (a.aux) {
+                a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
+                a.cstress[2 * P.plane + n] = o.stress[2];
+                if (P.rheology == 1) a.cimage[n] = o.yielded ? 0 : 5;
+            }
+            if (b >= 0) ((double *)a.erho_dst)[b] = r;
+        }
+    }
+    if (a.slots_yield) tile_monitors(a, tile, 0.0, yielded, false, true);
+}
+
+// DOMAIN_IMAGE from the particles alone (F6:438, F6:460-467).
+__device__ __forceinline__ int image_node(const DevP &P, const double *part, int i, int j) {
+    int img = 5;
+    const int m = P.maxp;
+    for (int p = 0; p < P.np; ++p) {
+        if (!in_reach(part, m, p, i, j)) continue;
+        double ph = phi_at(part[PX * m + p], part[PY * m + p], part[PDIAM * m + p],
+                           part[PZETA * m + p], i, j);
+        if (ph >= 0.5) {
+            img = 10;
+            double rp = 0.5 * part[PDIAM * m + p], tp = part[PANG * m + p];
+            double xpr = part[PX * m + p] + 0.75 * rp * cos(tp);
+            double ypr = part[PY * m + p] + 0.75 * rp * sin(tp);
+            double dp = 25.0 - (((double)i - xpr) * ((double)i - xpr) +
+                                ((double)j - ypr) * ((double)j - ypr));
+            if (dp >= 0) img = 20;
+        }
+    }
+    return img;
+}
+
+// One fused time step k over the grid: pull-stream + bounce-back (f_k),
+// density and predicted velocity, smoothed-profile forcing G_k, velocity
+// correction, then (FINAL = false) the MRT / Papanastasiou collision of step
+// k+1 stored as f*_{k+1}, or (FINAL = true) the FP64 state of step k.
+// Particle blocks compute the hydrodynamic force of step k and advance the
+// particles to state k+1.
+template <class St, bool DEV, bool FINAL>
+__global__ __launch_bounds__(NT) void k_step(StepArgs a) {
+    const DevP &P = a.P;
+    if ((int)blockIdx.x < a.n_pblocks) {
+        int p = blockIdx.x;
+        double F[3];
+        particle_force<St, DEV>(a, p, F);
+        hist_write(a, p, F);
+        if (threadIdx.x == 0) {
+            const int m = P.maxp;
+            if (FINAL) {
+                a.pout[PFX * m + p] = F[0]; a.pout[PFY * m + p] = F[1]; a.pout[PTQ * m + p] = F[2];
+            }
+        }
+        if (!FINAL) {
+            // the update uses FORCE_k: write it into the outgoing record first
+            if (threadIdx.x == 0) {
+                double rec[PCOUNT];
+                const int m = P.maxp;
+                for (int f = 0; f < PCOUNT; ++f) rec[f] = a.pin[f * m + p];
+                rec[PFX] = F[0]; rec[PFY] = F[1]; rec[PTQ] = F[2];
+                double fcx, fcy, dl;
+                int cnt = contact_one(P, a.pin, m, P.np, p, *a.dpw_in, fcx, fcy,
+                                      a.pairs + p * MAXNB, &dl);
+                a.paircnt[p] = cnt;
+                rec[PFCX] = fcx; rec[PFCY] = fcy;
+                update_one(P, rec);
+                for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
+                if (p == P.np - 1) *a.dpw_out = dl;
+            }
+        }
+        return;
+    }
+    long long tile = xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles);
+    long long n = tile * NT + threadIdx.x;
+    double mass = 0.0;
+    int yielded = 0;
+    if (n < P.nn) {
+        int j = (int)(n / P.W), i = (int)(n - (long long)j * P.W);
+        const typename St::T *src = (const typename St::T *)a.src;
+        int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
+        if constexpr (DEV) {
+            float h[9], rd, rho, ux, uy;
+            load_fk_dev<St>(P, src, (const float *)a.erho_src, n, i, j, h);
+            macros_dev(h, (float)P.rho0, rd, rho, ux, uy);
+            double gxd = 0.0, gyd = 0.0;
+            if (P.np > 0) spm_node(a.pin, P.maxp, P.np, i, j, (double)ux, (double)uy, gxd, gyd);
+            float gx = (float)gxd, gy = (float)gyd;
+            float ucx = ux + 0.5f * gx, ucy = uy + 0.5f * gy;
+            mass = (double)P.rho0 + (double)rd;
+            if (FINAL) {
+                const typename St::T *s0 = src;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    double w0 = (double)wdev(k) * P.rho0;
+                    a.cf[k * P.plane + n] = w0 + (double)h[k];
+                    a.cfp[k * P.plane + n] = w0 + (double)St::ld(s0 + k * P.plane + n);
+                }
+                a.crho[n] = mass; a.cux[n] = ucx; a.cuy[n] = ucy;
+                a.cg[n] = gx; a.cg[P.plane + n] = gy;
+                if (P.rheology == 1) a.cobn[n] = a.rate32[n];
+                if (a.aux && P.rheology != 1) a.cimage[n] = image_node(P, a.pin, i, j);
+            } else {
+                typename St::T *dst = (typename St::T *)a.dst;
+                float om = P.rheology == 1 ? a.rate32[n] : (float)P.omega, om2, str[3];
+                collide_dev(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k]);
+                if (P.rheology == 1) a.rate32[n] = om2;
+                if (a.aux) {
+                    a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
+                    a.cstress[2 * P.plane + n] = str[2];
+                    if (P.rheology == 1) a.cimage[n] = yielded ? 0 : 5;
+                }
+                if (b >= 0) ((float *)a.erho_dst)[b] = rho;
+            }
+        } else {
+            double f[9], rho, ux, uy;
+            load_fk64<St>(P, src, (const double *)a.erho_src, n, i, j, f);
+            macros64(f, rho, ux, uy);
+            double gx = 0.0, gy = 0.0;
+            if (P.np > 0) spm_node(a.pin, P.maxp, P.np, i, j, ux, uy, gx, gy);
+            double ucx = ux + 0.5 * 1.0 * (gx + 0.0), ucy = uy + 0.5 * 1.0 * (gy + 0.0);
+            mass = rho;
+            if (FINAL) {
+                double *cfk = (double *)a.dst;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) cfk[k * P.plane + n] = f[k];
+                a.crho[n] = rho; a.cux[n] = ucx; a.cuy[n] = ucy;
+                a.cg[n] = gx; a.cg[P.plane + n] = gy;
+                if (a.aux && P.rheology != 1) a.cimage[n] = image_node(P, a.pin, i, j);
+            } else {
+                typename St::T *dst = (typename St::T *)a.dst;
+                CollideOut o;
+                double om = P.rheology == 1 ? a.cobn[n] : P.omega;
+                collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k]);
+                if (P.rheology == 1) a.cobn[n] = o.omega_bn;
+                yielded = o.yielded;
+                if (a.aux) {
+                    a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
+                    a.cstress[2 * P.plane + n] = o.stress[2];
+                    if (P.rheology == 1) a.cimage[n] = o.yielded ? 0 : 5;
+                }
+                if (b >= 0) ((double *)a.erho_dst)[b] = rho;
+            }
+        }
+    }
+    if (a.slots_mass || (!FINAL && a.slots_yield))
+        tile_monitors(a, tile, mass, yielded, a.slots_mass != nullptr,
+                      !FINAL && a.slots_yield != nullptr);
+}
+
+}  // namespace lbm

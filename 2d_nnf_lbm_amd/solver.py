@@ -1,0 +1,244 @@
+"""Host-side mirror of the reference's per-step subroutine interface.
+
+`D2Q9` wraps one lbm_handle.  Its methods carry the reference subroutine
+names (F6 = 6_FreeFall_YieldStress/D2Q9_BinghamSediment_Validated_20191127.f90)
+and the same argument meaning; the arrays they act on live on the GPU and are
+exchanged in the reference layout (F(0:8,0:NX,0:NY) etc., returned here as
+numpy arrays indexed [j, i, q]).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+
+class LBMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("lbm error %d: %s" % (code, msg))
+        self.code = code
+
+
+def make_params(cfg):
+    """lbm_params from a dict of field values (missing fields are zero)."""
+    p = _abi.Params()
+    for name, _ in _abi.Params._fields_:
+        if name not in cfg:
+            continue
+        v = cfg[name]
+        if name in ("wall", "corner"):
+            arr = getattr(p, name)
+            for i, x in enumerate(v):
+                arr[i] = float(x)
+        elif name == "storage" and isinstance(v, str):
+            p.storage = _abi.STORAGE[v]
+        else:
+            setattr(p, name, v)
+    return p
+
+
+def _ptr(a, ct=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ct)) if a is not None else None
+
+
+class D2Q9:
+    """One simulation on the GPU.  cfg: dict of lbm_params fields."""
+
+    def __init__(self, cfg):
+        self.lib = _abi.load()
+        self.params = make_params(cfg)
+        self.nx, self.ny = self.params.nx, self.params.ny
+        self.shape = (self.ny + 1, self.nx + 1)
+        self._h = ctypes.c_void_p()
+        rc = self.lib.lbm_create(ctypes.byref(self.params), ctypes.byref(self._h))
+        if rc != 0:
+            msg = self.lib.lbm_last_error(self._h).decode() if self._h else "invalid parameters"
+            if self._h:
+                self.lib.lbm_destroy(self._h)
+                self._h = None
+            raise LBMError(rc, msg)
+        self.np = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(self._h, *args)
+        if rc != 0:
+            raise LBMError(rc, self.lib.lbm_last_error(self._h).decode())
+        return rc
+
+    # ------------------------------------------------------------ state
+    def initialize(self):
+        """INITIALIZE (F6:563-580)."""
+        self._call("lbm_initialize")
+
+    @property
+    def f(self):
+        out = np.empty(self.shape + (9,))
+        self._call("lbm_get_f", _ptr(out))
+        return out
+
+    @f.setter
+    def f(self, value):
+        a = np.ascontiguousarray(value, dtype=np.float64).reshape(self.shape + (9,))
+        self._call("lbm_set_f", _ptr(a))
+
+    @property
+    def fpost(self):
+        out = np.empty(self.shape + (9,))
+        self._call("lbm_get_fpost", _ptr(out))
+        return out
+
+    @fpost.setter
+    def fpost(self, value):
+        a = np.ascontiguousarray(value, dtype=np.float64).reshape(self.shape + (9,))
+        self._call("lbm_set_fpost", _ptr(a))
+
+    def get_fields(self):
+        """(RHO, UX, UY, G, OMEGA_BN)"""
+        rho, ux, uy, om = (np.empty(self.shape) for _ in range(4))
+        g = np.empty(self.shape + (2,))
+        self._call("lbm_get_fields", _ptr(rho), _ptr(ux), _ptr(uy), _ptr(g), _ptr(om))
+        return rho, ux, uy, g, om
+
+    def set_fields(self, rho=None, ux=None, uy=None, g=None, omega_bn=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (rho, ux, uy, g, omega_bn)]
+        self._call("lbm_set_fields", *[_ptr(a) for a in arrs])
+
+    @property
+    def stress(self):
+        out = np.empty(self.shape + (3,))
+        self._call("lbm_get_stress", _ptr(out))
+        return out
+
+    @property
+    def image(self):
+        out = np.empty(self.shape, dtype=np.int32)
+        self._call("lbm_get_image", _ptr(out, ctypes.c_int32))
+        return out
+
+    def set_particles(self, part):
+        """part: (PF_COUNT, n) array of particle records (LBM_PF_* order)."""
+        part = np.ascontiguousarray(part, dtype=np.float64)
+        assert part.shape[0] == _abi.PF_COUNT
+        n = part.shape[1]
+        self._call("lbm_set_particles", n, _ptr(part))
+        self.np = n
+
+    def get_particles(self):
+        out = np.empty((_abi.PF_COUNT, self.np))
+        if self.np:
+            self._call("lbm_get_particles", _ptr(out))
+        return out
+
+    def get_phi(self, ip):
+        out = np.empty(self.shape)
+        self._call("lbm_get_phi", ip, _ptr(out))
+        return out
+
+    # ---------------------------------------------- reference subroutines
+    def collision_mrt(self):
+        """Collision_MRT (F6:867-962)."""
+        self._call("lbm_collide")
+
+    def copy_post(self):
+        """F_POST = F (F6:329)."""
+        self._call("lbm_copy_post")
+
+    def streaming(self):
+        """STREAMING (F6:1154-1190)."""
+        self._call("lbm_stream")
+
+    def boundarycondition(self, kk):
+        """BOUNDARYCONDITION (F6:1195-1390)."""
+        self._call("lbm_boundary", int(kk))
+
+    def macroscopicvariables(self):
+        """MACROSCOPICVARIABLES (F6:1394-1412)."""
+        self._call("lbm_macros")
+
+    def compute_concentration_function(self):
+        """ComputeConcenterationFunction (F6:430-473)."""
+        self._call("lbm_spm_phi")
+
+    def compute_force_field_spm(self):
+        """ComputeForceField_SPM (F6:477-530)."""
+        self._call("lbm_spm_forcing")
+
+    def compute_force_hydro(self):
+        """ComputeForce_HYDRO (F6:534-558)."""
+        self._call("lbm_hydro_force")
+
+    def velocity_corrector(self):
+        """VelocityCorrector (F6:1111-1125)."""
+        self._call("lbm_correct")
+
+    def compute_collision_force(self, max_pairs=0):
+        """ComputeCollisionForce (F6:584-761, N-body F3:632-806).  Returns the
+        in-range pairs (i<j, ascending) when max_pairs > 0."""
+        n = ctypes.c_int32(0)
+        if max_pairs:
+            buf = np.zeros(2 * max_pairs, dtype=np.int32)
+            self._call("lbm_contact_force", _ptr(buf, ctypes.c_int32), max_pairs,
+                       ctypes.byref(n))
+            return buf[: 2 * n.value].reshape(-1, 2)
+        self._call("lbm_contact_force", None, 0, None)
+        return None
+
+    def particle_update(self):
+        """Particle update inline in MAIN (F6:299-312)."""
+        self._call("lbm_particle_update")
+
+    def total_mass(self):
+        """SUM(RHO) (F6:348)."""
+        m = ctypes.c_double(0.0)
+        self._call("lbm_total_mass", ctypes.byref(m))
+        return m.value
+
+    def step_parts(self, kk):
+        """One time step through the per-subroutine entries, in the order of
+        the reference's time loop (F6:293-345)."""
+        self.compute_collision_force()
+        self.particle_update()
+        self.compute_concentration_function()
+        self.collision_mrt()
+        self.copy_post()
+        self.streaming()
+        self.boundarycondition(kk)
+        self.macroscopicvariables()
+        self.compute_force_field_spm()
+        self.compute_force_hydro()
+        self.velocity_corrector()
+
+    # ------------------------------------------------------------ fused
+    def step(self, kk0, nsteps, record=True):
+        """Fused time steps kk0+1..kk0+nsteps on the device.  Returns the
+        per-step history (nsteps, np*12 + 2) when record."""
+        if record:
+            hist = np.zeros((nsteps, self.np * _abi.H_PCOUNT + 2))
+            self._call("lbm_step", int(kk0), int(nsteps), _ptr(hist))
+            return hist
+        self._call("lbm_step", int(kk0), int(nsteps), None)
+        return None
+
+    def step_timed(self, kk0, nsteps):
+        """(ms over all launches, ms inside the fused grid kernel)."""
+        t, m = ctypes.c_double(0), ctypes.c_double(0)
+        self._call("lbm_step_timed", int(kk0), int(nsteps), ctypes.byref(t), ctypes.byref(m))
+        return t.value, m.value
+
+    def last_pairs(self, max_pairs=4096):
+        n = ctypes.c_int32(0)
+        buf = np.zeros(2 * max_pairs, dtype=np.int32)
+        self._call("lbm_last_pairs", _ptr(buf, ctypes.c_int32), max_pairs, ctypes.byref(n))
+        return buf[: 2 * n.value].reshape(-1, 2)
+
+    def sync(self):
+        self._call("lbm_sync")

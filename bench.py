@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused D2Q9 MRT / smoothed-profile time step on one MI355X.
+
+Workload: two disks settling in a Papanastasiou-Bingham fluid on a 401 x 1601
+node grid (2d_nnf_lbm_amd.configs.two_disk_bingham).  A step is one pass of
+the whole time step (streaming, bounce-back, smoothed-profile coupling, force
+and torque, particle update, MRT/Papanastasiou collision) with the state
+resident in HBM.  The headline value is the MLUPS of the FP32-collide /
+FP16-stored path; the plain FP64 path is measured beside it, and the
+reference program is timed on one host core as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--accuracy-steps A]
+"""
+import argparse
+import importlib
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measure(pkg, configs, storage, steps, warmup):
+    cfg, part = configs.two_disk_bingham(storage)
+    s = pkg.D2Q9(cfg)
+    s.initialize()
+    s.set_particles(part)
+    s.step(0, warmup, record=False)
+    s.sync()
+    t0 = time.perf_counter()
+    ms_total, ms_main = s.step_timed(warmup, steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    n = configs.nodes(cfg)
+    bpn = configs.bytes_per_node(storage, True)
+    per_launch = ms_main / steps  # ms per fused grid kernel
+    out = dict(
+        storage=storage,
+        ms_per_step=ms_total / steps,
+        ms_main_kernel=per_launch,
+        wall_s=wall,
+        mlups=n * steps / (ms_total * 1e-3) / 1e6,
+        bytes_per_node=bpn,
+        achieved_gbs=bpn * n / (per_launch * 1e-3) / 1e9,
+    )
+    s.close()
+    return out
+
+
+def accuracy(pkg, configs, steps):
+    """FP16-stored vs FP64 on the same run: terminal velocity, drag
+    coefficient and yielded-area fraction after `steps` steps."""
+    res = {}
+    for storage in ("fp64", "fp16"):
+        cfg, part = configs.two_disk_bingham(storage)
+        s = pkg.D2Q9(cfg)
+        s.initialize()
+        s.set_particles(part)
+        hist = s.step(0, steps)
+        vy0, cd0 = configs.settling_metrics(hist, 2, cfg, 0)
+        vy1, cd1 = configs.settling_metrics(hist, 2, cfg, 1)
+        yielded = hist[-1][-1] / configs.nodes(cfg)
+        res[storage] = dict(vy=[vy0, vy1], cd=[cd0, cd1], yielded=yielded)
+        s.close()
+    a, b = res["fp64"], res["fp16"]
+    rel = lambda x, y: abs(x - y) / max(abs(y), 1e-300)
+    return dict(
+        steps=steps,
+        terminal_velocity_rel_err=max(rel(b["vy"][i], a["vy"][i]) for i in range(2)),
+        drag_coefficient_rel_err=max(rel(b["cd"][i], a["cd"][i]) for i in range(2)),
+        yielded_fraction_rel_err=rel(b["yielded"], a["yielded"]),
+        fp64_vy=a["vy"], fp16_vy=b["vy"], fp64_yielded_fraction=a["yielded"],
+    )
+
+
+def cpu_baseline():
+    """The reference program (oracle/_ref/f6_bing_400x1600: the settling-disk
+    Bingham program on the same 401 x 1601 node grid, 40 steps) timed on one
+    host core; falls back to the C restatement of the oracle when that build is
+    absent."""
+    exe = os.path.join(REPO, "oracle", "_ref", "f6_bing_400x1600")
+    nodes = 401 * 1601
+    if os.path.exists(exe):
+        steps = 40
+        with tempfile.TemporaryDirectory(prefix="refrun_") as d:
+            t0 = time.perf_counter()
+            r = subprocess.run([exe], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                               text=True, timeout=600)
+            wall = time.perf_counter() - t0
+        m = re.search(r"RUNTIME=\s*([0-9.Ee+-]+)", r.stdout)
+        cpu_t = float(m.group(1)) if m else wall
+        return dict(value=nodes * steps / wall / 1e6, unit="MLUPS", cores=1, kind="reference",
+                    sample="reference Bingham settling-disk program (F6, patched grid "
+                           "NX=400 NY=1600, one disk, per-step history I/O to tmpfs), "
+                           "%d steps, wall %.2f s, CPU_TIME %.2f s" % (steps, wall, cpu_t))
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    import numpy as np
+    oracle.build()
+    pkg = importlib.import_module("2d_nnf_lbm_amd")
+    cfg, part = pkg.configs.two_disk_bingham("fp64")
+    cfg = dict(cfg, rate_mode=0)
+    st = oracle.State(oracle.make_params(cfg), 2)
+    st.initialize()
+    st.set_particles(part)
+    steps = 20
+    t0 = time.perf_counter()
+    st.run(0, steps, record=False)
+    wall = time.perf_counter() - t0
+    return dict(value=nodes * steps / wall / 1e6, unit="MLUPS", cores=1, kind="port",
+                sample="C restatement of the reference step on the two-disk workload, "
+                       "%d steps, wall %.2f s" % (steps, wall))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--accuracy-steps", type=int, default=3000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the fused step runs on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    pkg = importlib.import_module("2d_nnf_lbm_amd")
+    configs = importlib.import_module("2d_nnf_lbm_amd.configs")
+    log("bench: fp64 leg")
+    r64 = measure(pkg, configs, "fp64", args.steps, args.warmup)
+    log("bench: fp16 leg")
+    r16 = measure(pkg, configs, "fp16", args.steps, args.warmup)
+    log("bench: accuracy leg")
+    acc = accuracy(pkg, configs, args.accuracy_steps) if args.accuracy_steps > 0 else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline()
+    cfg, _ = configs.two_disk_bingham("fp16")
+    n = configs.nodes(cfg)
+    line = {
+        "metric": "MLUPS + % HBM roofline; plain FP64 vs FP16-stored at <=1e-3 rel err",
+        "value": r16["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r16["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "two disks (D=40, density ratio 1.01) settling in a "
+                        "Papanastasiou-Bingham fluid (Bn=0.5, tau=0.8, m=1e5), "
+                        "401x1601 nodes, bounce-back walls, FP32 collide / FP16 deviation storage",
+            "nodes": n,
+            "storage": "fp16",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": r16["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": r16["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_node": r16["bytes_per_node"],
+        },
+        "cpu_baseline": cpu,
+        "fp64": {
+            "mlups": r64["mlups"],
+            "ms_per_step": r64["ms_per_step"],
+            "achieved_gbs": r64["achieved_gbs"],
+            "roofline_frac": r64["achieved_gbs"] / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_node": r64["bytes_per_node"],
+        },
+        "fp16_over_fp64": r16["mlups"] / r64["mlups"],
+        "accuracy_fp16_vs_fp64": acc,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * lbm_amd.h -- C ABI of the MI355X (gfx950) D2Q9 MRT / smoothed-profile time
+ * step.  It is the drop-in boundary for the per-step subroutines of
+ * taha-re/2D_NNF_LBM, whose Fortran driver calls them through implicit
+ * interfaces with arrays passed by address.  Reference routines are cited as
+ * F6:line (6_FreeFall_YieldStress/D2Q9_BinghamSediment_Validated_20191127.f90)
+ * and F3:line (3_MultiParticle_Sedimentation/D2Q9_RTParticle_20191003.f90).
+ *
+ * Conventions
+ *  - Grid: nx, ny cells; nodes 0..nx by 0..ny (F6:8-9).
+ *  - Host arrays use the reference layout: F(0:8,0:NX,0:NY) column-major
+ *    (q fastest, then i, then j); node fields (RHO, UX, ...) i fastest;
+ *    G(1:2,I,J) and STRESS(1:3,I,J) component fastest.
+ *  - Particles cross as one SoA block of LBM_PF_COUNT * n doubles:
+ *    field f of particle p at part[f*n + p] (fields below).  No full-grid
+ *    solid fraction crosses the boundary.
+ *  - The library owns all device memory.  Host arrays are caller-owned and are
+ *    read or written only inside the set/get call that receives them.
+ *  - Every call returns 0 on success or a negative LBM_ERR_* code; the text of
+ *    the last error is in lbm_last_error().  Nothing calls exit().
+ *  - One handle per host thread; calls are serialised on the handle's stream.
+ */
+#ifndef LBM_AMD_H
+#define LBM_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_PARAM (-1)       /* bad parameter, e.g. tau < 0.5 (F6:115-118) */
+#define LBM_ERR_HIP (-2)         /* HIP runtime failure */
+#define LBM_ERR_UNSUPPORTED (-3) /* combination not provided on the GPU path */
+#define LBM_ERR_CAPACITY (-4)    /* particle, pair or history capacity exceeded */
+#define LBM_ERR_NAN (-5)         /* non-finite value detected by a check */
+
+#define LBM_NEWTONIAN 0
+#define LBM_BINGHAM 1            /* Papanastasiou-regularised Bingham (F6:939-945) */
+
+#define LBM_RATE_LAGGED 0        /* module-global rate set by the previous node (F6:958) */
+#define LBM_RATE_LOCAL 1         /* each node's own previous-step rate */
+
+#define LBM_FP64 0               /* populations stored and collided in FP64 */
+#define LBM_FP32 1               /* FP32 deviations f - w_i*rho0, FP32 collide */
+#define LBM_FP16 2               /* FP16 deviations (scaled by 2^10), FP32 collide */
+#define LBM_BF16 3               /* BF16 deviations, FP32 collide */
+
+/* Particle record fields (TYPE PARTICLE, F6:69-81, plus VP_OLD and F_H_IN, F6:96). */
+enum {
+    LBM_PF_X = 0, LBM_PF_Y, LBM_PF_ANG, LBM_PF_VX, LBM_PF_VY, LBM_PF_W,
+    LBM_PF_DIAM, LBM_PF_FX, LBM_PF_FY, LBM_PF_TQ, LBM_PF_FCX, LBM_PF_FCY,
+    LBM_PF_RHOP, LBM_PF_MP, LBM_PF_IP, LBM_PF_ZETA,
+    LBM_PF_VOX, LBM_PF_VOY, LBM_PF_VOW, LBM_PF_FHX, LBM_PF_FHY, LBM_PF_FHW,
+    LBM_PF_COUNT
+};
+
+/* Per-step history record per particle (written by lbm_step). */
+enum {
+    LBM_H_X = 0, LBM_H_Y, LBM_H_ANG, LBM_H_VX, LBM_H_VY, LBM_H_W,
+    LBM_H_FX, LBM_H_FY, LBM_H_TQ, LBM_H_FHX, LBM_H_FHY, LBM_H_FHW,
+    LBM_H_PCOUNT
+};
+
+/* Run parameters; the compile-time PARAMETERs of MODULE PARAM (F6:2-83)
+ * become run-time values. */
+typedef struct lbm_params {
+    int32_t nx, ny;            /* F6:8-9 */
+    int32_t rheology;          /* LBM_NEWTONIAN / LBM_BINGHAM (F6:4) */
+    int32_t rate_mode;         /* LBM_RATE_*; the GPU path implements LOCAL */
+    int32_t q_mode;            /* 1: s4=s6=8(2-w)/(8-w) (F6:1009); 0: sq */
+    int32_t update_angle;      /* 1: integrate the angle (F6:312); 0: F3:327 */
+    int32_t storage;           /* LBM_FP64 / LBM_FP32 / LBM_FP16 / LBM_BF16 */
+    int32_t max_particles;     /* capacity of the particle arrays */
+    double omega;              /* OMEGA = 1/(3 nu + 0.5) (F6:101) */
+    double s0, s1, s2, s3, s5, sq;  /* other MRT rates (F6:1009-1011) */
+    double nu;                 /* NU, Newtonian stress factor (F6:918) */
+    double mu0, tau_y, m_pap, gamma_min;  /* Papanastasiou (F6:939-945) */
+    double rho0;               /* RHO0 (F6:60) */
+    double gy;                 /* A_G, gravity along y (F6:28, F6:308) */
+    double wall[8];            /* uxS,uyS, uxR,uyR, uxN,uyN, uxL,uyL (F6:1204-1264) */
+    double corner[2];          /* ux_w, uy_w at the four corners (F6:1365) */
+    double zeta_c, cij, eps_p, eps_w, e_p, e_w;  /* contact (F6:691-693, F3:739-742) */
+} lbm_params;
+
+typedef struct lbm_handle lbm_handle;
+
+/* Lifecycle.  lbm_create validates the parameters (tau >= 0.5, F6:115-118),
+ * allocates device state and leaves it zeroed; call lbm_initialize next. */
+int lbm_create(const lbm_params *p, lbm_handle **out);
+int lbm_destroy(lbm_handle *h);
+const char *lbm_last_error(const lbm_handle *h);
+int lbm_sync(lbm_handle *h);
+
+/* INITIALIZE, F6:563-580 (f = feq(rho0, 0), G = 0), with OMEGA_BN = OMEGA
+ * (F6:102) and the initial relaxation vector (F6:199). */
+int lbm_initialize(lbm_handle *h);
+
+/* State transfer (reference layout on the host, SoA on the device). */
+int lbm_set_f(lbm_handle *h, const double *f);        /* F(0:8,0:NX,0:NY) */
+int lbm_get_f(lbm_handle *h, double *f);
+int lbm_set_fpost(lbm_handle *h, const double *f);    /* F_POST */
+int lbm_get_fpost(lbm_handle *h, double *f);
+/* RHO, UX, UY, G(1:2,:,:), OMEGA_BN; any pointer may be NULL to skip it. */
+int lbm_set_fields(lbm_handle *h, const double *rho, const double *ux,
+                   const double *uy, const double *g, const double *omega_bn);
+int lbm_get_fields(lbm_handle *h, double *rho, double *ux, double *uy,
+                   double *g, double *omega_bn);
+int lbm_get_stress(lbm_handle *h, double *stress);    /* STRESS(1:3,0:NX,0:NY) */
+int lbm_get_image(lbm_handle *h, int32_t *image);     /* DOMAIN_IMAGE */
+int lbm_set_particles(lbm_handle *h, int32_t n, const double *part);
+int lbm_get_particles(lbm_handle *h, double *part);
+/* Solid fraction of particle ip on the full grid (PHI(0:NX,0:NY) of F6:79),
+ * as computed by the last lbm_spm_phi. */
+int lbm_get_phi(lbm_handle *h, int32_t ip, double *phi);
+
+/* Drop-in per-subroutine entries (FP64 state, reference semantics). */
+int lbm_collide(lbm_handle *h);           /* Collision_MRT, F6:867-962 */
+int lbm_copy_post(lbm_handle *h);         /* F_POST = F, F6:329 */
+int lbm_stream(lbm_handle *h);            /* STREAMING, F6:1154-1190 */
+int lbm_boundary(lbm_handle *h, int32_t kk);  /* BOUNDARYCONDITION, F6:1195-1390 */
+int lbm_macros(lbm_handle *h);            /* MACROSCOPICVARIABLES, F6:1394-1412 */
+int lbm_spm_phi(lbm_handle *h);           /* ComputeConcenterationFunction, F6:430-473 */
+int lbm_spm_forcing(lbm_handle *h);       /* ComputeForceField_SPM, F6:477-503 */
+int lbm_hydro_force(lbm_handle *h);       /* ComputeForce_HYDRO, F6:534-558 */
+int lbm_correct(lbm_handle *h);           /* VelocityCorrector, F6:1111-1125 */
+/* ComputeCollisionForce (F6:584-761; N-body loop F3:632-806).  pairs (may be
+ * NULL) receives the in-range pairs (i<j, ascending, 0-based) as 2*n ints. */
+int lbm_contact_force(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
+                      int32_t *npairs);
+int lbm_particle_update(lbm_handle *h);   /* inline update, F6:299-312 */
+int lbm_total_mass(lbm_handle *h, double *mass);  /* SUM(RHO), F6:348 */
+
+/* Fused time steps kk0+1 .. kk0+nsteps (the loop body F6:293-349), all on the
+ * device, in the storage precision of the handle.  hist (may be NULL):
+ * nsteps rows of (n_particles*LBM_H_PCOUNT + 2) doubles -- the particle
+ * records, then SUM(RHO) and the number of yielded nodes. */
+int lbm_step(lbm_handle *h, int32_t kk0, int32_t nsteps, double *hist);
+/* Pairs found by the last contact evaluation of lbm_step. */
+int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
+                   int32_t *npairs);
+/* Like lbm_step (no history), timing on the handle's stream with HIP events:
+ * ms_total covers all launches, ms_main the fused stream/collide kernel only. */
+int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps,
+                   double *ms_total, double *ms_main);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

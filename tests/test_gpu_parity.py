@@ -1,0 +1,188 @@
+"""GPU parity of the FP64 HIP path against the oracle (and, where the
+reference's own dumps exist, against the reference program itself).
+
+Tolerances (FP64): populations <= 1e-12 relative after 1000 steps, particle
+trajectories <= 1e-9 relative over the run.  The two paths differ only in the
+last bit of sin/exp (device libm vs glibc) and in the summation order of the
+fused force reduction.
+"""
+import numpy as np
+import pytest
+
+import cases
+from lbmtest import disk_record, f6_case, gpu_solver, oracle_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+POP_TOL = 1e-12
+TRAJ_TOL = 1e-9
+
+
+def _compare_state(g, st, tol=POP_TOL, label=""):
+    rho, ux, uy, gg, om = g.get_fields()
+    errs = {
+        "f": rel_err(g.f, st.f),
+        "rho": rel_err(rho, st.rho),
+        "ux": rel_err(ux, st.ux),
+        "uy": rel_err(uy, st.uy),
+        "G": rel_err(gg, st.g),
+        "omega_bn": rel_err(om, st.omega_bn),
+    }
+    print(label, {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= tol, (label, k, v)
+
+
+@pytest.mark.parametrize("bingham", [False, True])
+def test_dropin_subroutines_match_oracle(orc, bingham):
+    """Per-subroutine entries, called in the reference's loop order."""
+    nx, ny = 40, 120
+    cfg, part = f6_case(nx, ny, bingham)
+    print("case: dropin bingham=%s %dx%d" % (bingham, nx, ny), flush=True)
+    st = oracle_state(orc, cfg, part)
+    g = gpu_solver(cfg)
+    g.initialize()
+    g.set_particles(part)
+    for kk in range(1, 41):
+        st.step_parts(kk)
+        g.step_parts(kk)
+    _compare_state(g, st, label="dropin")
+    assert np.array_equal(g.image, st.image)
+    assert rel_err(g.stress, st.stress) <= 1e-10
+    p = g.get_particles()[:, 0]
+    q = st.part[:, 0]
+    for fld in (orc.PF_X, orc.PF_Y, orc.PF_VX, orc.PF_VY, orc.PF_W, orc.PF_FX, orc.PF_FY):
+        assert abs(p[fld] - q[fld]) <= 1e-11 * max(abs(q[fld]), 1e-300) + 1e-300, fld
+    assert abs(g.total_mass() - st.total_mass()) <= 1e-12 * st.total_mass()
+
+
+@pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local", "f6_newt_lid"])
+def test_fused_fp64_matches_reference_dump(orc, name):
+    """Fused FP64 steps against the reference program's own final dump
+    (LOCAL rate for Bingham, which is what the GPU implements)."""
+    meta, ref = cases.load_golden(name)
+    cfg = cases.golden_params(meta)
+    nx, ny, n = meta["nx"], meta["ny"], meta["time_max"]
+    print("case: fused vs reference dump", name, nx, ny, n, flush=True)
+    g = gpu_solver(cfg)
+    g.initialize()
+    g.set_particles(cases.f6_disk(nx, ny)[:, None])
+    hist = g.step(0, n)
+    rho, ux, uy, gg, om = g.get_fields()
+    errs = dict(f=rel_err(g.f, ref["f"]), rho=rel_err(rho, ref["rho"]),
+                ux=rel_err(ux, ref["ux"]), uy=rel_err(uy, ref["uy"]),
+                G=rel_err(gg, ref["g"]), omega_bn=rel_err(om, ref["omega_bn"]))
+    print({k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= POP_TOL, (k, v)
+    # trajectory: history columns Y, X, Vy/U_C over every step (F6:315-316)
+    c = cases.f6_constants(nx, ny)
+    y = hist[:, 1]
+    vy = hist[:, 4] / c["u_c"]
+    assert rel_err(y, ref["hist"][:, 1]) <= TRAJ_TOL
+    assert rel_err(vy, ref["hist"][:, 5]) <= TRAJ_TOL
+    part = g.get_particles()[:, 0]
+    assert rel_err(part[[0, 1]], ref["particle"][[0, 1]]) <= TRAJ_TOL
+    # mean density per step (F6:348)
+    nodes = (nx + 1) * (ny + 1)
+    assert rel_err(hist[:, -2] / nodes, ref["avgden"]) <= 1e-13
+    if meta["non_newtonian"]:
+        assert np.mean(g.image == ref["image"]) > 0.999
+
+
+@pytest.mark.parametrize("bingham", [False, True])
+def test_fused_fp64_matches_oracle_1000_steps(orc, bingham):
+    nx, ny = 60, 160
+    cfg, part = f6_case(nx, ny, bingham, uxn=0.002)
+    print("case: fused vs oracle bingham=%s" % bingham, flush=True)
+    st = oracle_state(orc, cfg, part)
+    ho = st.run(0, 1000)
+    g = gpu_solver(cfg)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, 1000)
+    _compare_state(g, st, label="fused1000")
+    for col in (orc.H_X, orc.H_Y, orc.H_VX, orc.H_VY, orc.H_W, orc.H_FY):
+        assert rel_err(hg[:, col], ho[:, col]) <= TRAJ_TOL, col
+    # yielded-node counts may differ only where tau_mag sits on tau_y
+    if bingham:
+        assert np.max(np.abs(hg[:, -1] - ho[:, -1])) <= 2
+
+
+def test_fused_split_calls_equal_one_call(orc):
+    """lbm_step(0,n) == lbm_step(0,a) + lbm_step(a,n-a): the FP64 state kept
+    between calls is the reference's loop-end state."""
+    nx, ny = 40, 120
+    cfg, part = f6_case(nx, ny, True)
+    g1 = gpu_solver(cfg)
+    g1.initialize(); g1.set_particles(part)
+    g1.step(0, 37)
+    g2 = gpu_solver(cfg)
+    g2.initialize(); g2.set_particles(part)
+    g2.step(0, 20)
+    g2.step(20, 17)
+    assert np.array_equal(g1.f, g2.f)
+    assert np.array_equal(g1.get_particles(), g2.get_particles())
+
+
+def test_fused_then_dropin_continues(orc):
+    """Fused steps followed by per-subroutine steps equal the oracle."""
+    nx, ny = 40, 120
+    cfg, part = f6_case(nx, ny, True)
+    st = oracle_state(orc, cfg, part)
+    st.run(0, 30, record=False)
+    for kk in range(31, 36):
+        st.step_parts(kk)
+    g = gpu_solver(cfg)
+    g.initialize(); g.set_particles(part)
+    g.step(0, 30, record=False)
+    for kk in range(31, 36):
+        g.step_parts(kk)
+    _compare_state(g, st, label="fused+dropin")
+
+
+def test_mass_invariant_closed_box(orc):
+    """Closed box, no particles, no forcing: total mass scales by exactly
+    36*fl32(1/36) per step in exact arithmetic (float literal d_mrt, F6:873).
+    Checked on the per-step history to 1e-13 relative."""
+    nx, ny = 64, 64
+    cfg = cases.f6_params(nx, ny, False)
+    cfg["wall"] = [0.0, 0.0, 0.0, 0.0, 0.01, 0.0, 0.0, 0.0]
+    g = gpu_solver(cfg, max_particles=1)
+    g.initialize()
+    m0 = g.total_mass()
+    n = 200
+    hist = g.step(0, n)
+    fac = 36.0 * float(np.float32(1.0 / 36.0))
+    # a moving lid adds momentum, not mass; the bounce-back term is mass-free
+    # only for tangential wall motion, so the factor must hold step by step
+    expect = m0 * fac ** np.arange(1, n + 1)
+    assert rel_err(hist[:, -1], expect) <= 1e-11
+
+
+def test_contact_force_and_pairs_match_oracle(orc):
+    """N-body contact (F3:746-804): forces bit-exact and pair list identical
+    to the oracle's brute-force loop, including wall contacts."""
+    rng = np.random.default_rng(7)
+    nx, ny = 120, 120
+    cfg = cases.f6_params(nx, ny, False)
+    n = 40
+    recs = []
+    d = 8.0
+    while len(recs) < n:
+        x, y = rng.uniform(3, nx - 3), rng.uniform(3, ny - 3)
+        if all((x - r[0]) ** 2 + (y - r[1]) ** 2 > (0.8 * d) ** 2 for r in recs):
+            recs.append(disk_record(x, y, d, 1.5))
+    part = np.stack(recs, axis=1)
+    st = oracle_state(orc, cfg, part)
+    g = gpu_solver(cfg, max_particles=64)
+    g.initialize(); g.set_particles(part)
+    for it in range(3):
+        pairs_o, n_o = st.contact(max_pairs=2048)
+        pairs_g = g.compute_collision_force(max_pairs=2048)
+        print("pairs", it, n_o, len(pairs_g), flush=True)
+        assert n_o > 0
+        assert np.array_equal(pairs_g, pairs_o)
+        pg = g.get_particles()
+        assert np.array_equal(pg[orc.PF_FCX], st.part[orc.PF_FCX, :n])
+        assert np.array_equal(pg[orc.PF_FCY], st.part[orc.PF_FCY, :n])
