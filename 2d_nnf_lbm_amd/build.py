@@ -20,6 +20,26 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wno-unused-command-line-argument"]
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN_DIR, "lbm_f6_driver")
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
+
+
+def build_fortran():
+    """Compile the iso_c_binding module and the Fortran driver, linked to
+    liblbm_amd.so (found next to the package through the run path)."""
+    import tempfile
+    srcs = [os.path.join(FORTRAN_DIR, "lbm_amd.f90"), os.path.join(FORTRAN_DIR, "lbm_f6_driver.f90")]
+    with tempfile.TemporaryDirectory(prefix="lbmmod_") as moddir:
+        cmd = [FLANG, "-O2", "-J", moddir] + srcs + [
+            "-L" + HERE, "-llbm_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stdout)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(force=True, verbose=False):
     """Compile liblbm_amd.so (always when force, else only when stale)."""
     if not force and os.path.exists(LIB):
@@ -34,6 +54,8 @@ def build(force=True, verbose=False):
     if verbose:
         print(r.stdout)
     os.replace(tmp, LIB)
+    if os.path.exists(FLANG):
+        build_fortran()
     return LIB
 
 

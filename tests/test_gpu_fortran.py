@@ -1,0 +1,53 @@
+"""End-to-end drop-in check: the Fortran driver (the reference's MAIN loop on
+the GPU through the iso_c_binding module) writes the reference's own output
+files, which must match the reference program's files for the same case."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import cases
+from lbmtest import rel_err
+from refio import parse_run
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(REPO, "2d_nnf_lbm_amd", "fortran", "lbm_f6_driver")
+
+
+@pytest.mark.parametrize("name,fused", [("f6_bing_local", True), ("f6_bing_local", False),
+                                        ("f6_newt", True), ("f6_newt_lid", False)])
+def test_fortran_driver_matches_reference_files(name, fused):
+    assert os.path.exists(DRIVER), "build the Fortran driver first"
+    meta, ref = cases.load_golden(name)
+    nx, ny, n = meta["nx"], meta["ny"], meta["time_max"]
+    uxn = meta.get("uxn") or "0D0"
+    print("case: fortran driver", name, "fused" if fused else "subroutines", flush=True)
+    with tempfile.TemporaryDirectory(prefix="fdrv_") as d:
+        nml = os.path.join(d, "run.nml")
+        with open(nml, "w") as fh:
+            fh.write("&RUN NX=%d, NY=%d, TIME_MAX=%d, NON_NEWTONIAN=%s, WRITE_OUTPUT=.TRUE., "
+                     "FUSED=%s, STORAGE=0, UXN=%s /\n"
+                     % (nx, ny, n, ".TRUE." if meta["non_newtonian"] else ".FALSE.",
+                        ".TRUE." if fused else ".FALSE.", uxn))
+        r = subprocess.run([DRIVER, nml], cwd=d, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True, timeout=300)
+        print(r.stdout[-2000:], flush=True)
+        assert r.returncode == 0
+        out, _ = parse_run(d, nx, ny)
+    for k in ("f", "rho", "ux", "uy", "g", "omega_bn"):
+        e = rel_err(out[k], ref[k])
+        print(k, "%.2e" % e, flush=True)
+        assert e <= 1e-12, k
+    assert rel_err(out["stress"], ref["stress"]) <= 1e-9
+    h, hr = out["hist"], ref["hist"]
+    assert h.shape == hr.shape
+    for col in (1, 2, 4, 5, 6):     # Y, X, Ux, Uy/U_C, OMG (F6:315-316)
+        assert rel_err(h[:, col], hr[:, col]) <= 1e-9, col
+    assert rel_err(out["avgden"], ref["avgden"]) <= 1e-13
+    assert rel_err(out["particle"][:2], ref["particle"][:2]) <= 1e-9
+    if not meta["non_newtonian"]:
+        assert np.array_equal(out["omega_mrt"], ref["omega_mrt"])
