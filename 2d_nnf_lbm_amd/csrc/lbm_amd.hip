@@ -49,6 +49,11 @@ struct lbm_handle {
     size_t hist_rows = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> evk;
+    // per-tile candidate lists of many-particle runs
+    unsigned long long *tl_hdr[2] = {nullptr, nullptr};
+    int *tl_ids[2] = {nullptr, nullptr};
+    int *err_flag = nullptr;
+    unsigned gen_base = 1;
 };
 
 namespace {
@@ -141,9 +146,19 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->paircnt, (size_t)P.maxp * 4)) ||
         (rc = al((void **)&h->porig, (size_t)P.maxp * 2 * 4)) ||
         (rc = al((void **)&h->partial, (size_t)nblk(P.nn) * 8)) ||
-        (rc = al((void **)&h->scalar, 8))) {
+        (rc = al((void **)&h->scalar, 8)) ||
+        (rc = al((void **)&h->err_flag, 4))) {
         *out = h;
         return rc;
+    }
+    if (p->max_particles > NP_LOOP) {
+        size_t nt = (size_t)nblk(P.nn);
+        for (int b = 0; b < 2; ++b)
+            if ((rc = al((void **)&h->tl_hdr[b], nt * 8)) ||
+                (rc = al((void **)&h->tl_ids[b], nt * MAXC * 4))) {
+                *out = h;
+                return rc;
+            }
     }
     if (p->storage != LBM_FP64) {
         size_t sb = storage_bytes(p->storage);
@@ -168,7 +183,8 @@ int lbm_destroy(lbm_handle *h) {
     void *bufs[] = {h->f, h->fp, h->rho, h->ux, h->uy, h->g, h->obn, h->stress, h->image,
                     h->sA, h->sB, h->rate32, h->erho[0], h->erho[1], h->pbuf[0], h->pbuf[1],
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
-                    h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots};
+                    h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
+                    h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -508,6 +524,16 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, bool time
     a.pairs = h->pairs; a.paircnt = h->paircnt;
     a.n_pblocks = np;
     a.ntiles = ntiles;
+    a.err = h->err_flag;
+    const unsigned g0 = h->gen_base;
+    auto lists = [&](unsigned gin, unsigned gout) {
+        a.gen_in = gin;
+        a.gen_out = gout;
+        a.tl_hdr_in = h->tl_hdr[gin & 1];
+        a.tl_ids_in = h->tl_ids[gin & 1];
+        a.tl_hdr_out = h->tl_hdr[gout & 1];
+        a.tl_ids_out = h->tl_ids[gout & 1];
+    };
     dim3 grid((unsigned)(np + ntiles)), block(NT);
     const size_t rowp = (size_t)np * HPC;
 
@@ -536,6 +562,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, bool time
     a.slots_mass = nullptr;
     a.hist_part = nullptr;
     a.aux = nsteps == 1;
+    lists(g0, g0 + 1);
     hipLaunchKernelGGL((k_prologue<St, DEV>), grid, block, 0, h->stream, a);
     int rc = check_launch(h, "k_prologue");
     if (rc) return rc;
@@ -554,6 +581,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, bool time
         a.slots_mass = record ? h->hist_slots + (size_t)(s - 1) * 2 * NSLOT : nullptr;
         a.slots_yield = (record && !fin) ? h->hist_slots + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
         a.aux = (s == nsteps - 1) || fin;
+        lists(g0 + (unsigned)s, g0 + (unsigned)s + 1);
         if (timed) (void)hipEventRecord(h->evk[2 * (s - 1)], h->stream);
         if (fin)
             hipLaunchKernelGGL((k_step<St, DEV, true>), grid, block, 0, h->stream, a);
@@ -576,6 +604,16 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, bool time
         h->fp = fs;
     }
     h->pcur = pc;
+    h->gen_base = g0 + (unsigned)nsteps + 2;
+    if (np > NP_LOOP) {
+        int ef = 0;
+        HIPCHECK(h, hipMemcpyAsync(&ef, h->err_flag, 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        if (ef) {
+            (void)hipMemsetAsync(h->err_flag, 0, 4, h->stream);
+            return fail(h, LBM_ERR_CAPACITY, "more particles reach one grid tile than its candidate list holds");
+        }
+    }
     if (timed) {
         HIPCHECK(h, hipEventSynchronize(h->ev1));
         float t = 0.0f;

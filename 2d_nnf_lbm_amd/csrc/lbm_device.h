@@ -233,6 +233,65 @@ __device__ __forceinline__ bool spm_node(const double *part, int maxp, int np, i
     return any;
 }
 
+// Same rule for a short candidate list (many-particle runs): the largest
+// index with phi > 0 sets G, whatever the order of the list.
+__device__ __forceinline__ bool spm_node_cand(const double *part, int maxp, const int *ids,
+                                              int cnt, int i, int j, double ux, double uy,
+                                              double &gx, double &gy) {
+    gx = 0.0;
+    gy = 0.0;
+    int best = -1;
+    double bphi = 0.0;
+    for (int c = 0; c < cnt; ++c) {
+        int p = ids[c];
+        if (p <= best || !in_reach(part, maxp, p, i, j)) continue;
+        double ph = phi_at(part[PX * maxp + p], part[PY * maxp + p], part[PDIAM * maxp + p],
+                           part[PZETA * maxp + p], i, j);
+        if (ph > 0.0) { best = p; bphi = ph; }
+    }
+    if (best < 0) return false;
+    double xc = part[PX * maxp + best], yc = part[PY * maxp + best], w = part[PW * maxp + best];
+    double uxp = part[PVX * maxp + best] - w * ((double)j - yc);
+    double uyp = part[PVY * maxp + best] + w * ((double)i - xc);
+    gx = bphi * (uxp - ux);
+    gy = bphi * (uyp - uy);
+    return true;
+}
+
+constexpr int NP_LOOP = 16;    // above this many particles, tiles use candidate lists
+constexpr int MAXC = 32;       // candidate particles per tile
+
+// Registers particle p (record rec, stride 1) in the candidate list of every
+// grid tile its reach box touches, for generation gen.  Each tile header packs
+// (generation << 32 | count); the first registration of a new generation
+// resets the count (lock-free CAS loop, one contender per touching particle).
+__device__ __forceinline__ void register_tiles(const DevP &P, const double *rec, int p,
+                                               unsigned gen, unsigned long long *hdr,
+                                               int *ids, int *err) {
+    double reach = 0.5 * rec[PDIAM] + 0.5 * rec[PZETA] + 1.0;
+    int i0 = max(0, (int)ceil(rec[PX] - reach)), i1 = min(P.nx, (int)floor(rec[PX] + reach));
+    int j0 = max(0, (int)ceil(rec[PY] - reach)), j1 = min(P.ny, (int)floor(rec[PY] + reach));
+    if (i0 > i1 || j0 > j1) return;
+    long long last = -1;
+    for (int j = j0; j <= j1; ++j) {
+        long long tlo = ((long long)j * P.W + i0) / NT, thi = ((long long)j * P.W + i1) / NT;
+        for (long long t = (tlo > last ? tlo : last + 1); t <= thi; ++t) {
+            unsigned long long old = hdr[t], assumed, nv;
+            unsigned slot;
+            do {
+                assumed = old;
+                unsigned g = (unsigned)(assumed >> 32);
+                slot = (g == gen) ? (unsigned)(assumed & 0xffffffffu) : 0u;
+                nv = ((unsigned long long)gen << 32) | (unsigned long long)(slot + 1);
+                old = atomicCAS(hdr + t, assumed, nv);
+            } while (old != assumed);
+            if (slot < (unsigned)MAXC) ids[t * MAXC + slot] = p;
+            else atomicOr(err, 1);
+            last = t;
+        }
+    }
+}
+
 struct CollideOut {
     double omega_bn;     // new node rate (Bingham)
     double stress[3];

@@ -350,7 +350,33 @@ struct StepArgs {
     int n_pblocks;
     long long ntiles;
     int aux;                      // write STRESS / DOMAIN_IMAGE
+    // per-tile candidate lists (more than NP_LOOP particles): read for the
+    // state the tiles use (generation gen_in), written for the next state
+    const unsigned long long *tl_hdr_in;
+    const int *tl_ids_in;
+    unsigned long long *tl_hdr_out;
+    int *tl_ids_out;
+    unsigned gen_in, gen_out;
+    int *err;                     // bit 0: a tile list overflowed
 };
+
+// G at a node of a tile: all particles tested directly when there are few,
+// else the tile's candidate list for the current generation.
+__device__ __forceinline__ void spm_tile_node(const StepArgs &a, long long tile, int i, int j,
+                                              double ux, double uy, double &gx, double &gy) {
+    const DevP &P = a.P;
+    gx = 0.0;
+    gy = 0.0;
+    if (P.np == 0) return;
+    if (P.np <= NP_LOOP) {
+        spm_node(a.pin, P.maxp, P.np, i, j, ux, uy, gx, gy);
+        return;
+    }
+    unsigned long long h = a.tl_hdr_in[tile];
+    if ((unsigned)(h >> 32) != a.gen_in) return;
+    int cnt = min((int)(h & 0xffffffffu), MAXC);
+    spm_node_cand(a.pin, P.maxp, a.tl_ids_in + tile * MAXC, cnt, i, j, ux, uy, gx, gy);
+}
 
 // Particle block: thread 0 runs the contact force and the Newton-Euler update
 // of particle p from state k (pin) into state k+1 (pout).
@@ -367,6 +393,7 @@ __device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
     update_one(P, rec);
     for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
     if (p == P.np - 1) *a.dpw_out = dl;
+    if (P.np > NP_LOOP) register_tiles(P, rec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
 // Hydrodynamic force and torque on particle p at step k (ComputeForce_HYDRO,
@@ -561,6 +588,8 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
                 update_one(P, rec);
                 for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
                 if (p == P.np - 1) *a.dpw_out = dl;
+                if (P.np > NP_LOOP)
+                    register_tiles(P, rec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
             }
         }
         return;
@@ -577,8 +606,8 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
             float h[9], rd, rho, ux, uy;
             load_fk_dev<St>(P, src, (const float *)a.erho_src, n, i, j, h);
             macros_dev(h, (float)P.rho0, rd, rho, ux, uy);
-            double gxd = 0.0, gyd = 0.0;
-            if (P.np > 0) spm_node(a.pin, P.maxp, P.np, i, j, (double)ux, (double)uy, gxd, gyd);
+            double gxd, gyd;
+            spm_tile_node(a, tile, i, j, (double)ux, (double)uy, gxd, gyd);
             float gx = (float)gxd, gy = (float)gyd;
             float ucx = ux + 0.5f * gx, ucy = uy + 0.5f * gy;
             mass = (double)P.rho0 + (double)rd;
@@ -612,8 +641,8 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
             double f[9], rho, ux, uy;
             load_fk64<St>(P, src, (const double *)a.erho_src, n, i, j, f);
             macros64(f, rho, ux, uy);
-            double gx = 0.0, gy = 0.0;
-            if (P.np > 0) spm_node(a.pin, P.maxp, P.np, i, j, ux, uy, gx, gy);
+            double gx, gy;
+            spm_tile_node(a, tile, i, j, ux, uy, gx, gy);
             double ucx = ux + 0.5 * 1.0 * (gx + 0.0), ucy = uy + 0.5 * 1.0 * (gy + 0.0);
             mass = rho;
             if (FINAL) {

@@ -1,0 +1,51 @@
+"""Reduced-precision storage (FP32 collide; FP32 / FP16 / BF16 deviations
+f - w_i*rho0 in HBM) against the FP64 oracle on a settling-disk Bingham run.
+
+Measured quantities: settling velocity and position of the disk, mean
+density, and the yielded-node count.  Bounds: FP32 storage 1e-4, FP16 5e-3,
+BF16 5e-2 relative on the settling velocity (the FP16 target on the two-disk
+benchmark workload is 1e-3 and is reported by bench.py).
+"""
+import numpy as np
+import pytest
+
+import cases
+from lbmtest import f6_case, gpu_solver, oracle_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+VY_TOL = {"fp32": 1e-4, "fp16": 5e-3, "bf16": 5e-2}
+
+
+@pytest.fixture(scope="module")
+def fp64_reference(orc):
+    nx, ny, n = 60, 180, 3000
+    cfg, part = f6_case(nx, ny, True)
+    st = oracle_state(orc, cfg, part)
+    hist = st.run(0, n)
+    return cfg, part, hist, st
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_reduced_storage_settling(orc, fp64_reference, storage):
+    cfg, part, ho, st = fp64_reference
+    n = ho.shape[0]
+    print("case: reduced storage", storage, flush=True)
+    g = gpu_solver(cfg, storage=storage)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, n)
+    vy_g, vy_o = hg[-1, orc.H_VY], ho[-1, orc.H_VY]
+    e_vy = abs(vy_g - vy_o) / abs(vy_o)
+    e_y = abs(hg[-1, orc.H_Y] - ho[-1, orc.H_Y]) / abs(ho[-1, orc.H_Y])
+    e_mass = rel_err(hg[:, -2], ho[:, -2])
+    e_yield = abs(hg[-1, -1] - ho[-1, -1]) / max(ho[-1, -1], 1.0)
+    print("%s: vy %.3e (ref %.6e)  y %.3e  mass %.3e  yielded %.3e"
+          % (storage, e_vy, vy_o, e_y, e_mass, e_yield), flush=True)
+    assert np.all(np.isfinite(hg))
+    assert e_vy <= VY_TOL[storage]
+    assert e_y <= VY_TOL[storage]
+    assert e_mass <= 1e-4
+    # FP64 state is rebuilt from the stored deviations at the end of the run
+    rho, ux, uy, gg, om = g.get_fields()
+    assert rel_err(rho, st.rho) <= 1e-3
