@@ -71,33 +71,46 @@ __device__ __forceinline__ int bnd_index(const DevP &P, int i, int j) {
 // Storage formats of the fused path.  FP64 stores f itself; the reduced
 // formats store h = f - w_i*rho0 (times 2^10 for FP16 to stay clear of
 // subnormals), so load/store are exact inverses up to the format's rounding.
+// 32-bit integer hash (xorshift-multiply finaliser) for stochastic rounding.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// st(p, v, r): r is a per-value random word; only BF16 uses it.
 struct StF64 {
     using T = double;
     __device__ static double ld(const T *p) { return *p; }
-    __device__ static void st(T *p, double v) { *p = v; }
+    __device__ static void st(T *p, double v, uint32_t) { *p = v; }
 };
 struct StF32 {
     using T = float;
     __device__ static float ld(const T *p) { return *p; }
-    __device__ static void st(T *p, float v) { *p = v; }
+    __device__ static void st(T *p, float v, uint32_t) { *p = v; }
 };
 struct StF16 {
     using T = uint16_t;
     __device__ static float ld(const T *p) {
         return (float)__builtin_bit_cast(_Float16, *p) * (1.0f / 1024.0f);
     }
-    __device__ static void st(T *p, float v) {
+    __device__ static void st(T *p, float v, uint32_t) {
         *p = __builtin_bit_cast(uint16_t, (_Float16)(v * 1024.0f));
     }
 };
+// BF16 keeps 8 significant bits.  Round-to-nearest of a quasi-steady field
+// repeats the same rounding error every step, which acts as a persistent
+// source term; stochastic rounding (add 16 random low bits, truncate) makes
+// the stored value unbiased.
 struct StBF16 {
     using T = uint16_t;
     __device__ static float ld(const T *p) {
         return __builtin_bit_cast(float, (uint32_t)(*p) << 16);
     }
-    __device__ static void st(T *p, float v) {
+    __device__ static void st(T *p, float v, uint32_t r) {
         uint32_t u = __builtin_bit_cast(uint32_t, v);
-        u += 0x7FFFu + ((u >> 16) & 1u);           // round to nearest even
+        u += (r & 0xFFFFu);
         *p = (uint16_t)(u >> 16);
     }
 };

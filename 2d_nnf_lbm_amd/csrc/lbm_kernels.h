@@ -502,7 +502,7 @@ __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
             collide_dev(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy, om, om2,
                         str, yielded);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k]);
+            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], mix32((uint32_t)(n * 9 + k) ^ mix32(a.gen_out)));
             if (P.rheology == 1) a.rate32[n] = om2;
             if (a.aux) {
                 a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
             double om = P.rheology == 1 ? a.cobn[n] : P.omega;
             collide64(P, f, r, ux, uy, gx, gy, om, o);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k]);
+            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
             if (P.rheology == 1) a.cobn[n] = o.omega_bn;
             yielded = o.yielded;
             if (a.aux) {
@@ -628,7 +628,7 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
                 float om = P.rheology == 1 ? a.rate32[n] : (float)P.omega, om2, str[3];
                 collide_dev(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k]);
+                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], mix32((uint32_t)(n * 9 + k) ^ mix32(a.gen_out)));
                 if (P.rheology == 1) a.rate32[n] = om2;
                 if (a.aux) {
                     a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
                 double om = P.rheology == 1 ? a.cobn[n] : P.omega;
                 collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k]);
+                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
                 if (P.rheology == 1) a.cobn[n] = o.omega_bn;
                 yielded = o.yielded;
                 if (a.aux) {

@@ -62,6 +62,16 @@ PROGRAM LBM_F6_DRIVER
     STOP
   END IF
 
+  OPEN(12, FILE='00_Simulation_Parameters.txt')                  ! F6:104-107
+  WRITE(12, *) '**********************'
+  WRITE(12, *) 'Lattice Parameter'
+  WRITE(12, *) 'TAU_LB=', 1.0 / OMEGA
+  WRITE(12, *) 'U_LB_max=', U_C
+  WRITE(12, *) 'Tau_Y_LB=', TAU_Y
+  WRITE(12, *) 'A_G=', A_G
+  WRITE(12, *) 'MAX_ITR =', TIME_MAX
+  CLOSE(12)
+
   P%NX = NX
   P%NY = NY
   P%RHEOLOGY = MERGE(LBM_BINGHAM, LBM_NEWTONIAN, NON_NEWTONIAN)

@@ -54,7 +54,10 @@ def parse_run(d, nx, ny):
     toks = " ".join(rows).split()
     out["avgden"] = np.array([float(t) for t in toks]).reshape(-1, 2)[:, 1]
     consts = {}
-    with open(os.path.join(d, "00_Simulation_Parameters.txt")) as fh:
+    pfile = os.path.join(d, "00_Simulation_Parameters.txt")
+    if not os.path.exists(pfile):
+        return out, consts
+    with open(pfile) as fh:
         for line in fh:
             if "=" in line:
                 k, v = line.split("=", 1)

@@ -7,6 +7,7 @@ import pytest
 
 import cases
 from lbmtest import disk_record, gpu_solver, oracle_state, rel_err
+from test_gpu_parity import check_field_errors, field_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -46,12 +47,9 @@ def test_many_particles_fused_matches_oracle(orc, bingham):
     g.initialize()
     g.set_particles(part)
     hg = g.step(0, steps)
-    errs = dict(f=rel_err(g.f, st.f))
     rho, ux, uy, gg, om = g.get_fields()
-    errs.update(ux=rel_err(ux, st.ux), uy=rel_err(uy, st.uy), G=rel_err(gg, st.g))
-    print({k: "%.2e" % v for k, v in errs.items()}, flush=True)
-    for k, v in errs.items():
-        assert v <= 1e-11, k
+    ref = dict(f=st.f, rho=st.rho, ux=st.ux, uy=st.uy, g=st.g, omega_bn=st.omega_bn)
+    check_field_errors(field_errors(g.f, rho, ux, uy, gg, om, ref), "many")
     H = orc.H_PCOUNT
     for fld in (orc.H_X, orc.H_Y, orc.H_VX, orc.H_VY):
         cols = [p * H + fld for p in range(n)]
