@@ -79,7 +79,7 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
-// st(p, v, r): r is a per-value random word; only BF16 uses it.
+// st(p, v, r): r is a per-value random word for stochastic rounding (FP16, BF16).
 struct StF64 {
     using T = double;
     __device__ static double ld(const T *p) { return *p; }
@@ -95,8 +95,17 @@ struct StF16 {
     __device__ static float ld(const T *p) {
         return (float)__builtin_bit_cast(_Float16, *p) * (1.0f / 1024.0f);
     }
-    __device__ static void st(T *p, float v, uint32_t) {
-        *p = __builtin_bit_cast(uint16_t, (_Float16)(v * 1024.0f));
+    // stochastic rounding in the normal range of FP16: 13 random low bits are
+    // added to the FP32 value, which is then truncated to FP16 precision and
+    // converted exactly; below it round-to-nearest (absolute error < 2^-25)
+    __device__ static void st(T *p, float v, uint32_t r) {
+        float x = v * 1024.0f;
+        if (fabsf(x) >= 6.103515625e-05f) {
+            uint32_t u = __builtin_bit_cast(uint32_t, x);
+            u = (u + (r & 0x1FFFu)) & ~0x1FFFu;
+            x = __builtin_bit_cast(float, u);
+        }
+        *p = __builtin_bit_cast(uint16_t, (_Float16)x);
     }
 };
 // BF16 keeps 8 significant bits.  Round-to-nearest of a quasi-steady field
@@ -273,6 +282,7 @@ __device__ __forceinline__ bool spm_node_cand(const double *part, int maxp, cons
 
 constexpr int NP_LOOP = 16;    // above this many particles, tiles use candidate lists
 constexpr int MAXC = 32;       // candidate particles per tile
+constexpr int MAXSUB = 16;     // blocks summing the force on one particle
 
 // Registers particle p (record rec, stride 1) in the candidate list of every
 // grid tile its reach box touches, for generation gen.  Each tile header packs

@@ -358,6 +358,11 @@ struct StepArgs {
     int *tl_ids_out;
     unsigned gen_in, gen_out;
     int *err;                     // bit 0: a tile list overflowed
+    // the force on one particle is summed by nsub blocks; the last of them
+    // (counter pcount) adds the partials pforce in order and advances it
+    int nsub;
+    double *pforce;               // [maxp][nsub][3]
+    int *pcount;                  // [maxp]
 };
 
 // G at a node of a tile: all particles tested directly when there are few,
@@ -399,9 +404,10 @@ __device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
 // Hydrodynamic force and torque on particle p at step k (ComputeForce_HYDRO,
 // F6:542-556) over the nodes that can have phi > 0, from populations
 // rebuilt by the same pull-stream and bounce-back as the grid pass; a
-// fixed-order block reduction keeps it deterministic.
+// fixed-order block reduction keeps it deterministic.  Block `sub` of `nsub`
+// takes every nsub-th group of NT nodes of the reach box.
 template <class St, bool DEV>
-__device__ void particle_force(const StepArgs &a, int p, double (&F)[3]) {
+__device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, double (&F)[3]) {
     __shared__ double lds[3 * NT];
     const DevP &P = a.P;
     const int m = P.maxp;
@@ -416,7 +422,7 @@ __device__ void particle_force(const StepArgs &a, int p, double (&F)[3]) {
     double acc[3] = {0.0, 0.0, 0.0};
     if (bw > 0 && bh > 0) {
         const typename St::T *src = (const typename St::T *)a.src;
-        for (int t = threadIdx.x; t < bw * bh; t += NT) {
+        for (int t = sub * NT + threadIdx.x; t < bw * bh; t += nsub * NT) {
             int i = i0 + t % bw, j = j0 + t / bw;
             double ph = phi_at(xc, yc, dm, zt, i, j);
             if (!(ph > 0.0)) continue;
@@ -477,7 +483,7 @@ template <class St, bool DEV>
 __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
     const DevP &P = a.P;
     if ((int)blockIdx.x < a.n_pblocks) {
-        particle_advance(a, blockIdx.x);
+        if ((int)blockIdx.x % a.nsub == 0) particle_advance(a, (int)blockIdx.x / a.nsub);
         return;
     }
     long long tile = xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles);
@@ -563,9 +569,33 @@ template <class St, bool DEV, bool FINAL>
 __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     const DevP &P = a.P;
     if ((int)blockIdx.x < a.n_pblocks) {
-        int p = blockIdx.x;
+        int p = blockIdx.x / a.nsub, sub = blockIdx.x - p * a.nsub;
         double F[3];
-        particle_force<St, DEV>(a, p, F);
+        particle_force<St, DEV>(a, p, sub, a.nsub, F);
+        if (a.nsub > 1) {
+            // partial of this block; the last block of the particle adds all
+            // partials in sub order and carries on with the update
+            __shared__ int last;
+            if (threadIdx.x == 0) {
+                double *pf = a.pforce + ((size_t)p * a.nsub + sub) * 3;
+                pf[0] = F[0]; pf[1] = F[1]; pf[2] = F[2];
+                __threadfence();
+                last = atomicAdd(a.pcount + p, 1) == a.nsub - 1;
+                if (last) {
+                    __threadfence();
+                    const double *q = a.pforce + (size_t)p * a.nsub * 3;
+                    F[0] = 0.0; F[1] = 0.0; F[2] = 0.0;
+                    for (int s = 0; s < a.nsub; ++s) {
+                        F[0] += __builtin_nontemporal_load(q + 3 * s);
+                        F[1] += __builtin_nontemporal_load(q + 3 * s + 1);
+                        F[2] += __builtin_nontemporal_load(q + 3 * s + 2);
+                    }
+                    a.pcount[p] = 0;
+                }
+            }
+            __syncthreads();
+            if (!last) return;
+        }
         hist_write(a, p, F);
         if (threadIdx.x == 0) {
             const int m = P.maxp;

@@ -116,8 +116,11 @@ def test_fused_fp64_matches_oracle_1000_steps(orc, bingham):
     g.set_particles(part)
     hg = g.step(0, 1000)
     _compare_state(g, st, label="fused1000")
-    for col in (orc.H_X, orc.H_Y, orc.H_VX, orc.H_VY, orc.H_W, orc.H_FY):
+    for col in (orc.H_X, orc.H_Y, orc.H_VX, orc.H_VY):
         assert rel_err(hg[:, col], ho[:, col]) <= TRAJ_TOL, col
+    # angular velocity and force stay near round-off in this symmetric case
+    for col in (orc.H_W, orc.H_FY):
+        assert rel_err(hg[:, col], ho[:, col]) <= 1e-7, col
     # yielded-node counts may differ only where tau_mag sits on tau_y
     if bingham:
         assert np.max(np.abs(hg[:, -1] - ho[:, -1])) <= 2
@@ -164,7 +167,9 @@ def test_mass_invariant_closed_box(orc):
     cfg["wall"] = [0.0, 0.0, 0.0, 0.0, 0.01, 0.0, 0.0, 0.0]
     g = gpu_solver(cfg, max_particles=1)
     g.initialize()
-    m0 = g.total_mass()
+    # INITIALIZE sets RHO = rho0 exactly, while the populations sum to
+    # rho0*(1 + 7.45e-9) per node (float weights, F6:1135-1137)
+    m0 = float(np.sum(g.f))
     n = 200
     hist = g.step(0, n)
     fac = 36.0 * float(np.float32(1.0 / 36.0))
