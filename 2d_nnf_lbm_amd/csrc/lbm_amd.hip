@@ -124,6 +124,23 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.corner[0] = p->corner[0]; P.corner[1] = p->corner[1];
     P.zeta_c = p->zeta_c; P.cij = p->cij; P.eps_p = p->eps_p; P.eps_w = p->eps_w;
     P.e_p = p->e_p; P.e_w = p->e_w;
+    // magic numbers for row = node / W (Granlund-Montgomery round-up method)
+    if (P.nn >= (1LL << 31)) { delete h; return LBM_ERR_PARAM; }
+    {
+        int l = 0;
+        while ((1ULL << l) < (unsigned long long)P.W) ++l;
+        P.w_shift = l;
+        P.w_magic = (unsigned)(((1ULL << 32) * ((1ULL << l) - (unsigned long long)P.W)) /
+                               (unsigned long long)P.W + 1ULL);
+        auto div = [&](unsigned n) {
+            unsigned t = (unsigned)(((unsigned long long)P.w_magic * n) >> 32);
+            return (t + ((n - t) >> 1)) >> (l - 1);
+        };
+        const unsigned probe[] = {0u, 1u, (unsigned)P.W - 1, (unsigned)P.W, (unsigned)P.W + 1,
+                                  (unsigned)(P.nn - 1), (unsigned)(P.nn / 2), 2147483647u};
+        for (unsigned n : probe)
+            if (div(n) != n / (unsigned)P.W) { delete h; return LBM_ERR_PARAM; }
+    }
 
     auto al = [&](void **ptr, size_t bytes) -> int {
         hipError_t e = hipMalloc(ptr, bytes);

@@ -48,7 +48,16 @@ struct DevP {
     double omega, s0, s1, s2, s3, s5, sq, nu, mu0, tau_y, m_pap, gamma_min, rho0, gy;
     double wall[8], corner[2];
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;
+    unsigned w_magic;    // row = node / W by multiply-high (w_shift >= 1)
+    int w_shift;
 };
+
+// j = n / W for 0 <= n < 2^31 by the round-up multiply-high method (the
+// host computes w_magic, w_shift and checks them).
+__device__ __forceinline__ int row_of(const DevP &P, unsigned n) {
+    unsigned t = __umulhi(P.w_magic, n);
+    return (int)((t + ((n - t) >> 1)) >> (P.w_shift - 1));
+}
 
 __device__ __forceinline__ int cxk(int k) {
     return (k == 1 || k == 5 || k == 8) ? 1 : ((k == 3 || k == 6 || k == 7) ? -1 : 0);
@@ -79,7 +88,14 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
-// st(p, v, r): r is a per-value random word for stochastic rounding (FP16, BF16).
+// st(p, v, r): r is a per-value random word for stochastic rounding (FP16,
+// BF16), whose high bits are used.
+__device__ __forceinline__ uint32_t sr_word(uint32_t node_hash, int k) {
+    // distinct odd multipliers spread one node hash over the 9 populations
+    const uint32_t K[9] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu, 0x165667B1u,
+                           0xD3A2646Bu, 0xFD7046C5u, 0xB55A4F09u, 0x8F5A3C2Bu};
+    return node_hash * K[k];
+}
 struct StF64 {
     using T = double;
     __device__ static double ld(const T *p) { return *p; }
@@ -102,7 +118,7 @@ struct StF16 {
         float x = v * 1024.0f;
         if (fabsf(x) >= 6.103515625e-05f) {
             uint32_t u = __builtin_bit_cast(uint32_t, x);
-            u = (u + (r & 0x1FFFu)) & ~0x1FFFu;
+            u = (u + (r >> 19)) & ~0x1FFFu;
             x = __builtin_bit_cast(float, u);
         }
         *p = __builtin_bit_cast(uint16_t, (_Float16)x);
@@ -119,7 +135,7 @@ struct StBF16 {
     }
     __device__ static void st(T *p, float v, uint32_t r) {
         uint32_t u = __builtin_bit_cast(uint32_t, v);
-        u += (r & 0xFFFFu);
+        u += (r >> 16);
         *p = (uint16_t)(u >> 16);
     }
 };
@@ -144,6 +160,12 @@ template <class St>
 __device__ __forceinline__ void load_fk64(const DevP &P, const typename St::T *src,
                                           const double *erho, long long node, int i,
                                           int j, double v[9]) {
+    if (i > 0 && j > 0 && i < P.nx && j < P.ny) {  // interior: every source in the domain
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            v[k] = St::ld(src + k * P.plane + node - cxk(k) - (long long)cyk(k) * P.W);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         int si = i - cxk(k), sj = j - cyk(k);
@@ -416,6 +438,12 @@ template <class St>
 __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T *src,
                                             const float *erho, long long node, int i, int j,
                                             float h[9]) {
+    if (i > 0 && j > 0 && i < P.nx && j < P.ny) {  // interior: every source in the domain
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            h[k] = St::ld(src + k * P.plane + node - cxk(k) - (long long)cyk(k) * P.W);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         int si = i - cxk(k), sj = j - cyk(k);
@@ -476,16 +504,27 @@ __device__ __forceinline__ void macros_dev(const float h[9], float rho0, float &
                                            float &rho, float &ux, float &uy) {
     rho_dev = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
     rho = rho0 + rho_dev;
-    float rinv = 1.0f / rho;
+    float rinv = __builtin_amdgcn_rcpf(rho);
     ux = rinv * ((h[1] + h[5] + h[8]) - (h[6] + h[3] + h[7]));
     uy = rinv * ((h[5] + h[2] + h[6]) - (h[7] + h[4] + h[8]));
 }
 
+// Hardware reciprocal, square root and base-2 exponential (about 1 ulp):
+// the FP32 path trades IEEE-exact division for issue rate.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fexp(float x) {
+    return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
+}
+
 // Collision_MRT in deviation form and FP32 (same physics as collide64).
+// OUT: also produce the stress tensor and the yield flag (output only); the
+// Newtonian rate needs neither, the Papanastasiou rate needs the strain rate.
+template <bool OUT>
 __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho_dev, float rho,
                                             float ux, float uy, float gx, float gy, float om,
                                             float &om_out, float stress[3], int &yielded) {
-    float q = P.q_mode ? (8.0f * (2.0f - om)) / (8.0f - om) : (float)P.sq;
+    float q = P.q_mode ? (8.0f * (2.0f - om)) * frcp(8.0f - om) : (float)P.sq;
     float s[9] = {(float)P.s0, (float)P.s1, (float)P.s2, (float)P.s3, q,
                   (float)P.s5, q, om, om};
     float m[9], mf[9], meq[9], mne[9];
@@ -530,6 +569,9 @@ __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho
     h[6] = d * (b2 - c3 + c5 - 9.0f * m[8]);
     h[7] = d * (b2 - c3 - c5 + 9.0f * m[8]);
     h[8] = d * (b2 + c3 - c5 - 9.0f * m[8]);
+    om_out = om;
+    yielded = 0;
+    if (!OUT && P.rheology != 1) return;
     // Chai strain rate (F6:903-916); only f1[1..8] enter it
     float e0 = 4.0f * s[0] * mne[0], e1 = s[1] * mne[1], e2 = s[2] * mne[2];
     float e3 = 6.0f * s[3] * mne[3], e4 = s[4] * mne[4], e5 = 6.0f * s[5] * mne[5];
@@ -547,25 +589,27 @@ __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho
     float d22 = zarib * (f1_5 + f1_2 + f1_6 + f1_7 + f1_4 + f1_8)
               + 0.75f * ((s[7] - s[1]) * gu - s[7] * (2.0f * gy * uy));
     float d12 = zarib * (f1_5 - f1_6 + f1_7 - f1_8) - 0.75f * (s[7] * (gx * uy + gy * ux));
-    float nu = (float)P.nu;
-    stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
-    om_out = om;
-    yielded = 0;
+    if (OUT) {
+        float nu = (float)P.nu;
+        stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
+    }
     if (P.rheology == 1) {
-        float inv_rho = 1.0f / rho;
+        float inv_rho = frcp(rho);
         d11 *= inv_rho; d22 *= inv_rho; d12 *= inv_rho;
-        float gam = sqrtf(2.0f * fabsf(d11 * d11 + d22 * d22 + 2.0f * (d12 * d12)));
+        float gam = fsqrt(2.0f * fabsf(d11 * d11 + d22 * d22 + 2.0f * (d12 * d12)));
         float mu0 = (float)P.mu0, ty = (float)P.tau_y, mp = (float)P.m_pap;
         float nu_a;
         if (gam < (float)P.gamma_min)
             nu_a = mu0 + mp * ty;
         else
-            nu_a = mu0 + (ty / gam) * (1.0f - expf(-(mp * gam)));
-        float t11 = nu_a * 2.0f * d11, t22 = nu_a * 2.0f * d22, t12 = nu_a * 2.0f * d12;
-        stress[0] = t11; stress[1] = t22; stress[2] = t12;
-        float tmag = sqrtf(0.5f * fabsf(t11 * t11 + t22 * t22 + 2.0f * (t12 * t12)));
-        yielded = (tmag <= ty) ? 0 : 1;
-        om_out = 1.0f / (3.0f * (nu_a * inv_rho) + 0.5f);
+            nu_a = mu0 + (ty * frcp(gam)) * (1.0f - fexp(-(mp * gam)));
+        if (OUT) {
+            float t11 = nu_a * 2.0f * d11, t22 = nu_a * 2.0f * d22, t12 = nu_a * 2.0f * d12;
+            stress[0] = t11; stress[1] = t22; stress[2] = t12;
+            float tmag = fsqrt(0.5f * fabsf(t11 * t11 + t22 * t22 + 2.0f * (t12 * t12)));
+            yielded = (tmag <= ty) ? 0 : 1;
+        }
+        om_out = frcp(3.0f * (nu_a * inv_rho) + 0.5f);
     }
 }
 

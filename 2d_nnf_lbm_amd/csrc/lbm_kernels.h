@@ -365,14 +365,32 @@ struct StepArgs {
     int *pcount;                  // [maxp]
 };
 
+// Can any particle reach the rows of this tile?  (Uniform over the block;
+// with few particles it lets most tiles skip the smoothed-profile work.)
+__device__ __forceinline__ bool tile_has_particles(const StepArgs &a, long long tile) {
+    const DevP &P = a.P;
+    if (P.np == 0) return false;
+    if (P.np > NP_LOOP) return true;
+    long long n0 = tile * NT, n1 = min(n0 + NT - 1, P.nn - 1);
+    double jlo = (double)row_of(P, (unsigned)n0), jhi = (double)row_of(P, (unsigned)n1);
+    const int m = P.maxp;
+    for (int p = 0; p < P.np; ++p) {
+        double reach = 0.5 * a.pin[PDIAM * m + p] + 0.5 * a.pin[PZETA * m + p] + 1.0;
+        double y = a.pin[PY * m + p];
+        if (y + reach >= jlo && y - reach <= jhi) return true;
+    }
+    return false;
+}
+
 // G at a node of a tile: all particles tested directly when there are few,
 // else the tile's candidate list for the current generation.
-__device__ __forceinline__ void spm_tile_node(const StepArgs &a, long long tile, int i, int j,
-                                              double ux, double uy, double &gx, double &gy) {
+__device__ __forceinline__ void spm_tile_node(const StepArgs &a, long long tile, bool any, int i,
+                                              int j, double ux, double uy, double &gx,
+                                              double &gy) {
     const DevP &P = a.P;
     gx = 0.0;
     gy = 0.0;
-    if (P.np == 0) return;
+    if (!any) return;
     if (P.np <= NP_LOOP) {
         spm_node(a.pin, P.maxp, P.np, i, j, ux, uy, gx, gy);
         return;
@@ -490,7 +508,7 @@ __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
     long long n = tile * NT + threadIdx.x;
     int yielded = 0;
     if (n < P.nn) {
-        int j = (int)(n / P.W), i = (int)(n - (long long)j * P.W);
+        int j = row_of(P, (unsigned)n), i = (int)(n - (long long)j * P.W);
         typename St::T *dst = (typename St::T *)a.dst;
         double r = a.crho[n], ux = a.cux[n], uy = a.cuy[n];
         double gx = a.cg[n], gy = a.cg[P.plane + n];
@@ -505,10 +523,15 @@ __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
             // density deviation from the same populations
             float rd = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
             float om = (float)(P.rheology == 1 ? a.cobn[n] : P.omega), om2, str[3];
-            collide_dev(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy, om, om2,
-                        str, yielded);
+            if (a.aux || a.slots_yield)
+                collide_dev<true>(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy,
+                                  om, om2, str, yielded);
+            else
+                collide_dev<false>(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy,
+                                   om, om2, str, yielded);
+            uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
 #pragma unroll
-            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], mix32((uint32_t)(n * 9 + k) ^ mix32(a.gen_out)));
+            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], sr_word(nh, k));
             if (P.rheology == 1) a.rate32[n] = om2;
             if (a.aux) {
                 a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
@@ -628,8 +651,9 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     long long n = tile * NT + threadIdx.x;
     double mass = 0.0;
     int yielded = 0;
+    const bool any = tile_has_particles(a, tile);
     if (n < P.nn) {
-        int j = (int)(n / P.W), i = (int)(n - (long long)j * P.W);
+        int j = row_of(P, (unsigned)n), i = (int)(n - (long long)j * P.W);
         const typename St::T *src = (const typename St::T *)a.src;
         int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
         if constexpr (DEV) {
@@ -637,7 +661,7 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
             load_fk_dev<St>(P, src, (const float *)a.erho_src, n, i, j, h);
             macros_dev(h, (float)P.rho0, rd, rho, ux, uy);
             double gxd, gyd;
-            spm_tile_node(a, tile, i, j, (double)ux, (double)uy, gxd, gyd);
+            spm_tile_node(a, tile, any, i, j, (double)ux, (double)uy, gxd, gyd);
             float gx = (float)gxd, gy = (float)gyd;
             float ucx = ux + 0.5f * gx, ucy = uy + 0.5f * gy;
             mass = (double)P.rho0 + (double)rd;
@@ -656,9 +680,13 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
             } else {
                 typename St::T *dst = (typename St::T *)a.dst;
                 float om = P.rheology == 1 ? a.rate32[n] : (float)P.omega, om2, str[3];
-                collide_dev(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
+                if (a.aux || a.slots_yield)
+                    collide_dev<true>(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
+                else
+                    collide_dev<false>(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
+                uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
 #pragma unroll
-                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], mix32((uint32_t)(n * 9 + k) ^ mix32(a.gen_out)));
+                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], sr_word(nh, k));
                 if (P.rheology == 1) a.rate32[n] = om2;
                 if (a.aux) {
                     a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
@@ -672,7 +700,7 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
             load_fk64<St>(P, src, (const double *)a.erho_src, n, i, j, f);
             macros64(f, rho, ux, uy);
             double gx, gy;
-            spm_tile_node(a, tile, i, j, ux, uy, gx, gy);
+            spm_tile_node(a, tile, any, i, j, ux, uy, gx, gy);
             double ucx = ux + 0.5 * 1.0 * (gx + 0.0), ucy = uy + 0.5 * 1.0 * (gy + 0.0);
             mass = rho;
             if (FINAL) {

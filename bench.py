@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--accuracy-steps", type=int, default=3000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one device-to-device copy of 9 FP64 planes (known byte "
+                         "count) so hardware traffic counters can be scaled")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the fused step runs on a single MI355X; "
@@ -137,6 +140,14 @@ def main():
         sys.exit(2)
     pkg = importlib.import_module("2d_nnf_lbm_amd")
     configs = importlib.import_module("2d_nnf_lbm_amd.configs")
+    if args.copy_calibration:
+        cfg, _ = configs.two_disk_bingham("fp64")
+        s = pkg.D2Q9(cfg)
+        s.copy_post()
+        s.sync()
+        plane = (configs.nodes(cfg) + 255) // 256 * 256
+        log("bench: copy calibration, %d bytes read and written" % (9 * plane * 8))
+        s.close()
     log("bench: fp64 leg")
     r64 = measure(pkg, configs, "fp64", args.steps, args.warmup)
     log("bench: fp16 leg")
