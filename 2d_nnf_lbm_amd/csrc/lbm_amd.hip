@@ -124,6 +124,16 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.corner[0] = p->corner[0]; P.corner[1] = p->corner[1];
     P.zeta_c = p->zeta_c; P.cij = p->cij; P.eps_p = p->eps_p; P.eps_w = p->eps_w;
     P.e_p = p->e_p; P.e_w = p->e_w;
+    P.fomega = (float)P.omega; P.fs0 = (float)P.s0; P.fs1 = (float)P.s1; P.fs2 = (float)P.s2;
+    P.fs3 = (float)P.s3; P.fs5 = (float)P.s5; P.fsq = (float)P.sq; P.fnu = (float)P.nu;
+    P.fmu0 = (float)P.mu0; P.ftau_y = (float)P.tau_y; P.fm_pap = (float)P.m_pap;
+    P.fgamma_min = (float)P.gamma_min; P.frho0 = (float)P.rho0;
+    {
+        unsigned long long bb = 9ULL * (unsigned long long)P.plane *
+                                (p->storage == LBM_FP64 ? 8ULL : p->storage == LBM_FP32 ? 4ULL : 2ULL);
+        if (p->storage != LBM_FP64 && bb >= (1ULL << 32)) { delete h; return LBM_ERR_PARAM; }
+        P.buf_bytes = (unsigned)std::min<unsigned long long>(bb, 0xFFFFFFFFULL);
+    }
     // magic numbers for row = node / W (Granlund-Montgomery round-up method)
     if (P.nn >= (1LL << 31)) { delete h; return LBM_ERR_PARAM; }
     {

@@ -50,7 +50,19 @@ struct DevP {
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;
     unsigned w_magic;    // row = node / W by multiply-high (w_shift >= 1)
     int w_shift;
+    // FP32 copies of the parameters used by the reduced-precision path
+    float fomega, fs0, fs1, fs2, fs3, fs5, fsq, fnu, fmu0, ftau_y, fm_pap, fgamma_min, frho0;
+    unsigned buf_bytes;  // bytes of one 9-plane storage buffer (< 2^32)
 };
+
+// Raw buffer access to the storage planes: one buffer descriptor per array,
+// a per-node 32-bit vector offset and a uniform per-population offset, so the
+// nine loads and stores share one address computation.  Offsets beyond
+// num_records read as zero and drop writes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
 
 // j = n / W for 0 <= n < 2^31 by the round-up multiply-high method (the
 // host computes w_magic, w_shift and checks them).
@@ -105,24 +117,38 @@ struct StF32 {
     using T = float;
     __device__ static float ld(const T *p) { return *p; }
     __device__ static void st(T *p, float v, uint32_t) { *p = v; }
+    __device__ static float bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t r, float v, unsigned vo, unsigned so,
+                               uint32_t) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo,
+                                              (int)so, 0);
+    }
 };
 struct StF16 {
     using T = uint16_t;
     __device__ static float ld(const T *p) {
         return (float)__builtin_bit_cast(_Float16, *p) * (1.0f / 1024.0f);
     }
-    // stochastic rounding in the normal range of FP16: 13 random low bits are
-    // added to the FP32 value, which is then truncated to FP16 precision and
-    // converted exactly; below it round-to-nearest (absolute error < 2^-25)
+    // stochastic rounding (gfx950 v_cvt_sr_f16_f32): in the normal FP16 range
+    // the top 13 bits of r are added below the kept mantissa bits before
+    // truncation, which makes the stored value unbiased
     __device__ static void st(T *p, float v, uint32_t r) {
-        float x = v * 1024.0f;
-        if (fabsf(x) >= 6.103515625e-05f) {
-            uint32_t u = __builtin_bit_cast(uint32_t, x);
-            u = (u + (r >> 19)) & ~0x1FFFu;
-            x = __builtin_bit_cast(float, u);
-        }
-        *p = __builtin_bit_cast(uint16_t, (_Float16)x);
+        h2 w = __builtin_amdgcn_cvt_sr_f16_f32((h2){}, v * 1024.0f, r, false);
+        *p = __builtin_bit_cast(uint16_t, w[0]);
     }
+    __device__ static float bld(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so) {
+        unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)vo, (int)so, 0);
+        return (float)__builtin_bit_cast(_Float16, b) * (1.0f / 1024.0f);
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t rs, float v, unsigned vo, unsigned so,
+                               uint32_t r) {
+        h2 w = __builtin_amdgcn_cvt_sr_f16_f32((h2){}, v * 1024.0f, r, false);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, w[0]), rs,
+                                              (int)vo, (int)so, 0);
+    }
+    typedef __attribute__((ext_vector_type(2))) _Float16 h2;
 };
 // BF16 keeps 8 significant bits.  Round-to-nearest of a quasi-steady field
 // repeats the same rounding error every step, which acts as a persistent
@@ -133,11 +159,23 @@ struct StBF16 {
     __device__ static float ld(const T *p) {
         return __builtin_bit_cast(float, (uint32_t)(*p) << 16);
     }
+    // gfx950 v_cvt_sr_bf16_f32: adds the top 16 bits of r below the kept
+    // mantissa bits and truncates
     __device__ static void st(T *p, float v, uint32_t r) {
-        uint32_t u = __builtin_bit_cast(uint32_t, v);
-        u += (r >> 16);
-        *p = (uint16_t)(u >> 16);
+        b2 w = __builtin_amdgcn_cvt_sr_bf16_f32((b2){}, v, r, false);
+        *p = __builtin_bit_cast(uint16_t, w[0]);
     }
+    __device__ static float bld(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so) {
+        unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)vo, (int)so, 0);
+        return __builtin_bit_cast(float, (uint32_t)b << 16);
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t rs, float v, unsigned vo, unsigned so,
+                               uint32_t r) {
+        b2 w = __builtin_amdgcn_cvt_sr_bf16_f32((b2){}, v, r, false);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, w[0]), rs,
+                                              (int)vo, (int)so, 0);
+    }
+    typedef __attribute__((ext_vector_type(2))) __bf16 b2;
 };
 
 __device__ __forceinline__ float wdev(int k) {
@@ -439,9 +477,15 @@ __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T 
                                             const float *erho, long long node, int i, int j,
                                             float h[9]) {
     if (i > 0 && j > 0 && i < P.nx && j < P.ny) {  // interior: every source in the domain
+        // one vector offset for the node, a uniform offset per population
+        const auto rs = make_rsrc(src, P.buf_bytes);
+        const unsigned vo = (unsigned)node * (unsigned)sizeof(typename St::T);
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
-            h[k] = St::ld(src + k * P.plane + node - cxk(k) - (long long)cyk(k) * P.W);
+        for (int k = 0; k < 9; ++k) {
+            unsigned so = (unsigned)((k * P.plane - cxk(k) - (long long)cyk(k) * P.W) *
+                                     (long long)sizeof(typename St::T));
+            h[k] = St::bld(rs, vo, so);
+        }
         return;
     }
 #pragma unroll
@@ -524,9 +568,9 @@ template <bool OUT>
 __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho_dev, float rho,
                                             float ux, float uy, float gx, float gy, float om,
                                             float &om_out, float stress[3], int &yielded) {
-    float q = P.q_mode ? (8.0f * (2.0f - om)) * frcp(8.0f - om) : (float)P.sq;
-    float s[9] = {(float)P.s0, (float)P.s1, (float)P.s2, (float)P.s3, q,
-                  (float)P.s5, q, om, om};
+#pragma clang fp contract(fast)
+    float q = P.q_mode ? (8.0f * (2.0f - om)) * frcp(8.0f - om) : P.fsq;
+    float s[9] = {P.fs0, P.fs1, P.fs2, P.fs3, q, P.fs5, q, om, om};
     float m[9], mf[9], meq[9], mne[9];
     m[0] = rho_dev;
     m[1] = -4.0f * h[0] - (h[1] + h[2] + h[3] + h[4]) + 2.0f * (h[5] + h[6] + h[7] + h[8]);
@@ -590,16 +634,16 @@ __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho
               + 0.75f * ((s[7] - s[1]) * gu - s[7] * (2.0f * gy * uy));
     float d12 = zarib * (f1_5 - f1_6 + f1_7 - f1_8) - 0.75f * (s[7] * (gx * uy + gy * ux));
     if (OUT) {
-        float nu = (float)P.nu;
+        float nu = P.fnu;
         stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
     }
     if (P.rheology == 1) {
         float inv_rho = frcp(rho);
         d11 *= inv_rho; d22 *= inv_rho; d12 *= inv_rho;
         float gam = fsqrt(2.0f * fabsf(d11 * d11 + d22 * d22 + 2.0f * (d12 * d12)));
-        float mu0 = (float)P.mu0, ty = (float)P.tau_y, mp = (float)P.m_pap;
+        float mu0 = P.fmu0, ty = P.ftau_y, mp = P.fm_pap;
         float nu_a;
-        if (gam < (float)P.gamma_min)
+        if (gam < P.fgamma_min)
             nu_a = mu0 + mp * ty;
         else
             nu_a = mu0 + (ty * frcp(gam)) * (1.0f - fexp(-(mp * gam)));

@@ -530,8 +530,11 @@ __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
                 collide_dev<false>(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy,
                                    om, om2, str, yielded);
             uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
+            const auto drs = make_rsrc(dst, P.buf_bytes);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], sr_word(nh, k));
+            for (int k = 0; k < 9; ++k)
+                St::bst(drs, h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
+                        (unsigned)(k * P.plane * (long long)sizeof(typename St::T)), sr_word(nh, k));
             if (P.rheology == 1) a.rate32[n] = om2;
             if (a.aux) {
                 a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
@@ -659,7 +662,7 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
         if constexpr (DEV) {
             float h[9], rd, rho, ux, uy;
             load_fk_dev<St>(P, src, (const float *)a.erho_src, n, i, j, h);
-            macros_dev(h, (float)P.rho0, rd, rho, ux, uy);
+            macros_dev(h, P.frho0, rd, rho, ux, uy);
             double gxd, gyd;
             spm_tile_node(a, tile, any, i, j, (double)ux, (double)uy, gxd, gyd);
             float gx = (float)gxd, gy = (float)gyd;
@@ -679,14 +682,18 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
                 if (a.aux && P.rheology != 1) a.cimage[n] = image_node(P, a.pin, i, j);
             } else {
                 typename St::T *dst = (typename St::T *)a.dst;
-                float om = P.rheology == 1 ? a.rate32[n] : (float)P.omega, om2, str[3];
+                float om = P.rheology == 1 ? a.rate32[n] : P.fomega, om2, str[3];
                 if (a.aux || a.slots_yield)
                     collide_dev<true>(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
                 else
                     collide_dev<false>(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
                 uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
+                const auto drs = make_rsrc(dst, P.buf_bytes);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, h[k], sr_word(nh, k));
+                for (int k = 0; k < 9; ++k)
+                    St::bst(drs, h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
+                            (unsigned)(k * P.plane * (long long)sizeof(typename St::T)),
+                            sr_word(nh, k));
                 if (P.rheology == 1) a.rate32[n] = om2;
                 if (a.aux) {
                     a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
