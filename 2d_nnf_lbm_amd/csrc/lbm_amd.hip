@@ -625,7 +625,14 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         if (fin)
             hipLaunchKernelGGL((k_step<St, DEV, true>), grid, block, 0, h->stream, a);
         else
-            hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
+        {
+            if constexpr (DEV) {
+                dim3 gp((unsigned)(a.n_pblocks + (ntiles + NPT - 1) / NPT));
+                hipLaunchKernelGGL((k_step_pipe<St>), gp, block, 0, h->stream, a);
+            } else {
+                hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
+            }
+        }
         if (timed == 2) (void)hipEventRecord(h->evk[2 * (s - 1) + 1], h->stream);
         rc = check_launch(h, "k_step");
         if (rc) return rc;

@@ -592,9 +592,9 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
 // Particle blocks compute the hydrodynamic force of step k and advance the
 // particles to state k+1.
 template <class St, bool DEV, bool FINAL>
-__global__ __launch_bounds__(NT) void k_step(StepArgs a) {
+__device__ void particle_block(const StepArgs &a) {
     const DevP &P = a.P;
-    if ((int)blockIdx.x < a.n_pblocks) {
+    {
         int p = blockIdx.x / a.nsub, sub = blockIdx.x - p * a.nsub;
         double F[3];
         particle_force<St, DEV>(a, p, sub, a.nsub, F);
@@ -648,6 +648,14 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
                     register_tiles(P, rec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
             }
         }
+    }
+}
+
+template <class St, bool DEV, bool FINAL>
+__global__ __launch_bounds__(NT) void k_step(StepArgs a) {
+    const DevP &P = a.P;
+    if ((int)blockIdx.x < a.n_pblocks) {
+        particle_block<St, DEV, FINAL>(a);
         return;
     }
     long long tile = xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles);
@@ -738,6 +746,92 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     if (a.slots_mass || (!FINAL && a.slots_yield))
         tile_monitors(a, tile, mass, yielded, a.slots_mass != nullptr,
                       !FINAL && a.slots_yield != nullptr);
+}
+
+// ------------------------------------------- pipelined reduced-precision pass
+// The reduced-precision step is issue-bound (several hundred FP32 VALU
+// instructions per node against 44 bytes of traffic).  With one node per
+// thread every wave loads first and computes afterwards, so memory and
+// arithmetic hardly overlap.  Here each thread takes NPT nodes, NT apart,
+// and issues the loads of node q+1 before colliding node q.
+constexpr int NPT = 4;
+
+struct NodeIn {
+    float h[9];
+    float om;
+    long long n;
+    int i, j;
+};
+
+template <class St>
+__device__ __forceinline__ void pipe_load(const StepArgs &a, long long tile, NodeIn &s) {
+    const DevP &P = a.P;
+    s.n = tile * NT + threadIdx.x;
+    if (s.n >= P.nn) return;
+    s.j = row_of(P, (unsigned)s.n);
+    s.i = (int)(s.n - (long long)s.j * P.W);
+    load_fk_dev<St>(P, (const typename St::T *)a.src, (const float *)a.erho_src, s.n, s.i, s.j,
+                    s.h);
+    s.om = P.rheology == 1 ? a.rate32[s.n] : P.fomega;
+}
+
+template <class St>
+__device__ __forceinline__ void pipe_node(const StepArgs &a, long long tile, bool any, NodeIn &s,
+                                          double &mass, int &yielded) {
+    const DevP &P = a.P;
+    const long long n = s.n;
+    if (n >= P.nn) return;
+    float rd, rho, ux, uy;
+    macros_dev(s.h, P.frho0, rd, rho, ux, uy);
+    double gxd, gyd;
+    spm_tile_node(a, tile, any, s.i, s.j, (double)ux, (double)uy, gxd, gyd);
+    float gx = (float)gxd, gy = (float)gyd;
+    float ucx = ux + 0.5f * gx, ucy = uy + 0.5f * gy;
+    mass = (double)P.rho0 + (double)rd;
+    float om2, str[3];
+    if (a.aux || a.slots_yield)
+        collide_dev<true>(P, s.h, rd, rho, ucx, ucy, gx, gy, s.om, om2, str, yielded);
+    else
+        collide_dev<false>(P, s.h, rd, rho, ucx, ucy, gx, gy, s.om, om2, str, yielded);
+    uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
+    const auto drs = make_rsrc(a.dst, P.buf_bytes);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        St::bst(drs, s.h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
+                (unsigned)(k * P.plane * (long long)sizeof(typename St::T)), sr_word(nh, k));
+    if (P.rheology == 1) a.rate32[n] = om2;
+    if (a.aux) {
+        a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
+        a.cstress[2 * P.plane + n] = str[2];
+        if (P.rheology == 1) a.cimage[n] = yielded ? 0 : 5;
+    }
+    if (s.i == 0 || s.j == 0 || s.i == P.nx || s.j == P.ny)
+        ((float *)a.erho_dst)[bnd_index(P, s.i, s.j)] = rho;
+}
+
+template <class St>
+__global__ __launch_bounds__(NT) void k_step_pipe(StepArgs a) {
+    if ((int)blockIdx.x < a.n_pblocks) {
+        particle_block<St, true, false>(a);
+        return;
+    }
+    const long long nb = (a.ntiles + NPT - 1) / NPT;
+    const long long b = xcd_tile(blockIdx.x - a.n_pblocks, nb);
+    const bool mon = a.slots_mass || a.slots_yield;
+    NodeIn s[2];
+    pipe_load<St>(a, b * NPT, s[0]);
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const long long tile = b * NPT + q;
+        if (q + 1 < NPT) pipe_load<St>(a, tile + 1, s[(q + 1) & 1]);
+        const bool any = tile < a.ntiles && tile_has_particles(a, tile);
+        double mass = 0.0;
+        int yielded = 0;
+        pipe_node<St>(a, tile, any, s[q & 1], mass, yielded);
+        if (mon)
+            tile_monitors(a, tile, mass, yielded, a.slots_mass != nullptr,
+                          a.slots_yield != nullptr);
+    }
 }
 
 }  // namespace lbm
