@@ -57,6 +57,8 @@ struct lbm_handle {
     int nsub = 1;
     double *pforce = nullptr;
     int *pcount = nullptr;
+    // FP32 smoothed-profile records, same parity as pbuf
+    float *spf[2] = {nullptr, nullptr};
     unsigned gen_base = 1;
 };
 
@@ -180,7 +182,9 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->scalar, 8)) ||
         (rc = al((void **)&h->err_flag, 4)) ||
         (rc = al((void **)&h->pforce, (size_t)P.maxp * MAXSUB * 3 * 8)) ||
-        (rc = al((void **)&h->pcount, (size_t)P.maxp * 4))) {
+        (rc = al((void **)&h->pcount, (size_t)P.maxp * 4)) ||
+        (rc = al((void **)&h->spf[0], (size_t)P.maxp * SPF * 4)) ||
+        (rc = al((void **)&h->spf[1], (size_t)P.maxp * SPF * 4))) {
         *out = h;
         return rc;
     }
@@ -218,7 +222,7 @@ int lbm_destroy(lbm_handle *h) {
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
-                    h->pforce, h->pcount};
+                    h->pforce, h->pcount, h->spf[0], h->spf[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -339,8 +343,9 @@ int lbm_set_particles(lbm_handle *h, int32_t n, const double *part) {
     HIPCHECK(h, hipMemcpyAsync(h->pbuf[h->pcur], buf.data(), buf.size() * 8,
                                hipMemcpyHostToDevice, h->stream));
     int PS = (int)std::ceil(2.0 * reach) + 2;
-    // blocks per particle for the force sum: about two reach-box nodes per thread
-    h->nsub = std::max(1, std::min(MAXSUB, (PS * PS + 2 * NT - 1) / (2 * NT)));
+    // blocks per particle for the force sum: one reach-box node per thread,
+    // so no thread waits on two dependent gathers in a row
+    h->nsub = std::max(1, std::min(MAXSUB, (PS * PS + NT - 1) / NT));
     if (n > 0 && PS > h->PS) {
         if (h->phi) (void)hipFree(h->phi);
         h->phi = nullptr;
@@ -596,6 +601,8 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
     a.dst = DEV ? buf[0] : buf[1];
     a.erho_dst = h->erho[e];
     a.pin = h->pbuf[pc]; a.pout = h->pbuf[pc ^ 1];
+    a.spf_in = nullptr;
+    a.spf_out = DEV ? h->spf[pc ^ 1] : nullptr;
     a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
     a.slots_yield = record ? h->hist_slots + NSLOT : nullptr;  // row 0, yield half
     a.slots_mass = nullptr;
@@ -615,6 +622,8 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         a.erho_dst = h->erho[e ^ 1];
         a.pin = h->pbuf[pc];
         a.pout = fin ? h->pbuf[pc] : h->pbuf[pc ^ 1];
+        a.spf_in = DEV ? h->spf[pc] : nullptr;
+        a.spf_out = (DEV && !fin) ? h->spf[pc ^ 1] : nullptr;
         a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
         a.hist_part = record ? h->hist_part + (size_t)(s - 1) * rowp : nullptr;
         a.slots_mass = record ? h->hist_slots + (size_t)(s - 1) * 2 * NSLOT : nullptr;
@@ -625,14 +634,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         if (fin)
             hipLaunchKernelGGL((k_step<St, DEV, true>), grid, block, 0, h->stream, a);
         else
-        {
-            if constexpr (DEV) {
-                dim3 gp((unsigned)(a.n_pblocks + (ntiles + NPT - 1) / NPT));
-                hipLaunchKernelGGL((k_step_pipe<St>), gp, block, 0, h->stream, a);
-            } else {
-                hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
-            }
-        }
+            hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
         if (timed == 2) (void)hipEventRecord(h->evk[2 * (s - 1) + 1], h->stream);
         rc = check_launch(h, "k_step");
         if (rc) return rc;

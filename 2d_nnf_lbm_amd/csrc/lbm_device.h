@@ -340,9 +340,64 @@ __device__ __forceinline__ bool spm_node_cand(const double *part, int maxp, cons
     return true;
 }
 
+// ------------------------------------------ FP32 smoothed-profile records
+// The reduced-precision path evaluates phi and G in FP32 from a compact
+// per-particle record.  Centres are split into an integer and a fractional
+// part so node offsets (i - X) stay exact to FP32 rounding of the fraction.
+enum { SX = 0, SXF, SY, SYF, SVX, SVY, SW, SRP, SHZ, SPZ, SREACH, SPAD, SPF };
+
+__device__ __forceinline__ void make_spf(const double *rec /* PCOUNT */, float *out) {
+    double x = rec[PX], y = rec[PY];
+    double xi = floor(x), yi = floor(y);
+    out[SX] = (float)xi; out[SXF] = (float)(x - xi);
+    out[SY] = (float)yi; out[SYF] = (float)(y - yi);
+    out[SVX] = (float)rec[PVX]; out[SVY] = (float)rec[PVY]; out[SW] = (float)rec[PW];
+    out[SRP] = (float)(0.5 * rec[PDIAM]);
+    out[SHZ] = (float)(0.5 * rec[PZETA]);
+    out[SPZ] = (float)(PI_F / rec[PZETA]);
+    out[SREACH] = (float)(0.5 * rec[PDIAM] + 0.5 * rec[PZETA] + 1.0);
+    out[SPAD] = 0.0f;
+}
+
+// phi of one particle at node (i, j) in FP32 (same piecewise profile as
+// phi_at); dx, dy receive the node offset from the centre.
+__device__ __forceinline__ float phi_f32(const float *s, int i, int j, float &dx, float &dy) {
+    dx = ((float)i - s[SX]) - s[SXF];
+    dy = ((float)j - s[SY]) - s[SYF];
+    if (fabsf(dx) > s[SREACH] || fabsf(dy) > s[SREACH]) return 0.0f;
+    float di = s[SRP] - __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+    if (di <= -s[SHZ]) return 0.0f;
+    if (fabsf(di) < s[SHZ]) return 0.5f * (__sinf(s[SPZ] * di) + 1.0f);
+    return 1.0f;
+}
+
+// G at a node in FP32 (F6:485-495): the largest index with phi > 0 wins.
+// ids == nullptr: test particles 0..cnt-1; else the listed ones.
+__device__ __forceinline__ void spm_node_f32(const float *spf, const int *ids, int cnt, int i,
+                                             int j, float ux, float uy, float &gx, float &gy) {
+    gx = 0.0f;
+    gy = 0.0f;
+    int best = -1;
+    float bphi = 0.0f, bdx = 0.0f, bdy = 0.0f;
+    for (int c = 0; c < cnt; ++c) {
+        int p = ids ? ids[c] : c;
+        if (p <= best) continue;
+        float dx, dy;
+        float ph = phi_f32(spf + p * SPF, i, j, dx, dy);
+        if (ph > 0.0f) { best = p; bphi = ph; bdx = dx; bdy = dy; }
+    }
+    if (best < 0) return;
+    const float *s = spf + best * SPF;
+    float uxp = s[SVX] - s[SW] * bdy;
+    float uyp = s[SVY] + s[SW] * bdx;
+    gx = bphi * (uxp - ux);
+    gy = bphi * (uyp - uy);
+}
+
 constexpr int NP_LOOP = 16;    // above this many particles, tiles use candidate lists
 constexpr int MAXC = 32;       // candidate particles per tile
 constexpr int MAXSUB = 16;     // blocks summing the force on one particle
+constexpr int NP_LDS = 128;    // up to this many particles, contact inputs are staged in LDS
 
 // Registers particle p (record rec, stride 1) in the candidate list of every
 // grid tile its reach box touches, for generation gen.  Each tile header packs
@@ -681,17 +736,18 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*NT 
 // F3:746-770, walls of F3:773-804), evaluated by one thread.  The pair branch
 // reads dpw, the D_PW left by the previous call (F3:760).  Writes in-range
 // partners j>p into nb[0..MAXNB) and returns their count; *dpw_last receives
-// this particle's last wall distance.
-__device__ __forceinline__ int contact_one(const DevP &P, const double *part, int maxp, int np,
-                                           int p, double dpw, double &fcx, double &fcy,
-                                           int *nb, double *dpw_last) {
+// this particle's last wall distance.  X, Y, D: centres and diameters of all
+// particles (global memory or an LDS copy).
+__device__ __forceinline__ int contact_one(const DevP &P, const double *X, const double *Y,
+                                           const double *D, int np, int p, double dpw,
+                                           double &fcx, double &fcy, int *nb, double *dpw_last) {
     const double zeta = P.zeta_c, cij = P.cij;
     fcx = 0.0; fcy = 0.0;
     int cnt = 0;
     auto pair_force = [&](int ip, int jp, double &f0, double &f1) -> bool {
-        double xi = part[PX * maxp + ip], yi = part[PY * maxp + ip];
-        double xj = part[PX * maxp + jp], yj = part[PY * maxp + jp];
-        double di = part[PDIAM * maxp + ip], dj = part[PDIAM * maxp + jp];
+        double xi = X[ip], yi = Y[ip];
+        double xj = X[jp], yj = Y[jp];
+        double di = D[ip], dj = D[jp];
         double dpp = sqrt((xi - xj) * (xi - xj) + (yi - yj) * (yi - yj));
         double rsum = 0.5 * di + 0.5 * dj;
         f0 = 0.0; f1 = 0.0;
@@ -720,7 +776,7 @@ __device__ __forceinline__ int contact_one(const DevP &P, const double *part, in
             ++cnt;
         }
     }
-    double x = part[PX * maxp + p], y = part[PY * maxp + p], dm = part[PDIAM * maxp + p];
+    double x = X[p], y = Y[p], dm = D[p];
     double d = 0.0;
     for (int iw = 1; iw <= 4; ++iw) {
         double rw0, rw1;

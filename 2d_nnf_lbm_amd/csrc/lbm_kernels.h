@@ -11,7 +11,7 @@ namespace lbm {
 
 // Tiles are handed to the 8 XCDs in contiguous ranges so that rows shared
 // between neighbouring tiles meet in the same L2 (blocks are dispatched to
-// XCDs round-robin by block index).
+// XCDs round-robin by block index).  A bijection of [0, T).
 __device__ __forceinline__ long long xcd_tile(long long b, long long T) {
     long long x = b & 7, per = T >> 3, rem = T & 7;
     long long start = x * per + (x < rem ? x : rem);
@@ -278,7 +278,9 @@ __global__ void k_contact(DevP P, double *part, const double *dpw_in, double *dp
     int p = blockIdx.x * NT + threadIdx.x;
     if (p >= P.np) return;
     double fcx, fcy, dl;
-    int cnt = contact_one(P, part, P.maxp, P.np, p, *dpw_in, fcx, fcy, pairs + p * MAXNB, &dl);
+    const int m = P.maxp;
+    int cnt = contact_one(P, part + PX * m, part + PY * m, part + PDIAM * m, P.np, p, *dpw_in, fcx,
+                          fcy, pairs + p * MAXNB, &dl);
     paircnt[p] = cnt;
     // all threads have read the centres before any writes the contact force:
     // the force fields are not read by contact_one
@@ -363,7 +365,43 @@ struct StepArgs {
     int nsub;
     double *pforce;               // [maxp][nsub][3]
     int *pcount;                  // [maxp]
+    // FP32 smoothed-profile records (reduced-precision runs; null for FP64)
+    const float *spf_in;          // state k
+    float *spf_out;               // state k+1
 };
+
+// FP32 variant of tile_has_particles for the reduced-precision path.
+__device__ __forceinline__ bool tile_has_particles_f32(const StepArgs &a, long long tile) {
+    const DevP &P = a.P;
+    if (P.np == 0) return false;
+    if (P.np > NP_LOOP) return true;
+    long long n0 = tile * NT, n1 = min(n0 + NT - 1, P.nn - 1);
+    float jlo = (float)row_of(P, (unsigned)n0), jhi = (float)row_of(P, (unsigned)n1);
+    for (int p = 0; p < P.np; ++p) {
+        const float *s = a.spf_in + p * SPF;
+        float y = s[SY] + s[SYF], r = s[SREACH];
+        if (y + r >= jlo && y - r <= jhi) return true;
+    }
+    return false;
+}
+
+// G at a node of a tile, FP32 (reduced-precision path).
+__device__ __forceinline__ void spm_tile_node_f32(const StepArgs &a, long long tile, bool any,
+                                                  int i, int j, float ux, float uy, float &gx,
+                                                  float &gy) {
+    const DevP &P = a.P;
+    gx = 0.0f;
+    gy = 0.0f;
+    if (!any) return;
+    if (P.np <= NP_LOOP) {
+        spm_node_f32(a.spf_in, nullptr, P.np, i, j, ux, uy, gx, gy);
+        return;
+    }
+    unsigned long long h = a.tl_hdr_in[tile];
+    if ((unsigned)(h >> 32) != a.gen_in) return;
+    int cnt = min((int)(h & 0xffffffffu), MAXC);
+    spm_node_f32(a.spf_in, a.tl_ids_in + tile * MAXC, cnt, i, j, ux, uy, gx, gy);
+}
 
 // Can any particle reach the rows of this tile?  (Uniform over the block;
 // with few particles it lets most tiles skip the smoothed-profile work.)
@@ -410,11 +448,13 @@ __device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
     double rec[PCOUNT];
     for (int f = 0; f < PCOUNT; ++f) rec[f] = a.pin[f * m + p];
     double fcx, fcy, dl;
-    int cnt = contact_one(P, a.pin, m, P.np, p, *a.dpw_in, fcx, fcy, a.pairs + p * MAXNB, &dl);
+    int cnt = contact_one(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
+                          *a.dpw_in, fcx, fcy, a.pairs + p * MAXNB, &dl);
     a.paircnt[p] = cnt;
     rec[PFCX] = fcx; rec[PFCY] = fcy;
     update_one(P, rec);
     for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
+    if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
     if (p == P.np - 1) *a.dpw_out = dl;
     if (P.np > NP_LOOP) register_tiles(P, rec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
@@ -442,25 +482,37 @@ __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, doub
         const typename St::T *src = (const typename St::T *)a.src;
         for (int t = sub * NT + threadIdx.x; t < bw * bh; t += nsub * NT) {
             int i = i0 + t % bw, j = j0 + t / bw;
-            double ph = phi_at(xc, yc, dm, zt, i, j);
-            if (!(ph > 0.0)) continue;
             long long node = (long long)j * P.W + i;
             double r, u, v;
             if constexpr (DEV) {
+                // same FP32 profile and rigid-body velocity as the grid pass
+                float dx, dy;
+                const float *s = a.spf_in + p * SPF;
+                float phf = phi_f32(s, i, j, dx, dy);
+                if (!(phf > 0.0f)) continue;
                 float h[9], rd, rf, uf, vf;
                 load_fk_dev<St>(P, src, (const float *)a.erho_src, node, i, j, h);
-                macros_dev(h, (float)P.rho0, rd, rf, uf, vf);
-                r = rf; u = uf; v = vf;
+                macros_dev(h, P.frho0, rd, rf, uf, vf);
+                float uxpf = s[SVX] - s[SW] * dy, uypf = s[SVY] + s[SW] * dx;
+                double ph = phf;
+                r = rf;
+                double du = (double)(uf - uxpf), dv = (double)(vf - uypf);
+                acc[0] = acc[0] + r * ph * du;
+                acc[1] = acc[1] + r * ph * dv;
+                acc[2] = acc[2] + r * ph * ((double)dx * dv - (double)dy * du);
+                continue;
             } else {
+                double ph = phi_at(xc, yc, dm, zt, i, j);
+                if (!(ph > 0.0)) continue;
                 double f[9];
                 load_fk64<St>(P, src, (const double *)a.erho_src, node, i, j, f);
                 macros64(f, r, u, v);
+                double uxp = vx - w * ((double)j - yc);
+                double uyp = vy + w * ((double)i - xc);
+                acc[0] = acc[0] + r * ph * (u - uxp);
+                acc[1] = acc[1] + r * ph * (v - uyp);
+                acc[2] = acc[2] + r * ph * (((double)i - xc) * (v - uyp) - ((double)j - yc) * (u - uxp));
             }
-            double uxp = vx - w * ((double)j - yc);
-            double uyp = vy + w * ((double)i - xc);
-            acc[0] = acc[0] + r * ph * (u - uxp);
-            acc[1] = acc[1] + r * ph * (v - uyp);
-            acc[2] = acc[2] + r * ph * (((double)i - xc) * (v - uyp) - ((double)j - yc) * (u - uxp));
         }
     }
     block_sum<3>(acc, lds);
@@ -596,8 +648,28 @@ __device__ void particle_block(const StepArgs &a) {
     const DevP &P = a.P;
     {
         int p = blockIdx.x / a.nsub, sub = blockIdx.x - p * a.nsub;
+        const int m = P.maxp;
+        // Stage the inputs of the update in LDS while the force gather runs:
+        // this particle's record, the contact inputs of all particles and the
+        // stale wall distance (the thread that finishes the update then reads
+        // LDS instead of waiting on a chain of dependent global loads).
+        __shared__ double srec[PCOUNT];
+        __shared__ double scon[3 * NP_LDS];
+        __shared__ double sdpw;
+        const bool staged = !FINAL && P.np <= NP_LDS;
+        if (!FINAL) {
+            const int t = threadIdx.x;
+            if (t < PCOUNT) srec[t] = a.pin[t * m + p];
+            if (staged)
+                for (int q = t; q < P.np; q += NT) {
+                    scon[q] = a.pin[PX * m + q];
+                    scon[NP_LDS + q] = a.pin[PY * m + q];
+                    scon[2 * NP_LDS + q] = a.pin[PDIAM * m + q];
+                }
+            if (t == 0) sdpw = *a.dpw_in;
+        }
         double F[3];
-        particle_force<St, DEV>(a, p, sub, a.nsub, F);
+        particle_force<St, DEV>(a, p, sub, a.nsub, F);  // ends with block barriers
         if (a.nsub > 1) {
             // partial of this block; the last block of the particle adds all
             // partials in sub order and carries on with the update
@@ -624,7 +696,6 @@ __device__ void particle_block(const StepArgs &a) {
         }
         hist_write(a, p, F);
         if (threadIdx.x == 0) {
-            const int m = P.maxp;
             if (FINAL) {
                 a.pout[PFX * m + p] = F[0]; a.pout[PFY * m + p] = F[1]; a.pout[PTQ * m + p] = F[2];
             }
@@ -633,16 +704,19 @@ __device__ void particle_block(const StepArgs &a) {
             // the update uses FORCE_k: write it into the outgoing record first
             if (threadIdx.x == 0) {
                 double rec[PCOUNT];
-                const int m = P.maxp;
-                for (int f = 0; f < PCOUNT; ++f) rec[f] = a.pin[f * m + p];
+                for (int f = 0; f < PCOUNT; ++f) rec[f] = srec[f];
                 rec[PFX] = F[0]; rec[PFY] = F[1]; rec[PTQ] = F[2];
                 double fcx, fcy, dl;
-                int cnt = contact_one(P, a.pin, m, P.np, p, *a.dpw_in, fcx, fcy,
-                                      a.pairs + p * MAXNB, &dl);
+                int cnt = staged
+                    ? contact_one(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw,
+                                  fcx, fcy, a.pairs + p * MAXNB, &dl)
+                    : contact_one(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
+                                  sdpw, fcx, fcy, a.pairs + p * MAXNB, &dl);
                 a.paircnt[p] = cnt;
                 rec[PFCX] = fcx; rec[PFCY] = fcy;
                 update_one(P, rec);
                 for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
+                if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
                 if (p == P.np - 1) *a.dpw_out = dl;
                 if (P.np > NP_LOOP)
                     register_tiles(P, rec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
@@ -662,7 +736,6 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     long long n = tile * NT + threadIdx.x;
     double mass = 0.0;
     int yielded = 0;
-    const bool any = tile_has_particles(a, tile);
     if (n < P.nn) {
         int j = row_of(P, (unsigned)n), i = (int)(n - (long long)j * P.W);
         const typename St::T *src = (const typename St::T *)a.src;
@@ -670,10 +743,12 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
         if constexpr (DEV) {
             float h[9], rd, rho, ux, uy;
             load_fk_dev<St>(P, src, (const float *)a.erho_src, n, i, j, h);
+            float om = (!FINAL && P.rheology == 1) ? a.rate32[n] : P.fomega;
+            // particle test after the population loads are in flight
+            const bool any = tile_has_particles_f32(a, tile);
             macros_dev(h, P.frho0, rd, rho, ux, uy);
-            double gxd, gyd;
-            spm_tile_node(a, tile, any, i, j, (double)ux, (double)uy, gxd, gyd);
-            float gx = (float)gxd, gy = (float)gyd;
+            float gx, gy;
+            spm_tile_node_f32(a, tile, any, i, j, ux, uy, gx, gy);
             float ucx = ux + 0.5f * gx, ucy = uy + 0.5f * gy;
             mass = (double)P.rho0 + (double)rd;
             if (FINAL) {
@@ -690,7 +765,7 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
                 if (a.aux && P.rheology != 1) a.cimage[n] = image_node(P, a.pin, i, j);
             } else {
                 typename St::T *dst = (typename St::T *)a.dst;
-                float om = P.rheology == 1 ? a.rate32[n] : P.fomega, om2, str[3];
+                float om2, str[3];
                 if (a.aux || a.slots_yield)
                     collide_dev<true>(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
                 else
@@ -713,6 +788,7 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
         } else {
             double f[9], rho, ux, uy;
             load_fk64<St>(P, src, (const double *)a.erho_src, n, i, j, f);
+            const bool any = tile_has_particles(a, tile);
             macros64(f, rho, ux, uy);
             double gx, gy;
             spm_tile_node(a, tile, any, i, j, ux, uy, gx, gy);
@@ -746,92 +822,6 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     if (a.slots_mass || (!FINAL && a.slots_yield))
         tile_monitors(a, tile, mass, yielded, a.slots_mass != nullptr,
                       !FINAL && a.slots_yield != nullptr);
-}
-
-// ------------------------------------------- pipelined reduced-precision pass
-// The reduced-precision step is issue-bound (several hundred FP32 VALU
-// instructions per node against 44 bytes of traffic).  With one node per
-// thread every wave loads first and computes afterwards, so memory and
-// arithmetic hardly overlap.  Here each thread takes NPT nodes, NT apart,
-// and issues the loads of node q+1 before colliding node q.
-constexpr int NPT = 4;
-
-struct NodeIn {
-    float h[9];
-    float om;
-    long long n;
-    int i, j;
-};
-
-template <class St>
-__device__ __forceinline__ void pipe_load(const StepArgs &a, long long tile, NodeIn &s) {
-    const DevP &P = a.P;
-    s.n = tile * NT + threadIdx.x;
-    if (s.n >= P.nn) return;
-    s.j = row_of(P, (unsigned)s.n);
-    s.i = (int)(s.n - (long long)s.j * P.W);
-    load_fk_dev<St>(P, (const typename St::T *)a.src, (const float *)a.erho_src, s.n, s.i, s.j,
-                    s.h);
-    s.om = P.rheology == 1 ? a.rate32[s.n] : P.fomega;
-}
-
-template <class St>
-__device__ __forceinline__ void pipe_node(const StepArgs &a, long long tile, bool any, NodeIn &s,
-                                          double &mass, int &yielded) {
-    const DevP &P = a.P;
-    const long long n = s.n;
-    if (n >= P.nn) return;
-    float rd, rho, ux, uy;
-    macros_dev(s.h, P.frho0, rd, rho, ux, uy);
-    double gxd, gyd;
-    spm_tile_node(a, tile, any, s.i, s.j, (double)ux, (double)uy, gxd, gyd);
-    float gx = (float)gxd, gy = (float)gyd;
-    float ucx = ux + 0.5f * gx, ucy = uy + 0.5f * gy;
-    mass = (double)P.rho0 + (double)rd;
-    float om2, str[3];
-    if (a.aux || a.slots_yield)
-        collide_dev<true>(P, s.h, rd, rho, ucx, ucy, gx, gy, s.om, om2, str, yielded);
-    else
-        collide_dev<false>(P, s.h, rd, rho, ucx, ucy, gx, gy, s.om, om2, str, yielded);
-    uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
-    const auto drs = make_rsrc(a.dst, P.buf_bytes);
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-        St::bst(drs, s.h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
-                (unsigned)(k * P.plane * (long long)sizeof(typename St::T)), sr_word(nh, k));
-    if (P.rheology == 1) a.rate32[n] = om2;
-    if (a.aux) {
-        a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
-        a.cstress[2 * P.plane + n] = str[2];
-        if (P.rheology == 1) a.cimage[n] = yielded ? 0 : 5;
-    }
-    if (s.i == 0 || s.j == 0 || s.i == P.nx || s.j == P.ny)
-        ((float *)a.erho_dst)[bnd_index(P, s.i, s.j)] = rho;
-}
-
-template <class St>
-__global__ __launch_bounds__(NT) void k_step_pipe(StepArgs a) {
-    if ((int)blockIdx.x < a.n_pblocks) {
-        particle_block<St, true, false>(a);
-        return;
-    }
-    const long long nb = (a.ntiles + NPT - 1) / NPT;
-    const long long b = xcd_tile(blockIdx.x - a.n_pblocks, nb);
-    const bool mon = a.slots_mass || a.slots_yield;
-    NodeIn s[2];
-    pipe_load<St>(a, b * NPT, s[0]);
-#pragma unroll
-    for (int q = 0; q < NPT; ++q) {
-        const long long tile = b * NPT + q;
-        if (q + 1 < NPT) pipe_load<St>(a, tile + 1, s[(q + 1) & 1]);
-        const bool any = tile < a.ntiles && tile_has_particles(a, tile);
-        double mass = 0.0;
-        int yielded = 0;
-        pipe_node<St>(a, tile, any, s[q & 1], mass, yielded);
-        if (mon)
-            tile_monitors(a, tile, mass, yielded, a.slots_mass != nullptr,
-                          a.slots_yield != nullptr);
-    }
 }
 
 }  // namespace lbm
