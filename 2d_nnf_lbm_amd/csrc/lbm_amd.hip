@@ -47,8 +47,8 @@ struct lbm_handle {
     double *hist_part = nullptr;
     unsigned long long *hist_slots = nullptr;
     size_t hist_rows = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<hipEvent_t> evk;
+    // timing events: whole sequence (ev0..ev1), back-to-back main launches (evA..evB)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
     // per-tile candidate lists of many-particle runs
     unsigned long long *tl_hdr[2] = {nullptr, nullptr};
     int *tl_ids[2] = {nullptr, nullptr};
@@ -210,6 +210,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     (void)hipMemcpy(h->dpw[1], &inf, 8, hipMemcpyHostToDevice);
     (void)hipEventCreate(&h->ev0);
     (void)hipEventCreate(&h->ev1);
+    (void)hipEventCreate(&h->evA);
+    (void)hipEventCreate(&h->evB);
     *out = h;
     return LBM_OK;
 }
@@ -227,7 +229,8 @@ int lbm_destroy(lbm_handle *h) {
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
-    for (auto e : h->evk) (void)hipEventDestroy(e);
+    if (h->evA) (void)hipEventDestroy(h->evA);
+    if (h->evB) (void)hipEventDestroy(h->evB);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return LBM_OK;
@@ -588,13 +591,6 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
     int e = 0;        // erho parity
     int pc = h->pcur; // particle buffer holding the current state
 
-    if (timed == 2 && (int)h->evk.size() < 2 * nsteps) {
-        {
-            for (auto ev : h->evk) (void)hipEventDestroy(ev);
-            h->evk.assign(2 * nsteps, nullptr);
-            for (auto &ev : h->evk) (void)hipEventCreate(&ev);
-        }
-    }
     if (timed) (void)hipEventRecord(h->ev0, h->stream);
     // prologue: collide step kk0's state into f*_{kk0+1}; particles to kk0+1
     a.src = DEV ? nullptr : buf[0];
@@ -630,12 +626,13 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         a.slots_yield = (record && !fin) ? h->hist_slots + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
         a.aux = (s == nsteps - 1) || fin;
         lists(g0 + (unsigned)s, g0 + (unsigned)s + 1);
-        if (timed == 2) (void)hipEventRecord(h->evk[2 * (s - 1)], h->stream);
+        // the non-final launches run back to back between evA and evB
+        if (timed && s == 1 && !fin) (void)hipEventRecord(h->evA, h->stream);
+        if (timed && fin && s > 1) (void)hipEventRecord(h->evB, h->stream);
         if (fin)
             hipLaunchKernelGGL((k_step<St, DEV, true>), grid, block, 0, h->stream, a);
         else
             hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
-        if (timed == 2) (void)hipEventRecord(h->evk[2 * (s - 1) + 1], h->stream);
         rc = check_launch(h, "k_step");
         if (rc) return rc;
         if (!fin) {
@@ -667,14 +664,10 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         float t = 0.0f;
         (void)hipEventElapsedTime(&t, h->ev0, h->ev1);
         *ms_total = t;
-        if (timed == 2) {
-            double main = 0.0;
-            for (int s = 0; s < nsteps; ++s) {
-                float tk = 0.0f;
-                (void)hipEventElapsedTime(&tk, h->evk[2 * s], h->evk[2 * s + 1]);
-                main += tk;
-            }
-            *ms_main = main;
+        if (ms_main) {
+            float tm = 0.0f;
+            if (nsteps > 1) (void)hipEventElapsedTime(&tm, h->evA, h->evB);
+            *ms_main = tm;
         }
     }
     return LBM_OK;
@@ -749,13 +742,7 @@ int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps, double *ms_total,
     if (!h || nsteps <= 0 || !ms_total) return LBM_ERR_PARAM;
     if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
         return fail(h, LBM_ERR_UNSUPPORTED, "rate_mode LBM_RATE_LAGGED is not provided on the GPU");
-    // first run: events only around the whole sequence
-    int rc = fused_dispatch(h, kk0, nsteps, false, 1, ms_total, nullptr);
-    if (rc || !ms_main) return rc;
-    // second run (the state advances by another nsteps): an event pair
-    // around every fused grid kernel
-    double t2 = 0.0;
-    return fused_dispatch(h, kk0 + nsteps, nsteps, false, 2, &t2, ms_main);
+    return fused_dispatch(h, kk0, nsteps, false, 1, ms_total, ms_main);
 }
 
 }  // extern "C"

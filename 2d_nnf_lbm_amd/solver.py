@@ -229,7 +229,8 @@ class D2Q9:
         return None
 
     def step_timed(self, kk0, nsteps):
-        """(ms over all launches, ms inside the fused grid kernel)."""
+        """Fused steps timed with HIP events: (ms over all launches of the call,
+        ms over the nsteps-1 back-to-back non-final fused step kernels)."""
         t, m = ctypes.c_double(0), ctypes.c_double(0)
         self._call("lbm_step_timed", int(kk0), int(nsteps), ctypes.byref(t), ctypes.byref(m))
         return t.value, m.value

@@ -25,6 +25,23 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join("profiles", "k_step_two_disk_bingham.md")
+
+
+def pmc_traffic(storage):
+    """HBM bytes per launch of the fused step kernel from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
+    counter read for a copy of known size); None when the summary is absent."""
+    try:
+        with open(os.path.join(REPO, TRAFFIC_SUMMARY)) as fh:
+            text = fh.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s+(.*)", text)
+    if not m:
+        return None
+    vals = dict(kv.split("=") for kv in m.group(1).split())
+    return float(vals[storage]) if storage in vals else None
 
 
 def log(*a):
@@ -44,7 +61,7 @@ def measure(pkg, configs, storage, steps, warmup):
     wall = time.perf_counter() - t0
     n = configs.nodes(cfg)
     bpn = configs.bytes_per_node(storage, True)
-    per_launch = ms_main / steps  # ms per fused grid kernel
+    per_launch = ms_main / (steps - 1)  # ms per launch of the fused step kernel
     out = dict(
         storage=storage,
         ms_per_step=ms_total / steps,
@@ -186,8 +203,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": r16["achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": pmc_traffic("fp16"),
+            "traffic_source": TRAFFIC_SUMMARY,
+            "algorithmic_bytes_per_launch": r16["bytes_per_node"] * n,
             "algorithmic_bytes_per_node": r16["bytes_per_node"],
+            "kernel_us_per_launch": r16["ms_main_kernel"] * 1e3,
         },
         "cpu_baseline": cpu,
         "fp64": {
@@ -195,7 +215,9 @@ def main():
             "ms_per_step": r64["ms_per_step"],
             "achieved_gbs": r64["achieved_gbs"],
             "roofline_frac": r64["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": pmc_traffic("fp64"),
             "algorithmic_bytes_per_node": r64["bytes_per_node"],
+            "kernel_us_per_launch": r64["ms_main_kernel"] * 1e3,
         },
         "fp16_over_fp64": r16["mlups"] / r64["mlups"],
         "accuracy_fp16_vs_fp64": acc,

@@ -139,8 +139,11 @@ int lbm_step(lbm_handle *h, int32_t kk0, int32_t nsteps, double *hist);
 /* Pairs found by the last contact evaluation of lbm_step. */
 int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
                    int32_t *npairs);
-/* Like lbm_step (no history), timing on the handle's stream with HIP events:
- * ms_total covers all launches, ms_main the fused stream/collide kernel only. */
+/* Like lbm_step (no history), timed on the handle's stream with HIP events:
+ * ms_total covers every launch of the call; ms_main covers the nsteps-1
+ * non-final fused step kernels, which run back to back with no other work
+ * between them (0 when nsteps == 1), so ms_main/(nsteps-1) is the average
+ * duration of one launch of the main kernel. */
 int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps,
                    double *ms_total, double *ms_main);
 
