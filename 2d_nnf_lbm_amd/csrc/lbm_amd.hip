@@ -87,6 +87,14 @@ size_t storage_bytes(int storage) {
     }
 }
 
+// message of the last failed lbm_create on this thread (no handle exists)
+thread_local std::string t_create_err;
+
+int bad_create(const char *msg) {
+    t_create_err = msg;
+    return LBM_ERR_PARAM;
+}
+
 int check_launch(lbm_handle *h, const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(h, LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -97,17 +105,24 @@ int check_launch(lbm_handle *h, const char *what) {
 
 extern "C" {
 
-const char *lbm_last_error(const lbm_handle *h) { return h ? h->err.c_str() : "null handle"; }
+const char *lbm_last_error(const lbm_handle *h) {
+    if (h) return h->err.c_str();
+    return t_create_err.empty() ? "null handle" : t_create_err.c_str();
+}
 
 int lbm_create(const lbm_params *p, lbm_handle **out) {
-    if (!p || !out) return LBM_ERR_PARAM;
+    t_create_err.clear();
+    if (!p || !out) return bad_create("null params or output pointer");
     *out = nullptr;
-    if (p->nx < 2 || p->ny < 2) return LBM_ERR_PARAM;
+    if (p->nx < 2 || p->ny < 2) return bad_create("nx and ny must be at least 2");
     // F6:115-118: tau = 1/omega must not be below 0.5
-    if (!(p->omega > 0.0) || 1.0 / p->omega < 0.5) return LBM_ERR_PARAM;
-    if (p->rheology != LBM_NEWTONIAN && p->rheology != LBM_BINGHAM) return LBM_ERR_PARAM;
-    if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return LBM_ERR_PARAM;
-    if (p->max_particles < 0 || p->max_particles > 65535) return LBM_ERR_PARAM;
+    if (!(p->omega > 0.0) || 1.0 / p->omega < 0.5)
+        return bad_create("relaxation time 1/omega below 0.5 (F6:115-118)");
+    if (p->rheology != LBM_NEWTONIAN && p->rheology != LBM_BINGHAM)
+        return bad_create("unknown rheology");
+    if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return bad_create("unknown storage");
+    if (p->max_particles < 0 || p->max_particles > 65535)
+        return bad_create("max_particles outside 0..65535");
     lbm_handle *h = new lbm_handle();
     h->prm = *p;
     DevP &P = h->P;
@@ -133,11 +148,14 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     {
         unsigned long long bb = 9ULL * (unsigned long long)P.plane *
                                 (p->storage == LBM_FP64 ? 8ULL : p->storage == LBM_FP32 ? 4ULL : 2ULL);
-        if (p->storage != LBM_FP64 && bb >= (1ULL << 32)) { delete h; return LBM_ERR_PARAM; }
+        if (p->storage != LBM_FP64 && bb >= (1ULL << 32)) {
+            delete h;
+            return bad_create("reduced-format population buffer of 4 GiB or more");
+        }
         P.buf_bytes = (unsigned)std::min<unsigned long long>(bb, 0xFFFFFFFFULL);
     }
     // magic numbers for row = node / W (Granlund-Montgomery round-up method)
-    if (P.nn >= (1LL << 31)) { delete h; return LBM_ERR_PARAM; }
+    if (P.nn >= (1LL << 31)) { delete h; return bad_create("2^31 or more nodes"); }
     {
         int l = 0;
         while ((1ULL << l) < (unsigned long long)P.W) ++l;
@@ -151,7 +169,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         const unsigned probe[] = {0u, 1u, (unsigned)P.W - 1, (unsigned)P.W, (unsigned)P.W + 1,
                                   (unsigned)(P.nn - 1), (unsigned)(P.nn / 2), 2147483647u};
         for (unsigned n : probe)
-            if (div(n) != n / (unsigned)P.W) { delete h; return LBM_ERR_PARAM; }
+            if (div(n) != n / (unsigned)P.W) { delete h; return bad_create("row division check failed"); }
     }
 
     auto al = [&](void **ptr, size_t bytes) -> int {

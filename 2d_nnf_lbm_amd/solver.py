@@ -52,7 +52,7 @@ class D2Q9:
         self._h = ctypes.c_void_p()
         rc = self.lib.lbm_create(ctypes.byref(self.params), ctypes.byref(self._h))
         if rc != 0:
-            msg = self.lib.lbm_last_error(self._h).decode() if self._h else "invalid parameters"
+            msg = self.lib.lbm_last_error(self._h if self._h else None).decode()
             if self._h:
                 self.lib.lbm_destroy(self._h)
                 self._h = None

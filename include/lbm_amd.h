@@ -89,6 +89,8 @@ typedef struct lbm_handle lbm_handle;
  * allocates device state and leaves it zeroed; call lbm_initialize next. */
 int lbm_create(const lbm_params *p, lbm_handle **out);
 int lbm_destroy(lbm_handle *h);
+/* Text of the last error on h; with h == NULL, the reason the last
+ * lbm_create on this thread failed. */
 const char *lbm_last_error(const lbm_handle *h);
 int lbm_sync(lbm_handle *h);
 

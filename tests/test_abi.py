@@ -49,6 +49,9 @@ def test_create_rejects_bad_parameters(bad):
     rc = lib.lbm_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == abi.LBM_ERR_PARAM
     assert not h.value
+    # with no handle, the reason is kept per thread for lbm_last_error(NULL)
+    msg = lib.lbm_last_error(None)
+    assert msg and msg != b"null handle", msg
 
 
 def test_solver_raises_on_bad_tau():
