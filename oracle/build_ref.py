@@ -19,6 +19,19 @@ Patches (each is a parameter or syntax edit, none changes the algorithm):
   * Optional ("local" rate mode): one line at the top of the Collision_MRT node
     loop (after F6:877) that loads the node's own previous-step rate,
     CALL ComputeRelaxation_MRT(OMEGA_BN(I,J)).
+  * Optional ("mohammad" rates): in ComputeRelaxation_MRT the active rate
+    lines (F6:1009-1011) are commented out and the commented "Mohammad MRT"
+    set (F6:1017-1018) is enabled, the rate set of the one-disk Newtonian
+    workload (also F1:1044-1045).
+
+A second recipe builds the N-body contact force alone: MODULE PARAM
+(F3:2-83) and SUBROUTINE ComputeCollisionForce (F3:632-806) cut from F3,
+compiled with oracle/f3_contact_driver.f90 into oracle/_ref/f3_contact.
+It is compiled with -fno-automatic, which gives the routine's local variables
+the SAVE attribute: the pair branch (F3:760) reads D_PW before this call
+assigns it, so it sees the value the previous call left (the "stale D_PW"
+the oracle and the GPU path implement).  Without SAVE the value is whatever
+the stack held, i.e. undefined.
 
 Usage: python oracle/build_ref.py [--timing]   (no-op when /root/reference is
 absent, e.g. on a machine that only runs the prebuilt files).
@@ -33,6 +46,9 @@ import tempfile
 REF_ROOT = "/root/reference"
 F6 = os.path.join(REF_ROOT, "6_FreeFall_YieldStress",
                   "D2Q9_BinghamSediment_Validated_20191127.f90")
+F3 = os.path.join(REF_ROOT, "3_MultiParticle_Sedimentation",
+                  "D2Q9_RTParticle_20191003.f90")
+HERE = os.path.dirname(os.path.abspath(__file__))
 OUT_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref")
 FLANG = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
 
@@ -44,6 +60,8 @@ VARIANTS = {
                           local_rate=True),
     "f6_newt_lid": dict(nx=40, ny=60, non_newtonian=False, time_max=300,
                         uxn="0.01D0"),
+    "f6_newt_mohammad": dict(nx=40, ny=120, non_newtonian=False, time_max=1000,
+                             rates="mohammad"),
 }
 # CPU-baseline build: the reference's Bingham settling disk on the 401x1601
 # node grid (NX=400, NY=1600), timed for a bounded number of steps.
@@ -54,12 +72,27 @@ TIMING_VARIANTS = {
 
 
 def patch_source(text, nx, ny, non_newtonian, time_max, local_rate=False,
-                 uxn=None, write_output=True):
+                 uxn=None, write_output=True, rates=None):
     lines = text.replace("\r", "").split("\n")
     out = []
     in_collide = False
+    in_relax = False
+    enable = 0
     for n, line in enumerate(lines, start=1):
         s = line
+        if rates == "mohammad":
+            if re.match(r"\s*SUBROUTINE ComputeRelaxation_MRT\(", s):
+                in_relax = True
+            elif in_relax and re.match(r"\s*END SUBROUTINE ComputeRelaxation_MRT", s):
+                in_relax = False
+            elif in_relax:
+                if re.match(r"\s*(omega_e=1\.1;omega_eps=1\.25|Omega_mrt\(0\)=0\.0;|Omega_mrt\(4\)=omega_q;)", s):
+                    s = "!" + s
+                elif re.match(r"\s*!\*+Mohammad MRT", s):
+                    enable = 2
+                elif enable:
+                    s = s.replace("!", "", 1)
+                    enable -= 1
         if n == 1481 and "WRITE(15,*)" in s:
             s = "!" + s
         s = re.sub(r"\bIF\(0\)THEN", "IF(.FALSE.)THEN", s)
@@ -105,6 +138,41 @@ def build(name, cfg, flags):
     return exe
 
 
+def cut(text, start, end):
+    """Lines from the first matching `start` through the next `end`."""
+    lines = text.replace("\r", "").split("\n")
+    out, on = [], False
+    for line in lines:
+        if not on and re.match(start, line):
+            on = True
+        if on:
+            out.append(line)
+            if re.match(end, line):
+                return out
+    raise RuntimeError("section %r not found" % start)
+
+
+def build_f3_contact(flags):
+    with open(F3, "rb") as fh:
+        text = fh.read().decode("ascii", errors="replace")
+    src = cut(text, r"\s*MODULE PARAM\b", r"\s*END MODULE PARAM\b")
+    src += cut(text, r"\s*SUBROUTINE ComputeCollisionForce\(",
+               r"\s*END SUBROUTINE ComputeCollisionForce\b")
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")
+    try:
+        path = os.path.join(tmpdir, "f3_contact_ref.f90")
+        with open(path, "w") as fh:
+            fh.write("\n".join(src) + "\n")
+        exe = os.path.join(OUT_DIR, "f3_contact")
+        cmd = [FLANG] + flags + [path, os.path.join(HERE, "f3_contact_driver.f90"), "-o", exe]
+        subprocess.run(cmd, check=True, cwd=tmpdir,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    return exe
+
+
 def main(argv):
     if not os.path.exists(F6):
         print("build_ref: reference source not present; nothing to build")
@@ -114,6 +182,8 @@ def main(argv):
     built = []
     for name, cfg in VARIANTS.items():
         built.append(build(name, cfg, parity_flags))
+    if os.path.exists(F3):
+        built.append(build_f3_contact(parity_flags + ["-fno-automatic"]))
     if "--timing" in argv or "--all" in argv:
         for name, cfg in TIMING_VARIANTS.items():
             built.append(build(name, cfg, timing_flags))

@@ -73,7 +73,14 @@ def load_golden(name):
     return meta, {k: z[k] for k in z.files if k != "meta"}
 
 
+MOHAMMAD_RATES = dict(q_mode=0, s0=1.0, s1=f32(1.4), s2=f32(1.4), s3=1.0, s5=1.0,
+                      sq=f32(1.2))  # F6:1017-1018, F1:1044-1045
+
+
 def golden_params(meta):
-    return f6_params(meta["nx"], meta["ny"], meta["non_newtonian"],
-                     meta.get("local_rate", False),
-                     float(meta["uxn"].replace("D", "e")) if meta.get("uxn") else 0.0)
+    cfg = f6_params(meta["nx"], meta["ny"], meta["non_newtonian"],
+                    meta.get("local_rate", False),
+                    float(meta["uxn"].replace("D", "e")) if meta.get("uxn") else 0.0)
+    if meta.get("rates") == "mohammad":
+        cfg.update(MOHAMMAD_RATES)
+    return cfg

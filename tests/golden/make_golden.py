@@ -11,6 +11,10 @@ parsed doubles are exact), and stores them as compressed .npz fixtures:
   per-step particle history (F6:315-318) and mean density (F6:348-349);
   run constants printed at start-up (F6:104-111): TAU_LB, U_C, TAU_Y, A_G.
 
+and f3_contact.npz from oracle/_ref/f3_contact (the reference's N-body
+ComputeCollisionForce, F3:632-806, driven by oracle/f3_contact_driver.f90):
+disk positions and diameters in, F_COL of the second and third call out.
+
 Usage: python tests/golden/make_golden.py   (needs oracle/_ref/ built here)
 """
 import json
@@ -29,8 +33,50 @@ import build_ref  # noqa: E402
 from refio import parse_run  # noqa: E402
 
 
+def contact_positions():
+    """40 disks in the reference's 512 x 512 box: clusters with touching and
+    overlapping pairs (distances inside r_i + r_j + 1), disks within contact
+    range of each wall, mixed diameters.  Coordinates are multiples of 1/64
+    so their decimal text is exact.  Disk 1 sits far from the left wall
+    (its left-wall distance exceeds every r_i + r_j); the last disk touches
+    the left wall (distance 7.5 + 8 <= 16)."""
+    rng = np.random.default_rng(20191003)
+    pts = [(200.0, 300.0, 16.0)]
+    centres = [(120.0, 200.0), (300.0, 100.0), (420.0, 380.0), (60.0, 450.0)]
+    for cx, cy in centres:
+        for _ in range(8):
+            d = float(rng.choice([12.0, 16.0, 16.0, 20.0]))
+            x = cx + np.round(rng.uniform(-14, 14) * 64) / 64
+            y = cy + np.round(rng.uniform(-14, 14) * 64) / 64
+            pts.append((x, y, d))
+    pts += [(256.0, 8.5, 16.0), (503.25, 256.0, 16.0), (300.0, 503.0, 16.0),
+            (9.0, 9.0, 16.0), (502.0, 505.0, 12.0), (100.0, 60.0, 16.0), (7.5, 100.0, 16.0)]
+    assert len(pts) == 40
+    return np.array(pts)
+
+
+def make_contact():
+    exe = os.path.join(build_ref.OUT_DIR, "f3_contact")
+    if not os.path.exists(exe):
+        raise SystemExit("missing %s: run python oracle/build_ref.py" % exe)
+    pos = contact_positions()
+    text = "%d\n" % len(pos) + "".join("%r %r %r\n" % tuple(map(float, p)) for p in pos)
+    r = subprocess.run([exe], input=text, stdout=subprocess.PIPE, check=True, text=True)
+    rows = [ln.split() for ln in r.stdout.strip().split("\n")]
+    a_g, rho_s, rho_f, pi = (float(v) for v in rows[0][:4])
+    nx, ny = int(rows[0][4]), int(rows[0][5])
+    fc = np.array([[float(v) for v in row] for row in rows[1:]])
+    n = len(pos)
+    path = os.path.join(HERE, "f3_contact.npz")
+    meta = dict(a_g=a_g, rho_s=rho_s, rho_f=rho_f, pi=pi, nx=nx, ny=ny)
+    np.savez_compressed(path, meta=np.array(json.dumps(meta)), x=pos[:, 0], y=pos[:, 1],
+                        diam=pos[:, 2], fcol_call2=fc[:n], fcol_call3=fc[n:])
+    print("wrote", os.path.relpath(path, REPO), os.path.getsize(path), "bytes")
+
+
 def main():
     os.makedirs(HERE, exist_ok=True)
+    make_contact()
     for name, cfg in build_ref.VARIANTS.items():
         exe = os.path.join(build_ref.OUT_DIR, name)
         if not os.path.exists(exe):

@@ -75,7 +75,7 @@ def test_dropin_subroutines_match_oracle(orc, bingham):
     assert abs(g.total_mass() - st.total_mass()) <= 1e-12 * st.total_mass()
 
 
-@pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local", "f6_newt_lid"])
+@pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local", "f6_newt_lid", "f6_newt_mohammad"])
 def test_fused_fp64_matches_reference_dump(orc, name):
     """Fused FP64 steps against the reference program's own final dump
     (LOCAL rate for Bingham, which is what the GPU implements)."""
@@ -205,3 +205,28 @@ def test_contact_force_and_pairs_match_oracle(orc):
         pg = g.get_particles()
         assert np.array_equal(pg[orc.PF_FCX], st.part[orc.PF_FCX, :n])
         assert np.array_equal(pg[orc.PF_FCY], st.part[orc.PF_FCY, :n])
+
+
+def test_contact_force_matches_reference_routine(orc):
+    """The reference's own N-body ComputeCollisionForce (F3:632-806, compiled
+    from F3; fixture f3_contact): the same three calls on the GPU, the stale
+    D_PW carried between them, every F_COL bit-exact."""
+    meta, z = cases.load_golden("f3_contact")
+    n = len(z["x"])
+    d1 = float(z["diam"][0])
+    cij = -((((((meta["rho_s"] - meta["rho_f"]) * meta["a_g"]) * meta["pi"]) * 0.5) * d1) * 0.5) * d1
+    cfg = cases.f6_params(meta["nx"], meta["ny"], False)
+    cfg.update(zeta_c=1.0, cij=cij, eps_p=2.0, eps_w=1.0, e_p=20.0, e_w=10.0)
+    g = gpu_solver(cfg, max_particles=n)
+    g.initialize()
+    recs = np.stack([disk_record(float(z["x"][i]), float(z["y"][i]), float(z["diam"][i]), 1.5)
+                     for i in range(n)], axis=1)
+    g.set_particles(recs[:, :1])
+    g.compute_collision_force()
+    g.set_particles(recs)
+    for call, key in ((2, "fcol_call2"), (3, "fcol_call3")):
+        pairs = g.compute_collision_force(max_pairs=4096)
+        pg = g.get_particles()
+        got = np.stack([pg[orc.PF_FCX], pg[orc.PF_FCY]], axis=1)
+        print("call", call, "pairs", len(pairs), flush=True)
+        assert np.array_equal(got, z[key]), call

@@ -11,7 +11,7 @@ import pytest
 
 import cases
 
-VARIANTS = ["f6_newt", "f6_bing_lagged", "f6_bing_local", "f6_newt_lid"]
+VARIANTS = ["f6_newt", "f6_bing_lagged", "f6_bing_local", "f6_newt_lid", "f6_newt_mohammad"]
 
 
 def _run_oracle(orc, meta):
@@ -79,3 +79,49 @@ def test_oracle_history_vs_reference(orc, name):
     # mean density per step, F6:348
     nodes = (meta["nx"] + 1) * (meta["ny"] + 1)
     assert np.array_equal(g["avgden"], hist[:, H] / float(nodes))
+
+
+def _contact_setup(orc, z, meta, n):
+    """Oracle state holding the first n disks of the f3_contact fixture, with
+    F3's contact constants (F3:739-742)."""
+    d1 = float(z["diam"][0])
+    # CIJ = -(RHO_S-RHO_F)*A_G*PI*0.5*D1*0.5*D1, evaluated left to right (F3:739)
+    cij = -((((((meta["rho_s"] - meta["rho_f"]) * meta["a_g"]) * meta["pi"]) * 0.5) * d1) * 0.5) * d1
+    cfg = cases.f6_params(meta["nx"], meta["ny"], False)
+    cfg.update(zeta_c=1.0, cij=cij, eps_p=2.0, eps_w=1.0, e_p=20.0, e_w=10.0)
+    st = orc.State(orc.make_params(cfg), n)
+    part = np.zeros((orc.PF_COUNT, n))
+    part[orc.PF_X] = z["x"][:n]
+    part[orc.PF_Y] = z["y"][:n]
+    part[orc.PF_DIAM] = z["diam"][:n]
+    st.set_particles(part)
+    return st
+
+
+def test_oracle_contact_vs_reference_routine(orc):
+    """N-body contact force (F3:632-806) against the reference routine itself,
+    compiled from F3 and driven by oracle/f3_contact_driver.f90: three calls,
+    the stale D_PW of the pair branch (F3:760) carried from call to call.  The
+    second call sees particle 1's left-wall distance (> r_i + r_j, spring term
+    only); the third sees the last particle's (<= r_i + r_j, with the E_P
+    term).  Every F_COL must agree bit for bit."""
+    meta, z = cases.load_golden("f3_contact")
+    n = len(z["x"])
+    first = _contact_setup(orc, z, meta, 1)
+    first.contact()
+    st = _contact_setup(orc, z, meta, n)
+    st.c.dpw_stale = first.c.dpw_stale
+    pairs, npairs = st.contact(max_pairs=4096)
+    got2 = np.stack([st.part[orc.PF_FCX, :n], st.part[orc.PF_FCY, :n]], axis=1)
+    st.contact()
+    got3 = np.stack([st.part[orc.PF_FCX, :n], st.part[orc.PF_FCY, :n]], axis=1)
+    assert np.array_equal(got2, z["fcol_call2"])
+    assert np.array_equal(got3, z["fcol_call3"])
+    # the two calls take different pair branches, and both have pairs
+    assert npairs > 10
+    assert not np.array_equal(got2, got3)
+    # pair list: brute force over all i < j, d <= r_i + r_j + zeta
+    x, y, d = z["x"], z["y"], z["diam"]
+    brute = [(i, j) for i in range(n) for j in range(i + 1, n)
+             if np.sqrt((x[i] - x[j]) ** 2 + (y[i] - y[j]) ** 2) <= 0.5 * d[i] + 0.5 * d[j] + 1.0]
+    assert [tuple(p) for p in pairs] == brute
