@@ -5,6 +5,7 @@ Every float literal that the reference writes as a default REAL is rounded to
 float32 first (f32), as in the Fortran source.
 """
 import math
+import os
 
 import numpy as np
 
@@ -73,6 +74,93 @@ def two_disk_bingham(storage="fp64"):
     part = np.stack([disk(198.0, 1300.0, d, rho_s, zeta), disk(200.0, 1380.0, d, rho_s, zeta)],
                     axis=1)
     return cfg, part
+
+
+def one_disk_newtonian(storage="fp64"):
+    """One disk falling from rest in a Newtonian fluid in a closed channel.
+
+    Grid 200 x 600 cells (201 x 601 nodes), disk of diameter 25 at (100, 480),
+    density ratio 1.5, rho0 = 1, tau = 0.6 (nu = 1/30), the MRT rate set
+    {1, 1.4, 1.4, 1, 1.2, 1, 1.2, 1/tau, 1/tau} (F6:1017-1018, F1:1044-1045,
+    float literals as in the source), interface width 2, gravity 1e-5 along
+    -y, bounce-back walls, F6's Glowinski constants (F6:691-693).
+    """
+    nx, ny = 200, 600
+    d = 25.0
+    rho_s, rho_f = 1.5, 1.0
+    tau = 0.6
+    nu = (tau - 0.5) / 3.0
+    a_g = -1e-5
+    cij = -((((((rho_s - rho_f) * a_g) * PI) * 0.5) * d) * 0.5) * d
+    cfg = dict(nx=nx, ny=ny, rheology=_abi.LBM_NEWTONIAN, rate_mode=_abi.LBM_RATE_LOCAL,
+               update_angle=1, storage=storage, max_particles=1, omega=1.0 / tau, nu=nu,
+               mu0=nu, tau_y=0.0, m_pap=0.0, gamma_min=f32(1e-15), rho0=1.0, gy=a_g,
+               wall=[0.0] * 8, corner=[0.0, 0.0], zeta_c=1.0, cij=cij,
+               eps_p=0.5, eps_w=0.5, e_p=10.0, e_w=10.0)
+    cfg.update(mohammad_rates())
+    part = disk(100.0, 480.0, d, rho_s, 2.0)[:, None]
+    return cfg, part
+
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DENSE_FILE = os.path.join(DATA, "dense_504_seed0.txt")
+
+
+def dense_positions(n=504, diam=16.0, gap=4.0, nx=2048, ny=2048, frac=0.6, seed=0):
+    """Rejection sampling of n disk centres (numpy PCG64, default_rng(seed)):
+    uniform in the upper `frac` of the cavity, at least `gap` lattice units
+    between disks and between a disk and a wall.  Candidates are drawn as
+    (x, y) pairs and accepted in order."""
+    rng = np.random.default_rng(seed)
+    r = 0.5 * diam
+    xlo, xhi = r + gap, nx - r - gap
+    ylo, yhi = (1.0 - frac) * ny + r + gap, ny - r - gap
+    pts = np.zeros((0, 2))
+    while len(pts) < n:
+        x, y = rng.uniform(xlo, xhi), rng.uniform(ylo, yhi)
+        if len(pts) == 0 or np.min(np.hypot(pts[:, 0] - x, pts[:, 1] - y)) >= diam + gap:
+            pts = np.vstack([pts, [x, y]])
+    return pts
+
+
+def dense_sedimentation(bn=0.1, storage="bf16", positions=None, size=2048):
+    """504 disks (D = 16, density ratio 1.1) settling from the upper 60 % of
+    a closed 2048 x 2048 cavity (2049^2 nodes).  Positions: the committed
+    file data/dense_504_seed0.txt (dense_positions() with seed 0).  Fluid:
+    tau = 0.8 (mu0 = 0.1); Bn = 0 runs Newtonian, Bn > 0 Papanastasiou-Bingham
+    with m = 1e5 and TAU_Y = Bn*drho*|g|*D (F6:30); gravity from
+    A_G = -Re*mu0^2/(D^3 drho rho) with Re = 0.78 (F6:16, F6:28).  Contact: the
+    N-body Glowinski model with F3's constants (F3:739-742).  `size` and
+    `positions` give smaller cavities of the same fluid and disks for tests.
+    """
+    nx = ny = size
+    d = 16.0
+    rho_s, rho_f = 1.1, 1.0
+    tau = 0.8
+    mu0 = (tau - 0.5) / 3.0
+    re_s = f32(0.78)
+    a_g = -re_s * mu0 * mu0 / ((d * d * d) * (rho_s - rho_f) * rho_f)
+    tau_y = bn * (rho_s - rho_f) * (-a_g) * d
+    cij = -((((((rho_s - rho_f) * a_g) * PI) * 0.5) * d) * 0.5) * d
+    if positions is None:
+        positions = np.loadtxt(DENSE_FILE)
+    cfg = dict(nx=nx, ny=ny, rheology=_abi.LBM_BINGHAM if bn > 0 else _abi.LBM_NEWTONIAN,
+               rate_mode=_abi.LBM_RATE_LOCAL, update_angle=1, storage=storage,
+               max_particles=len(positions), omega=1.0 / tau, nu=mu0, mu0=mu0, tau_y=tau_y,
+               m_pap=1e5, gamma_min=f32(1e-15), rho0=1.0, gy=a_g, wall=[0.0] * 8,
+               corner=[0.0, 0.0], zeta_c=1.0, cij=cij, eps_p=2.0, eps_w=1.0, e_p=20.0,
+               e_w=10.0)
+    cfg.update(f6_rates())
+    part = np.stack([disk(float(x), float(y), d, rho_s, 2.0) for x, y in positions], axis=1)
+    return cfg, part
+
+
+def kinetic_energy(hist, np_, cfg, part, p=0):
+    """Translational kinetic energy 1/2 m V^2 of particle p per step."""
+    H = _abi.H
+    base = p * _abi.H_PCOUNT
+    vx, vy = hist[:, base + H["vx"]], hist[:, base + H["vy"]]
+    return 0.5 * part[_abi.PF["mp"], p] * (vx * vx + vy * vy)
 
 
 def nodes(cfg):

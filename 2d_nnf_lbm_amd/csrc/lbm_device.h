@@ -395,7 +395,7 @@ __device__ __forceinline__ void spm_node_f32(const float *spf, const int *ids, i
 }
 
 constexpr int NP_LOOP = 16;    // above this many particles, tiles use candidate lists
-constexpr int MAXC = 32;       // candidate particles per tile
+constexpr int MAXC = 64;       // candidate particles per tile
 constexpr int MAXSUB = 16;     // blocks summing the force on one particle
 constexpr int NP_LDS = 128;    // up to this many particles, contact inputs are staged in LDS
 
@@ -403,6 +403,9 @@ constexpr int NP_LDS = 128;    // up to this many particles, contact inputs are 
 // grid tile its reach box touches, for generation gen.  Each tile header packs
 // (generation << 32 | count); the first registration of a new generation
 // resets the count (lock-free CAS loop, one contender per touching particle).
+// Called by all NT threads of a block: thread r takes box rows j0+r, j0+r+NT,
+// ...; a tile that row j shares with row j-1 is left to row j-1, so each
+// tile is registered once per particle.
 __device__ __forceinline__ void register_tiles(const DevP &P, const double *rec, int p,
                                                unsigned gen, unsigned long long *hdr,
                                                int *ids, int *err) {
@@ -410,10 +413,13 @@ __device__ __forceinline__ void register_tiles(const DevP &P, const double *rec,
     int i0 = max(0, (int)ceil(rec[PX] - reach)), i1 = min(P.nx, (int)floor(rec[PX] + reach));
     int j0 = max(0, (int)ceil(rec[PY] - reach)), j1 = min(P.ny, (int)floor(rec[PY] + reach));
     if (i0 > i1 || j0 > j1) return;
-    long long last = -1;
-    for (int j = j0; j <= j1; ++j) {
+    for (int j = j0 + (int)threadIdx.x; j <= j1; j += NT) {
         long long tlo = ((long long)j * P.W + i0) / NT, thi = ((long long)j * P.W + i1) / NT;
-        for (long long t = (tlo > last ? tlo : last + 1); t <= thi; ++t) {
+        if (j > j0) {
+            long long prev = ((long long)(j - 1) * P.W + i1) / NT;
+            if (prev >= tlo) tlo = prev + 1;
+        }
+        for (long long t = tlo; t <= thi; ++t) {
             unsigned long long old = hdr[t], assumed, nv;
             unsigned slot;
             do {
@@ -425,7 +431,6 @@ __device__ __forceinline__ void register_tiles(const DevP &P, const double *rec,
             } while (old != assumed);
             if (slot < (unsigned)MAXC) ids[t * MAXC + slot] = p;
             else atomicOr(err, 1);
-            last = t;
         }
     }
 }
@@ -732,51 +737,37 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*NT 
     __syncthreads();
 }
 
-// Contact force on particle p (Glowinski model; N-body pair order of
-// F3:746-770, walls of F3:773-804), evaluated by one thread.  The pair branch
-// reads dpw, the D_PW left by the previous call (F3:760).  Writes in-range
-// partners j>p into nb[0..MAXNB) and returns their count; *dpw_last receives
-// this particle's last wall distance.  X, Y, D: centres and diameters of all
-// particles (global memory or an LDS copy).
-__device__ __forceinline__ int contact_one(const DevP &P, const double *X, const double *Y,
-                                           const double *D, int np, int p, double dpw,
-                                           double &fcx, double &fcy, int *nb, double *dpw_last) {
+// Glowinski pair force of the pair (ip, jp), ip < jp (F3:746-770), with the
+// stale wall distance dpw choosing the branch (F3:760).  Returns false (and a
+// zero force) when the pair is out of range.
+__device__ __forceinline__ bool pair_force(const DevP &P, double dpw, const double *X,
+                                           const double *Y, const double *D, int ip, int jp,
+                                           double &f0, double &f1) {
     const double zeta = P.zeta_c, cij = P.cij;
-    fcx = 0.0; fcy = 0.0;
-    int cnt = 0;
-    auto pair_force = [&](int ip, int jp, double &f0, double &f1) -> bool {
-        double xi = X[ip], yi = Y[ip];
-        double xj = X[jp], yj = Y[jp];
-        double di = D[ip], dj = D[jp];
-        double dpp = sqrt((xi - xj) * (xi - xj) + (yi - yj) * (yi - yj));
-        double rsum = 0.5 * di + 0.5 * dj;
-        f0 = 0.0; f1 = 0.0;
-        if (dpp > rsum + zeta) return false;
-        double t = (dpp - 0.5 * di - 0.5 * dj - zeta) / zeta;
-        if (dpw > rsum) {
-            f0 = P.eps_p * cij * (t * t) * (xi - xj) / dpp;
-            f1 = P.eps_p * cij * (t * t) * (yi - yj) / dpp;
-        } else {
-            f0 = (P.eps_p * cij * (t * t) + P.e_p * cij * (rsum - dpp) / zeta) * (xi - xj) / dpp;
-            f1 = (P.eps_p * cij * (t * t) + P.e_p * cij * (rsum - dpp) / zeta) * (yi - yj) / dpp;
-        }
-        return true;
-    };
-    // accumulation order seen by particle p in the reference loop: pairs
-    // (j, p) for j < p subtract, then pairs (p, j) for j > p add
-    for (int q = 0; q < np; ++q) {
-        if (q == p) continue;
-        double f0, f1;
-        bool hit = (q < p) ? pair_force(q, p, f0, f1) : pair_force(p, q, f0, f1);
-        if (!hit) continue;
-        if (q < p) { fcx = fcx - f0; fcy = fcy - f1; }
-        else {
-            fcx = fcx + f0; fcy = fcy + f1;
-            if (cnt < MAXNB) nb[cnt] = q;
-            ++cnt;
-        }
+    double xi = X[ip], yi = Y[ip];
+    double xj = X[jp], yj = Y[jp];
+    double di = D[ip], dj = D[jp];
+    double dpp = sqrt((xi - xj) * (xi - xj) + (yi - yj) * (yi - yj));
+    double rsum = 0.5 * di + 0.5 * dj;
+    f0 = 0.0; f1 = 0.0;
+    if (dpp > rsum + zeta) return false;
+    double t = (dpp - 0.5 * di - 0.5 * dj - zeta) / zeta;
+    if (dpw > rsum) {
+        f0 = P.eps_p * cij * (t * t) * (xi - xj) / dpp;
+        f1 = P.eps_p * cij * (t * t) * (yi - yj) / dpp;
+    } else {
+        f0 = (P.eps_p * cij * (t * t) + P.e_p * cij * (rsum - dpp) / zeta) * (xi - xj) / dpp;
+        f1 = (P.eps_p * cij * (t * t) + P.e_p * cij * (rsum - dpp) / zeta) * (yi - yj) / dpp;
     }
-    double x = X[p], y = Y[p], dm = D[p];
+    return true;
+}
+
+// Wall forces on a disk at (x, y), diameter dm, added in the wall order of
+// F3:773-804 (bottom, right, top, left); returns the last wall distance,
+// which becomes the next call's stale D_PW.
+__device__ __forceinline__ double wall_forces(const DevP &P, double x, double y, double dm,
+                                              double &fcx, double &fcy) {
+    const double zeta = P.zeta_c, cij = P.cij;
     double d = 0.0;
     for (int iw = 1; iw <= 4; ++iw) {
         double rw0, rw1;
@@ -799,7 +790,99 @@ __device__ __forceinline__ int contact_one(const DevP &P, const double *X, const
         }
         fcx = fcx + f0; fcy = fcy + f1;
     }
-    *dpw_last = d;
+    return d;
+}
+
+// Contact force on particle p (Glowinski model; N-body pair order of
+// F3:746-770, walls of F3:773-804), evaluated by one thread.  The pair branch
+// reads dpw, the D_PW left by the previous call (F3:760).  Writes in-range
+// partners j>p into nb[0..MAXNB) and returns their count; *dpw_last receives
+// this particle's last wall distance.  X, Y, D: centres and diameters of all
+// particles (global memory or an LDS copy).
+__device__ __forceinline__ int contact_one(const DevP &P, const double *X, const double *Y,
+                                           const double *D, int np, int p, double dpw,
+                                           double &fcx, double &fcy, int *nb, double *dpw_last) {
+    fcx = 0.0; fcy = 0.0;
+    int cnt = 0;
+    // accumulation order seen by particle p in the reference loop: pairs
+    // (j, p) for j < p subtract, then pairs (p, j) for j > p add
+    for (int q = 0; q < np; ++q) {
+        if (q == p) continue;
+        double f0, f1;
+        bool hit = (q < p) ? pair_force(P, dpw, X, Y, D, q, p, f0, f1)
+                           : pair_force(P, dpw, X, Y, D, p, q, f0, f1);
+        if (!hit) continue;
+        if (q < p) { fcx = fcx - f0; fcy = fcy - f1; }
+        else {
+            fcx = fcx + f0; fcy = fcy + f1;
+            if (cnt < MAXNB) nb[cnt] = q;
+            ++cnt;
+        }
+    }
+    *dpw_last = wall_forces(P, X[p], Y[p], D[p], fcx, fcy);
+    return cnt;
+}
+
+// Block-wide form of contact_one (all NT threads of the block call it; the
+// results are valid in thread 0).  Each thread tests one partner per round of
+// NT; the in-range partners are compacted into LDS in ascending index order
+// and thread 0 adds them in that order, which is the order contact_one (and
+// the reference loop) uses: out-of-range partners contribute +0.0, which
+// leaves the sums unchanged.  More than CCAP in-range partners fall back to
+// the serial loop.
+constexpr int CCAP = 64;
+struct ContactLDS {
+    double f0[CCAP], f1[CCAP];
+    int q[CCAP];
+    int wcount[NT / 64];
+    int total;
+};
+
+__device__ __forceinline__ int contact_block(const DevP &P, const double *X, const double *Y,
+                                             const double *D, int np, int p, double dpw,
+                                             ContactLDS &L, double &fcx, double &fcy, int *nb,
+                                             double *dpw_last) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) L.total = 0;
+    __syncthreads();
+    for (int base = 0; base < np; base += NT) {
+        const int q = base + t;
+        double f0 = 0.0, f1 = 0.0;
+        bool hit = false;
+        if (q < np && q != p)
+            hit = (q < p) ? pair_force(P, dpw, X, Y, D, q, p, f0, f1)
+                          : pair_force(P, dpw, X, Y, D, p, q, f0, f1);
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) L.wcount[w] = __popcll(m);
+        __syncthreads();
+        int off = L.total;
+        for (int v = 0; v < w; ++v) off += L.wcount[v];
+        if (hit) {
+            int pos = off + __popcll(m & ((1ULL << lane) - 1ULL));
+            if (pos < CCAP) { L.q[pos] = q; L.f0[pos] = f0; L.f1[pos] = f1; }
+        }
+        __syncthreads();
+        if (t == 0) {
+            int s = 0;
+            for (int v = 0; v < NT / 64; ++v) s += L.wcount[v];
+            L.total += s;
+        }
+        __syncthreads();
+    }
+    int cnt = 0;
+    if (t == 0) {
+        if (L.total > CCAP) return contact_one(P, X, Y, D, np, p, dpw, fcx, fcy, nb, dpw_last);
+        fcx = 0.0; fcy = 0.0;
+        for (int k = 0; k < L.total; ++k) {
+            if (L.q[k] < p) { fcx = fcx - L.f0[k]; fcy = fcy - L.f1[k]; }
+            else {
+                fcx = fcx + L.f0[k]; fcy = fcy + L.f1[k];
+                if (cnt < MAXNB) nb[cnt] = L.q[k];
+                ++cnt;
+            }
+        }
+        *dpw_last = wall_forces(P, X[p], Y[p], D[p], fcx, fcy);
+    }
     return cnt;
 }
 

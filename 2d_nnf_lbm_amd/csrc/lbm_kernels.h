@@ -439,24 +439,34 @@ __device__ __forceinline__ void spm_tile_node(const StepArgs &a, long long tile,
     spm_node_cand(a.pin, P.maxp, a.tl_ids_in + tile * MAXC, cnt, i, j, ux, uy, gx, gy);
 }
 
-// Particle block: thread 0 runs the contact force and the Newton-Euler update
-// of particle p from state k (pin) into state k+1 (pout).
+// Particle block (all NT threads): contact force of particle p over all
+// partners, then thread 0 runs the Newton-Euler update from state k (pin) into
+// state k+1 (pout); with many particles the block then registers p in the
+// candidate lists of the tiles it reaches.
 __device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
     const DevP &P = a.P;
-    if (threadIdx.x != 0) return;
     const int m = P.maxp;
-    double rec[PCOUNT];
-    for (int f = 0; f < PCOUNT; ++f) rec[f] = a.pin[f * m + p];
-    double fcx, fcy, dl;
-    int cnt = contact_one(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
-                          *a.dpw_in, fcx, fcy, a.pairs + p * MAXNB, &dl);
-    a.paircnt[p] = cnt;
-    rec[PFCX] = fcx; rec[PFCY] = fcy;
-    update_one(P, rec);
-    for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
-    if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
-    if (p == P.np - 1) *a.dpw_out = dl;
-    if (P.np > NP_LOOP) register_tiles(P, rec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
+    __shared__ double srec[PCOUNT];
+    __shared__ double sdpw;
+    __shared__ ContactLDS cl;
+    if (threadIdx.x < PCOUNT) srec[threadIdx.x] = a.pin[threadIdx.x * m + p];
+    if (threadIdx.x == 0) sdpw = *a.dpw_in;
+    __syncthreads();
+    double fcx = 0.0, fcy = 0.0, dl = 0.0;
+    int cnt = contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p, sdpw,
+                            cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
+    if (threadIdx.x == 0) {
+        double rec[PCOUNT];
+        for (int f = 0; f < PCOUNT; ++f) rec[f] = srec[f];
+        a.paircnt[p] = cnt;
+        rec[PFCX] = fcx; rec[PFCY] = fcy;
+        update_one(P, rec);
+        for (int f = 0; f < PCOUNT; ++f) { a.pout[f * m + p] = rec[f]; srec[f] = rec[f]; }
+        if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
+        if (p == P.np - 1) *a.dpw_out = dl;
+    }
+    __syncthreads();
+    if (P.np > NP_LOOP) register_tiles(P, srec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
 // Hydrodynamic force and torque on particle p at step k (ComputeForce_HYDRO,
@@ -701,26 +711,29 @@ __device__ void particle_block(const StepArgs &a) {
             }
         }
         if (!FINAL) {
-            // the update uses FORCE_k: write it into the outgoing record first
+            // contact over all partners with the whole block, then the update
+            // (it uses FORCE_k: written into the outgoing record first)
+            __shared__ ContactLDS cl;
+            double fcx = 0.0, fcy = 0.0, dl = 0.0;
+            int cnt = staged
+                ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl,
+                                fcx, fcy, a.pairs + p * MAXNB, &dl)
+                : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
+                                sdpw, cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
             if (threadIdx.x == 0) {
                 double rec[PCOUNT];
                 for (int f = 0; f < PCOUNT; ++f) rec[f] = srec[f];
                 rec[PFX] = F[0]; rec[PFY] = F[1]; rec[PTQ] = F[2];
-                double fcx, fcy, dl;
-                int cnt = staged
-                    ? contact_one(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw,
-                                  fcx, fcy, a.pairs + p * MAXNB, &dl)
-                    : contact_one(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
-                                  sdpw, fcx, fcy, a.pairs + p * MAXNB, &dl);
                 a.paircnt[p] = cnt;
                 rec[PFCX] = fcx; rec[PFCY] = fcy;
                 update_one(P, rec);
-                for (int f = 0; f < PCOUNT; ++f) a.pout[f * m + p] = rec[f];
+                for (int f = 0; f < PCOUNT; ++f) { a.pout[f * m + p] = rec[f]; srec[f] = rec[f]; }
                 if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
                 if (p == P.np - 1) *a.dpw_out = dl;
-                if (P.np > NP_LOOP)
-                    register_tiles(P, rec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
             }
+            __syncthreads();
+            if (P.np > NP_LOOP)
+                register_tiles(P, srec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
         }
     }
 }

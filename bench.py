@@ -48,8 +48,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measure(pkg, configs, storage, steps, warmup):
-    cfg, part = configs.two_disk_bingham(storage)
+def measure(pkg, configs, storage, steps, warmup, workload=None):
+    cfg, part = workload if workload is not None else configs.two_disk_bingham(storage)
     s = pkg.D2Q9(cfg)
     s.initialize()
     s.set_particles(part)
@@ -60,7 +60,7 @@ def measure(pkg, configs, storage, steps, warmup):
     s.sync()
     wall = time.perf_counter() - t0
     n = configs.nodes(cfg)
-    bpn = configs.bytes_per_node(storage, True)
+    bpn = configs.bytes_per_node(storage, cfg["rheology"] == 1)
     per_launch = ms_main / (steps - 1)  # ms per launch of the fused step kernel
     out = dict(
         storage=storage,
@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--accuracy-steps", type=int, default=3000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dense-steps", type=int, default=200,
+                    help="timed steps of the 504-disk BF16 cavity (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of 9 FP64 planes (known byte "
                          "count) so hardware traffic counters can be scaled")
@@ -169,6 +171,20 @@ def main():
     r64 = measure(pkg, configs, "fp64", args.steps, args.warmup)
     log("bench: fp16 leg")
     r16 = measure(pkg, configs, "fp16", args.steps, args.warmup)
+    dense = None
+    if args.dense_steps > 0:
+        log("bench: 504-disk cavity, bf16")
+        wl = configs.dense_sedimentation(0.1, "bf16")
+        rd = measure(pkg, configs, "bf16", args.dense_steps, 20, workload=wl)
+        dense = dict(workload="504 disks (D=16, density ratio 1.1) settling in a "
+                              "Papanastasiou-Bingham fluid (Bn=0.1), 2049x2049 nodes, "
+                              "FP32 collide / BF16 deviation storage",
+                     nodes=configs.nodes(wl[0]), steps=args.dense_steps,
+                     mlups=rd["mlups"], ms_per_step=rd["ms_per_step"],
+                     kernel_us_per_launch=rd["ms_main_kernel"] * 1e3,
+                     achieved_gbs=rd["achieved_gbs"],
+                     roofline_frac=rd["achieved_gbs"] / HBM_PEAK_GBS,
+                     algorithmic_bytes_per_node=rd["bytes_per_node"])
     log("bench: accuracy leg")
     acc = accuracy(pkg, configs, args.accuracy_steps) if args.accuracy_steps > 0 else None
     cpu = None
@@ -220,6 +236,7 @@ def main():
             "kernel_us_per_launch": r64["ms_main_kernel"] * 1e3,
         },
         "fp16_over_fp64": r16["mlups"] / r64["mlups"],
+        "dense_sedimentation_bf16": dense,
         "accuracy_fp16_vs_fp64": acc,
     }
     print(json.dumps(line), flush=True)
