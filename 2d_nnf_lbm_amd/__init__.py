@@ -6,6 +6,6 @@ subroutine interface (solver.D2Q9) and the run configurations.  Import the
 package with importlib.import_module("2d_nnf_lbm_amd").
 """
 from . import _abi  # noqa: F401
-from .solver import D2Q9, LBMError, make_params  # noqa: F401
+from .solver import Batch, D2Q9, LBMError, make_params  # noqa: F401
 
-__all__ = ["D2Q9", "LBMError", "make_params"]
+__all__ = ["Batch", "D2Q9", "LBMError", "make_params"]

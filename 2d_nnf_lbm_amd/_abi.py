@@ -14,6 +14,7 @@ LBM_ERR_NAN = -5
 
 LBM_NEWTONIAN, LBM_BINGHAM = 0, 1
 LBM_RATE_LAGGED, LBM_RATE_LOCAL = 0, 1
+LBM_MOTION_FREE, LBM_MOTION_OSC_X = 0, 1
 LBM_FP64, LBM_FP32, LBM_FP16, LBM_BF16 = 0, 1, 2, 3
 STORAGE = {"fp64": LBM_FP64, "fp32": LBM_FP32, "fp16": LBM_FP16, "bf16": LBM_BF16}
 
@@ -33,6 +34,7 @@ class Params(ctypes.Structure):
         ("rheology", ctypes.c_int32), ("rate_mode", ctypes.c_int32),
         ("q_mode", ctypes.c_int32), ("update_angle", ctypes.c_int32),
         ("storage", ctypes.c_int32), ("max_particles", ctypes.c_int32),
+        ("motion", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("omega", ctypes.c_double),
         ("s0", ctypes.c_double), ("s1", ctypes.c_double), ("s2", ctypes.c_double),
         ("s3", ctypes.c_double), ("s5", ctypes.c_double), ("sq", ctypes.c_double),
@@ -44,6 +46,7 @@ class Params(ctypes.Structure):
         ("zeta_c", ctypes.c_double), ("cij", ctypes.c_double),
         ("eps_p", ctypes.c_double), ("eps_w", ctypes.c_double),
         ("e_p", ctypes.c_double), ("e_w", ctypes.c_double),
+        ("osc_u", ctypes.c_double), ("osc_period", ctypes.c_double),
     ]
 
 
@@ -70,11 +73,21 @@ SIGNATURES = {
     "lbm_macros": [], "lbm_spm_phi": [], "lbm_spm_forcing": [],
     "lbm_hydro_force": [], "lbm_correct": [],
     "lbm_contact_force": [_ip, ctypes.c_int32, _ip],
-    "lbm_particle_update": [],
+    "lbm_particle_update": [ctypes.c_int32],
     "lbm_total_mass": [_dp],
     "lbm_step": [ctypes.c_int32, ctypes.c_int32, _dp],
     "lbm_last_pairs": [_ip, ctypes.c_int32, _ip],
     "lbm_step_timed": [ctypes.c_int32, ctypes.c_int32, _dp, _dp],
+    # batch entries (first argument: the lbm_batch pointer)
+    "lbm_batch_destroy": [],
+    "lbm_batch_size": [_ip],
+    "lbm_batch_initialize": [],
+    "lbm_batch_step": [ctypes.c_int32, ctypes.c_int32, _dp],
+    "lbm_batch_step_timed": [ctypes.c_int32, ctypes.c_int32, _dp],
+    "lbm_batch_set_graph": [ctypes.c_int32],
+    "lbm_batch_set_tiles": [_ip],
+    "lbm_batch_get_tiles": [_ip],
+    "lbm_batch_tune": [ctypes.c_int32, _ip],
 }
 
 _lib = None
@@ -94,6 +107,12 @@ def load():
     lib.lbm_create.restype = ctypes.c_int
     lib.lbm_last_error.argtypes = [_hp]
     lib.lbm_last_error.restype = ctypes.c_char_p
+    lib.lbm_batch_create.argtypes = [ctypes.c_int32, ctypes.POINTER(Params), ctypes.POINTER(_hp)]
+    lib.lbm_batch_create.restype = ctypes.c_int
+    lib.lbm_batch_last_error.argtypes = [_hp]
+    lib.lbm_batch_last_error.restype = ctypes.c_char_p
+    lib.lbm_batch_sim.argtypes = [_hp, ctypes.c_int32]
+    lib.lbm_batch_sim.restype = _hp
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = [_hp] + args
@@ -103,4 +122,5 @@ def load():
 
 
 def exported_names():
-    return ["lbm_create", "lbm_last_error"] + list(SIGNATURES)
+    return (["lbm_create", "lbm_last_error", "lbm_batch_create", "lbm_batch_last_error",
+             "lbm_batch_sim"] + list(SIGNATURES))

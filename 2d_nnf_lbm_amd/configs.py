@@ -155,6 +155,42 @@ def dense_sedimentation(bn=0.1, storage="bf16", positions=None, size=2048):
     return cfg, part
 
 
+def oscillating_cavity(side, bn, storage="fp16", wall_speed=0.05, tau=0.8, rho_p=1.5):
+    """A square lid- and side-wall-driven cavity of `side` cells with one
+    rigid cylinder (D = side/8) at the centre in prescribed x-oscillation:
+    amplitude 0.1 D, period 2000 steps, V_x = -U cos(2 pi KK / T) with
+    U = 2 pi A / T (F4:221-226).  The top wall moves along +x and the right
+    wall along +y at `wall_speed`; Papanastasiou-Bingham fluid with
+    mu0 = (tau - 1/2)/3, m = 1e5 and TAU_Y = Bn mu0 U_w / side; no gravity."""
+    d = side / 8.0
+    mu0 = (tau - 0.5) / 3.0
+    period = 2000.0
+    u_osc = 2.0 * PI * (0.1 * d) / period
+    wall = [0.0] * 8
+    wall[4] = wall_speed   # uxN, lid
+    wall[3] = wall_speed   # uyR, right wall
+    cfg = dict(nx=side, ny=side, rheology=_abi.LBM_BINGHAM, rate_mode=_abi.LBM_RATE_LOCAL,
+               update_angle=1, storage=storage, max_particles=1,
+               motion=_abi.LBM_MOTION_OSC_X, osc_u=u_osc, osc_period=period,
+               omega=1.0 / tau, nu=mu0, mu0=mu0, tau_y=bn * mu0 * wall_speed / side,
+               m_pap=1e5, gamma_min=f32(1e-15), rho0=1.0, gy=0.0, wall=wall,
+               corner=[0.0, 0.0], zeta_c=1.0, cij=0.0, eps_p=0.5, eps_w=0.5, e_p=10.0,
+               e_w=10.0)
+    cfg.update(f6_rates())
+    part = disk(side / 2.0, side / 2.0, d, rho_p, 2.0)[:, None]
+    return cfg, part
+
+
+def cavity_ensemble(n=256, seed=1, storage="fp16"):
+    """n oscillating-cylinder cavities with sides drawn uniformly from
+    {128..1024} and Bn uniform in [0, 10] (numpy PCG64, default_rng(seed):
+    sides first, then Bn)."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(128, 1025, size=n)
+    bns = rng.uniform(0.0, 10.0, size=n)
+    return [oscillating_cavity(int(s), float(b), storage) for s, b in zip(sides, bns)]
+
+
 def kinetic_energy(hist, np_, cfg, part, p=0):
     """Translational kinetic energy 1/2 m V^2 of particle p per step."""
     H = _abi.H

@@ -123,6 +123,10 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return bad_create("unknown storage");
     if (p->max_particles < 0 || p->max_particles > 65535)
         return bad_create("max_particles outside 0..65535");
+    if (p->motion != LBM_MOTION_FREE && p->motion != LBM_MOTION_OSC_X)
+        return bad_create("unknown particle motion");
+    if (p->motion == LBM_MOTION_OSC_X && !(p->osc_period > 0.0))
+        return bad_create("oscillation period must be positive");
     lbm_handle *h = new lbm_handle();
     h->prm = *p;
     DevP &P = h->P;
@@ -131,6 +135,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.nn = (long long)P.W * P.H;
     P.plane = (P.nn + 255) / 256 * 256;
     P.rheology = p->rheology; P.q_mode = p->q_mode; P.update_angle = p->update_angle;
+    P.motion = p->motion; P.osc_u = p->osc_u; P.osc_period = p->osc_period;
     P.np = 0;
     P.maxp = std::max(1, p->max_particles);
     P.nbnd = 2 * P.W + 2 * (P.H - 2);
@@ -543,12 +548,12 @@ int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t *np
     return gather_pairs(h, pairs, max_pairs, npairs);
 }
 
-int lbm_particle_update(lbm_handle *h) {
+int lbm_particle_update(lbm_handle *h, int32_t kk) {
     if (!h) return LBM_ERR_PARAM;
     const DevP &P = h->P;
     if (P.np == 0) return LBM_OK;
     hipLaunchKernelGGL(k_update, dim3(nblk(P.np)), dim3(NT), 0, h->stream, P,
-                       h->pbuf[h->pcur]);
+                       h->pbuf[h->pcur], (int)kk);
     return check_launch(h, "k_update");
 }
 
@@ -571,13 +576,14 @@ int lbm_total_mass(lbm_handle *h, double *mass) {
 // ----------------------------------------------------------------- fused
 namespace {
 
-template <class St, bool DEV>
-int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed,
-              double *ms_total, double *ms_main) {
-    (void)kk0;
+// Arguments of launch s of a fused call of nsteps steps on handle h (s = 0:
+// the prologue; fin: the last step), starting from the handle's current
+// buffers.  Buffers alternate with the parity of s; step_fields() sets the
+// per-step fields.
+template <bool DEV>
+StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fin) {
     const DevP &P = h->P;
     const int np = P.np;
-    long long ntiles = (P.nn + NT - 1) / NT;
     StepArgs a;
     std::memset(&a, 0, sizeof(a));
     a.P = P;
@@ -588,62 +594,98 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
     a.pforce = h->pforce;
     a.pcount = h->pcount;
     a.n_pblocks = np * a.nsub;
-    a.ntiles = ntiles;
+    a.ntiles = (P.nn + NT - 1) / NT;
     a.err = h->err_flag;
-    const unsigned g0 = h->gen_base;
-    auto lists = [&](unsigned gin, unsigned gout) {
-        a.gen_in = gin;
-        a.gen_out = gout;
-        a.tl_hdr_in = h->tl_hdr[gin & 1];
-        a.tl_ids_in = h->tl_ids[gin & 1];
-        a.tl_hdr_out = h->tl_hdr[gout & 1];
-        a.tl_ids_out = h->tl_ids[gout & 1];
-    };
-    dim3 grid((unsigned)(a.n_pblocks + ntiles)), block(NT);
-    const size_t rowp = (size_t)np * HPC;
-
+    a.hist_base = record ? h->hist_part : nullptr;
+    a.slots_base = record ? h->hist_slots : nullptr;
+    a.rowp = np * HPC;
+    a.kk0 = kk0;
+    a.g0 = h->gen_base;
+    a.tpb = 1;
+    const unsigned g = h->gen_base + (unsigned)s;
+    a.tl_hdr_in = h->tl_hdr[g & 1]; a.tl_ids_in = h->tl_ids[g & 1];
+    a.tl_hdr_out = h->tl_hdr[(g + 1) & 1]; a.tl_ids_out = h->tl_ids[(g + 1) & 1];
     // storage buffers: FP64 runs ping-pong between F and F_POST
     void *buf[2];
     if (DEV) { buf[0] = h->sA; buf[1] = h->sB; }
     else { buf[0] = h->f; buf[1] = h->fp; }
-    int e = 0;        // erho parity
-    int pc = h->pcur; // particle buffer holding the current state
+    const int flip = np > 0 ? (s & 1) : 0;  // particle buffers advance once per launch
+    if (s == 0) {
+        // collide step kk0's state into f*_{kk0+1}; particles to kk0+1
+        a.src = DEV ? nullptr : buf[0];
+        a.dst = DEV ? buf[0] : buf[1];
+        a.erho_dst = h->erho[0];
+        a.pin = h->pbuf[h->pcur]; a.pout = h->pbuf[h->pcur ^ 1];
+        a.spf_in = nullptr;
+        a.spf_out = DEV ? h->spf[h->pcur ^ 1] : nullptr;
+        a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
+        return a;
+    }
+    const int cur = DEV ? ((s - 1) & 1) : (s & 1);  // buffer holding f*_k
+    a.src = buf[cur];
+    a.dst = buf[cur ^ 1];
+    const int e = (s - 1) & 1;
+    a.erho_src = h->erho[e];
+    a.erho_dst = h->erho[e ^ 1];
+    const int pc = h->pcur ^ flip;
+    a.pin = h->pbuf[pc];
+    a.pout = fin ? h->pbuf[pc] : h->pbuf[pc ^ 1];
+    a.spf_in = DEV ? h->spf[pc] : nullptr;
+    a.spf_out = (DEV && !fin) ? h->spf[pc ^ 1] : nullptr;
+    const int dc = h->dcur ^ flip;
+    a.dpw_in = h->dpw[dc]; a.dpw_out = h->dpw[dc ^ 1];
+    return a;
+}
+
+// Handle state after a fused call of nsteps steps: FP64 runs leave F in the
+// final step's destination and F_POST in its source; particle and wall-
+// distance buffers moved once per launch when there are particles.
+template <bool DEV>
+void finish_fused(lbm_handle *h, int32_t nsteps) {
+    if (!DEV) {
+        double *b0 = h->f, *b1 = h->fp;
+        const int cur = nsteps & 1;
+        h->f = cur ? b0 : b1;
+        h->fp = cur ? b1 : b0;
+    }
+    if (h->P.np > 0) {
+        h->pcur ^= nsteps & 1;
+        h->dcur ^= nsteps & 1;
+    }
+    h->gen_base += (unsigned)nsteps + 2u;
+}
+
+// Tile-list overflow reported by the kernels of the last call.
+int check_lists(lbm_handle *h) {
+    if (h->P.np <= NP_LOOP) return LBM_OK;
+    int ef = 0;
+    HIPCHECK(h, hipMemcpyAsync(&ef, h->err_flag, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (ef) {
+        (void)hipMemsetAsync(h->err_flag, 0, 4, h->stream);
+        return fail(h, LBM_ERR_CAPACITY, "more particles reach one grid tile than its candidate list holds");
+    }
+    return LBM_OK;
+}
+
+template <class St, bool DEV>
+int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed,
+              double *ms_total, double *ms_main) {
+    const DevP &P = h->P;
+    const long long ntiles = (P.nn + NT - 1) / NT;
+    const int n_pblocks = P.np > 0 ? P.np * h->nsub : 0;
+    dim3 grid((unsigned)(n_pblocks + ntiles)), block(NT);
 
     if (timed) (void)hipEventRecord(h->ev0, h->stream);
-    // prologue: collide step kk0's state into f*_{kk0+1}; particles to kk0+1
-    a.src = DEV ? nullptr : buf[0];
-    a.dst = DEV ? buf[0] : buf[1];
-    a.erho_dst = h->erho[e];
-    a.pin = h->pbuf[pc]; a.pout = h->pbuf[pc ^ 1];
-    a.spf_in = nullptr;
-    a.spf_out = DEV ? h->spf[pc ^ 1] : nullptr;
-    a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
-    a.slots_yield = record ? h->hist_slots + NSLOT : nullptr;  // row 0, yield half
-    a.slots_mass = nullptr;
-    a.hist_part = nullptr;
-    a.aux = nsteps == 1;
-    lists(g0, g0 + 1);
+    StepArgs a = make_args<DEV>(h, kk0, record, 0, nsteps == 0);
+    step_fields(a, 0, nsteps);
     hipLaunchKernelGGL((k_prologue<St, DEV>), grid, block, 0, h->stream, a);
     int rc = check_launch(h, "k_prologue");
     if (rc) return rc;
-    if (np > 0) { pc ^= 1; h->dcur ^= 1; }
-    int cur = DEV ? 0 : 1;  // buffer holding f*_{k}
     for (int s = 1; s <= nsteps; ++s) {
-        bool fin = s == nsteps;
-        a.src = buf[cur];
-        a.dst = buf[cur ^ 1];
-        a.erho_src = h->erho[e];
-        a.erho_dst = h->erho[e ^ 1];
-        a.pin = h->pbuf[pc];
-        a.pout = fin ? h->pbuf[pc] : h->pbuf[pc ^ 1];
-        a.spf_in = DEV ? h->spf[pc] : nullptr;
-        a.spf_out = (DEV && !fin) ? h->spf[pc ^ 1] : nullptr;
-        a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
-        a.hist_part = record ? h->hist_part + (size_t)(s - 1) * rowp : nullptr;
-        a.slots_mass = record ? h->hist_slots + (size_t)(s - 1) * 2 * NSLOT : nullptr;
-        a.slots_yield = (record && !fin) ? h->hist_slots + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
-        a.aux = (s == nsteps - 1) || fin;
-        lists(g0 + (unsigned)s, g0 + (unsigned)s + 1);
+        const bool fin = s == nsteps;
+        a = make_args<DEV>(h, kk0, record, s, fin);
+        step_fields(a, s, nsteps);
         // the non-final launches run back to back between evA and evB
         if (timed && s == 1 && !fin) (void)hipEventRecord(h->evA, h->stream);
         if (timed && fin && s > 1) (void)hipEventRecord(h->evB, h->stream);
@@ -653,30 +695,10 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
             hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
         rc = check_launch(h, "k_step");
         if (rc) return rc;
-        if (!fin) {
-            cur ^= 1;
-            e ^= 1;
-            if (np > 0) { pc ^= 1; h->dcur ^= 1; }
-        }
     }
     if (timed) (void)hipEventRecord(h->ev1, h->stream);
-    // canonical state: FP64 runs wrote F into buf[cur^1] and F_POST is buf[cur]
-    if (!DEV) {
-        double *fk = (double *)buf[cur ^ 1], *fs = (double *)buf[cur];
-        h->f = fk;
-        h->fp = fs;
-    }
-    h->pcur = pc;
-    h->gen_base = g0 + (unsigned)nsteps + 2;
-    if (np > NP_LOOP) {
-        int ef = 0;
-        HIPCHECK(h, hipMemcpyAsync(&ef, h->err_flag, 4, hipMemcpyDeviceToHost, h->stream));
-        HIPCHECK(h, hipStreamSynchronize(h->stream));
-        if (ef) {
-            (void)hipMemsetAsync(h->err_flag, 0, 4, h->stream);
-            return fail(h, LBM_ERR_CAPACITY, "more particles reach one grid tile than its candidate list holds");
-        }
-    }
+    finish_fused<DEV>(h, nsteps);
+    if ((rc = check_lists(h))) return rc;
     if (timed) {
         HIPCHECK(h, hipEventSynchronize(h->ev1));
         float t = 0.0f;
@@ -701,6 +723,58 @@ int fused_dispatch(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int 
     }
 }
 
+int check_fused_mode(lbm_handle *h) {
+    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
+        return fail(h, LBM_ERR_UNSUPPORTED,
+                    "the lagged module-global rate is a sequential recurrence; the GPU path "
+                    "implements rate_mode LBM_RATE_LOCAL");
+    return LBM_OK;
+}
+
+// Device history buffers for nsteps rows, monitor slots cleared.
+int prepare_history(lbm_handle *h, int32_t nsteps) {
+    const size_t rowp = (size_t)h->P.np * HPC;
+    if ((size_t)nsteps > h->hist_rows || !h->hist_part) {
+        if (h->hist_part) (void)hipFree(h->hist_part);
+        if (h->hist_slots) (void)hipFree(h->hist_slots);
+        h->hist_part = nullptr;
+        h->hist_slots = nullptr;
+        size_t rows = std::max<size_t>(nsteps, 16);
+        HIPCHECK(h, hipMalloc((void **)&h->hist_part, std::max<size_t>(rows * rowp, 1) * 8));
+        HIPCHECK(h, hipMalloc((void **)&h->hist_slots, rows * 2 * NSLOT * 8));
+        h->hist_rows = rows;
+    }
+    HIPCHECK(h, hipMemsetAsync(h->hist_slots, 0, (size_t)nsteps * 2 * NSLOT * 8, h->stream));
+    return LBM_OK;
+}
+
+// History rows of the last recorded call: per step the particle records,
+// then SUM(RHO) and the yielded-node count.
+int collect_history(lbm_handle *h, int32_t nsteps, double *hist) {
+    const size_t rowp = (size_t)h->P.np * HPC;
+    std::vector<double> hp(std::max<size_t>((size_t)nsteps * rowp, 1));
+    std::vector<unsigned long long> hs((size_t)nsteps * 2 * NSLOT);
+    if (rowp)
+        HIPCHECK(h, hipMemcpyAsync(hp.data(), h->hist_part, (size_t)nsteps * rowp * 8,
+                                   hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipMemcpyAsync(hs.data(), h->hist_slots, hs.size() * 8, hipMemcpyDeviceToHost,
+                               h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    const size_t row = rowp + 2;
+    for (int s = 0; s < nsteps; ++s) {
+        double *out = hist + (size_t)s * row;
+        for (size_t c = 0; c < rowp; ++c) out[c] = hp[(size_t)s * rowp + c];
+        long long msum = 0, ysum = 0;
+        for (int q = 0; q < NSLOT; ++q) {
+            msum += (long long)hs[(size_t)s * 2 * NSLOT + q];
+            ysum += (long long)hs[(size_t)s * 2 * NSLOT + NSLOT + q];
+        }
+        out[rowp] = (double)msum / MASS_SCALE;
+        out[rowp + 1] = (double)ysum;
+    }
+    return LBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -708,59 +782,382 @@ extern "C" {
 int lbm_step(lbm_handle *h, int32_t kk0, int32_t nsteps, double *hist) {
     if (!h || nsteps < 0) return LBM_ERR_PARAM;
     if (nsteps == 0) return LBM_OK;
-    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
-        return fail(h, LBM_ERR_UNSUPPORTED,
-                    "the lagged module-global rate is a sequential recurrence; the GPU path "
-                    "implements rate_mode LBM_RATE_LOCAL");
-    const DevP &P = h->P;
-    bool record = hist != nullptr;
-    const size_t rowp = (size_t)P.np * HPC;
-    if (record) {
-        if ((size_t)nsteps > h->hist_rows || !h->hist_part) {
-            if (h->hist_part) (void)hipFree(h->hist_part);
-            if (h->hist_slots) (void)hipFree(h->hist_slots);
-            h->hist_part = nullptr;
-            h->hist_slots = nullptr;
-            size_t rows = std::max<size_t>(nsteps, 16);
-            HIPCHECK(h, hipMalloc((void **)&h->hist_part, std::max<size_t>(rows * rowp, 1) * 8));
-            HIPCHECK(h, hipMalloc((void **)&h->hist_slots, rows * 2 * NSLOT * 8));
-            h->hist_rows = rows;
-        }
-        HIPCHECK(h, hipMemsetAsync(h->hist_slots, 0, (size_t)nsteps * 2 * NSLOT * 8, h->stream));
-    }
-    int rc = fused_dispatch(h, kk0, nsteps, record, 0, nullptr, nullptr);
+    int rc = check_fused_mode(h);
     if (rc) return rc;
-    if (record) {
-        std::vector<double> hp(std::max<size_t>((size_t)nsteps * rowp, 1));
-        std::vector<unsigned long long> hs((size_t)nsteps * 2 * NSLOT);
-        if (rowp)
-            HIPCHECK(h, hipMemcpyAsync(hp.data(), h->hist_part, (size_t)nsteps * rowp * 8,
-                                       hipMemcpyDeviceToHost, h->stream));
-        HIPCHECK(h, hipMemcpyAsync(hs.data(), h->hist_slots, hs.size() * 8, hipMemcpyDeviceToHost,
-                                   h->stream));
-        HIPCHECK(h, hipStreamSynchronize(h->stream));
-        const size_t row = rowp + 2;
-        for (int s = 0; s < nsteps; ++s) {
-            double *out = hist + (size_t)s * row;
-            for (size_t c = 0; c < rowp; ++c) out[c] = hp[(size_t)s * rowp + c];
-            long long msum = 0, ysum = 0;
-            for (int q = 0; q < NSLOT; ++q) {
-                msum += (long long)hs[(size_t)s * 2 * NSLOT + q];
-                ysum += (long long)hs[(size_t)s * 2 * NSLOT + NSLOT + q];
-            }
-            out[rowp] = (double)msum / MASS_SCALE;
-            out[rowp + 1] = (double)ysum;
-        }
-    }
+    const bool record = hist != nullptr;
+    if (record && (rc = prepare_history(h, nsteps))) return rc;
+    if ((rc = fused_dispatch(h, kk0, nsteps, record, 0, nullptr, nullptr))) return rc;
+    if (record) return collect_history(h, nsteps, hist);
     return LBM_OK;
 }
 
 int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps, double *ms_total,
                    double *ms_main) {
     if (!h || nsteps <= 0 || !ms_total) return LBM_ERR_PARAM;
-    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
-        return fail(h, LBM_ERR_UNSUPPORTED, "rate_mode LBM_RATE_LAGGED is not provided on the GPU");
+    int rc0 = check_fused_mode(h);
+    if (rc0) return rc0;
     return fused_dispatch(h, kk0, nsteps, false, 1, ms_total, ms_main);
+}
+
+}  // extern "C"
+
+// ================================================================= batch
+// Independent simulations of different sizes and parameters stepped together:
+// one launch per time step covers every grid (block map bmap), the per-step
+// arguments of each simulation come from a small device table, and the
+// steps between the first and the last two of a call can be replayed from a
+// captured hipGraph.
+struct lbm_batch {
+    std::vector<lbm_handle *> sims;
+    int storage = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<int> tpb;                 // 256-node tiles per block, per simulation
+    int2 *bmap = nullptr;                 // block -> (simulation, block within it)
+    size_t bmap_cap = 0, nblocks = 0;
+    bool map_dirty = true;
+    std::vector<long long> map_key;       // (particles, blocks per particle, tiles per block) the map was built for
+    StepArgs *dargs = nullptr;            // [4][n]: prologue, odd steps, even steps, final
+    int *sdev = nullptr;                  // first step index of a replayed chunk
+    int chunk = 0;                        // graph chunk length (even), 0: no graph
+    hipGraphExec_t gexec = nullptr;
+    int gexec_chunk = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+constexpr int BIG_NSTEPS = 0x3fffffff;  // chunk launches are never the last two steps
+
+int bfail(lbm_batch *b, int code, const std::string &msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+#define BHIPCHECK(b, call)                                                               \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return bfail((b), LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void drop_graph(lbm_batch *b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    b->gexec_chunk = 0;
+}
+
+// Block map of the simulations in `which`: every particle block first (they
+// start early and carry the particle update), then the tile blocks, placed so
+// that each of the 8 XCDs runs a contiguous range of (simulation, tile) order.
+int build_map(lbm_batch *b, const std::vector<int> &which) {
+    std::vector<int2> map;
+    for (int i : which) {
+        const lbm_handle *h = b->sims[i];
+        const int np = h->P.np;
+        const int npb = np > 0 ? np * h->nsub : 0;
+        for (int k = 0; k < npb; ++k) map.push_back(make_int2(i, k));
+    }
+    std::vector<int2> tiles;
+    for (int i : which) {
+        const lbm_handle *h = b->sims[i];
+        const int np = h->P.np;
+        const int npb = np > 0 ? np * h->nsub : 0;
+        const long long ntiles = (h->P.nn + NT - 1) / NT;
+        const long long tb = (ntiles + b->tpb[i] - 1) / b->tpb[i];
+        for (long long k = 0; k < tb; ++k) tiles.push_back(make_int2(i, npb + (int)k));
+    }
+    const long long T = (long long)tiles.size();
+    const size_t base = map.size();
+    map.resize(base + tiles.size());
+    for (long long g = 0; g < T; ++g) {
+        long long x = g & 7, per = T >> 3, rem = T & 7;
+        long long c = x * per + (x < rem ? x : rem) + (g >> 3);
+        map[base + g] = tiles[c];
+    }
+    if (map.size() > b->bmap_cap) {
+        if (b->bmap) (void)hipFree(b->bmap);
+        b->bmap = nullptr;
+        BHIPCHECK(b, hipMalloc((void **)&b->bmap, map.size() * sizeof(int2)));
+        b->bmap_cap = map.size();
+    }
+    BHIPCHECK(b, hipMemcpy(b->bmap, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice));
+    b->nblocks = map.size();
+    drop_graph(b);
+    return LBM_OK;
+}
+
+template <class St, bool DEV>
+int batch_launches(lbm_batch *b, int32_t nsteps) {
+    const size_t n = b->sims.size();
+    const dim3 grid((unsigned)b->nblocks), block(NT);
+    const StepArgs *A0 = b->dargs, *Aodd = b->dargs + n, *Aeven = b->dargs + 2 * n,
+                   *Afin = b->dargs + 3 * n;
+    auto step_args = [&](int s) { return (s & 1) ? Aodd : Aeven; };
+    hipLaunchKernelGGL((k_batch_prologue<St, DEV>), grid, block, 0, b->stream, A0, b->bmap, nsteps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, LBM_ERR_HIP, std::string("k_batch_prologue: ") + hipGetErrorString(e));
+    int s = 1;
+    const int K = b->chunk;
+    if (K > 0 && nsteps - 2 >= K) {
+        const int nchunks = (nsteps - 2) / K;
+        if (!b->gexec || b->gexec_chunk != K) {
+            drop_graph(b);
+            hipGraph_t g = nullptr;
+            BHIPCHECK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            for (int j = 0; j < K; ++j)
+                hipLaunchKernelGGL((k_batch_step<St, DEV, false>), grid, block, 0, b->stream,
+                                   step_args(1 + j), b->bmap, b->sdev, 1 + j, BIG_NSTEPS);
+            hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, b->stream, b->sdev, K);
+            BHIPCHECK(b, hipStreamEndCapture(b->stream, &g));
+            hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess)
+                return bfail(b, LBM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            b->gexec_chunk = K;
+        }
+        hipLaunchKernelGGL(k_counter_set, dim3(1), dim3(64), 0, b->stream, b->sdev, 0);
+        for (int c = 0; c < nchunks; ++c) BHIPCHECK(b, hipGraphLaunch(b->gexec, b->stream));
+        s = 1 + nchunks * K;
+    }
+    for (; s < nsteps; ++s)
+        hipLaunchKernelGGL((k_batch_step<St, DEV, false>), grid, block, 0, b->stream,
+                           step_args(s), b->bmap, (const int *)nullptr, s, nsteps);
+    hipLaunchKernelGGL((k_batch_step<St, DEV, true>), grid, block, 0, b->stream, Afin, b->bmap,
+                       (const int *)nullptr, nsteps, nsteps);
+    e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, LBM_ERR_HIP, std::string("k_batch_step: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+// nsteps fused steps of the simulations in `which` (every simulation when
+// empty); hist (optional) receives each simulation's history rows in turn.
+int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool timed,
+              double *ms_total, const std::vector<int> &which_in) {
+    if (nsteps <= 0) return LBM_OK;
+    std::vector<int> which = which_in;
+    if (which.empty())
+        for (size_t i = 0; i < b->sims.size(); ++i) which.push_back((int)i);
+    const bool all = which.size() == b->sims.size();
+    const bool record = hist != nullptr;
+    const size_t n = b->sims.size();
+    for (int i : which) {
+        lbm_handle *h = b->sims[i];
+        int rc = check_fused_mode(h);
+        if (rc) return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
+        if (record && (rc = prepare_history(h, nsteps)))
+            return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
+        BHIPCHECK(b, hipStreamSynchronize(h->stream));
+    }
+    // argument tables
+    std::vector<StepArgs> args(4 * n);
+    std::memset(args.data(), 0, args.size() * sizeof(StepArgs));
+    for (int i : which) {
+        const lbm_handle *h = b->sims[i];
+        const bool dev = b->storage != LBM_FP64;
+        auto mk = [&](int s, bool fin) {
+            StepArgs a = dev ? make_args<true>(h, kk0, record, s, fin)
+                             : make_args<false>(h, kk0, record, s, fin);
+            a.tpb = b->tpb[i];
+            return a;
+        };
+        args[i] = mk(0, false);
+        args[n + i] = mk(1, false);
+        args[2 * n + i] = mk(2, false);
+        args[3 * n + i] = mk(nsteps, true);
+    }
+    BHIPCHECK(b, hipMemcpyAsync(b->dargs, args.data(), args.size() * sizeof(StepArgs),
+                                hipMemcpyHostToDevice, b->stream));
+    // the map depends on each simulation's particle count and tiling
+    std::vector<long long> key;
+    for (size_t i = 0; i < n; ++i)
+        key.push_back(((long long)b->sims[i]->P.np << 24) ^ ((long long)b->sims[i]->nsub << 8) ^
+                      b->tpb[i]);
+    if (b->map_dirty || !all || key != b->map_key) {
+        int rc = build_map(b, which);
+        if (rc) return rc;
+        b->map_dirty = !all;
+        b->map_key = key;
+    }
+    if (timed) (void)hipEventRecord(b->ev0, b->stream);
+    int rc;
+    switch (b->storage) {
+        case LBM_FP64: rc = batch_launches<StF64, false>(b, nsteps); break;
+        case LBM_FP32: rc = batch_launches<StF32, true>(b, nsteps); break;
+        case LBM_FP16: rc = batch_launches<StF16, true>(b, nsteps); break;
+        default: rc = batch_launches<StBF16, true>(b, nsteps); break;
+    }
+    if (rc) return rc;
+    if (timed) (void)hipEventRecord(b->ev1, b->stream);
+    BHIPCHECK(b, hipStreamSynchronize(b->stream));
+    for (int i : which) {
+        lbm_handle *h = b->sims[i];
+        if (b->storage == LBM_FP64) finish_fused<false>(h, nsteps);
+        else finish_fused<true>(h, nsteps);
+        if ((rc = check_lists(h))) return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
+    }
+    if (record) {
+        double *out = hist;
+        for (int i : which) {
+            lbm_handle *h = b->sims[i];
+            if ((rc = collect_history(h, nsteps, out)))
+                return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
+            out += (size_t)nsteps * ((size_t)h->P.np * HPC + 2);
+        }
+    }
+    if (timed) {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, b->ev0, b->ev1);
+        *ms_total = t;
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_batch_last_error(const lbm_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int lbm_batch_create(int32_t n, const lbm_params *params, lbm_batch **out) {
+    if (!out || !params || n <= 0) return LBM_ERR_PARAM;
+    *out = nullptr;
+    for (int32_t i = 1; i < n; ++i)
+        if (params[i].storage != params[0].storage) {
+            t_create_err = "all simulations of a batch use one storage format";
+            return LBM_ERR_PARAM;
+        }
+    lbm_batch *b = new lbm_batch();
+    b->storage = params[0].storage;
+    for (int32_t i = 0; i < n; ++i) {
+        lbm_handle *h = nullptr;
+        int rc = lbm_create(&params[i], &h);
+        if (rc) {
+            t_create_err = "simulation " + std::to_string(i) + ": " +
+                           (h ? h->err : t_create_err);
+            if (h) lbm_destroy(h);
+            lbm_batch_destroy(b);
+            return rc;
+        }
+        b->sims.push_back(h);
+    }
+    b->tpb.assign(n, 1);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&b->dargs, 4 * (size_t)n * sizeof(StepArgs)) != hipSuccess ||
+        hipMalloc((void **)&b->sdev, sizeof(int)) != hipSuccess) {
+        t_create_err = "batch stream or argument table allocation failed";
+        lbm_batch_destroy(b);
+        return LBM_ERR_HIP;
+    }
+    (void)hipEventCreate(&b->ev0);
+    (void)hipEventCreate(&b->ev1);
+    *out = b;
+    return LBM_OK;
+}
+
+int lbm_batch_destroy(lbm_batch *b) {
+    if (!b) return LBM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (lbm_handle *h : b->sims) lbm_destroy(h);
+    if (b->bmap) (void)hipFree(b->bmap);
+    if (b->dargs) (void)hipFree(b->dargs);
+    if (b->sdev) (void)hipFree(b->sdev);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LBM_OK;
+}
+
+int lbm_batch_size(const lbm_batch *b, int32_t *n) {
+    if (!b || !n) return LBM_ERR_PARAM;
+    *n = (int32_t)b->sims.size();
+    return LBM_OK;
+}
+
+lbm_handle *lbm_batch_sim(lbm_batch *b, int32_t i) {
+    if (!b || i < 0 || i >= (int32_t)b->sims.size()) return nullptr;
+    return b->sims[i];
+}
+
+int lbm_batch_initialize(lbm_batch *b) {
+    if (!b) return LBM_ERR_PARAM;
+    for (size_t i = 0; i < b->sims.size(); ++i) {
+        int rc = lbm_initialize(b->sims[i]);
+        if (rc) return bfail(b, rc, "simulation " + std::to_string(i) + ": " + b->sims[i]->err);
+    }
+    return LBM_OK;
+}
+
+int lbm_batch_step(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist) {
+    if (!b || nsteps < 0) return LBM_ERR_PARAM;
+    return batch_run(b, kk0, nsteps, hist, false, nullptr, {});
+}
+
+int lbm_batch_step_timed(lbm_batch *b, int32_t kk0, int32_t nsteps, double *ms_total) {
+    if (!b || nsteps <= 0 || !ms_total) return LBM_ERR_PARAM;
+    return batch_run(b, kk0, nsteps, nullptr, true, ms_total, {});
+}
+
+int lbm_batch_set_graph(lbm_batch *b, int32_t chunk) {
+    if (!b || chunk < 0 || (chunk & 1)) return LBM_ERR_PARAM;
+    if (chunk != b->chunk) drop_graph(b);
+    b->chunk = chunk;
+    return LBM_OK;
+}
+
+int lbm_batch_set_tiles(lbm_batch *b, const int32_t *tiles_per_block) {
+    if (!b || !tiles_per_block) return LBM_ERR_PARAM;
+    for (size_t i = 0; i < b->sims.size(); ++i)
+        if (tiles_per_block[i] < 1 || tiles_per_block[i] > 64) return LBM_ERR_PARAM;
+    for (size_t i = 0; i < b->sims.size(); ++i) b->tpb[i] = tiles_per_block[i];
+    b->map_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_batch_get_tiles(const lbm_batch *b, int32_t *tiles_per_block) {
+    if (!b || !tiles_per_block) return LBM_ERR_PARAM;
+    for (size_t i = 0; i < b->sims.size(); ++i) tiles_per_block[i] = b->tpb[i];
+    return LBM_OK;
+}
+
+int lbm_batch_tune(lbm_batch *b, int32_t steps, int32_t *tiles_per_block) {
+    if (!b || steps < 2) return LBM_ERR_PARAM;
+    // size buckets: floor(log2(nodes)) of each grid
+    std::vector<int> bucket(b->sims.size());
+    for (size_t i = 0; i < b->sims.size(); ++i) {
+        long long nn = b->sims[i]->P.nn;
+        int k = 0;
+        while ((1LL << (k + 1)) <= nn) ++k;
+        bucket[i] = k;
+    }
+    std::vector<int> keys(bucket);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int cand[] = {1, 2, 4, 8};
+    for (int key : keys) {
+        std::vector<int> which;
+        for (size_t i = 0; i < b->sims.size(); ++i)
+            if (bucket[i] == key) which.push_back((int)i);
+        double best = 1e300;
+        int best_t = 1;
+        for (int t : cand) {
+            for (int i : which) b->tpb[i] = t;
+            double ms = 0.0;
+            // one untimed call to settle, then the timed one
+            int rc = batch_run(b, 0, 2, nullptr, false, nullptr, which);
+            if (!rc) rc = batch_run(b, 0, steps, nullptr, true, &ms, which);
+            if (rc) return rc;
+            if (ms < best) { best = ms; best_t = t; }
+        }
+        for (int i : which) b->tpb[i] = best_t;
+    }
+    b->map_dirty = true;
+    drop_graph(b);
+    if (tiles_per_block)
+        for (size_t i = 0; i < b->sims.size(); ++i) tiles_per_block[i] = b->tpb[i];
+    return LBM_OK;
 }
 
 }  // extern "C"

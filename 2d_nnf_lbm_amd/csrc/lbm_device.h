@@ -43,6 +43,8 @@ struct DevP {
     long long nn;        // nodes (W*H)
     long long plane;     // SoA plane stride in elements
     int rheology, q_mode, update_angle, np;
+    int motion;          // 0 free, 1 prescribed x-oscillation
+    double osc_u, osc_period;
     int maxp;            // particle SoA stride
     int nbnd;            // boundary nodes
     double omega, s0, s1, s2, s3, s5, sq, nu, mu0, tau_y, m_pap, gamma_min, rho0, gy;
@@ -886,9 +888,27 @@ __device__ __forceinline__ int contact_block(const DevP &P, const double *X, con
     return cnt;
 }
 
-// Newton-Euler update of one particle record in place (F6:302-312).
-__device__ __forceinline__ void update_one(const DevP &P, double *rec /* PCOUNT, stride 1 */) {
+// Update of one particle record in place for loop iteration kk: Newton-Euler
+// (F6:302-312) or the prescribed motion (F4:221-226).
+__device__ __forceinline__ void update_one(const DevP &P, double *rec /* PCOUNT, stride 1 */,
+                                           int kk) {
     const double rho0 = P.rho0;
+    if (P.motion == 1) {
+        // prescribed oscillation of loop iteration kk (F4:221-226) and the
+        // added-mass term from the velocity change (F4:256-258)
+        double rhop = rec[PRHOP], mp = rec[PMP], ipm = rec[PIP];
+        double v0 = rec[PVX], v1 = rec[PVY], v2 = rec[PW];
+        rec[PVOX] = v0; rec[PVOY] = v1; rec[PVOW] = v2;
+        double n0 = -P.osc_u * cos(2.0 * PI_F * (double)kk / P.osc_period);
+        double n1 = 0.0, n2 = 0.0;
+        rec[PVX] = n0; rec[PVY] = n1; rec[PW] = n2;
+        rec[PX] = rec[PX] + 0.5 * (n0 + v0);
+        rec[PY] = rec[PY] + 0.5 * (n1 + v1);
+        rec[PFHX] = (rho0 / rhop) * mp * (n0 - v0);
+        rec[PFHY] = (rho0 / rhop) * mp * (n1 - v1);
+        rec[PFHW] = (rho0 / rhop) * ipm * (n2 - v2);
+        return;
+    }
     double rhop = rec[PRHOP], mp = rec[PMP], ipm = rec[PIP];
     double v0 = rec[PVX], v1 = rec[PVY], v2 = rec[PW];
     double fh0 = (rho0 / rhop) * mp * (v0 - rec[PVOX]);

@@ -290,12 +290,12 @@ __global__ void k_contact(DevP P, double *part, const double *dpw_in, double *dp
 }
 
 // Particle update, F6:299-312: one thread per particle.
-__global__ void k_update(DevP P, double *part) {
+__global__ void k_update(DevP P, double *part, int kk) {
     int p = blockIdx.x * NT + threadIdx.x;
     if (p >= P.np) return;
     double rec[PCOUNT];
     for (int f = 0; f < PCOUNT; ++f) rec[f] = part[f * P.maxp + p];
-    update_one(P, rec);
+    update_one(P, rec, kk);
     for (int f = 0; f < PCOUNT; ++f) part[f * P.maxp + p] = rec[f];
 }
 
@@ -350,6 +350,7 @@ struct StepArgs {
     unsigned long long *slots_mass;   // NSLOT fixed-point mass sums of step k, or null
     unsigned long long *slots_yield;  // NSLOT yielded-node counts of the collide's step
     int n_pblocks;
+    int kk_upd;                   // loop iteration of the particle update in this launch
     long long ntiles;
     int aux;                      // write STRESS / DOMAIN_IMAGE
     // per-tile candidate lists (more than NP_LOOP particles): read for the
@@ -368,7 +369,37 @@ struct StepArgs {
     // FP32 smoothed-profile records (reduced-precision runs; null for FP64)
     const float *spf_in;          // state k
     float *spf_out;               // state k+1
+    // per-call bases from which step_fields() derives the per-step fields
+    double *hist_base;            // history rows of the call, or null
+    unsigned long long *slots_base;  // monitor slots of the call, or null
+    int rowp;                     // history doubles per step (particles)
+    int kk0;                      // step index before the call
+    unsigned g0;                  // tile-list generation at the start of the call
+    int tpb;                      // 256-node tiles per block (batched launches)
 };
+
+// Per-step fields of launch s of a call of nsteps fused steps (s = 0: the
+// prologue; s = nsteps: the final step): tile-list generations, the loop
+// iteration of the particle update, the history row and monitor slots, and
+// whether the launch writes the output-only fields.
+__host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nsteps) {
+    const bool rec = a.slots_base != nullptr;
+    const bool fin = s == nsteps;
+    a.gen_in = a.g0 + (unsigned)s;
+    a.gen_out = a.gen_in + 1u;
+    a.kk_upd = a.kk0 + s + 1;
+    if (s == 0) {
+        a.hist_part = nullptr;
+        a.slots_mass = nullptr;
+        a.slots_yield = rec ? a.slots_base + NSLOT : nullptr;
+        a.aux = nsteps == 1;
+    } else {
+        a.hist_part = rec ? a.hist_base + (size_t)(s - 1) * (size_t)a.rowp : nullptr;
+        a.slots_mass = rec ? a.slots_base + (size_t)(s - 1) * 2 * NSLOT : nullptr;
+        a.slots_yield = (rec && !fin) ? a.slots_base + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
+        a.aux = (s == nsteps - 1) || fin;
+    }
+}
 
 // FP32 variant of tile_has_particles for the reduced-precision path.
 __device__ __forceinline__ bool tile_has_particles_f32(const StepArgs &a, long long tile) {
@@ -460,7 +491,7 @@ __device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
         for (int f = 0; f < PCOUNT; ++f) rec[f] = srec[f];
         a.paircnt[p] = cnt;
         rec[PFCX] = fcx; rec[PFCY] = fcy;
-        update_one(P, rec);
+        update_one(P, rec, a.kk_upd);
         for (int f = 0; f < PCOUNT; ++f) { a.pout[f * m + p] = rec[f]; srec[f] = rec[f]; }
         if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
         if (p == P.np - 1) *a.dpw_out = dl;
@@ -560,13 +591,8 @@ __device__ __forceinline__ void tile_monitors(const StepArgs &a, long long tile,
 // UX/UY, G, OMEGA_BN) into f*_{k+1} in the storage format; particle blocks
 // take the particles from state k to k+1.
 template <class St, bool DEV>
-__global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
+__device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile) {
     const DevP &P = a.P;
-    if ((int)blockIdx.x < a.n_pblocks) {
-        if ((int)blockIdx.x % a.nsub == 0) particle_advance(a, (int)blockIdx.x / a.nsub);
-        return;
-    }
-    long long tile = xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles);
     long long n = tile * NT + threadIdx.x;
     int yielded = 0;
     if (n < P.nn) {
@@ -626,6 +652,15 @@ __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
     if (a.slots_yield) tile_monitors(a, tile, 0.0, yielded, false, true);
 }
 
+template <class St, bool DEV>
+__global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
+    if ((int)blockIdx.x < a.n_pblocks) {
+        if ((int)blockIdx.x % a.nsub == 0) particle_advance(a, (int)blockIdx.x / a.nsub);
+        return;
+    }
+    prologue_tile<St, DEV>(a, xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles));
+}
+
 // DOMAIN_IMAGE from the particles alone (F6:438, F6:460-467).
 __device__ __forceinline__ int image_node(const DevP &P, const double *part, int i, int j) {
     int img = 5;
@@ -654,10 +689,10 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
 // Particle blocks compute the hydrodynamic force of step k and advance the
 // particles to state k+1.
 template <class St, bool DEV, bool FINAL>
-__device__ void particle_block(const StepArgs &a) {
+__device__ void particle_block(const StepArgs &a, int pb) {
     const DevP &P = a.P;
     {
-        int p = blockIdx.x / a.nsub, sub = blockIdx.x - p * a.nsub;
+        int p = pb / a.nsub, sub = pb - p * a.nsub;
         const int m = P.maxp;
         // Stage the inputs of the update in LDS while the force gather runs:
         // this particle's record, the contact inputs of all particles and the
@@ -726,7 +761,7 @@ __device__ void particle_block(const StepArgs &a) {
                 rec[PFX] = F[0]; rec[PFY] = F[1]; rec[PTQ] = F[2];
                 a.paircnt[p] = cnt;
                 rec[PFCX] = fcx; rec[PFCY] = fcy;
-                update_one(P, rec);
+                update_one(P, rec, a.kk_upd);
                 for (int f = 0; f < PCOUNT; ++f) { a.pout[f * m + p] = rec[f]; srec[f] = rec[f]; }
                 if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
                 if (p == P.np - 1) *a.dpw_out = dl;
@@ -739,13 +774,8 @@ __device__ void particle_block(const StepArgs &a) {
 }
 
 template <class St, bool DEV, bool FINAL>
-__global__ __launch_bounds__(NT) void k_step(StepArgs a) {
+__device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
     const DevP &P = a.P;
-    if ((int)blockIdx.x < a.n_pblocks) {
-        particle_block<St, DEV, FINAL>(a);
-        return;
-    }
-    long long tile = xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles);
     long long n = tile * NT + threadIdx.x;
     double mass = 0.0;
     int yielded = 0;
@@ -835,6 +865,63 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     if (a.slots_mass || (!FINAL && a.slots_yield))
         tile_monitors(a, tile, mass, yielded, a.slots_mass != nullptr,
                       !FINAL && a.slots_yield != nullptr);
+}
+
+template <class St, bool DEV, bool FINAL>
+__global__ __launch_bounds__(NT) void k_step(StepArgs a) {
+    if ((int)blockIdx.x < a.n_pblocks) {
+        particle_block<St, DEV, FINAL>(a, (int)blockIdx.x);
+        return;
+    }
+    step_tile<St, DEV, FINAL>(a, xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles));
+}
+
+// ================================================================= batch
+// Many independent grids in one launch.  bmap[blockIdx.x] = (simulation,
+// block within it); each simulation's arguments for this launch's step parity
+// are read from args[], and the per-step fields are derived from the step
+// index s = s_const + *s_dev (s_dev lets one captured graph serve every
+// chunk of steps).  A tile block handles tpb consecutive 256-node tiles.
+template <class St, bool DEV>
+__global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *args, const int2 *bmap,
+                                                       int nsteps) {
+    const int2 m = bmap[blockIdx.x];
+    StepArgs a = args[m.x];
+    step_fields(a, 0, nsteps);
+    if (m.y < a.n_pblocks) {
+        if (m.y % a.nsub == 0) particle_advance(a, m.y / a.nsub);
+        return;
+    }
+    const long long tb = m.y - a.n_pblocks;
+    for (int r = 0; r < a.tpb; ++r) {
+        const long long t = tb * a.tpb + r;
+        if (t < a.ntiles) prologue_tile<St, DEV>(a, t);
+    }
+}
+
+template <class St, bool DEV, bool FINAL>
+__global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *args, const int2 *bmap,
+                                                   const int *s_dev, int s_const, int nsteps) {
+    const int2 m = bmap[blockIdx.x];
+    StepArgs a = args[m.x];
+    step_fields(a, s_const + (s_dev ? *s_dev : 0), nsteps);
+    if (m.y < a.n_pblocks) {
+        particle_block<St, DEV, FINAL>(a, m.y);
+        return;
+    }
+    const long long tb = m.y - a.n_pblocks;
+    for (int r = 0; r < a.tpb; ++r) {
+        const long long t = tb * a.tpb + r;
+        if (t < a.ntiles) step_tile<St, DEV, FINAL>(a, t);
+    }
+}
+
+// Step counter of captured chunks: one thread adds the chunk length.
+__global__ void k_counter_add(int *c, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c += v;
+}
+__global__ void k_counter_set(int *c, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
 }
 
 }  // namespace lbm

@@ -11,6 +11,7 @@ MODULE LBM_AMD
   INTEGER, PARAMETER :: LBM_PF_COUNT = 22, LBM_H_PCOUNT = 12
   INTEGER, PARAMETER :: LBM_NEWTONIAN = 0, LBM_BINGHAM = 1
   INTEGER, PARAMETER :: LBM_RATE_LAGGED = 0, LBM_RATE_LOCAL = 1
+  INTEGER, PARAMETER :: LBM_MOTION_FREE = 0, LBM_MOTION_OSC_X = 1
   INTEGER, PARAMETER :: LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16 = 2, LBM_BF16 = 3
   ! particle record fields, 1-based (LBM_PF_* + 1)
   INTEGER, PARAMETER :: PF_X = 1, PF_Y = 2, PF_ANG = 3, PF_VX = 4, PF_VY = 5, PF_W = 6
@@ -24,11 +25,12 @@ MODULE LBM_AMD
 
   TYPE, BIND(C) :: LBM_PARAMS
     INTEGER(C_INT32_T) :: NX, NY, RHEOLOGY, RATE_MODE, Q_MODE, UPDATE_ANGLE
-    INTEGER(C_INT32_T) :: STORAGE, MAX_PARTICLES
+    INTEGER(C_INT32_T) :: STORAGE, MAX_PARTICLES, MOTION, RESERVED
     REAL(C_DOUBLE) :: OMEGA, S0, S1, S2, S3, S5, SQ, NU, MU0, TAU_Y, M_PAP, GAMMA_MIN
     REAL(C_DOUBLE) :: RHO0, GY
     REAL(C_DOUBLE) :: WALL(8), CORNER(2)
     REAL(C_DOUBLE) :: ZETA_C, CIJ, EPS_P, EPS_W, E_P, E_W
+    REAL(C_DOUBLE) :: OSC_U, OSC_PERIOD
   END TYPE LBM_PARAMS
 
   INTERFACE
@@ -177,9 +179,10 @@ MODULE LBM_AMD
       TYPE(C_PTR), VALUE :: npairs
       INTEGER(C_INT) :: rc
     END FUNCTION
-    FUNCTION lbm_particle_update(h) BIND(C, NAME='lbm_particle_update') RESULT(rc)
-      IMPORT :: C_PTR, C_INT
+    FUNCTION lbm_particle_update(h, kk) BIND(C, NAME='lbm_particle_update') RESULT(rc)
+      IMPORT :: C_PTR, C_INT, C_INT32_T
       TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: kk
       INTEGER(C_INT) :: rc
     END FUNCTION
     FUNCTION lbm_total_mass(h, mass) BIND(C, NAME='lbm_total_mass') RESULT(rc)

@@ -80,6 +80,10 @@ PROGRAM LBM_F6_DRIVER
   P%UPDATE_ANGLE = 1
   P%STORAGE = STORAGE
   P%MAX_PARTICLES = 1
+  P%MOTION = LBM_MOTION_FREE
+  P%RESERVED = 0
+  P%OSC_U = 0D0
+  P%OSC_PERIOD = 0D0
   P%OMEGA = OMEGA
   P%S0 = 0.0                                                     ! F6:1009-1011
   P%S1 = 1.1
@@ -151,7 +155,7 @@ PROGRAM LBM_F6_DRIVER
     DO KK = 1, TIME_MAX
       RC = lbm_contact_force(H, C_NULL_PTR, 0, C_NULL_PTR)
       CALL LBM_CHECK(H, RC, 'ComputeCollisionForce')
-      RC = lbm_particle_update(H)
+      RC = lbm_particle_update(H, KK)
       CALL LBM_CHECK(H, RC, 'particle update')
       RC = lbm_get_particles(H, PART)
       CALL LBM_CHECK(H, RC, 'lbm_get_particles')

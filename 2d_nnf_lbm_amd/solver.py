@@ -44,11 +44,17 @@ def _ptr(a, ct=ctypes.c_double):
 class D2Q9:
     """One simulation on the GPU.  cfg: dict of lbm_params fields."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, _handle=None):
         self.lib = _abi.load()
         self.params = make_params(cfg)
         self.nx, self.ny = self.params.nx, self.params.ny
         self.shape = (self.ny + 1, self.nx + 1)
+        self.np = 0
+        self._owner = _handle is None
+        if _handle is not None:
+            # a simulation of a Batch: the batch owns the handle
+            self._h = _handle
+            return
         self._h = ctypes.c_void_p()
         rc = self.lib.lbm_create(ctypes.byref(self.params), ctypes.byref(self._h))
         if rc != 0:
@@ -61,7 +67,8 @@ class D2Q9:
 
     def close(self):
         if getattr(self, "_h", None):
-            self.lib.lbm_destroy(self._h)
+            if self._owner:
+                self.lib.lbm_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -192,9 +199,10 @@ class D2Q9:
         self._call("lbm_contact_force", None, 0, None)
         return None
 
-    def particle_update(self):
-        """Particle update inline in MAIN (F6:299-312)."""
-        self._call("lbm_particle_update")
+    def particle_update(self, kk=0):
+        """Particle update of loop iteration kk: inline in MAIN (F6:299-312),
+        or the prescribed motion of F4:221-226."""
+        self._call("lbm_particle_update", int(kk))
 
     def total_mass(self):
         """SUM(RHO) (F6:348)."""
@@ -206,7 +214,7 @@ class D2Q9:
         """One time step through the per-subroutine entries, in the order of
         the reference's time loop (F6:293-345)."""
         self.compute_collision_force()
-        self.particle_update()
+        self.particle_update(kk)
         self.compute_concentration_function()
         self.collision_mrt()
         self.copy_post()
@@ -243,3 +251,85 @@ class D2Q9:
 
     def sync(self):
         self._call("lbm_sync")
+
+
+class Batch:
+    """Independent simulations stepped together, one launch per time step
+    (lbm_batch_* in include/lbm_amd.h).  cfgs: one lbm_params dict per
+    simulation (same storage for all).  sims[i] is a D2Q9 view of simulation
+    i for setting particles and reading state."""
+
+    def __init__(self, cfgs):
+        self.lib = _abi.load()
+        n = len(cfgs)
+        arr = (_abi.Params * n)(*[make_params(c) for c in cfgs])
+        self._b = ctypes.c_void_p()
+        rc = self.lib.lbm_batch_create(n, arr, ctypes.byref(self._b))
+        if rc != 0:
+            raise LBMError(rc, self.lib.lbm_last_error(None).decode())
+        self.n = n
+        self.sims = []
+        for i, c in enumerate(cfgs):
+            h = self.lib.lbm_batch_sim(self._b, i)
+            self.sims.append(D2Q9(c, _handle=ctypes.c_void_p(h)))
+
+    def close(self):
+        if getattr(self, "_b", None):
+            for s in self.sims:
+                s.close()
+            self.lib.lbm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        self.close()
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(self._b, *args)
+        if rc != 0:
+            raise LBMError(rc, self.lib.lbm_batch_last_error(self._b).decode())
+        return rc
+
+    def set_particles(self, i, part):
+        self.sims[i].set_particles(part)
+
+    def initialize(self):
+        self._call("lbm_batch_initialize")
+
+    def step(self, kk0, nsteps, record=True):
+        """Fused steps of every simulation; with record, a list of per-
+        simulation histories (nsteps, np_i*12 + 2)."""
+        if not record:
+            self._call("lbm_batch_step", int(kk0), int(nsteps), None)
+            return None
+        widths = [s.np * _abi.H_PCOUNT + 2 for s in self.sims]
+        buf = np.zeros(nsteps * sum(widths))
+        self._call("lbm_batch_step", int(kk0), int(nsteps), _ptr(buf))
+        out, o = [], 0
+        for w in widths:
+            out.append(buf[o:o + nsteps * w].reshape(nsteps, w))
+            o += nsteps * w
+        return out
+
+    def step_timed(self, kk0, nsteps):
+        t = ctypes.c_double(0)
+        self._call("lbm_batch_step_timed", int(kk0), int(nsteps), ctypes.byref(t))
+        return t.value
+
+    def set_graph(self, chunk):
+        self._call("lbm_batch_set_graph", int(chunk))
+
+    @property
+    def tiles(self):
+        t = np.zeros(self.n, dtype=np.int32)
+        self._call("lbm_batch_get_tiles", _ptr(t, ctypes.c_int32))
+        return t
+
+    @tiles.setter
+    def tiles(self, values):
+        t = np.ascontiguousarray(values, dtype=np.int32)
+        self._call("lbm_batch_set_tiles", _ptr(t, ctypes.c_int32))
+
+    def tune(self, steps=10):
+        t = np.zeros(self.n, dtype=np.int32)
+        self._call("lbm_batch_tune", int(steps), _ptr(t, ctypes.c_int32))
+        return t

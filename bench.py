@@ -21,6 +21,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
@@ -73,6 +75,37 @@ def measure(pkg, configs, storage, steps, warmup, workload=None):
     )
     s.close()
     return out
+
+
+def measure_ensemble(pkg, configs, steps):
+    """256 oscillating-cylinder cavities (sides U{128..1024}, Bn U[0,10],
+    seed 1) in one batch: autotuned tiling, graph-replayed steps."""
+    ens = configs.cavity_ensemble()
+    nodes = sum(configs.nodes(c) for c, _ in ens)
+    b = pkg.Batch([c for c, _ in ens])
+    b.initialize()
+    for i, (_, part) in enumerate(ens):
+        b.set_particles(i, part)
+    t0 = time.perf_counter()
+    tiles = b.tune(steps=4)
+    tune_s = time.perf_counter() - t0
+    b.initialize()
+    for i, (_, part) in enumerate(ens):
+        b.set_particles(i, part)
+    chunk = 16
+    b.set_graph(chunk)
+    b.step(0, chunk + 2, record=False)
+    ms = b.step_timed(chunk + 2, steps)
+    b.close()
+    vals, counts = np.unique(tiles, return_counts=True)
+    return dict(workload="256 lid- and side-wall-driven cavities, each with one cylinder "
+                         "(D=side/8) in prescribed x-oscillation, sides U{128..1024}, "
+                         "Bn U[0,10], FP32 collide / FP16 storage, one launch per step",
+                simulations=len(ens), nodes_per_step=nodes, steps=steps, graph_chunk=chunk,
+                mlups=nodes * steps / (ms * 1e-3) / 1e6, ms_per_step=ms / steps,
+                achieved_gbs=configs.bytes_per_node("fp16", True) * nodes * steps / (ms * 1e-3) / 1e9,
+                tiles_per_block={int(v): int(c) for v, c in zip(vals, counts)},
+                tune_seconds=tune_s)
 
 
 def accuracy(pkg, configs, steps):
@@ -149,6 +182,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dense-steps", type=int, default=200,
                     help="timed steps of the 504-disk BF16 cavity (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=40,
+                    help="timed steps of the 256-cavity FP16 batch (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of 9 FP64 planes (known byte "
                          "count) so hardware traffic counters can be scaled")
@@ -185,6 +220,10 @@ def main():
                      achieved_gbs=rd["achieved_gbs"],
                      roofline_frac=rd["achieved_gbs"] / HBM_PEAK_GBS,
                      algorithmic_bytes_per_node=rd["bytes_per_node"])
+    ensemble = None
+    if args.ensemble_steps > 0:
+        log("bench: 256-cavity batch, fp16")
+        ensemble = measure_ensemble(pkg, configs, args.ensemble_steps)
     log("bench: accuracy leg")
     acc = accuracy(pkg, configs, args.accuracy_steps) if args.accuracy_steps > 0 else None
     cpu = None
@@ -237,6 +276,7 @@ def main():
         },
         "fp16_over_fp64": r16["mlups"] / r64["mlups"],
         "dense_sedimentation_bf16": dense,
+        "cavity_ensemble_fp16": ensemble,
         "accuracy_fp16_vs_fp64": acc,
     }
     print(json.dumps(line), flush=True)

@@ -38,6 +38,9 @@ extern "C" {
 #define LBM_NEWTONIAN 0
 #define LBM_BINGHAM 1            /* Papanastasiou-regularised Bingham (F6:939-945) */
 
+#define LBM_MOTION_FREE 0        /* Newton-Euler update of the particles (F6:299-312) */
+#define LBM_MOTION_OSC_X 1       /* prescribed x-oscillation V_x = -U cos(2 pi KK / T) (F4:221-226) */
+
 #define LBM_RATE_LAGGED 0        /* module-global rate set by the previous node (F6:958) */
 #define LBM_RATE_LOCAL 1         /* each node's own previous-step rate */
 
@@ -72,6 +75,8 @@ typedef struct lbm_params {
     int32_t update_angle;      /* 1: integrate the angle (F6:312); 0: F3:327 */
     int32_t storage;           /* LBM_FP64 / LBM_FP32 / LBM_FP16 / LBM_BF16 */
     int32_t max_particles;     /* capacity of the particle arrays */
+    int32_t motion;            /* LBM_MOTION_FREE / LBM_MOTION_OSC_X */
+    int32_t reserved;          /* zero */
     double omega;              /* OMEGA = 1/(3 nu + 0.5) (F6:101) */
     double s0, s1, s2, s3, s5, sq;  /* other MRT rates (F6:1009-1011) */
     double nu;                 /* NU, Newtonian stress factor (F6:918) */
@@ -81,6 +86,7 @@ typedef struct lbm_params {
     double wall[8];            /* uxS,uyS, uxR,uyR, uxN,uyN, uxL,uyL (F6:1204-1264) */
     double corner[2];          /* ux_w, uy_w at the four corners (F6:1365) */
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;  /* contact (F6:691-693, F3:739-742) */
+    double osc_u, osc_period;  /* U_C, T_C of LBM_MOTION_OSC_X (F4:224) */
 } lbm_params;
 
 typedef struct lbm_handle lbm_handle;
@@ -130,7 +136,9 @@ int lbm_correct(lbm_handle *h);           /* VelocityCorrector, F6:1111-1125 */
  * NULL) receives the in-range pairs (i<j, ascending, 0-based) as 2*n ints. */
 int lbm_contact_force(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
                       int32_t *npairs);
-int lbm_particle_update(lbm_handle *h);   /* inline update, F6:299-312 */
+/* Particle update of loop iteration kk: inline in MAIN (F6:299-312), or the
+ * prescribed motion of F4:221-226 and its added-mass term (F4:256-258). */
+int lbm_particle_update(lbm_handle *h, int32_t kk);
 int lbm_total_mass(lbm_handle *h, double *mass);  /* SUM(RHO), F6:348 */
 
 /* Fused time steps kk0+1 .. kk0+nsteps (the loop body F6:293-349), all on the
@@ -148,6 +156,37 @@ int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
  * duration of one launch of the main kernel. */
 int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps,
                    double *ms_total, double *ms_main);
+
+/* ------------------------------------------------------------------ batch
+ * Many independent simulations (grid sizes, rheology, walls and particle
+ * motion may differ; one storage format) stepped together: one launch per
+ * time step covers every grid.  The reference runs one simulation per
+ * program; these entries have no counterpart there and serve ensembles. */
+typedef struct lbm_batch lbm_batch;
+/* params: n records; each simulation gets its own handle (lbm_batch_sim),
+ * through which particles are set and state is read with the calls above. */
+int lbm_batch_create(int32_t n, const lbm_params *params, lbm_batch **out);
+int lbm_batch_destroy(lbm_batch *b);
+const char *lbm_batch_last_error(const lbm_batch *b);
+int lbm_batch_size(const lbm_batch *b, int32_t *n);
+lbm_handle *lbm_batch_sim(lbm_batch *b, int32_t i);
+int lbm_batch_initialize(lbm_batch *b);
+/* nsteps fused steps (kk0+1 .. kk0+nsteps) of every simulation, as lbm_step
+ * per simulation; hist (may be NULL) receives each simulation's history rows
+ * (nsteps rows of n_particles*LBM_H_PCOUNT + 2) one simulation after another. */
+int lbm_batch_step(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist);
+int lbm_batch_step_timed(lbm_batch *b, int32_t kk0, int32_t nsteps, double *ms_total);
+/* chunk > 0 (even): steps 1 .. nsteps-2 of a call are replayed in chunks of
+ * `chunk` launches from one captured hipGraph; 0: every launch eager. */
+int lbm_batch_set_graph(lbm_batch *b, int32_t chunk);
+/* 256-node tiles handled by one 256-thread block, per simulation (1..64). */
+int lbm_batch_set_tiles(lbm_batch *b, const int32_t *tiles_per_block);
+int lbm_batch_get_tiles(const lbm_batch *b, int32_t *tiles_per_block);
+/* Autotuner: for each size bucket (floor(log2(nodes))) times `steps` steps of
+ * that bucket's simulations with 1, 2, 4 and 8 tiles per block and keeps the
+ * fastest.  The timed steps advance those simulations: initialise and set the
+ * particles again afterwards.  tiles_per_block (may be NULL) gets the choice. */
+int lbm_batch_tune(lbm_batch *b, int32_t steps, int32_t *tiles_per_block);
 
 #ifdef __cplusplus
 }

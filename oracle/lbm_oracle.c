@@ -421,10 +421,30 @@ int32_t orc_contact(const orc_params *p, orc_state *st, int32_t *pairs,
     return npairs;
 }
 
-/* Particle update, inline in MAIN, F6:299-312 / F3:312-328. */
-void orc_update(const orc_params *p, orc_state *st)
+/* Particle update, inline in MAIN, F6:299-312 / F3:312-328; with motion = 1
+ * the prescribed oscillation of F4:221-226 for loop iteration kk, with the
+ * added-mass term of F4:256-258. */
+void orc_update(const orc_params *p, orc_state *st, int32_t kk)
 {
     const double rho0 = p->rho0;
+    if (p->motion == 1) {
+        const double pi = F32(3.14159265359f);
+        for (int ip = 0; ip < st->np; ++ip) {
+            double rhop = PF(st, ORC_PF_RHOP, ip), mp = PF(st, ORC_PF_MP, ip);
+            double ipm = PF(st, ORC_PF_IP, ip);
+            double v0 = PF(st, ORC_PF_VX, ip), v1 = PF(st, ORC_PF_VY, ip), v2 = PF(st, ORC_PF_W, ip);
+            PF(st, ORC_PF_VOX, ip) = v0; PF(st, ORC_PF_VOY, ip) = v1; PF(st, ORC_PF_VOW, ip) = v2;
+            double n0 = -p->osc_u * cos(2.0 * pi * (double)kk / p->osc_period);
+            double n1 = 0.0, n2 = 0.0;
+            PF(st, ORC_PF_VX, ip) = n0; PF(st, ORC_PF_VY, ip) = n1; PF(st, ORC_PF_W, ip) = n2;
+            PF(st, ORC_PF_X, ip) = PF(st, ORC_PF_X, ip) + 0.5 * (n0 + v0);
+            PF(st, ORC_PF_Y, ip) = PF(st, ORC_PF_Y, ip) + 0.5 * (n1 + v1);
+            PF(st, ORC_PF_FHX, ip) = (rho0 / rhop) * mp * (n0 - v0);
+            PF(st, ORC_PF_FHY, ip) = (rho0 / rhop) * mp * (n1 - v1);
+            PF(st, ORC_PF_FHW, ip) = (rho0 / rhop) * ipm * (n2 - v2);
+        }
+        return;
+    }
     for (int ip = 0; ip < st->np; ++ip) {
         double rhop = PF(st, ORC_PF_RHOP, ip), mp = PF(st, ORC_PF_MP, ip), ipm = PF(st, ORC_PF_IP, ip);
         double v0 = PF(st, ORC_PF_VX, ip), v1 = PF(st, ORC_PF_VY, ip), v2 = PF(st, ORC_PF_W, ip);
@@ -471,7 +491,7 @@ void orc_run(const orc_params *p, orc_state *st, int32_t kk0, int32_t nsteps,
     for (int32_t s = 0; s < nsteps; ++s) {
         int32_t kk = kk0 + s + 1;
         orc_contact(p, st, NULL, 0);
-        orc_update(p, st);
+        orc_update(p, st, kk);
         orc_phi(p, st);
         orc_collide(p, st);
         orc_copy_post(p, st);

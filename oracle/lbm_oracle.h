@@ -48,6 +48,8 @@ typedef struct orc_params {
                                 1 each node's own previous-step rate */
     int32_t q_mode;          /* 1: s4=s6=8(2-w)/(8-w) (F6:1009); 0: sq */
     int32_t update_angle;    /* 1: F6:312 integrates the angle; 0: F3:327 */
+    int32_t motion;          /* 0 free (F6:299-312); 1 prescribed x-oscillation (F4:221-226) */
+    int32_t reserved;
     double omega;            /* OMEGA = 1/(3 nu + 0.5) (F6:101) */
     double s0, s1, s2, s3, s5, sq;  /* remaining MRT rates (F6:1009-1011) */
     double nu;               /* NU, Newtonian stress factor (F6:918) */
@@ -57,6 +59,7 @@ typedef struct orc_params {
     double wall[8];          /* uxS,uyS, uxR,uyR, uxN,uyN, uxL,uyL (F6:1204-1264) */
     double corner[2];        /* ux_w, uy_w at the four corners (F6:1365) */
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;  /* contact (F6:691-693) */
+    double osc_u, osc_period;  /* U_C and T_C of the prescribed velocity (F4:224) */
 } orc_params;
 
 typedef struct orc_state {
@@ -87,7 +90,7 @@ void orc_hydro(const orc_params *p, orc_state *st);
 void orc_correct(const orc_params *p, orc_state *st);
 int32_t orc_contact(const orc_params *p, orc_state *st, int32_t *pairs,
                     int32_t max_pairs);
-void orc_update(const orc_params *p, orc_state *st);
+void orc_update(const orc_params *p, orc_state *st, int32_t kk);
 double orc_total_mass(const orc_params *p, const orc_state *st);
 int64_t orc_yielded_nodes(const orc_params *p, const orc_state *st);
 /* Runs steps kk0+1 .. kk0+nsteps in the reference order (F6:291-370).

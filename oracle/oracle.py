@@ -38,6 +38,7 @@ class Params(ctypes.Structure):
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
         ("rheology", ctypes.c_int32), ("rate_mode", ctypes.c_int32),
         ("q_mode", ctypes.c_int32), ("update_angle", ctypes.c_int32),
+        ("motion", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("omega", ctypes.c_double),
         ("s0", ctypes.c_double), ("s1", ctypes.c_double), ("s2", ctypes.c_double),
         ("s3", ctypes.c_double), ("s5", ctypes.c_double), ("sq", ctypes.c_double),
@@ -49,6 +50,7 @@ class Params(ctypes.Structure):
         ("zeta_c", ctypes.c_double), ("cij", ctypes.c_double),
         ("eps_p", ctypes.c_double), ("eps_w", ctypes.c_double),
         ("e_p", ctypes.c_double), ("e_w", ctypes.c_double),
+        ("osc_u", ctypes.c_double), ("osc_period", ctypes.c_double),
     ]
 
 
@@ -78,9 +80,11 @@ def lib():
         S = ctypes.POINTER(_CState)
         for name in ("orc_initialize", "orc_collide", "orc_copy_post", "orc_stream",
                      "orc_macros", "orc_phi", "orc_forcing", "orc_hydro",
-                     "orc_correct", "orc_update"):
+                     "orc_correct"):
             getattr(L, name).argtypes = [P, S]
             getattr(L, name).restype = None
+        L.orc_update.argtypes = [P, S, ctypes.c_int32]
+        L.orc_update.restype = None
         L.orc_boundary.argtypes = [P, S, ctypes.c_int32]
         L.orc_contact.argtypes = [P, S, _ip, ctypes.c_int32]
         L.orc_contact.restype = ctypes.c_int32
@@ -161,7 +165,7 @@ class State:
     def step_parts(self, kk):
         """One time step, routine by routine, in the reference order."""
         self.contact()
-        self.call("orc_update")
+        self.call("orc_update", kk)
         self.call("orc_phi")
         self.call("orc_collide")
         self.call("orc_copy_post")

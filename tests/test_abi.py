@@ -13,14 +13,14 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     text = open(os.path.join(REPO, "include", "lbm_amd.h")).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(lbm_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|const char \*|lbm_handle \*)\s*(lbm_\w+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
     abi = pkg()._abi
     lib = abi.load()
     names = header_functions()
-    assert len(names) >= 30
+    assert len(names) >= 40
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(abi.exported_names()) == names
@@ -28,8 +28,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_params_struct_matches_header_size():
     abi = pkg()._abi
-    # 8 int32 + 20 doubles + wall[8] + corner[2] + 6 contact doubles
-    assert ctypes.sizeof(abi.Params) == 8 * 4 + (1 + 6 + 1 + 4 + 2 + 8 + 2 + 6) * 8
+    # 10 int32 + 14 doubles + wall[8] + corner[2] + 6 contact + 2 oscillation doubles
+    assert ctypes.sizeof(abi.Params) == 10 * 4 + (1 + 6 + 1 + 4 + 2 + 8 + 2 + 6 + 2) * 8
 
 
 @pytest.mark.parametrize("bad", [

@@ -125,3 +125,35 @@ def test_oracle_contact_vs_reference_routine(orc):
     brute = [(i, j) for i in range(n) for j in range(i + 1, n)
              if np.sqrt((x[i] - x[j]) ** 2 + (y[i] - y[j]) ** 2) <= 0.5 * d[i] + 0.5 * d[j] + 1.0]
     assert [tuple(p) for p in pairs] == brute
+
+
+def test_oracle_prescribed_oscillation_closed_form(orc):
+    """Prescribed motion of F4:221-226: V_x(KK) = -U cos(2 pi KK / T), centre
+    advanced by the mean of the old and new velocity, and the added-mass
+    term (rho0/rho_p) m (V - V_old) of F4:256-258.  Checked against the
+    integral of the velocity law: after n updates from rest,
+    X_n - X_0 = -U T/(2 pi) sin(2 pi n/T) + U/2 (the first half-step starts
+    from V = 0 instead of -U) up to the trapezoid error, which is below
+    n U (2 pi/T)^2 / 12.  pi is F4's default-REAL literal (float32)."""
+    cfg = cases.f6_params(40, 40, False)
+    u, period = 0.003, 200.0
+    cfg.update(motion=1, osc_u=u, osc_period=period)
+    st = orc.State(orc.make_params(cfg), 1)
+    rec = np.zeros(orc.PF_COUNT)
+    rec[orc.PF_X], rec[orc.PF_Y], rec[orc.PF_DIAM] = 20.0, 20.0, 5.0
+    rec[orc.PF_RHOP], rec[orc.PF_MP], rec[orc.PF_IP], rec[orc.PF_ZETA] = 1.5, 29.45, 92.0, 1.0
+    st.set_particles(rec[:, None])
+    n = 150
+    xs, fh = [], []
+    for kk in range(1, n + 1):
+        st.call("orc_update", kk)
+        xs.append(st.part[orc.PF_X, 0])
+        fh.append(st.part[orc.PF_FHX, 0])
+    k = np.arange(1, n + 1)
+    pi = cases.PI
+    x_exact = 20.0 - u * period / (2 * pi) * np.sin(2 * pi * k / period) + 0.5 * u
+    assert np.max(np.abs(np.array(xs) - x_exact)) <= n * u * (2 * pi / period) ** 2 / 12
+    v = -u * np.cos(2 * pi * k / period)
+    dv = np.diff(np.concatenate([[0.0], v]))
+    assert np.allclose(fh, (1.0 / 1.5) * 29.45 * dv, rtol=1e-12, atol=1e-18)
+    assert st.part[orc.PF_Y, 0] == 20.0 and st.part[orc.PF_VY, 0] == 0.0
