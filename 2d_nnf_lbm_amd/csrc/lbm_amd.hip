@@ -270,7 +270,21 @@ int lbm_initialize(lbm_handle *h) {
     const DevP &P = h->P;
     hipLaunchKernelGGL(k_init, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->rho,
                        h->ux, h->uy, h->g, h->obn);
-    return check_launch(h, "k_init");
+    int rc = check_launch(h, "k_init");
+    if (rc) return rc;
+    // run state back to its start: the stochastic-rounding sequence and the
+    // tile-list generations restart (headers cleared so no stale generation
+    // matches), and the stale wall distance of the contact force is unset
+    h->gen_base = 1;
+    for (int b = 0; b < 2; ++b)
+        if (h->tl_hdr[b])
+            HIPCHECK(h, hipMemsetAsync(h->tl_hdr[b], 0, (size_t)nblk(P.nn) * 8, h->stream));
+    double inf = INFINITY;
+    HIPCHECK(h, hipMemcpyAsync(h->dpw[0], &inf, 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, hipMemcpyAsync(h->dpw[1], &inf, 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    h->dcur = 0;
+    return LBM_OK;
 }
 
 static int upload_planes(lbm_handle *h, const double *host, int nc, double *dev) {

@@ -101,7 +101,10 @@ const char *lbm_last_error(const lbm_handle *h);
 int lbm_sync(lbm_handle *h);
 
 /* INITIALIZE, F6:563-580 (f = feq(rho0, 0), G = 0), with OMEGA_BN = OMEGA
- * (F6:102) and the initial relaxation vector (F6:199). */
+ * (F6:102) and the initial relaxation vector (F6:199).  Also restarts the run
+ * state: the stochastic-rounding sequence of the reduced formats and the
+ * stale wall distance of the contact force, so a run after lbm_initialize
+ * does not depend on what the handle ran before. */
 int lbm_initialize(lbm_handle *h);
 
 /* State transfer (reference layout on the host, SoA on the device). */
