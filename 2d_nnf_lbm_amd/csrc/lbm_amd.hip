@@ -903,12 +903,25 @@ int build_map(lbm_batch *b, const std::vector<int> &which) {
 
 template <class St, bool DEV>
 int batch_launches(lbm_batch *b, int32_t nsteps) {
-    const size_t n = b->sims.size();
-    const dim3 grid((unsigned)b->nblocks), block(NT);
-    const StepArgs *A0 = b->dargs, *Aodd = b->dargs + n, *Aeven = b->dargs + 2 * n,
-                   *Afin = b->dargs + 3 * n;
+    const int n = (int)b->sims.size();
+    const dim3 grid((unsigned)b->nblocks), block(NT), fgrid((unsigned)((n + 63) / 64)), fblock(64);
+    StepArgs *A0 = b->dargs, *Aodd = b->dargs + n, *Aeven = b->dargs + 2 * n,
+             *Afin = b->dargs + 3 * n;
     auto step_args = [&](int s) { return (s & 1) ? Aodd : Aeven; };
-    hipLaunchKernelGGL((k_batch_prologue<St, DEV>), grid, block, 0, b->stream, A0, b->bmap, nsteps);
+    // per-step fields, then the launch that reads them
+    auto launch = [&](StepArgs *A, const int *sdev, int s_const, int ns, bool fin) {
+        hipLaunchKernelGGL(k_batch_fields, fgrid, fblock, 0, b->stream, A, n, sdev, s_const, ns);
+        if (fin)
+            hipLaunchKernelGGL((k_batch_step<St, DEV, true>), grid, block, 0, b->stream,
+                               (const StepArgs *)A, (const int2 *)b->bmap);
+        else
+            hipLaunchKernelGGL((k_batch_step<St, DEV, false>), grid, block, 0, b->stream,
+                               (const StepArgs *)A, (const int2 *)b->bmap);
+    };
+    hipLaunchKernelGGL(k_batch_fields, fgrid, fblock, 0, b->stream, A0, n, (const int *)nullptr, 0,
+                       nsteps);
+    hipLaunchKernelGGL((k_batch_prologue<St, DEV>), grid, block, 0, b->stream,
+                       (const StepArgs *)A0, (const int2 *)b->bmap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, LBM_ERR_HIP, std::string("k_batch_prologue: ") + hipGetErrorString(e));
     int s = 1;
@@ -919,9 +932,7 @@ int batch_launches(lbm_batch *b, int32_t nsteps) {
             drop_graph(b);
             hipGraph_t g = nullptr;
             BHIPCHECK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            for (int j = 0; j < K; ++j)
-                hipLaunchKernelGGL((k_batch_step<St, DEV, false>), grid, block, 0, b->stream,
-                                   step_args(1 + j), b->bmap, b->sdev, 1 + j, BIG_NSTEPS);
+            for (int j = 0; j < K; ++j) launch(step_args(1 + j), b->sdev, 1 + j, BIG_NSTEPS, false);
             hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, b->stream, b->sdev, K);
             BHIPCHECK(b, hipStreamEndCapture(b->stream, &g));
             hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
@@ -934,11 +945,8 @@ int batch_launches(lbm_batch *b, int32_t nsteps) {
         for (int c = 0; c < nchunks; ++c) BHIPCHECK(b, hipGraphLaunch(b->gexec, b->stream));
         s = 1 + nchunks * K;
     }
-    for (; s < nsteps; ++s)
-        hipLaunchKernelGGL((k_batch_step<St, DEV, false>), grid, block, 0, b->stream,
-                           step_args(s), b->bmap, (const int *)nullptr, s, nsteps);
-    hipLaunchKernelGGL((k_batch_step<St, DEV, true>), grid, block, 0, b->stream, Afin, b->bmap,
-                       (const int *)nullptr, nsteps, nsteps);
+    for (; s < nsteps; ++s) launch(step_args(s), nullptr, s, nsteps, false);
+    launch(Afin, nullptr, nsteps, nsteps, true);
     e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, LBM_ERR_HIP, std::string("k_batch_step: ") + hipGetErrorString(e));
     return LBM_OK;

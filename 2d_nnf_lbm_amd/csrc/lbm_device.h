@@ -630,7 +630,7 @@ template <bool OUT>
 __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho_dev, float rho,
                                             float ux, float uy, float gx, float gy, float om,
                                             float &om_out, float stress[3], int &yielded) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)  // fused within one expression: the same in every kernel
     float q = P.q_mode ? (8.0f * (2.0f - om)) * frcp(8.0f - om) : P.fsq;
     float s[9] = {P.fs0, P.fs1, P.fs2, P.fs3, q, P.fs5, q, om, om};
     float m[9], mf[9], meq[9], mne[9];

@@ -878,16 +878,23 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
 
 // ================================================================= batch
 // Many independent grids in one launch.  bmap[blockIdx.x] = (simulation,
-// block within it); each simulation's arguments for this launch's step parity
-// are read from args[], and the per-step fields are derived from the step
-// index s = s_const + *s_dev (s_dev lets one captured graph serve every
-// chunk of steps).  A tile block handles tpb consecutive 256-node tiles.
+// block within it); each simulation's arguments for this launch's step
+// parity are read in place from args[] (k_batch_fields writes their per-step
+// fields just before the launch, from the step index s = s_const + *s_dev, so
+// one captured graph serves every chunk of steps).  A tile block handles tpb
+// consecutive 256-node tiles.
+__global__ void k_batch_fields(StepArgs *args, int n, const int *s_dev, int s_const,
+                               int nsteps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    step_fields(args[i], s_const + (s_dev ? *s_dev : 0), nsteps);
+}
+
 template <class St, bool DEV>
-__global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *args, const int2 *bmap,
-                                                       int nsteps) {
+__global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *__restrict__ args,
+                                                       const int2 *__restrict__ bmap) {
     const int2 m = bmap[blockIdx.x];
-    StepArgs a = args[m.x];
-    step_fields(a, 0, nsteps);
+    const StepArgs &a = args[m.x];
     if (m.y < a.n_pblocks) {
         if (m.y % a.nsub == 0) particle_advance(a, m.y / a.nsub);
         return;
@@ -900,11 +907,10 @@ __global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *args, con
 }
 
 template <class St, bool DEV, bool FINAL>
-__global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *args, const int2 *bmap,
-                                                   const int *s_dev, int s_const, int nsteps) {
+__global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ args,
+                                                   const int2 *__restrict__ bmap) {
     const int2 m = bmap[blockIdx.x];
-    StepArgs a = args[m.x];
-    step_fields(a, s_const + (s_dev ? *s_dev : 0), nsteps);
+    const StepArgs &a = args[m.x];
     if (m.y < a.n_pblocks) {
         particle_block<St, DEV, FINAL>(a, m.y);
         return;
