@@ -191,6 +191,32 @@ def cavity_ensemble(n=256, seed=1, storage="fp16"):
     return [oscillating_cavity(int(s), float(b), storage) for s, b in zip(sides, bns)]
 
 
+def settling_adjoint(storage="fp32", tau_y=None, mu0=0.1, rho_p=1.2, nx=512, ny=1024,
+                     diam=32.0):
+    """One disk (D = 32, density ratio 1.2) settling from rest at
+    (nx/2, ny - 7 D) in a closed Papanastasiou-Bingham channel of 512 x 1024
+    cells; the parameters of the sensitivity study are the yield stress
+    tau_y, the plastic viscosity mu0 and the particle density rho_p.
+    Gravity is fixed at A_G = -Re mu^2/(D^3 drho rho) with Re = 0.78,
+    mu = 0.1, drho = 0.2 (F6:16, F6:28) whatever mu0 and rho_p are; the
+    default tau_y is Bn = 0.5 on that scale (F6:30).  OMEGA = 1/(3 mu0 + 1/2)
+    (F6:101); m = 1e5; interface width 2; walls at rest."""
+    rho_f = 1.0
+    re_s = f32(0.78)
+    a_g = -re_s * 0.1 * 0.1 / ((diam * diam * diam) * 0.2 * rho_f)
+    if tau_y is None:
+        tau_y = 0.5 * 0.2 * (-a_g) * diam
+    cij = -((((((rho_p - rho_f) * a_g) * PI) * 0.5) * diam) * 0.5) * diam
+    cfg = dict(nx=nx, ny=ny, rheology=_abi.LBM_BINGHAM, rate_mode=_abi.LBM_RATE_LOCAL,
+               update_angle=1, storage=storage, max_particles=1, omega=1.0 / (3.0 * mu0 + 0.5),
+               nu=mu0, mu0=mu0, tau_y=tau_y, m_pap=1e5, gamma_min=f32(1e-15), rho0=1.0,
+               gy=a_g, wall=[0.0] * 8, corner=[0.0, 0.0], zeta_c=1.0, cij=cij,
+               eps_p=0.5, eps_w=0.5, e_p=10.0, e_w=10.0)
+    cfg.update(f6_rates())
+    part = disk(nx / 2.0, ny - 7.0 * diam, diam, rho_p, 2.0)[:, None]
+    return cfg, part
+
+
 def kinetic_energy(hist, np_, cfg, part, p=0):
     """Translational kinetic energy 1/2 m V^2 of particle p per step."""
     H = _abi.H

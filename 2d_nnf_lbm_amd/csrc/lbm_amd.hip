@@ -12,6 +12,7 @@
 
 #include "../../include/lbm_amd.h"
 #include "lbm_kernels.h"
+#include "lbm_adjoint.h"
 
 using namespace lbm;
 
@@ -1183,3 +1184,219 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps, int32_t *tiles_per_block) {
 }
 
 }  // extern "C"
+
+
+// ================================================================ adjoint
+// Gradient of L = (1/W) sum_{k=N-W+1..N} V_y(P_k) for one disk with respect
+// to (tau_y, mu0, rho_p) by the discrete adjoint of the fused step
+// (lbm_adjoint.h).  The forward pass keeps a checkpoint every `chk` steps;
+// the backward pass restores each checkpoint in turn, recomputes its
+// segment one step per call and walks the segment backwards.
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+int adj_alloc(lbm_handle *h, DevBuf &b, size_t bytes, size_t &total) {
+    HIPCHECK(h, hipMalloc(&b.p, std::max<size_t>(bytes, 8)));
+    total += bytes;
+    return LBM_OK;
+}
+
+// canonical state needed to resume: F, RHO, UX, UY, G(2), OMEGA_BN planes
+constexpr int CK_PLANES = 15;
+
+struct Checkpoint {
+    size_t off = 0;  // into the checkpoint buffer (doubles)
+    int pcur = 0, dcur = 0;
+    unsigned gen = 0;
+};
+
+int save_state(lbm_handle *h, double *dst, Checkpoint &c) {
+    const size_t pl = (size_t)h->P.plane;
+    const size_t pc = (size_t)PCOUNT * h->P.maxp;
+    double *d = dst;
+    HIPCHECK(h, hipMemcpyAsync(d, h->f, 9 * pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += 9 * pl;
+    HIPCHECK(h, hipMemcpyAsync(d, h->rho, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(d, h->ux, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(d, h->uy, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(d, h->g, 2 * pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += 2 * pl;
+    HIPCHECK(h, hipMemcpyAsync(d, h->obn, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(d, h->pbuf[h->pcur], pc * 8, hipMemcpyDeviceToDevice, h->stream)); d += pc;
+    HIPCHECK(h, hipMemcpyAsync(d, h->dpw[h->dcur], 8, hipMemcpyDeviceToDevice, h->stream));
+    c.pcur = h->pcur; c.dcur = h->dcur; c.gen = h->gen_base;
+    return LBM_OK;
+}
+
+int restore_state(lbm_handle *h, const double *src, const Checkpoint &c) {
+    const size_t pl = (size_t)h->P.plane;
+    const size_t pc = (size_t)PCOUNT * h->P.maxp;
+    h->pcur = c.pcur; h->dcur = c.dcur; h->gen_base = c.gen;
+    const double *d = src;
+    HIPCHECK(h, hipMemcpyAsync(h->f, d, 9 * pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += 9 * pl;
+    HIPCHECK(h, hipMemcpyAsync(h->rho, d, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(h->ux, d, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(h->uy, d, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(h->g, d, 2 * pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += 2 * pl;
+    HIPCHECK(h, hipMemcpyAsync(h->obn, d, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
+    HIPCHECK(h, hipMemcpyAsync(h->pbuf[h->pcur], d, pc * 8, hipMemcpyDeviceToDevice, h->stream)); d += pc;
+    HIPCHECK(h, hipMemcpyAsync(h->dpw[h->dcur], d, 8, hipMemcpyDeviceToDevice, h->stream));
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" int lbm_adjoint_velocity(lbm_handle *h, int32_t nsteps, int32_t window, int32_t chk,
+                                    double *loss, double *grad, double *mem_bytes) {
+    if (!h || nsteps < 1 || window < 1 || window > nsteps || chk < 1 || !loss || !grad)
+        return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    const lbm_params &pr = h->prm;
+    if (pr.storage != LBM_FP64 && pr.storage != LBM_FP32)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint recomputes from FP64 or FP32 storage only");
+    if (P.np != 1) return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint follows exactly one particle");
+    if (P.motion != LBM_MOTION_FREE)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs the free (Newton-Euler) particle update");
+    for (int k = 0; k < 8; ++k)
+        if (P.wall[k] != 0.0) return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs walls at rest");
+    if (P.corner[0] != 0.0 || P.corner[1] != 0.0)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs walls at rest");
+    int rc = check_fused_mode(h);
+    if (rc) return rc;
+
+    const size_t pl = (size_t)P.plane;
+    const size_t pc = (size_t)PCOUNT * P.maxp;
+    const size_t ck_doubles = CK_PLANES * pl + pc + 1;
+    const size_t snap_doubles = 10 * pl + pc;  // f_k (9), om_k, P_k
+    const int nck = (nsteps + chk - 1) / chk;
+    const unsigned nb = nblk(P.nn);
+    size_t total = 0;
+    DevBuf ckb, segb, lf[2], lw[2], mu[2], fbar, theta, partial, dstmap, ids, zero;
+    if ((rc = adj_alloc(h, ckb, (size_t)(nck + 1) * ck_doubles * 8, total)) ||
+        (rc = adj_alloc(h, segb, (size_t)chk * snap_doubles * 8, total)) ||
+        (rc = adj_alloc(h, lf[0], 9 * pl * 8, total)) || (rc = adj_alloc(h, lf[1], 9 * pl * 8, total)) ||
+        (rc = adj_alloc(h, lw[0], pl * 8, total)) || (rc = adj_alloc(h, lw[1], pl * 8, total)) ||
+        (rc = adj_alloc(h, mu[0], ANP * 8, total)) || (rc = adj_alloc(h, mu[1], ANP * 8, total)) ||
+        (rc = adj_alloc(h, fbar, 3 * 8, total)) || (rc = adj_alloc(h, theta, 3 * 8, total)) ||
+        (rc = adj_alloc(h, partial, (size_t)nb * AR * 8, total)) ||
+        (rc = adj_alloc(h, dstmap, 9 * pl * 4, total)) || (rc = adj_alloc(h, ids, 9 * pl * 8, total)) ||
+        (rc = adj_alloc(h, zero, (size_t)P.nbnd * 8, total)))
+        return rc;
+    double *ck = (double *)ckb.p, *seg = (double *)segb.p;
+
+    // inverse stream map (a permutation when the walls are at rest)
+    HIPCHECK(h, hipMemsetAsync(zero.p, 0, (size_t)P.nbnd * 8, h->stream));
+    HIPCHECK(h, hipMemsetAsync(ids.p, 0, 9 * pl * 8, h->stream));
+    hipLaunchKernelGGL(k_adj_ids, dim3(nb), dim3(NT), 0, h->stream, P, (double *)ids.p);
+    hipLaunchKernelGGL(k_adj_srcmap, dim3(nb), dim3(NT), 0, h->stream, P, (const double *)ids.p,
+                       (const double *)zero.p, (int *)dstmap.p);
+    if ((rc = check_launch(h, "k_adj_srcmap"))) return rc;
+    {
+        std::vector<int> src(9 * pl), dst(9 * pl, -1);
+        HIPCHECK(h, hipMemcpyAsync(src.data(), dstmap.p, 9 * pl * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        for (int k = 0; k < 9; ++k)
+            for (long long x = 0; x < P.nn; ++x) {
+                long long sid = src[(size_t)k * pl + x];
+                long long q = sid / P.nn, y = sid - q * P.nn;
+                size_t at = (size_t)q * pl + (size_t)y;
+                if (q < 0 || q > 8 || dst[at] != -1)
+                    return fail(h, LBM_ERR_UNSUPPORTED, "stream map is not a permutation");
+                dst[at] = (int)((size_t)k * pl + (size_t)x);
+            }
+        // padding entries point at a zero adjoint slot of their own plane
+        for (int q = 0; q < 9; ++q)
+            for (size_t y = (size_t)P.nn; y < pl; ++y) dst[(size_t)q * pl + y] = (int)((size_t)q * pl + y);
+        HIPCHECK(h, hipMemcpy(dstmap.p, dst.data(), 9 * pl * 4, hipMemcpyHostToDevice));
+    }
+
+    // forward pass with checkpoints; the loss from the history rows
+    std::vector<Checkpoint> cks(nck + 1);
+    const size_t rowp = (size_t)HPC + 2;
+    std::vector<double> hist((size_t)chk * rowp);
+    double lsum = 0.0;
+    for (int c = 0; c < nck; ++c) {
+        const int k0 = c * chk, n = std::min(chk, nsteps - k0);
+        cks[c].off = (size_t)c * ck_doubles;
+        if ((rc = save_state(h, ck + cks[c].off, cks[c]))) return rc;
+        if ((rc = lbm_step(h, k0, n, hist.data()))) return rc;
+        for (int s = 0; s < n; ++s) {
+            const int k = k0 + s + 1;
+            if (k > nsteps - window) lsum += hist[(size_t)s * rowp + LBM_H_VY];
+        }
+    }
+    cks[nck].off = (size_t)nck * ck_doubles;
+    if ((rc = save_state(h, ck + cks[nck].off, cks[nck]))) return rc;
+    *loss = lsum / window;
+
+    // backward pass
+    adj::LocalP L;
+    L.rheology = P.rheology; L.q_mode = P.q_mode;
+    L.s0 = P.s0; L.s1 = P.s1; L.s2 = P.s2; L.s3 = P.s3; L.s5 = P.s5; L.sq = P.sq;
+    L.mu0 = P.mu0; L.tau_y = P.tau_y; L.m_pap = P.m_pap; L.gamma_min = P.gamma_min;
+    L.d_mrt = D_MRT;
+    for (int b = 0; b < 2; ++b) {
+        HIPCHECK(h, hipMemsetAsync(lf[b].p, 0, 9 * pl * 8, h->stream));
+        HIPCHECK(h, hipMemsetAsync(lw[b].p, 0, pl * 8, h->stream));
+        HIPCHECK(h, hipMemsetAsync(mu[b].p, 0, ANP * 8, h->stream));
+    }
+    HIPCHECK(h, hipMemsetAsync(theta.p, 0, 3 * 8, h->stream));
+    {
+        // the loss term of step N seeds the particle adjoint
+        double seed[ANP] = {0, 0, 0, 1.0 / window, 0, 0, 0, 0};
+        HIPCHECK(h, hipMemcpy(mu[0].p, seed, sizeof(seed), hipMemcpyHostToDevice));
+    }
+    int cur = 0;  // buffers [cur] hold the adjoint of the later state
+    std::vector<double> fcol(2 * P.maxp);
+    for (int c = nck - 1; c >= 0; --c) {
+        const int k0 = c * chk, k1 = std::min(k0 + chk, nsteps);
+        if ((rc = restore_state(h, ck + cks[c].off, cks[c]))) return rc;
+        for (int k = k0; k < k1; ++k) {
+            if (k > k0 && (rc = lbm_step(h, k - 1, 1, nullptr))) return rc;
+            double *sn = seg + (size_t)(k - k0) * snap_doubles;
+            HIPCHECK(h, hipMemcpyAsync(sn, h->f, 9 * pl * 8, hipMemcpyDeviceToDevice, h->stream));
+            HIPCHECK(h, hipMemcpyAsync(sn + 9 * pl, h->obn, pl * 8, hipMemcpyDeviceToDevice, h->stream));
+            HIPCHECK(h, hipMemcpyAsync(sn + 10 * pl, h->pbuf[h->pcur], pc * 8, hipMemcpyDeviceToDevice,
+                                       h->stream));
+            // the contact force is not differentiated: it must stay zero
+            HIPCHECK(h, hipMemcpyAsync(fcol.data(), h->pbuf[h->pcur] + (size_t)PFCX * P.maxp,
+                                       2 * P.maxp * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHECK(h, hipStreamSynchronize(h->stream));
+            if (fcol[0] != 0.0 || fcol[P.maxp] != 0.0)
+                return fail(h, LBM_ERR_UNSUPPORTED,
+                            "the particle came within contact range; the adjoint treats the "
+                            "contact force as absent");
+        }
+        for (int k = k1 - 1; k >= k0; --k) {
+            const double *sn = seg + (size_t)(k - k0) * snap_doubles;
+            const double *rec = sn + 10 * pl;
+            const int nx = cur ^ 1;
+            hipLaunchKernelGGL(k_adj_particle, dim3(1), dim3(64), 0, h->stream, P, rec,
+                               (const double *)mu[cur].p, (double *)mu[nx].p, (double *)fbar.p,
+                               (double *)theta.p);
+            hipLaunchKernelGGL(k_adj_node, dim3(nb), dim3(NT), 0, h->stream, P, L,
+                               (const double *)lf[cur].p, (const double *)lw[cur].p,
+                               (const int *)dstmap.p, sn, sn + 9 * pl, rec,
+                               (const double *)fbar.p, (double *)lf[nx].p, (double *)lw[nx].p,
+                               (double *)partial.p);
+            const double lw_k = (k >= 1 && k > nsteps - window) ? 1.0 / window : 0.0;
+            hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(NT), 0, h->stream,
+                               (const double *)partial.p, (int)nb, (double *)mu[nx].p,
+                               (double *)theta.p, lw_k);
+            if ((rc = check_launch(h, "adjoint step"))) return rc;
+            cur = nx;
+        }
+    }
+    if (P.rheology == 1)
+        hipLaunchKernelGGL(k_adj_omega0, dim3(1), dim3(NT), 0, h->stream, P,
+                           (const double *)lw[cur].p, (double *)theta.p);
+    HIPCHECK(h, hipMemcpyAsync(grad, theta.p, 3 * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    // leave the handle in the state after nsteps
+    if ((rc = restore_state(h, ck + cks[nck].off, cks[nck]))) return rc;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (mem_bytes) *mem_bytes = (double)total;
+    return LBM_OK;
+}

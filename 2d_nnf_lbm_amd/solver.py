@@ -243,6 +243,18 @@ class D2Q9:
         self._call("lbm_step_timed", int(kk0), int(nsteps), ctypes.byref(t), ctypes.byref(m))
         return t.value, m.value
 
+    def adjoint_velocity(self, nsteps, window, checkpoint):
+        """Loss = mean V_y of the particle over the last `window` of nsteps
+        fused steps from the current state, and its gradient with respect to
+        (tau_y, mu0, rho_p) by the discrete adjoint with a checkpoint every
+        `checkpoint` steps.  Returns (loss, grad[3], device bytes used)."""
+        loss = ctypes.c_double(0.0)
+        mem = ctypes.c_double(0.0)
+        grad = np.zeros(3)
+        self._call("lbm_adjoint_velocity", int(nsteps), int(window), int(checkpoint),
+                   ctypes.byref(loss), _ptr(grad), ctypes.byref(mem))
+        return loss.value, grad, mem.value
+
     def last_pairs(self, max_pairs=4096):
         n = ctypes.c_int32(0)
         buf = np.zeros(2 * max_pairs, dtype=np.int32)

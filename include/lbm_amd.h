@@ -160,6 +160,24 @@ int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
 int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps,
                    double *ms_total, double *ms_main);
 
+/* ---------------------------------------------------------------- adjoint
+ * Sensitivity of the settling velocity of one disk (the reference has no
+ * adjoint; this entry serves parameter studies).  Runs nsteps fused steps
+ * from the handle's current state and returns
+ *   loss = (1/window) * sum over the last `window` steps of V_y,
+ *   grad = d loss / d (tau_y, mu0, rho_p),
+ * by the discrete adjoint of the step (FP64 arithmetic) with a checkpoint
+ * every `chk` steps and recomputation of each segment, so device memory is
+ * O(nsteps/chk + chk) states.  mu0 enters as the Papanastasiou plastic
+ * viscosity and through OMEGA = 1/(3 mu0 + 1/2), the initial node rate;
+ * rho_p moves the mass m = rho_p pi r^2 and inertia m r^2 / 2 with it.
+ * Requires FP64 or FP32 storage, one particle, free motion and walls at rest;
+ * returns LBM_ERR_UNSUPPORTED if the particle reaches contact range.  The
+ * handle ends in the state after nsteps.  mem_bytes (may be NULL) receives
+ * the device memory the adjoint allocated. */
+int lbm_adjoint_velocity(lbm_handle *h, int32_t nsteps, int32_t window, int32_t chk,
+                         double *loss, double *grad, double *mem_bytes);
+
 /* ------------------------------------------------------------------ batch
  * Many independent simulations (grid sizes, rheology, walls and particle
  * motion may differ; one storage format) stepped together: one launch per
