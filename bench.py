@@ -108,6 +108,29 @@ def measure_ensemble(pkg, configs, steps):
                 tune_seconds=tune_s)
 
 
+def measure_adjoint(pkg, configs, steps):
+    """Gradient of the mean settling velocity over the last 512 steps with
+    respect to (tau_y, mu0, rho_p): one disk, 513x1025 nodes, FP32 storage,
+    checkpoints every 64 steps."""
+    cfg, part = configs.settling_adjoint("fp32")
+    s = pkg.D2Q9(cfg)
+    s.initialize()
+    s.set_particles(part)
+    window, chk = min(512, steps), 64
+    t0 = time.perf_counter()
+    loss, grad, mem = s.adjoint_velocity(steps, window, chk)
+    wall = time.perf_counter() - t0
+    s.close()
+    nodes = configs.nodes(cfg)
+    return dict(workload="one disk (D=32, density ratio 1.2) settling in a 513x1025-node "
+                         "Papanastasiou-Bingham channel; loss = mean V_y over the last %d steps"
+                         % window,
+                steps=steps, checkpoint_every=chk, seconds=wall,
+                forward_equivalent_mlups=nodes * steps / wall / 1e6,
+                loss=loss, grad_tau_y=grad[0], grad_mu0=grad[1], grad_rho_p=grad[2],
+                device_bytes=mem)
+
+
 def accuracy(pkg, configs, steps):
     """FP16-stored vs FP64 on the same run: terminal velocity, drag
     coefficient and yielded-area fraction after `steps` steps."""
@@ -184,6 +207,8 @@ def main():
                     help="timed steps of the 504-disk BF16 cavity (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=40,
                     help="timed steps of the 256-cavity FP16 batch (0 skips it)")
+    ap.add_argument("--adjoint-steps", type=int, default=4096,
+                    help="steps of the settling-velocity adjoint run (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of 9 FP64 planes (known byte "
                          "count) so hardware traffic counters can be scaled")
@@ -224,6 +249,10 @@ def main():
     if args.ensemble_steps > 0:
         log("bench: 256-cavity batch, fp16")
         ensemble = measure_ensemble(pkg, configs, args.ensemble_steps)
+    adjoint = None
+    if args.adjoint_steps > 0:
+        log("bench: settling-velocity adjoint, fp32")
+        adjoint = measure_adjoint(pkg, configs, args.adjoint_steps)
     log("bench: accuracy leg")
     acc = accuracy(pkg, configs, args.accuracy_steps) if args.accuracy_steps > 0 else None
     cpu = None
@@ -277,6 +306,7 @@ def main():
         "fp16_over_fp64": r16["mlups"] / r64["mlups"],
         "dense_sedimentation_bf16": dense,
         "cavity_ensemble_fp16": ensemble,
+        "adjoint_fp32": adjoint,
         "accuracy_fp16_vs_fp64": acc,
     }
     print(json.dumps(line), flush=True)
