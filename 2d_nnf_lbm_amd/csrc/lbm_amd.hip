@@ -54,6 +54,7 @@ struct lbm_handle {
     unsigned long long *tl_hdr[2] = {nullptr, nullptr};
     int *tl_ids[2] = {nullptr, nullptr};
     int *err_flag = nullptr;
+    double *lag = nullptr;  // module-global rate of the lagged collide
     // split force reduction: blocks per particle, partials and counters
     int nsub = 1;
     double *pforce = nullptr;
@@ -204,7 +205,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->porig, (size_t)P.maxp * 2 * 4)) ||
         (rc = al((void **)&h->partial, (size_t)nblk(P.nn) * 8)) ||
         (rc = al((void **)&h->scalar, 8)) ||
-        (rc = al((void **)&h->err_flag, 4)) ||
+        (rc = al((void **)&h->err_flag, 4)) || (rc = al((void **)&h->lag, 8)) ||
         (rc = al((void **)&h->pforce, (size_t)P.maxp * MAXSUB * 3 * 8)) ||
         (rc = al((void **)&h->pcount, (size_t)P.maxp * 4)) ||
         (rc = al((void **)&h->spf[0], (size_t)P.maxp * SPF * 4)) ||
@@ -248,7 +249,7 @@ int lbm_destroy(lbm_handle *h) {
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
-                    h->pforce, h->pcount, h->spf[0], h->spf[1]};
+                    h->pforce, h->pcount, h->spf[0], h->spf[1], h->lag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -283,6 +284,8 @@ int lbm_initialize(lbm_handle *h) {
     double inf = INFINITY;
     HIPCHECK(h, hipMemcpyAsync(h->dpw[0], &inf, 8, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(h, hipMemcpyAsync(h->dpw[1], &inf, 8, hipMemcpyHostToDevice, h->stream));
+    // the module-global rate starts from OMEGA (F6:199)
+    HIPCHECK(h, hipMemcpyAsync(h->lag, &P.omega, 8, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     h->dcur = 0;
     return LBM_OK;
@@ -433,11 +436,12 @@ int lbm_get_phi(lbm_handle *h, int32_t ip, double *phi) {
 // --------------------------------------------------------------- drop-in
 int lbm_collide(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
-    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
-        return fail(h, LBM_ERR_UNSUPPORTED,
-                    "the lagged module-global rate is a sequential recurrence; the GPU path "
-                    "implements rate_mode LBM_RATE_LOCAL");
     const DevP &P = h->P;
+    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED) {
+        hipLaunchKernelGGL(k_collide_lagged, dim3(1), dim3(64), 0, h->stream, P, h->f, h->rho,
+                           h->ux, h->uy, h->g, h->obn, h->stress, h->image, h->lag);
+        return check_launch(h, "k_collide_lagged");
+    }
     hipLaunchKernelGGL(k_collide, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->rho,
                        h->ux, h->uy, h->g, h->obn, h->stress, h->image);
     return check_launch(h, "k_collide");

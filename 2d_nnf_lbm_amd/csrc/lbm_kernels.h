@@ -66,6 +66,35 @@ __global__ void k_collide(DevP P, double *f, const double *rho, const double *ux
     }
 }
 
+// Collision_MRT with the reference's module-global rate (F6:890, F6:958):
+// each node collides with the rate the node before it set, in the reference's
+// node order (J outer, I inner), which makes the pass one sequential chain.
+// One thread walks the grid; *lag carries the rate from call to call (the
+// module variable OMEGA_MRT, F6:58).  A fidelity mode for drop-in runs of
+// the unmodified reference; the fused step uses each node's own rate.
+__global__ void k_collide_lagged(DevP P, double *f, const double *rho, const double *ux,
+                                 const double *uy, const double *g, double *obn,
+                                 double *stress, int *image, double *lag) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double om = *lag;
+    for (long long n = 0; n < P.nn; ++n) {
+        double v[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = f[k * P.plane + n];
+        CollideOut o;
+        collide64(P, v, rho[n], ux[n], uy[n], g[n], g[P.plane + n], om, o);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k * P.plane + n] = v[k];
+        stress[n] = o.stress[0];
+        stress[P.plane + n] = o.stress[1];
+        stress[2 * P.plane + n] = o.stress[2];
+        image[n] = o.yielded ? 0 : 5;
+        obn[n] = o.omega_bn;
+        om = o.omega_bn;
+    }
+    *lag = om;
+}
+
 // STREAMING, F6:1164-1188.
 __global__ void k_stream(DevP P, double *f, const double *fp) {
     long long n = (long long)blockIdx.x * NT + threadIdx.x;

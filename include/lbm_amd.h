@@ -126,7 +126,11 @@ int lbm_get_particles(lbm_handle *h, double *part);
 int lbm_get_phi(lbm_handle *h, int32_t ip, double *phi);
 
 /* Drop-in per-subroutine entries (FP64 state, reference semantics). */
-int lbm_collide(lbm_handle *h);           /* Collision_MRT, F6:867-962 */
+/* Collision_MRT, F6:867-962.  With LBM_BINGHAM and LBM_RATE_LAGGED it keeps
+ * the reference's module-global rate (each node collides with the rate the
+ * node before it set, F6:958), which is a sequential chain over the grid:
+ * one GPU thread walks it, so use it for drop-in fidelity on small grids. */
+int lbm_collide(lbm_handle *h);
 int lbm_copy_post(lbm_handle *h);         /* F_POST = F, F6:329 */
 int lbm_stream(lbm_handle *h);            /* STREAMING, F6:1154-1190 */
 int lbm_boundary(lbm_handle *h, int32_t kk);  /* BOUNDARYCONDITION, F6:1195-1390 */

@@ -27,6 +27,13 @@ Patches (each is a parameter or syntax edit, none changes the algorithm):
 A second recipe builds the N-body contact force alone: MODULE PARAM
 (F3:2-83) and SUBROUTINE ComputeCollisionForce (F3:632-806) cut from F3,
 compiled with oracle/f3_contact_driver.f90 into oracle/_ref/f3_contact.
+A third recipe builds the reference's own MAIN program on the GPU: F6
+(patched as its variant) with the nine time-step subroutines that
+2d_nnf_lbm_amd/fortran/lbm_f6_shim.F90 replaces cut out, compiled with that
+shim and the LBM_AMD module and linked to liblbm_amd.so, into
+oracle/_ref/f6_shim_<variant>.  The MAIN program and every other routine
+are the reference's, unchanged.
+
 It is compiled with -fno-automatic, which gives the routine's local variables
 the SAVE attribute: the pair branch (F3:760) reads D_PW before this call
 assigns it, so it sees the value the previous call left (the "stale D_PW"
@@ -173,6 +180,60 @@ def build_f3_contact(flags):
     return exe
 
 
+SHIM_ROUTINES = ["Collision_MRT", "STREAMING", "BOUNDARYCONDITION", "MACROSCOPICVARIABLES",
+                 "ComputeConcenterationFunction", "ComputeForceField_SPM",
+                 "ComputeForce_HYDRO", "VelocityCorrector", "ComputeCollisionForce"]
+# variant -> extra preprocessor definitions of the shim
+SHIM_VARIANTS = {
+    "f6_bing_lagged": [],
+    "f6_newt_lid": ["-DLBM_SHIM_UXN=0.01D0"],
+    "f6_newt_mohammad": [],
+}
+PKG = os.path.join(os.path.dirname(HERE), "2d_nnf_lbm_amd")
+
+
+def drop_routines(text, names):
+    """The source without the listed subroutines (SUBROUTINE name( through
+    its END SUBROUTINE or bare END line)."""
+    out, skip = [], False
+    for line in text.split("\n"):
+        if not skip:
+            m = re.match(r"\s*SUBROUTINE\s+(\w+)\s*\(", line, re.I)
+            if m and m.group(1).lower() in [n.lower() for n in names]:
+                skip = True
+                continue
+            out.append(line)
+        elif re.match(r"\s*END(\s+SUBROUTINE\b.*)?\s*(!.*)?$", line, re.I):
+            skip = False
+    return "\n".join(out) + "\n"
+
+
+def build_shim(name, cfg, defines, flags):
+    lib = os.path.join(PKG, "liblbm_amd.so")
+    if not os.path.exists(lib):
+        raise RuntimeError("build liblbm_amd.so before the shim programs")
+    with open(F6, "rb") as fh:
+        text = fh.read().decode("ascii")
+    src = drop_routines(patch_source(text, **cfg), SHIM_ROUTINES)
+    tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")
+    try:
+        path = os.path.join(tmpdir, name + "_main.f90")
+        with open(path, "w") as fh:
+            fh.write(src)
+        exe = os.path.join(OUT_DIR, "f6_shim_" + name[3:])
+        cmd = ([FLANG] + flags + ["-cpp"] + defines + ["-J", tmpdir,
+               os.path.join(PKG, "fortran", "lbm_amd.f90"), path,
+               os.path.join(PKG, "fortran", "lbm_f6_shim.F90"),
+               "-L" + PKG, "-llbm_amd", "-Wl,-rpath,$ORIGIN/../../2d_nnf_lbm_amd", "-o", exe])
+        r = subprocess.run(cmd, cwd=tmpdir, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True)
+        if r.returncode:
+            raise RuntimeError("shim build failed:\n" + r.stdout[-3000:])
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    return exe
+
+
 def main(argv):
     if not os.path.exists(F6):
         print("build_ref: reference source not present; nothing to build")
@@ -184,6 +245,9 @@ def main(argv):
         built.append(build(name, cfg, parity_flags))
     if os.path.exists(F3):
         built.append(build_f3_contact(parity_flags + ["-fno-automatic"]))
+    if os.path.exists(os.path.join(PKG, "liblbm_amd.so")):
+        for name, defines in SHIM_VARIANTS.items():
+            built.append(build_shim(name, VARIANTS[name], defines, parity_flags))
     if "--timing" in argv or "--all" in argv:
         for name, cfg in TIMING_VARIANTS.items():
             built.append(build(name, cfg, timing_flags))
