@@ -678,23 +678,18 @@ __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho
     om_out = om;
     yielded = 0;
     if (!OUT && P.rheology != 1) return;
-    // Chai strain rate (F6:903-916); only f1[1..8] enter it
-    float e0 = 4.0f * s[0] * mne[0], e1 = s[1] * mne[1], e2 = s[2] * mne[2];
-    float e3 = 6.0f * s[3] * mne[3], e4 = s[4] * mne[4], e5 = 6.0f * s[5] * mne[5];
-    float e6 = s[6] * mne[6], e7 = 9.0f * s[7] * mne[7], e8 = 9.0f * s[8] * mne[8];
-    float ba = e0 - e1 - 2.0f * e2, bb = e0 + 2.0f * e1 + e2;
-    float f1_1 = ba + e3 - 6.0f * e4 + e7, f1_2 = ba + e5 - 6.0f * e6 - e7;
-    float f1_3 = ba - e3 + 6.0f * e4 + e7, f1_4 = ba - e5 + 6.0f * e6 - e7;
-    float q3 = e3 + 3.0f * e4, q5 = e5 + 3.0f * e6;
-    float f1_5 = bb + q3 + q5 + e8, f1_6 = bb - q3 + q5 - e8;
-    float f1_7 = bb - q3 - q5 + e8, f1_8 = bb + q3 - q5 - e8;
-    float zarib = -1.5f * d;
-    float gu = gx * ux + gy * uy;
-    float d11 = zarib * (f1_1 + f1_5 + f1_6 + f1_3 + f1_7 + f1_8)
-              + 0.75f * ((s[7] - s[1]) * gu - s[7] * (2.0f * gx * ux));
-    float d22 = zarib * (f1_5 + f1_2 + f1_6 + f1_7 + f1_4 + f1_8)
-              + 0.75f * ((s[7] - s[1]) * gu - s[7] * (2.0f * gy * uy));
-    float d12 = zarib * (f1_5 - f1_6 + f1_7 - f1_8) - 0.75f * (s[7] * (gx * uy + gy * ux));
+    // Chai strain rate (F6:903-924).  The strain moments f1 = C (s * mne)
+    // enter d11, d22, d12 only through three column sums of C, which leave
+    //   d11 = -(e0 + e1/4 + 3 e7/4), d22 = -(e0 + e1/4 - 3 e7/4), d12 = -3 e8/2
+    // (e_k = s_k mne_k) plus the forcing terms: the same quantities as the
+    // reference's sums, in 8 operations instead of about 50.
+    const float e0 = s[0] * mne[0], e1 = s[1] * mne[1], e7 = s[7] * mne[7], e8 = s[8] * mne[8];
+    const float gu = gx * ux + gy * uy;
+    const float base = -(e0 + 0.25f * e1);
+    const float gs = 0.75f * ((s[7] - s[1]) * gu);
+    float d11 = (base - 0.75f * e7) + (gs - 0.75f * (s[7] * (2.0f * gx * ux)));
+    float d22 = (base + 0.75f * e7) + (gs - 0.75f * (s[7] * (2.0f * gy * uy)));
+    float d12 = -1.5f * e8 - 0.75f * (s[7] * (gx * uy + gy * ux));
     if (OUT) {
         float nu = P.fnu;
         stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
