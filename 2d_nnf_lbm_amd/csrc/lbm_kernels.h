@@ -746,23 +746,31 @@ __device__ void particle_block(const StepArgs &a, int pb) {
         particle_force<St, DEV>(a, p, sub, a.nsub, F);  // ends with block barriers
         if (a.nsub > 1) {
             // partial of this block; the last block of the particle adds all
-            // partials in sub order and carries on with the update
+            // partials in sub order and carries on with the update.
+            // The partials are written and read with agent-scope atomic
+            // accesses, which bypass the per-XCD L2, and the counter add is
+            // issued only after those stores have completed (vmcnt drained).
+            // A __threadfence() here would instead write back every dirty
+            // line of this XCD's L2 (the tile stores of the whole launch) on
+            // every particle block; that cost ~40 us per step at 504 disks.
             __shared__ int last;
             if (threadIdx.x == 0) {
                 double *pf = a.pforce + ((size_t)p * a.nsub + sub) * 3;
-                pf[0] = F[0]; pf[1] = F[1]; pf[2] = F[2];
-                __threadfence();
-                last = atomicAdd(a.pcount + p, 1) == a.nsub - 1;
+                for (int c = 0; c < 3; ++c)
+                    __hip_atomic_store(pf + c, F[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                last = __hip_atomic_fetch_add(a.pcount + p, 1, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT) == a.nsub - 1;
                 if (last) {
-                    __threadfence();
                     const double *q = a.pforce + (size_t)p * a.nsub * 3;
                     F[0] = 0.0; F[1] = 0.0; F[2] = 0.0;
-                    for (int s = 0; s < a.nsub; ++s) {
-                        F[0] += __builtin_nontemporal_load(q + 3 * s);
-                        F[1] += __builtin_nontemporal_load(q + 3 * s + 1);
-                        F[2] += __builtin_nontemporal_load(q + 3 * s + 2);
-                    }
-                    a.pcount[p] = 0;
+                    for (int s = 0; s < a.nsub; ++s)
+                        for (int c = 0; c < 3; ++c)
+                            F[c] += __hip_atomic_load(q + 3 * s + c, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(a.pcount + p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
             __syncthreads();
