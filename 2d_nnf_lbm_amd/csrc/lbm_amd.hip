@@ -152,6 +152,14 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.fs3 = (float)P.s3; P.fs5 = (float)P.s5; P.fsq = (float)P.sq; P.fnu = (float)P.nu;
     P.fmu0 = (float)P.mu0; P.ftau_y = (float)P.tau_y; P.fm_pap = (float)P.m_pap;
     P.fgamma_min = (float)P.gamma_min; P.frho0 = (float)P.rho0;
+    // FP32 collide factors (D_k / N_k = n_k cDka + mf_k cDkb): mf_1 = 6 fu,
+    // mf_2 = -6 fu, mf_3 = fx, mf_4 = -fx, mf_5 = fy, mf_6 = -fy
+    P.cD1a = (float)(-P.s1 / 36.0); P.cD1b = (float)((1.0 - 0.5 * P.s1) * 6.0 / 36.0);
+    P.cD2a = (float)(-P.s2 / 36.0); P.cD2b = (float)(-(1.0 - 0.5 * P.s2) * 6.0 / 36.0);
+    P.cD3a = (float)(-P.s3 / 6.0); P.cD3b = (float)((1.0 - 0.5 * P.s3) / 6.0);
+    P.cD5a = (float)(-P.s5 / 6.0); P.cD5b = (float)((1.0 - 0.5 * P.s5) / 6.0);
+    P.cDq1 = (float)(-P.sq / 12.0); P.cDq2 = (float)(-(1.0 - 0.5 * P.sq) / 12.0);
+    P.cS1 = (float)(-0.25 * P.s1); P.cS2 = (float)(-0.75 * P.s1);
     {
         unsigned long long bb = 9ULL * (unsigned long long)P.plane *
                                 (p->storage == LBM_FP64 ? 8ULL : p->storage == LBM_FP32 ? 4ULL : 2ULL);

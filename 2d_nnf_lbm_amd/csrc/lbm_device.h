@@ -54,6 +54,10 @@ struct DevP {
     int w_shift;
     // FP32 copies of the parameters used by the reduced-precision path
     float fomega, fs0, fs1, fs2, fs3, fs5, fsq, fnu, fmu0, ftau_y, fm_pap, fgamma_min, frho0;
+    // relaxation factors of the FP32 collide: -s_k / N_k and (1 - s_k/2) c_k / N_k
+    // for the uniform rates (forcing-moment constants c_k folded in), the
+    // fixed-q factors, and the strain-rate factors -s_1/4 and -0.75 s_1
+    float cD1a, cD1b, cD2a, cD2b, cD3a, cD3b, cD5a, cD5b, cDq1, cDq2, cS1, cS2;
     unsigned buf_bytes;  // bytes of one 9-plane storage buffer (< 2^32)
 };
 
@@ -105,11 +109,17 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 // st(p, v, r): r is a per-value random word for stochastic rounding (FP16,
 // BF16), whose high bits are used.
 __device__ __forceinline__ uint32_t sr_word(uint32_t node_hash, int k) {
-    // distinct odd multipliers spread one node hash over the 9 populations
-    const uint32_t K[9] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu, 0x165667B1u,
-                           0xD3A2646Bu, 0xFD7046C5u, 0xB55A4F09u, 0x8F5A3C2Bu};
-    return node_hash * K[k];
+    // distinct odd 24-bit multipliers spread one node hash over the 9
+    // populations; the 24 x 24-bit product (low 32 bits) issues at full
+    // rate, a 32-bit integer multiply at quarter rate
+    const uint32_t K[9] = {0x3779B1u, 0xEBCA77u, 0xB2AE3Du, 0xD4EB2Fu, 0x5667B1u,
+                           0xA2646Bu, 0x7046C5u, 0x5A4F09u, 0x5A3C2Bu};
+    return __umul24(node_hash, K[k]);
 }
+// The conversions below store only the low half of their result; the other
+// half passes through from the input register itself (cvt_sr writes one half
+// of its destination, which is tied to that input), so no register is
+// cleared for it.
 struct StF64 {
     using T = double;
     __device__ static double ld(const T *p) { return *p; }
@@ -146,7 +156,8 @@ struct StF16 {
     }
     __device__ static void bst(__amdgpu_buffer_rsrc_t rs, float v, unsigned vo, unsigned so,
                                uint32_t r) {
-        h2 w = __builtin_amdgcn_cvt_sr_f16_f32((h2){}, v * 1024.0f, r, false);
+        const float sv = v * 1024.0f;
+        h2 w = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, sv), sv, r, false);
         __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, w[0]), rs,
                                               (int)vo, (int)so, 0);
     }
@@ -173,7 +184,7 @@ struct StBF16 {
     }
     __device__ static void bst(__amdgpu_buffer_rsrc_t rs, float v, unsigned vo, unsigned so,
                                uint32_t r) {
-        b2 w = __builtin_amdgcn_cvt_sr_bf16_f32((b2){}, v, r, false);
+        b2 w = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(b2, v), v, r, false);
         __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, w[0]), rs,
                                               (int)vo, (int)so, 0);
     }
@@ -626,70 +637,85 @@ __device__ __forceinline__ float fexp(float x) {
 // Collision_MRT in deviation form and FP32 (same physics as collide64).
 // OUT: also produce the stress tensor and the yield flag (output only); the
 // Newtonian rate needs neither, the Papanastasiou rate needs the strain rate.
+//
+// Written as a change of the populations: with n_k = m_k - meq_k the
+// post-collision moments differ from m_k by D_k = -s_k n_k + (1 - s_k/2) mf_k
+// (D_0 = 0: n_0 = mf_0 = 0), and since the rows of the moment matrix M are
+// orthogonal, M^-1 = M^T diag(1/N) with N = (9, 36, 36, 6, 12, 6, 12, 4, 4),
+// so h* = h + M^T (D / N).  The uniform factors s_k / N_k and
+// (1 - s_k/2) / N_k come precomputed in DevP (cD*).
 template <bool OUT>
 __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho_dev, float rho,
                                             float ux, float uy, float gx, float gy, float om,
                                             float &om_out, float stress[3], int &yielded) {
 #pragma clang fp contract(on)  // fused within one expression: the same in every kernel
-    float q = P.q_mode ? (8.0f * (2.0f - om)) * frcp(8.0f - om) : P.fsq;
-    float s[9] = {P.fs0, P.fs1, P.fs2, P.fs3, q, P.fs5, q, om, om};
-    float m[9], mf[9], meq[9], mne[9];
-    m[0] = rho_dev;
-    m[1] = -4.0f * h[0] - (h[1] + h[2] + h[3] + h[4]) + 2.0f * (h[5] + h[6] + h[7] + h[8]);
-    m[2] = 4.0f * h[0] - 2.0f * (h[1] + h[2] + h[3] + h[4]) + (h[5] + h[6] + h[7] + h[8]);
-    m[3] = (h[1] - h[3]) + (h[5] - h[6]) - (h[7] - h[8]);
-    m[4] = -2.0f * (h[1] - h[3]) + (h[5] - h[6]) - (h[7] - h[8]);
-    m[5] = (h[2] - h[4]) + (h[5] + h[6]) - (h[7] + h[8]);
-    m[6] = -2.0f * (h[2] - h[4]) + (h[5] + h[6]) - (h[7] + h[8]);
-    m[7] = (h[1] - h[2]) + (h[3] - h[4]);
-    m[8] = (h[5] - h[6]) + (h[7] - h[8]);
-    float fx = rho * gx, fy = rho * gy;
-    float fu = fx * ux + fy * uy;
-    mf[0] = 0.0f; mf[1] = 6.0f * fu; mf[2] = -6.0f * fu;
-    mf[3] = fx; mf[4] = -fx; mf[5] = fy; mf[6] = -fy;
-    mf[7] = 2.0f * (fx * ux - fy * uy);
-    mf[8] = fy * ux + fx * uy;
-    float u2 = ux * ux + uy * uy;
-    // equilibrium moments relative to the rest state (rho0, -2 rho0, rho0, 0...)
-    meq[0] = rho_dev;
-    meq[1] = -2.0f * rho_dev + 3.0f * rho * u2;
-    meq[2] = rho_dev - 3.0f * rho * u2;
-    meq[3] = rho * ux; meq[4] = -rho * ux;
-    meq[5] = rho * uy; meq[6] = -rho * uy;
-    meq[7] = rho * (ux * ux - uy * uy);
-    meq[8] = rho * (ux * uy);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) mne[k] = m[k] - meq[k];
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-        m[k] = s[k] * meq[k] + (1.0f - s[k]) * m[k] + (1.0f - 0.5f * s[k]) * mf[k];
-    const float d = 1.0f / 36.0f;
-    float a = 4.0f * m[0], b1 = a - m[1] - 2.0f * m[2], b2 = a + 2.0f * m[1] + m[2];
-    h[0] = d * (4.0f * (m[0] - m[1] + m[2]));
-    h[1] = d * (b1 + 6.0f * m[3] - 6.0f * m[4] + 9.0f * m[7]);
-    h[2] = d * (b1 + 6.0f * m[5] - 6.0f * m[6] - 9.0f * m[7]);
-    h[3] = d * (b1 - 6.0f * m[3] + 6.0f * m[4] + 9.0f * m[7]);
-    h[4] = d * (b1 - 6.0f * m[5] + 6.0f * m[6] - 9.0f * m[7]);
-    float c3 = 6.0f * m[3] + 3.0f * m[4], c5 = 6.0f * m[5] + 3.0f * m[6];
-    h[5] = d * (b2 + c3 + c5 + 9.0f * m[8]);
-    h[6] = d * (b2 - c3 + c5 - 9.0f * m[8]);
-    h[7] = d * (b2 - c3 - c5 + 9.0f * m[8]);
-    h[8] = d * (b2 + c3 - c5 - 9.0f * m[8]);
+    // moments of h (F6:873-881); m_0 = rho_dev
+    const float s14 = (h[1] + h[2]) + (h[3] + h[4]), s58 = (h[5] + h[6]) + (h[7] + h[8]);
+    const float a13 = h[1] - h[3], a24 = h[2] - h[4];
+    const float b56 = h[5] - h[6], b78 = h[7] - h[8], c56 = h[5] + h[6], c78 = h[7] + h[8];
+    const float m1 = -4.0f * h[0] - s14 + 2.0f * s58;
+    const float m2 = 4.0f * h[0] - 2.0f * s14 + s58;
+    const float m3 = a13 + (b56 - b78), m4 = -2.0f * a13 + (b56 - b78);
+    const float m5 = a24 + (c56 - c78), m6 = -2.0f * a24 + (c56 - c78);
+    const float m7 = (h[1] - h[2]) + (h[3] - h[4]), m8 = b56 + b78;
+    // non-equilibrium parts n_k = m_k - meq_k (equilibrium relative to the
+    // rest state) and the forcing moments (F6:883-900)
+    const float fx = rho * gx, fy = rho * gy;
+    const float rux = rho * ux, ruy = rho * uy;
+    const float ru2 = rux * ux + ruy * uy;
+    const float fu = fx * ux + fy * uy;
+    const float n1 = m1 - (3.0f * ru2 - 2.0f * rho_dev);
+    const float n2 = m2 - (rho_dev - 3.0f * ru2);
+    const float n3 = m3 - rux, n4 = m4 + rux, n5 = m5 - ruy, n6 = m6 + ruy;
+    const float n7 = m7 - (rux * ux - ruy * uy), n8 = m8 - rux * uy;
+    const float mf7 = fx * ux - fy * uy, mf8 = fy * ux + fx * uy;
+    // D_k / N_k; the q and om factors vary per node
+    float qa, qb;
+    if (P.q_mode) {
+        const float q = (8.0f * (2.0f - om)) * frcp(8.0f - om);
+        qa = q * (-1.0f / 12.0f);
+        qb = q * (1.0f / 24.0f) - (1.0f / 12.0f);
+    } else {
+        qa = P.cDq1; qb = P.cDq2;
+    }
+    const float oa = om * -0.25f;
+    const float A = n1 * P.cD1a + fu * P.cD1b;
+    const float B = n2 * P.cD2a + fu * P.cD2b;
+    const float C = n3 * P.cD3a + fx * P.cD3b;
+    const float E = n5 * P.cD5a + fy * P.cD5b;
+    const float Q = n4 * qa + fx * qb;
+    const float G = n6 * qa + fy * qb;
+    const float R7 = n7 * oa + mf7 * (0.5f - 0.25f * om);
+    const float R8 = n8 * oa + mf8 * (0.25f - 0.125f * om);
+    // h* = h + M^T (D / N)
+    const float ab1 = -A - 2.0f * B, ab5 = 2.0f * A + B;
+    const float x = C - 2.0f * Q, y = E - 2.0f * G, u = C + Q, v = E + G;
+    const float t13 = ab1 + R7, t24 = ab1 - R7, t57 = ab5 + R8, t68 = ab5 - R8;
+    const float uv = u + v, vu = v - u;
+    h[0] = h[0] + 4.0f * (B - A);
+    h[1] = h[1] + (t13 + x);
+    h[3] = h[3] + (t13 - x);
+    h[2] = h[2] + (t24 + y);
+    h[4] = h[4] + (t24 - y);
+    h[5] = h[5] + (t57 + uv);
+    h[7] = h[7] + (t57 - uv);
+    h[6] = h[6] + (t68 + vu);
+    h[8] = h[8] + (t68 - vu);
     om_out = om;
     yielded = 0;
     if (!OUT && P.rheology != 1) return;
-    // Chai strain rate (F6:903-924).  The strain moments f1 = C (s * mne)
+    // Chai strain rate (F6:903-924).  The strain moments f1 = C (s * n)
     // enter d11, d22, d12 only through three column sums of C, which leave
     //   d11 = -(e0 + e1/4 + 3 e7/4), d22 = -(e0 + e1/4 - 3 e7/4), d12 = -3 e8/2
-    // (e_k = s_k mne_k) plus the forcing terms: the same quantities as the
-    // reference's sums, in 8 operations instead of about 50.
-    const float e0 = s[0] * mne[0], e1 = s[1] * mne[1], e7 = s[7] * mne[7], e8 = s[8] * mne[8];
+    // (e_k = s_k n_k; e0 = 0) plus the forcing terms: the same quantities as
+    // the reference's sums, in a few operations instead of about 50.
     const float gu = gx * ux + gy * uy;
-    const float base = -(e0 + 0.25f * e1);
-    const float gs = 0.75f * ((s[7] - s[1]) * gu);
-    float d11 = (base - 0.75f * e7) + (gs - 0.75f * (s[7] * (2.0f * gx * ux)));
-    float d22 = (base + 0.75f * e7) + (gs - 0.75f * (s[7] * (2.0f * gy * uy)));
-    float d12 = -1.5f * e8 - 0.75f * (s[7] * (gx * uy + gy * ux));
+    const float base = n1 * P.cS1;  // -e1 / 4
+    const float e7 = om * n7;
+    const float gs = (0.75f * om + P.cS2) * gu;  // 0.75 (s7 - s1) gu
+    float d11 = (base - 0.75f * e7) + (gs - 1.5f * (om * (gx * ux)));
+    float d22 = (base + 0.75f * e7) + (gs - 1.5f * (om * (gy * uy)));
+    float d12 = -1.5f * (om * n8) - 0.75f * (om * (gx * uy + gy * ux));
     if (OUT) {
         float nu = P.fnu;
         stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
