@@ -11,6 +11,7 @@ LBM_ERR_HIP = -2
 LBM_ERR_UNSUPPORTED = -3
 LBM_ERR_CAPACITY = -4
 LBM_ERR_NAN = -5
+LBM_ERR_IO = -6
 
 LBM_NEWTONIAN, LBM_BINGHAM = 0, 1
 LBM_RATE_LAGGED, LBM_RATE_LOCAL = 0, 1
@@ -68,6 +69,10 @@ SIGNATURES = {
     "lbm_set_particles": [ctypes.c_int32, _dp],
     "lbm_get_particles": [_dp],
     "lbm_get_phi": [ctypes.c_int32, _dp],
+    "lbm_restart_write": [ctypes.c_char_p], "lbm_restart_read": [ctypes.c_char_p],
+    "lbm_snapshot_size": [ctypes.POINTER(ctypes.c_uint64)],
+    "lbm_snapshot_save": [ctypes.c_void_p, ctypes.c_uint64],
+    "lbm_snapshot_load": [ctypes.c_void_p, ctypes.c_uint64],
     "lbm_collide": [], "lbm_copy_post": [], "lbm_stream": [],
     "lbm_boundary": [ctypes.c_int32],
     "lbm_macros": [], "lbm_spm_phi": [], "lbm_spm_forcing": [],

@@ -441,6 +441,272 @@ int lbm_get_phi(lbm_handle *h, int32_t ip, double *phi) {
     return LBM_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------ checkpoint / restart
+// Text state files of the reference (WRITE_OUTPUT F6:377-405, READ_INPUT
+// F6:174-198): one record per node, I outer and J inner, list-directed
+// numbers; the particle file holds CENTER(2), VELOCITY(3), FORCE(3) per disk
+// and 0_OmegaMRT_output.txt the relaxation vector OMEGA_MRT(0:8).  Numbers are written
+// with 17 significant digits (exact round trip) and read as a stream of
+// tokens, as Fortran list-directed input does (D exponents accepted).
+namespace {
+
+const char *const RS_F = "0_F_output.txt", *const RS_UX = "0_UX_output.txt",
+                  *const RS_UY = "0_UY_output.txt", *const RS_RHO = "0_RHO_output.txt",
+                  *const RS_G = "0_Gspm_output.txt", *const RS_OBN = "0_OmegaBN_output.txt",
+                  *const RS_PART = "0_Particle_output.txt", *const RS_OM = "0_OmegaMRT_output.txt",
+                  *const RS_WALL = "0_Wall_output.txt", *const RS_IMG = "0_DomainImage_output.txt";
+
+std::string join_path(const char *dir, const char *name) {
+    std::string d(dir);
+    if (!d.empty() && d.back() != '/') d += '/';
+    return d + name;
+}
+
+// Node-major reference layout (component fastest, i fastest among nodes)
+// -> one text record per node in I-outer / J-inner order.
+bool write_nodes(const std::string &path, const DevP &P, int nc, const double *a) {
+    FILE *fp = std::fopen(path.c_str(), "w");
+    if (!fp) return false;
+    for (int i = 0; i <= P.nx; ++i)
+        for (int j = 0; j <= P.ny; ++j) {
+            const double *v = a + ((size_t)j * P.W + i) * nc;
+            for (int c = 0; c < nc; ++c) std::fprintf(fp, " %.17g", v[c]);
+            std::fputc('\n', fp);
+        }
+    return std::fclose(fp) == 0;
+}
+
+// Reads exactly n numbers; false if the file is missing, short or malformed.
+bool read_numbers(const std::string &path, size_t n, std::vector<double> &out,
+                  std::string &why) {
+    FILE *fp = std::fopen(path.c_str(), "r");
+    if (!fp) { why = "cannot open " + path; return false; }
+    out.clear();
+    out.reserve(n);
+    char tok[128];
+    while (out.size() < n && std::fscanf(fp, "%127s", tok) == 1) {
+        for (char *c = tok; *c; ++c)
+            if (*c == 'D' || *c == 'd') *c = 'E';
+        char *end = nullptr;
+        double v = std::strtod(tok, &end);
+        if (end == tok || *end != '\0') {
+            std::fclose(fp);
+            why = path + ": not a number: " + tok;
+            return false;
+        }
+        out.push_back(v);
+    }
+    std::fclose(fp);
+    if (out.size() < n) {
+        why = path + ": " + std::to_string(out.size()) + " numbers, expected " + std::to_string(n);
+        return false;
+    }
+    return true;
+}
+
+// I-outer / J-inner records -> node-major reference layout.
+void nodes_from_records(const DevP &P, int nc, const std::vector<double> &rec, double *a) {
+    size_t r = 0;
+    for (int i = 0; i <= P.nx; ++i)
+        for (int j = 0; j <= P.ny; ++j)
+            for (int c = 0; c < nc; ++c) a[((size_t)j * P.W + i) * nc + c] = rec[r++];
+}
+
+// fields of the particle file, in the order of F6:393/F6:192
+const int RS_PFIELDS[8] = {LBM_PF_X, LBM_PF_Y, LBM_PF_VX, LBM_PF_VY, LBM_PF_W,
+                           LBM_PF_FX, LBM_PF_FY, LBM_PF_TQ};
+
+struct SnapHeader {
+    char magic[8];
+    uint32_t version, gen_base;
+    int32_t nx, ny, maxp, np, storage, rheology;
+    uint64_t bytes;
+};
+const char SNAP_MAGIC[8] = {'L', 'B', 'M', 'S', 'N', 'A', 'P', '1'};
+
+// F, F_POST (9 each), RHO, UX, UY, G (2), OMEGA_BN, STRESS (3): 27 doubles
+// per node at these plane offsets, then DOMAIN_IMAGE as int32, the particle
+// records, D_PW and the module-global rate
+constexpr int SN_F = 0, SN_FP = 9, SN_RHO = 18, SN_UX = 19, SN_UY = 20, SN_G = 21, SN_OBN = 23,
+              SN_STRESS = 24, SN_PLANES = 27;
+
+uint64_t snapshot_bytes(long long nn, int np) {
+    return sizeof(SnapHeader) + (uint64_t)nn * SN_PLANES * 8 + (uint64_t)nn * 4 +
+           (uint64_t)PCOUNT * np * 8 + 2 * 8;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_restart_write(lbm_handle *h, const char *dir) {
+    if (!h || !dir) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    std::vector<double> f((size_t)P.nn * 9), rho(P.nn), ux(P.nn), uy(P.nn), g((size_t)P.nn * 2),
+        obn(P.nn), part((size_t)PCOUNT * std::max(1, P.np));
+    std::vector<int32_t> img(P.nn);
+    int rc;
+    if ((rc = lbm_get_f(h, f.data())) || (rc = lbm_get_fields(h, rho.data(), ux.data(), uy.data(),
+                                                                g.data(), obn.data())) ||
+        (rc = lbm_get_image(h, img.data())) || (rc = lbm_get_particles(h, part.data())))
+        return rc;
+    // OMEGA_MRT(0:8): the relaxation vector the last node of the J-outer /
+    // I-inner sweep leaves behind (F6:958, F6:1009-1011)
+    const double om = h->prm.rheology == LBM_BINGHAM ? obn[(size_t)P.ny * P.W + P.nx] : P.omega;
+    const double q = P.q_mode ? (8.0 * (2.0 - om)) / (8.0 - om) : P.sq;
+    const double omv[9] = {P.s0, P.s1, P.s2, P.s3, q, P.s5, q, om, om};
+    bool ok = write_nodes(join_path(dir, RS_F), P, 9, f.data()) &&
+              write_nodes(join_path(dir, RS_UX), P, 1, ux.data()) &&
+              write_nodes(join_path(dir, RS_UY), P, 1, uy.data()) &&
+              write_nodes(join_path(dir, RS_RHO), P, 1, rho.data()) &&
+              write_nodes(join_path(dir, RS_G), P, 2, g.data()) &&
+              write_nodes(join_path(dir, RS_OBN), P, 1, obn.data());
+    if (ok) {
+        // DOMAIN_IMAGE is an integer array: written as integers
+        FILE *fp = std::fopen(join_path(dir, RS_IMG).c_str(), "w");
+        ok = fp != nullptr;
+        for (int i = 0; ok && i <= P.nx; ++i)
+            for (int j = 0; j <= P.ny; ++j)
+                std::fprintf(fp, " %d\n", (int)img[(size_t)j * P.W + i]);
+        if (fp) ok = (std::fclose(fp) == 0) && ok;
+    }
+    if (ok) {
+        FILE *fp = std::fopen(join_path(dir, RS_PART).c_str(), "w");
+        ok = fp != nullptr;
+        for (int p = 0; ok && p < P.np; ++p) {
+            for (int c = 0; c < 8; ++c)
+                std::fprintf(fp, " %.17g", part[(size_t)RS_PFIELDS[c] * P.np + p]);
+            std::fputc('\n', fp);
+        }
+        if (fp) ok = (std::fclose(fp) == 0) && ok;
+    }
+    if (ok) {
+        FILE *fp = std::fopen(join_path(dir, RS_OM).c_str(), "w");
+        ok = fp != nullptr;
+        for (int k = 0; ok && k < 9; ++k) std::fprintf(fp, " %.17g", omv[k]);
+        if (fp) { std::fputc('\n', fp); ok = (std::fclose(fp) == 0) && ok; }
+    }
+    if (ok) {
+        // opened by WRITE_OUTPUT (F6:387) with nothing written to it
+        FILE *fp = std::fopen(join_path(dir, RS_WALL).c_str(), "w");
+        ok = fp != nullptr && std::fclose(fp) == 0;
+    }
+    if (!ok) return fail(h, LBM_ERR_IO, std::string("cannot write restart files in ") + dir);
+    return LBM_OK;
+}
+
+int lbm_restart_read(lbm_handle *h, const char *dir) {
+    if (!h || !dir) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    const size_t nn = (size_t)P.nn;
+    std::vector<double> rec, f(nn * 9), rho(nn), ux(nn), uy(nn), g(nn * 2), obn(nn);
+    std::string why;
+    struct Item { const char *name; int nc; double *dst; } items[] = {
+        {RS_F, 9, f.data()}, {RS_UX, 1, ux.data()}, {RS_UY, 1, uy.data()},
+        {RS_RHO, 1, rho.data()}, {RS_G, 2, g.data()}, {RS_OBN, 1, obn.data()}};
+    for (const Item &it : items) {
+        if (!read_numbers(join_path(dir, it.name), nn * it.nc, rec, why))
+            return fail(h, LBM_ERR_IO, why);
+        nodes_from_records(P, it.nc, rec, it.dst);
+    }
+    std::vector<double> prec, lag;
+    if (!read_numbers(join_path(dir, RS_PART), (size_t)8 * P.np, prec, why) ||
+        !read_numbers(join_path(dir, RS_OM), 9, lag, why))
+        return fail(h, LBM_ERR_IO, why);
+    // CENTER, VELOCITY and FORCE replace those fields of the particle
+    // record; the rest (diameter, density, mass, ...) is the record already
+    // set, as the reference sets them from its parameters before reading
+    std::vector<double> part((size_t)PCOUNT * std::max(1, P.np));
+    int rc = P.np > 0 ? lbm_get_particles(h, part.data()) : LBM_OK;
+    if (rc) return rc;
+    for (int p = 0; p < P.np; ++p)
+        for (int c = 0; c < 8; ++c) part[(size_t)RS_PFIELDS[c] * P.np + p] = prec[(size_t)8 * p + c];
+    if ((rc = lbm_set_f(h, f.data())) ||
+        (rc = lbm_set_fields(h, rho.data(), ux.data(), uy.data(), g.data(), obn.data())) ||
+        (P.np > 0 && (rc = lbm_set_particles(h, P.np, part.data()))))
+        return rc;
+    // the module-global rate is entry 7 (omega) of OMEGA_MRT(0:8)
+    HIPCHECK(h, hipMemcpyAsync(h->lag, lag.data() + 7, 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_snapshot_size(const lbm_handle *h, uint64_t *bytes) {
+    if (!h || !bytes) return LBM_ERR_PARAM;
+    *bytes = snapshot_bytes(h->P.nn, h->P.np);
+    return LBM_OK;
+}
+
+int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes) {
+    if (!h || !buf) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    const uint64_t need = snapshot_bytes(P.nn, P.np);
+    if (bytes < need) return fail(h, LBM_ERR_CAPACITY, "snapshot buffer too small");
+    SnapHeader hd;
+    std::memset(&hd, 0, sizeof(hd));
+    std::memcpy(hd.magic, SNAP_MAGIC, 8);
+    hd.version = 1; hd.gen_base = h->gen_base;
+    hd.nx = P.nx; hd.ny = P.ny; hd.maxp = P.maxp; hd.np = P.np;
+    hd.storage = h->prm.storage; hd.rheology = h->prm.rheology; hd.bytes = need;
+    char *o = (char *)buf;
+    std::memcpy(o, &hd, sizeof(hd));
+    double *d = (double *)(o + sizeof(hd));
+    const size_t nn = (size_t)P.nn;
+    int rc;
+    if ((rc = lbm_get_f(h, d + SN_F * nn)) || (rc = lbm_get_fpost(h, d + SN_FP * nn)) ||
+        (rc = lbm_get_fields(h, d + SN_RHO * nn, d + SN_UX * nn, d + SN_UY * nn, d + SN_G * nn,
+                             d + SN_OBN * nn)) ||
+        (rc = lbm_get_stress(h, d + SN_STRESS * nn)))
+        return rc;
+    int32_t *img = (int32_t *)(d + SN_PLANES * nn);
+    if ((rc = lbm_get_image(h, img))) return rc;
+    double *tail = (double *)(img + nn);
+    if (P.np > 0 && (rc = lbm_get_particles(h, tail))) return rc;
+    tail += (size_t)PCOUNT * P.np;
+    HIPCHECK(h, hipMemcpyAsync(tail, h->dpw[h->dcur], 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipMemcpyAsync(tail + 1, h->lag, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes) {
+    if (!h || !buf) return LBM_ERR_PARAM;
+    const DevP &P = h->P;
+    SnapHeader hd;
+    if (bytes < sizeof(hd)) return fail(h, LBM_ERR_PARAM, "snapshot too short");
+    std::memcpy(&hd, buf, sizeof(hd));
+    if (std::memcmp(hd.magic, SNAP_MAGIC, 8) != 0 || hd.version != 1)
+        return fail(h, LBM_ERR_PARAM, "not an lbm snapshot");
+    if (hd.nx != P.nx || hd.ny != P.ny)
+        return fail(h, LBM_ERR_PARAM, "snapshot grid differs from the handle's");
+    if (hd.np > h->prm.max_particles)
+        return fail(h, LBM_ERR_CAPACITY, "snapshot has more particles than max_particles");
+    const size_t nn = (size_t)P.nn;
+    const uint64_t need = hd.np >= 0 ? snapshot_bytes(P.nn, hd.np) : 0;
+    if (hd.np < 0 || hd.bytes != need || bytes < need)
+        return fail(h, LBM_ERR_PARAM, "snapshot size mismatch");
+    const double *d = (const double *)((const char *)buf + sizeof(hd));
+    int rc;
+    if ((rc = lbm_set_f(h, d + SN_F * nn)) || (rc = lbm_set_fpost(h, d + SN_FP * nn)) ||
+        (rc = lbm_set_fields(h, d + SN_RHO * nn, d + SN_UX * nn, d + SN_UY * nn, d + SN_G * nn,
+                             d + SN_OBN * nn)) ||
+        (rc = upload_planes(h, d + SN_STRESS * nn, 3, h->stress)))
+        return rc;
+    const int32_t *img = (const int32_t *)(d + SN_PLANES * nn);
+    HIPCHECK(h, hipMemcpyAsync(h->image, img, nn * 4, hipMemcpyHostToDevice, h->stream));
+    const double *tail = (const double *)(img + nn);
+    if ((rc = lbm_set_particles(h, hd.np, tail))) return rc;
+    tail += (size_t)PCOUNT * hd.np;
+    HIPCHECK(h, hipMemcpyAsync(h->dpw[h->dcur], tail, 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, hipMemcpyAsync(h->lag, tail + 1, 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    // the stochastic-rounding sequence continues where the saved run was
+    h->gen_base = hd.gen_base;
+    return LBM_OK;
+}
+
 // --------------------------------------------------------------- drop-in
 int lbm_collide(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;

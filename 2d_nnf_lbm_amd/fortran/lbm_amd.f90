@@ -212,6 +212,39 @@ MODULE LBM_AMD
       REAL(C_DOUBLE), INTENT(OUT) :: ms_total, ms_main
       INTEGER(C_INT) :: rc
     END FUNCTION
+    ! restart files of WRITE_OUTPUT / READ_INPUT (F6:377-405, F6:174-198);
+    ! dir is a NUL-terminated path: TRIM(DIR)//C_NULL_CHAR
+    FUNCTION lbm_restart_write(h, dir) BIND(C, NAME='lbm_restart_write') RESULT(rc)
+      IMPORT :: C_PTR, C_INT, C_CHAR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: dir
+      INTEGER(C_INT) :: rc
+    END FUNCTION
+    FUNCTION lbm_restart_read(h, dir) BIND(C, NAME='lbm_restart_read') RESULT(rc)
+      IMPORT :: C_PTR, C_INT, C_CHAR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: dir
+      INTEGER(C_INT) :: rc
+    END FUNCTION
+    ! binary snapshot of the run state in a caller buffer of `bytes` bytes
+    FUNCTION lbm_snapshot_size(h, bytes) BIND(C, NAME='lbm_snapshot_size') RESULT(rc)
+      IMPORT :: C_PTR, C_INT, C_INT64_T
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT64_T), INTENT(OUT) :: bytes
+      INTEGER(C_INT) :: rc
+    END FUNCTION
+    FUNCTION lbm_snapshot_save(h, buf, bytes) BIND(C, NAME='lbm_snapshot_save') RESULT(rc)
+      IMPORT :: C_PTR, C_INT, C_INT64_T
+      TYPE(C_PTR), VALUE :: h, buf
+      INTEGER(C_INT64_T), VALUE :: bytes
+      INTEGER(C_INT) :: rc
+    END FUNCTION
+    FUNCTION lbm_snapshot_load(h, buf, bytes) BIND(C, NAME='lbm_snapshot_load') RESULT(rc)
+      IMPORT :: C_PTR, C_INT, C_INT64_T
+      TYPE(C_PTR), VALUE :: h, buf
+      INTEGER(C_INT64_T), VALUE :: bytes
+      INTEGER(C_INT) :: rc
+    END FUNCTION
     FUNCTION c_strlen(s) BIND(C, NAME='strlen') RESULT(n)
       IMPORT :: C_PTR, C_SIZE_T
       TYPE(C_PTR), VALUE :: s

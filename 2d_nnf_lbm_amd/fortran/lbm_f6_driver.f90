@@ -10,14 +10,17 @@
 ! (F6:348-349) and, with WRITE_OUTPUT, the state dump of F6:389-403 plus the
 ! field file of WriteResults (F6:1416-1463).
 !
+! With READ_INPUT the run starts from the state files of F6:174-198 in the
+! working directory (as written by WRITE_OUTPUT).
+!
 ! Usage: lbm_f6_driver [namelist-file]   (namelist /RUN/ below)
 PROGRAM LBM_F6_DRIVER
   USE LBM_AMD
   IMPLICIT NONE
   INTEGER :: NX = 80, NY = 300, TIME_MAX = 1000, STORAGE = 0
-  LOGICAL :: NON_NEWTONIAN = .TRUE., WRITE_OUTPUT = .TRUE., FUSED = .TRUE.
+  LOGICAL :: NON_NEWTONIAN = .TRUE., WRITE_OUTPUT = .TRUE., FUSED = .TRUE., READ_INPUT = .FALSE.
   DOUBLE PRECISION :: UXN = 0D0
-  NAMELIST /RUN/ NX, NY, TIME_MAX, NON_NEWTONIAN, WRITE_OUTPUT, FUSED, STORAGE, UXN
+  NAMELIST /RUN/ NX, NY, TIME_MAX, NON_NEWTONIAN, WRITE_OUTPUT, FUSED, STORAGE, UXN, READ_INPUT
 
   DOUBLE PRECISION :: PI, RE_S, BN_S, Y_G, RHO_S, RHO_F, NU, L_C, A_G, U_C, TAU_Y
   DOUBLE PRECISION :: MU_0, M_PAP, RHO0, OMEGA, AVGDEN, MASS, M_P, I_P, D
@@ -128,6 +131,13 @@ PROGRAM LBM_F6_DRIVER
   PART(PF_IP, 1) = I_P
   RC = lbm_set_particles(H, 1, PART)
   CALL LBM_CHECK(H, RC, 'lbm_set_particles')
+  ! READ_INPUT (F6:174-198): state files of an earlier run in the working
+  ! directory replace the initial state
+  IF (READ_INPUT) THEN
+    RC = lbm_restart_read(H, '.' // C_NULL_CHAR)
+    CALL LBM_CHECK(H, RC, 'READ_INPUT')
+    PRINT *, 'Reading Initial Data Was Succsessful...'
+  END IF
 
   OPEN(8, FILE='01_AveragedDensity.PLT')
   WRITE(8, *) 'VARIABLES=Iteration,Density_Averaged'

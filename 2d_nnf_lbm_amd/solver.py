@@ -7,6 +7,7 @@ exchanged in the reference layout (F(0:8,0:NX,0:NY) etc., returned here as
 numpy arrays indexed [j, i, q]).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -149,6 +150,31 @@ class D2Q9:
         out = np.empty(self.shape)
         self._call("lbm_get_phi", ip, _ptr(out))
         return out
+
+    # ------------------------------------------------- checkpoint / restart
+    def write_output(self, directory):
+        """WRITE_OUTPUT (F6:377-405): the reference's text state files."""
+        self._call("lbm_restart_write", os.fsencode(str(directory)))
+
+    def read_input(self, directory):
+        """READ_INPUT (F6:174-198); the particles must be set first."""
+        self._call("lbm_restart_read", os.fsencode(str(directory)))
+
+    def snapshot(self):
+        """Binary snapshot of the whole run state (numpy uint8 array)."""
+        n = ctypes.c_uint64(0)
+        self._call("lbm_snapshot_size", ctypes.byref(n))
+        buf = np.empty(n.value, dtype=np.uint8)
+        self._call("lbm_snapshot_save", buf.ctypes.data_as(ctypes.c_void_p), n.value)
+        return buf
+
+    def load_snapshot(self, buf):
+        """Continue the run saved by snapshot() (same parameters)."""
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._call("lbm_snapshot_load", buf.ctypes.data_as(ctypes.c_void_p), buf.size)
+        # particle count: int32 after the 8-byte tag, version, sequence
+        # position, nx, ny and maxp of the snapshot header
+        self.np = int(buf[28:32].view(np.int32)[0])
 
     # ---------------------------------------------- reference subroutines
     def collision_mrt(self):

@@ -34,6 +34,7 @@ extern "C" {
 #define LBM_ERR_UNSUPPORTED (-3) /* combination not provided on the GPU path */
 #define LBM_ERR_CAPACITY (-4)    /* particle, pair or history capacity exceeded */
 #define LBM_ERR_NAN (-5)         /* non-finite value detected by a check */
+#define LBM_ERR_IO (-6)          /* a restart file could not be read or written */
 
 #define LBM_NEWTONIAN 0
 #define LBM_BINGHAM 1            /* Papanastasiou-regularised Bingham (F6:939-945) */
@@ -124,6 +125,36 @@ int lbm_get_particles(lbm_handle *h, double *part);
 /* Solid fraction of particle ip on the full grid (PHI(0:NX,0:NY) of F6:79),
  * as computed by the last lbm_spm_phi. */
 int lbm_get_phi(lbm_handle *h, int32_t ip, double *phi);
+
+/* Checkpoint and restart.
+ * lbm_restart_write replaces WRITE_OUTPUT (F6:377-405): it writes the text
+ * state files 0_F_output.txt, 0_UX/0_UY/0_RHO/0_Gspm/0_OmegaBN/
+ * 0_DomainImage_output.txt (one record per node, I outer, J inner),
+ * 0_Particle_output.txt (CENTER, VELOCITY, FORCE per disk),
+ * 0_OmegaMRT_output.txt (the relaxation vector OMEGA_MRT(0:8) that the last
+ * node of the sweep leaves, F6:958) and the empty 0_Wall_output.txt into
+ * directory dir, numbers with 17 significant digits.
+ * lbm_restart_read replaces READ_INPUT (F6:174-198): it reads the same files
+ * (not the image, as the reference) in any list-directed number format; the
+ * particles must be set first (the file replaces CENTER, VELOCITY, FORCE,
+ * the other fields stay, as the reference sets them from its parameters),
+ * and entry 7 of OMEGA_MRT becomes the module-global rate of
+ * LBM_RATE_LAGGED collisions.
+ * Both return LBM_ERR_IO with the file name in lbm_last_error on failure. */
+int lbm_restart_write(lbm_handle *h, const char *dir);
+int lbm_restart_read(lbm_handle *h, const char *dir);
+
+/* Binary snapshot of the whole run state in host memory: F, F_POST, RHO,
+ * UX, UY, G, OMEGA_BN, STRESS, DOMAIN_IMAGE, the particle records, the stale
+ * wall distance of the contact force, the module-global rate and the
+ * position of the stochastic-rounding sequence.  Loading it into a handle
+ * created with the same parameters continues the run exactly: the next
+ * lbm_step gives the same results bit for bit as it would have given on the
+ * saved handle (a handle with another grid is refused).  No counterpart in the reference (its restart
+ * files are the text ones above); for callers that checkpoint in memory. */
+int lbm_snapshot_size(const lbm_handle *h, uint64_t *bytes);
+int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes);
+int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes);
 
 /* Drop-in per-subroutine entries (FP64 state, reference semantics). */
 /* Collision_MRT, F6:867-962.  With LBM_BINGHAM and LBM_RATE_LAGGED it keeps

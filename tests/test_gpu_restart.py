@@ -1,0 +1,214 @@
+"""Checkpoint and restart: the reference's text state files (WRITE_OUTPUT
+F6:377-405, READ_INPUT F6:174-198) and the binary run-state snapshot.
+
+- Files the reference program itself wrote are read exactly (values as the
+  reference's list-directed output holds them).
+- Writing and reading back restores the state bit for bit, and a run
+  continued from the files equals the uninterrupted run.
+- A snapshot loaded into a fresh handle continues the run bit for bit, in
+  FP64 and in the reduced formats (whose stochastic-rounding sequence the
+  snapshot carries)."""
+import importlib
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import cases
+from lbmtest import pkg
+from refio import parse_run
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def small_two_disk(storage):
+    """Two disks in a 100 x 300 Bingham channel (the two-disk settling case
+    scaled down), far enough apart and from the walls to stay out of
+    contact for the run."""
+    cfg, part = importlib.import_module("2d_nnf_lbm_amd.configs").two_disk_bingham(storage)
+    cfg = dict(cfg, nx=100, ny=300)
+    part = part.copy()
+    part[0] = [48.0, 52.0]      # x of both disks
+    part[1] = [200.0, 250.0]    # y
+    return cfg, part
+
+
+def states_equal(a, b):
+    fa, fb = a.f, b.f
+    assert np.array_equal(fa, fb)
+    for x, y in zip(a.get_fields(), b.get_fields()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.get_particles(), b.get_particles())
+
+
+@pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local"])
+def test_reads_reference_state_files(name):
+    """The reference program's own WRITE_OUTPUT files (oracle/_ref/<case>,
+    compiled from the reference's source) are read value for value."""
+    exe = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(exe):
+        pytest.skip("reference programs are built by oracle/build_ref.py where the reference exists")
+    meta, _ = cases.load_golden(name)
+    nx, ny = meta["nx"], meta["ny"]
+    print("case: read reference state files", name, flush=True)
+    with tempfile.TemporaryDirectory(prefix="rst_") as d:
+        r = subprocess.run([exe], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-1500:]
+        ref, _ = parse_run(d, nx, ny)
+        s = pkg().D2Q9(dict(cases.golden_params(meta), max_particles=1))
+        s.initialize()
+        s.set_particles(cases.f6_disk(nx, ny).reshape(-1, 1))
+        s.read_input(d)
+    assert np.array_equal(s.f, ref["f"])
+    rho, ux, uy, g, om = s.get_fields()
+    for got, key in ((rho, "rho"), (ux, "ux"), (uy, "uy"), (g, "g"), (om, "omega_bn")):
+        assert np.array_equal(got, ref[key]), key
+    p = s.get_particles()[:, 0]
+    ab = pkg()._abi
+    fields = [ab.PF[n] for n in ("x", "y", "vx", "vy", "w", "fx", "fy", "tq")]
+    assert np.array_equal(p[fields], ref["particle"][:8])
+    # the fields the file does not hold keep the record that was set
+    assert p[ab.PF["diam"]] == cases.f6_disk(nx, ny)[ab.PF["diam"]]
+    s.close()
+
+
+def test_write_read_round_trip_and_continue():
+    """WRITE_OUTPUT then READ_INPUT into a fresh handle restores the state
+    exactly; both then run on to the same bits (FP64 fused steps)."""
+    cfg, part = small_two_disk("fp64")
+    lbm = pkg()
+    a = lbm.D2Q9(cfg)
+    a.initialize()
+    a.set_particles(part)
+    a.step(0, 30)
+    with tempfile.TemporaryDirectory(prefix="rst_") as d:
+        a.write_output(d)
+        names = sorted(os.listdir(d))
+        assert names == sorted(["0_F_output.txt", "0_UX_output.txt", "0_UY_output.txt",
+                                "0_RHO_output.txt", "0_Gspm_output.txt", "0_OmegaBN_output.txt",
+                                "0_Particle_output.txt", "0_OmegaMRT_output.txt",
+                                "0_Wall_output.txt", "0_DomainImage_output.txt"])
+        # the same layout as the reference's files
+        out, _ = parse_run_state(d, cfg["nx"], cfg["ny"])
+        assert np.array_equal(out["f"], a.f)
+        om = out["omega_mrt"]
+        assert om.shape == (9,) and om[7] == om[8] == a.get_fields()[4][-1, -1]
+        b = lbm.D2Q9(cfg)
+        b.initialize()
+        b.set_particles(a.get_particles())
+        b.read_input(d)
+    states_equal(a, b)
+    ha, hb = a.step(30, 20), b.step(30, 20)
+    assert np.array_equal(ha, hb)
+    states_equal(a, b)
+    a.close()
+    b.close()
+
+
+def parse_run_state(d, nx, ny):
+    """The state files of a directory (parse_run without the run's other
+    outputs)."""
+    nn = (nx + 1) * (ny + 1)
+    out = {}
+
+    def floats(name):
+        with open(os.path.join(d, name)) as fh:
+            return np.array([float(t) for t in fh.read().split()])
+
+    a = floats("0_F_output.txt")
+    assert a.size == nn * 9
+    out["f"] = np.ascontiguousarray(a.reshape(nx + 1, ny + 1, 9).transpose(1, 0, 2))
+    out["omega_mrt"] = floats("0_OmegaMRT_output.txt")
+    return out, None
+
+
+def test_read_input_reports_missing_files():
+    cfg, part = small_two_disk("fp64")
+    lbm = pkg()
+    s = lbm.D2Q9(cfg)
+    s.initialize()
+    s.set_particles(part)
+    with tempfile.TemporaryDirectory(prefix="rst_") as d:
+        with pytest.raises(lbm.LBMError) as e:
+            s.read_input(d)
+        assert e.value.code == lbm._abi.LBM_ERR_IO
+        assert "0_F_output.txt" in str(e.value)
+        # a short file is refused, naming the count
+        with open(os.path.join(d, "0_F_output.txt"), "w") as fh:
+            fh.write(" 1.0 2.0D0 3.0E-1\n")
+        with pytest.raises(lbm.LBMError) as e:
+            s.read_input(d)
+        assert "3 numbers" in str(e.value)
+    s.close()
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp16", "bf16"])
+def test_snapshot_continues_bit_for_bit(storage):
+    cfg, part = small_two_disk(storage)
+    lbm = pkg()
+    print("case: snapshot continuation", storage, flush=True)
+    a = lbm.D2Q9(cfg)
+    a.initialize()
+    a.set_particles(part)
+    a.step(0, 25)
+    snap = a.snapshot()
+    ha = a.step(25, 25)
+    b = lbm.D2Q9(cfg)
+    b.initialize()
+    b.load_snapshot(snap)
+    assert b.np == part.shape[1]
+    hb = b.step(25, 25)
+    assert np.array_equal(ha, hb)
+    states_equal(a, b)
+    # a handle with another grid refuses it
+    c = lbm.D2Q9(dict(cfg, nx=cfg["nx"] + 2))
+    with pytest.raises(lbm.LBMError):
+        c.load_snapshot(snap)
+    for s in (a, b, c):
+        s.close()
+
+
+def state_files(d, nx, ny):
+    nn = (nx + 1) * (ny + 1)
+    out = {}
+    for key, name, nc in (("f", "0_F_output.txt", 9), ("ux", "0_UX_output.txt", 1),
+                          ("uy", "0_UY_output.txt", 1), ("rho", "0_RHO_output.txt", 1),
+                          ("g", "0_Gspm_output.txt", 2), ("omega_bn", "0_OmegaBN_output.txt", 1),
+                          ("particle", "0_Particle_output.txt", 0)):
+        with open(os.path.join(d, name)) as fh:
+            a = np.array([float(t) for t in fh.read().split()])
+        if nc:
+            assert a.size == nn * nc, name
+        out[key] = a
+    return out
+
+
+def test_fortran_driver_read_input():
+    """The Fortran driver writes its state files (flang list-directed, as
+    the reference), then a second run with READ_INPUT and no steps reads
+    them and writes them again unchanged."""
+    drv = os.path.join(REPO, "2d_nnf_lbm_amd", "fortran", "lbm_f6_driver")
+    assert os.path.exists(drv), "build the Fortran driver first"
+    nx, ny = 40, 120
+    with tempfile.TemporaryDirectory(prefix="rstf_") as d:
+        def run(extra):
+            nml = os.path.join(d, "run.nml")
+            with open(nml, "w") as fh:
+                fh.write("&RUN NX=%d, NY=%d, NON_NEWTONIAN=.TRUE., WRITE_OUTPUT=.TRUE., "
+                         "STORAGE=0, %s /\n" % (nx, ny, extra))
+            r = subprocess.run([drv, nml], cwd=d, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True, timeout=300)
+            assert r.returncode == 0, r.stdout[-1500:]
+            return r.stdout
+        run("TIME_MAX=50, FUSED=.TRUE.")
+        first = state_files(d, nx, ny)
+        out = run("TIME_MAX=0, FUSED=.FALSE., READ_INPUT=.TRUE.")
+        assert "Reading Initial Data" in out
+        second = state_files(d, nx, ny)
+    for k in first:
+        assert np.array_equal(first[k], second[k]), k
