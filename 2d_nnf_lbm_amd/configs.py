@@ -42,6 +42,42 @@ def disk(x, y, diam, rho_p, zeta):
     return rec
 
 
+def read_initial_positions(path, n=None, diam=None, rho_p=None, zeta=1.0):
+    """Particle records from the reference's initial-condition file
+    0_Particle_Initial_Position.txt, read as its list-directed READs do.
+
+    With diam and rho_p given, each record holds CENTER only (F3:145-149:
+    READ(200,*) DISK(IP)%CENTER, with DIAM = L_C and RHO_P = RHO_S set from
+    the parameters, F3:152-153); without them, CENTER, DIAM, RHO_P
+    (F1:170-174).  Mass and inertia follow F1:177-178 / F3:154-155, with
+    ZITA = zeta (F1:176).  Each READ starts at a new line and takes its
+    values from as many lines as it needs (blank lines skipped); values left
+    on its last line are ignored.  n: number of particles to read
+    (N_PARTICLE); by default every record in the file.  Returns a
+    (PF_COUNT, n) array."""
+    per = 2 if diam is not None else 4
+    if (diam is None) != (rho_p is None):
+        raise ValueError("give both diam and rho_p (CENTER-only records) or neither")
+    with open(path) as fh:
+        lines = [ln.replace("D", "E").replace("d", "e").split() for ln in fh]
+    recs, k = [], 0
+    while (n is None or len(recs) < n) and k < len(lines):
+        vals = []
+        while len(vals) < per and k < len(lines):
+            vals.extend(lines[k])
+            k += 1
+        if len(vals) < per:
+            if vals:
+                raise ValueError("%s: incomplete record %d" % (path, len(recs) + 1))
+            break
+        vals = [float(v) for v in vals[:per]]
+        d, r = (diam, rho_p) if per == 2 else (vals[2], vals[3])
+        recs.append(disk(vals[0], vals[1], float(d), float(r), zeta))
+    if n is not None and len(recs) < n:
+        raise ValueError("%s: %d records, expected %d" % (path, len(recs), n))
+    return np.stack(recs, axis=1)
+
+
 def two_disk_bingham(storage="fp64"):
     """Two disks settling in a Papanastasiou-Bingham fluid.
 

@@ -70,6 +70,34 @@ def test_dense_cavity_fp64_matches_oracle(orc):
         assert rel_err(hg[:, idx], ho[:, idx]) <= TRAJ_TOL, col
 
 
+def test_reference_f3_initial_condition_fp64_matches_oracle(orc):
+    """The reference's own 504-disk initial condition (F3's
+    0_Particle_Initial_Position.txt, F3's 512 x 512 cavity, D = 16) with the
+    dense-sedimentation fluid: fused FP64 steps against the oracle."""
+    import os
+    c = configs()
+    pf = importlib.import_module("2d_nnf_lbm_amd._abi").PF
+    ic = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "f3_initial_positions.txt")
+    rec = c.read_initial_positions(ic, n=504, diam=16.0, rho_p=1.01)
+    cfg, part = c.dense_sedimentation(1.0, "fp64", positions=rec[[pf["x"], pf["y"]]].T,
+                                      size=512)
+    n = 20  # the oracle takes about 1 s per step here
+    print("case: reference F3 initial condition, 504 disks 513^2 FP64, %d steps" % n, flush=True)
+    st = oracle_state(orc, cfg, part)
+    ho = st.run(0, n)
+    g = gpu_solver(cfg, max_particles=504)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, n)
+    assert rel_err(g.f, st.f) <= POP_TOL
+    H = orc.H_PCOUNT
+    for col in (orc.H_X, orc.H_Y, orc.H_VY):
+        idx = [p * H + col for p in range(504)]
+        assert rel_err(hg[:, idx], ho[:, idx]) <= TRAJ_TOL, col
+    assert np.all(g.get_particles()[pf["vy"]] < 0)
+
+
 def test_dense_sedimentation_504_disks_bf16():
     c = configs()
     pf = importlib.import_module("2d_nnf_lbm_amd._abi").PF
