@@ -785,32 +785,41 @@ __device__ __forceinline__ bool pair_force(const DevP &P, double dpw, const doub
     return true;
 }
 
+// Force of wall iw (1 bottom, 2 right, 3 top, 4 left; F3:773-804) on a disk
+// at (x, y), diameter dm; returns the wall distance.
+__device__ __forceinline__ double wall_one(const DevP &P, double x, double y, double dm, int iw,
+                                           double &f0, double &f1) {
+    const double zeta = P.zeta_c, cij = P.cij;
+    double rw0, rw1;
+    if (iw == 1) { rw0 = x; rw1 = -(0.5 * dm); }
+    else if (iw == 2) { rw0 = (double)P.nx + 0.5 * dm; rw1 = y; }
+    else if (iw == 3) { rw0 = x; rw1 = (double)P.ny + 0.5 * dm; }
+    else { rw0 = -(0.5 * dm); rw1 = y; }
+    const double d = sqrt((x - rw0) * (x - rw0) + (y - rw1) * (y - rw1));
+    f0 = 0.0; f1 = 0.0;
+    if (d > dm + zeta) {
+        f0 = 0.0; f1 = 0.0;
+    } else if (d <= dm + zeta && d > dm) {
+        double t = (d - dm - zeta) / zeta;
+        f0 = P.eps_w * cij * (t * t) * (x - rw0) / d;
+        f1 = P.eps_w * cij * (t * t) * (y - rw1) / d;
+    } else if (d <= dm) {
+        double t = (d - dm - zeta) / zeta;
+        f0 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (x - rw0) / d;
+        f1 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (y - rw1) / d;
+    }
+    return d;
+}
+
 // Wall forces on a disk at (x, y), diameter dm, added in the wall order of
 // F3:773-804 (bottom, right, top, left); returns the last wall distance,
 // which becomes the next call's stale D_PW.
 __device__ __forceinline__ double wall_forces(const DevP &P, double x, double y, double dm,
                                               double &fcx, double &fcy) {
-    const double zeta = P.zeta_c, cij = P.cij;
     double d = 0.0;
     for (int iw = 1; iw <= 4; ++iw) {
-        double rw0, rw1;
-        if (iw == 1) { rw0 = x; rw1 = -(0.5 * dm); }
-        else if (iw == 2) { rw0 = (double)P.nx + 0.5 * dm; rw1 = y; }
-        else if (iw == 3) { rw0 = x; rw1 = (double)P.ny + 0.5 * dm; }
-        else { rw0 = -(0.5 * dm); rw1 = y; }
-        d = sqrt((x - rw0) * (x - rw0) + (y - rw1) * (y - rw1));
-        double f0 = 0.0, f1 = 0.0;
-        if (d > dm + zeta) {
-            f0 = 0.0; f1 = 0.0;
-        } else if (d <= dm + zeta && d > dm) {
-            double t = (d - dm - zeta) / zeta;
-            f0 = P.eps_w * cij * (t * t) * (x - rw0) / d;
-            f1 = P.eps_w * cij * (t * t) * (y - rw1) / d;
-        } else if (d <= dm) {
-            double t = (d - dm - zeta) / zeta;
-            f0 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (x - rw0) / d;
-            f1 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (y - rw1) / d;
-        }
+        double f0, f1;
+        d = wall_one(P, x, y, dm, iw, f0, f1);
         fcx = fcx + f0; fcy = fcy + f1;
     }
     return d;
@@ -907,6 +916,89 @@ __device__ __forceinline__ int contact_block(const DevP &P, const double *X, con
         *dpw_last = wall_forces(P, X[p], Y[p], D[p], fcx, fcy);
     }
     return cnt;
+}
+
+// contact_block for at most 64 particles, by one wavefront (threads 0..63,
+// no block barriers): lane q tests partner q, the in-range partners are
+// visited in ascending index order through the ballot mask, and the four
+// walls are evaluated by four lanes; every sum keeps the order of
+// contact_one.  Results are valid in lane 0 (they are the same in every lane).
+__device__ __forceinline__ int contact_wave(const DevP &P, const double *X, const double *Y,
+                                            const double *D, int np, int p, double dpw,
+                                            double &fcx, double &fcy, int *nb, double *dpw_last) {
+    const int lane = threadIdx.x & 63;
+    double f0 = 0.0, f1 = 0.0;
+    bool hit = false;
+    if (lane < np && lane != p)
+        hit = (lane < p) ? pair_force(P, dpw, X, Y, D, lane, p, f0, f1)
+                         : pair_force(P, dpw, X, Y, D, p, lane, f0, f1);
+    unsigned long long m = __ballot(hit);
+    double w0 = 0.0, w1 = 0.0, wd = 0.0;
+    if (lane < 4) wd = wall_one(P, X[p], Y[p], D[p], lane + 1, w0, w1);
+    fcx = 0.0; fcy = 0.0;
+    int cnt = 0;
+    while (m) {
+        const int k = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const double g0 = __shfl(f0, k), g1 = __shfl(f1, k);
+        if (k < p) { fcx = fcx - g0; fcy = fcy - g1; }
+        else {
+            fcx = fcx + g0; fcy = fcy + g1;
+            if (lane == 0 && nb && cnt < MAXNB) nb[cnt] = k;
+            ++cnt;
+        }
+    }
+    for (int w = 0; w < 4; ++w) {
+        fcx = fcx + __shfl(w0, w);
+        fcy = fcy + __shfl(w1, w);
+    }
+    const double dl = __shfl(wd, 3);
+    if (lane == 0) *dpw_last = dl;
+    return cnt;
+}
+
+// update_one by one wavefront: lane c < 3 updates component c (x, y,
+// rotation) with the operations of update_one in the same order, so the
+// result is bitwise the same.  rec: the record of state k (read by every
+// lane); F0..F2, fcx, fcy: FORCE/TORQUE and the contact force, the same in
+// every lane.  Lane t < PCOUNT returns field t of the new record.
+__device__ __forceinline__ double update_wave(const DevP &P, const double *rec, double F0,
+                                              double F1, double F2, double fcx, double fcy,
+                                              int kk) {
+    const int lane = threadIdx.x & 63;
+    const int c = lane < 3 ? lane : 0;
+    const double rho0 = P.rho0, rhop = rec[PRHOP];
+    const double mass = c < 2 ? rec[PMP] : rec[PIP];
+    const double v = rec[PVX + c], vo = rec[PVOX + c];
+    const double fk = c == 0 ? F0 : (c == 1 ? F1 : F2);
+    const double fc = c == 0 ? fcx : fcy;
+    double n, fh;
+    if (P.motion == 1) {
+        // prescribed oscillation (F4:221-226), added mass (F4:256-258)
+        n = c == 0 ? -P.osc_u * cos(2.0 * PI_F * (double)kk / P.osc_period) : 0.0;
+        fh = (rho0 / rhop) * mass * (n - v);
+    } else {
+        fh = (rho0 / rhop) * mass * (v - vo);
+        n = v + fk / mass + fh / mass;
+        if (c == 1) n = n + (1.0 - rho0 / rhop) * P.gy;
+        if (c < 2) n = n + fc / mass;
+    }
+    const double pos = rec[PX + c] + 0.5 * (n + v);
+    const int t = lane;
+    auto from = [&](double x, int base) { return __shfl(x, min(max(t - base, 0), 2)); };
+    const double g_pos = from(pos, PX), g_n = from(n, PVX), g_vo = from(v, PVOX),
+                 g_fh = from(fh, PFHX);
+    double val = t < PCOUNT ? rec[t] : 0.0;
+    if (t == PX || t == PY || (t == PANG && P.motion != 1 && P.update_angle)) val = g_pos;
+    if (t >= PVX && t <= PW) val = g_n;
+    if (t >= PVOX && t <= PVOW) val = g_vo;
+    if (t >= PFHX && t <= PFHW) val = g_fh;
+    if (t == PFX) val = F0;
+    if (t == PFY) val = F1;
+    if (t == PTQ) val = F2;
+    if (t == PFCX) val = fcx;
+    if (t == PFCY) val = fcy;
+    return val;
 }
 
 // Update of one particle record in place for loop iteration kk: Newton-Euler

@@ -787,23 +787,36 @@ __device__ void particle_block(const StepArgs &a, int pb) {
             // (it uses FORCE_k: written into the outgoing record first)
             __shared__ ContactLDS cl;
             double fcx = 0.0, fcy = 0.0, dl = 0.0;
-            int cnt = staged
-                ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl,
-                                fcx, fcy, a.pairs + p * MAXNB, &dl)
-                : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
-                                sdpw, cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
-            if (threadIdx.x == 0) {
-                double rec[PCOUNT];
-                for (int f = 0; f < PCOUNT; ++f) rec[f] = srec[f];
-                rec[PFX] = F[0]; rec[PFY] = F[1]; rec[PTQ] = F[2];
-                a.paircnt[p] = cnt;
-                rec[PFCX] = fcx; rec[PFCY] = fcy;
-                update_one(P, rec, a.kk_upd);
-                for (int f = 0; f < PCOUNT; ++f) { a.pout[f * m + p] = rec[f]; srec[f] = rec[f]; }
-                if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
-                if (p == P.np - 1) *a.dpw_out = dl;
+            int cnt = 0;
+            if (P.np <= 64) {
+                // one wavefront suffices (the staged copy holds every disk)
+                if (threadIdx.x < 64)
+                    cnt = contact_wave(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw,
+                                       fcx, fcy, a.pairs + p * MAXNB, &dl);
+            } else {
+                cnt = staged
+                    ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl,
+                                    fcx, fcy, a.pairs + p * MAXNB, &dl)
+                    : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np,
+                                    p, sdpw, cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
+            }
+            if (threadIdx.x < 64) {
+                // the update by wavefront 0; FORCE_k and the contact force
+                // are final in thread 0 (lane 0): broadcast them
+                const double val =
+                    update_wave(P, srec, __shfl(F[0], 0), __shfl(F[1], 0), __shfl(F[2], 0),
+                                __shfl(fcx, 0), __shfl(fcy, 0), a.kk_upd);
+                if (threadIdx.x < PCOUNT) {
+                    a.pout[threadIdx.x * m + p] = val;
+                    srec[threadIdx.x] = val;
+                }
+                if (threadIdx.x == 0) {
+                    a.paircnt[p] = cnt;
+                    if (p == P.np - 1) *a.dpw_out = dl;
+                }
             }
             __syncthreads();
+            if (threadIdx.x == 0 && a.spf_out) make_spf(srec, a.spf_out + p * SPF);
             if (P.np > NP_LOOP)
                 register_tiles(P, srec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
         }
