@@ -783,8 +783,9 @@ __device__ void particle_block(const StepArgs &a, int pb) {
             }
         }
         if (!FINAL) {
-            // contact over all partners with the whole block, then the update
-            // (it uses FORCE_k: written into the outgoing record first)
+            // contact force over all partners (one wavefront up to 64
+            // disks, the whole block beyond), then the update, which uses
+            // FORCE_k: written into the outgoing record first
             __shared__ ContactLDS cl;
             double fcx = 0.0, fcy = 0.0, dl = 0.0;
             int cnt = 0;
