@@ -105,17 +105,17 @@ __global__ __launch_bounds__(NT) void k_adj_node(DevP P, adj::LocalP L, const do
             f[q] = fk[q * P.plane + n];
             lfs[q] = lf_next[dst[q * P.plane + n]];
         }
-        const double om = P.rheology == 1 ? omk[n] : P.omega;
+        const double om = P.rheology != 0 ? omk[n] : P.omega;
         adj::Tape t;
         double fs[9], om2, fcx, fcy;
         adj::node_forward(L, f, om, phi, upx, upy, fs, om2, fcx, fcy, t);
         // force and torque adjoints seen by this node's contribution
         const double lfx = fbar[0] - fbar[2] * dy, lfy = fbar[1] + fbar[2] * dx;
-        const double lw = P.rheology == 1 ? lw_next[n] : 0.0;
+        const double lw = P.rheology != 0 ? lw_next[n] : 0.0;
         double fb[9], omb, phib, upxb, upyb, tyb = 0.0, mub = 0.0;
         adj::node_backward(L, t, lfs, lw, lfx, lfy, fb, omb, phib, upxb, upyb, tyb, mub);
         for (int q = 0; q < 9; ++q) lf_cur[q * P.plane + n] = fb[q];
-        if (P.rheology == 1) lw_cur[n] = omb;
+        if (P.rheology != 0) lw_cur[n] = omb;
         else mub += omb * (-3.0 * P.omega * P.omega);  // OMEGA = 1/(3 mu0 + 1/2)
         // phi(d), d = r - |x - X|: dphi/dX = phi'(d) dx / dist
         const double dist = sqrt(dx * dx + dy * dy);

@@ -120,7 +120,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     // F6:115-118: tau = 1/omega must not be below 0.5
     if (!(p->omega > 0.0) || 1.0 / p->omega < 0.5)
         return bad_create("relaxation time 1/omega below 0.5 (F6:115-118)");
-    if (p->rheology != LBM_NEWTONIAN && p->rheology != LBM_BINGHAM)
+    if (p->rheology != LBM_NEWTONIAN && p->rheology != LBM_BINGHAM && p->rheology != LBM_CASSON)
         return bad_create("unknown rheology");
     if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return bad_create("unknown storage");
     if (p->max_particles < 0 || p->max_particles > 65535)
@@ -152,6 +152,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.fs3 = (float)P.s3; P.fs5 = (float)P.s5; P.fsq = (float)P.sq; P.fnu = (float)P.nu;
     P.fmu0 = (float)P.mu0; P.ftau_y = (float)P.tau_y; P.fm_pap = (float)P.m_pap;
     P.fgamma_min = (float)P.gamma_min; P.frho0 = (float)P.rho0;
+    P.fsqrt_mu0 = (float)std::sqrt(P.mu0);
+    P.fcas_lim = (float)(std::sqrt(P.mu0) + std::sqrt(P.m_pap * P.tau_y));
     // FP32 collide factors (D_k / N_k = n_k cDka + mf_k cDkb): mf_1 = 6 fu,
     // mf_2 = -6 fu, mf_3 = fx, mf_4 = -fx, mf_5 = fy, mf_6 = -fy
     P.cD1a = (float)(-P.s1 / 36.0); P.cD1b = (float)((1.0 - 0.5 * P.s1) * 6.0 / 36.0);
@@ -554,7 +556,7 @@ int lbm_restart_write(lbm_handle *h, const char *dir) {
         return rc;
     // OMEGA_MRT(0:8): the relaxation vector the last node of the J-outer /
     // I-inner sweep leaves behind (F6:958, F6:1009-1011)
-    const double om = h->prm.rheology == LBM_BINGHAM ? obn[(size_t)P.ny * P.W + P.nx] : P.omega;
+    const double om = h->prm.rheology != LBM_NEWTONIAN ? obn[(size_t)P.ny * P.W + P.nx] : P.omega;
     const double q = P.q_mode ? (8.0 * (2.0 - om)) / (8.0 - om) : P.sq;
     const double omv[9] = {P.s0, P.s1, P.s2, P.s3, q, P.s5, q, om, om};
     bool ok = write_nodes(join_path(dir, RS_F), P, 9, f.data()) &&
@@ -711,7 +713,7 @@ int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes) {
 int lbm_collide(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
     const DevP &P = h->P;
-    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED) {
+    if (h->prm.rheology != LBM_NEWTONIAN && h->prm.rate_mode == LBM_RATE_LAGGED) {
         hipLaunchKernelGGL(k_collide_lagged, dim3(1), dim3(64), 0, h->stream, P, h->f, h->rho,
                            h->ux, h->uy, h->g, h->obn, h->stress, h->image, h->lag);
         return check_launch(h, "k_collide_lagged");
@@ -1017,7 +1019,7 @@ int fused_dispatch(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int 
 }
 
 int check_fused_mode(lbm_handle *h) {
-    if (h->prm.rheology == LBM_BINGHAM && h->prm.rate_mode == LBM_RATE_LAGGED)
+    if (h->prm.rheology != LBM_NEWTONIAN && h->prm.rate_mode == LBM_RATE_LAGGED)
         return fail(h, LBM_ERR_UNSUPPORTED,
                     "the lagged module-global rate is a sequential recurrence; the GPU path "
                     "implements rate_mode LBM_RATE_LOCAL");
@@ -1535,6 +1537,9 @@ extern "C" int lbm_adjoint_velocity(lbm_handle *h, int32_t nsteps, int32_t windo
     if (pr.storage != LBM_FP64 && pr.storage != LBM_FP32)
         return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint recomputes from FP64 or FP32 storage only");
     if (P.np != 1) return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint follows exactly one particle");
+    if (pr.rheology == LBM_CASSON)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint differentiates the Newtonian and "
+                                            "Papanastasiou-Bingham rates only");
     if (P.motion != LBM_MOTION_FREE)
         return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs the free (Newton-Euler) particle update");
     for (int k = 0; k < 8; ++k)
@@ -1667,7 +1672,7 @@ extern "C" int lbm_adjoint_velocity(lbm_handle *h, int32_t nsteps, int32_t windo
             cur = nx;
         }
     }
-    if (P.rheology == 1)
+    if (P.rheology == LBM_BINGHAM)
         hipLaunchKernelGGL(k_adj_omega0, dim3(1), dim3(NT), 0, h->stream, P,
                            (const double *)lw[cur].p, (double *)theta.p);
     HIPCHECK(h, hipMemcpyAsync(grad, theta.p, 3 * 8, hipMemcpyDeviceToHost, h->stream));

@@ -42,7 +42,7 @@ struct DevP {
     int nx, ny, W, H;
     long long nn;        // nodes (W*H)
     long long plane;     // SoA plane stride in elements
-    int rheology, q_mode, update_angle, np;
+    int rheology, q_mode, update_angle, np;  // rheology: LBM_NEWTONIAN/BINGHAM/CASSON
     int motion;          // 0 free, 1 prescribed x-oscillation
     double osc_u, osc_period;
     int maxp;            // particle SoA stride
@@ -58,6 +58,7 @@ struct DevP {
     // for the uniform rates (forcing-moment constants c_k folded in), the
     // fixed-q factors, and the strain-rate factors -s_1/4 and -0.75 s_1
     float cD1a, cD1b, cD2a, cD2b, cD3a, cD3b, cD5a, cD5b, cDq1, cDq2, cS1, cS2;
+    float fsqrt_mu0, fcas_lim;  // Casson: sqrt(mu0), sqrt(mu0) + sqrt(m tau_y)
     unsigned buf_bytes;  // bytes of one 9-plane storage buffer (< 2^32)
 };
 
@@ -523,13 +524,23 @@ __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho
     o.stress[2] = P.nu * 2.0 * d12;
     o.omega_bn = om;
     o.yielded = 0;
-    if (P.rheology == 1) {
-        // Papanastasiou-Bingham viscosity, F6:921-957
+    if (P.rheology != 0) {
+        // Papanastasiou-Bingham viscosity, F6:921-957, or (rheology 2) the
+        // Papanastasiou-Casson lines F6 carries commented out (F6:941,
+        // F6:944; the model of F3:986-989, whose zero-shear limit is written
+        // without the square)
         double inv_rho = 1.0 / rho;
         d11 = d11 * inv_rho; d22 = d22 * inv_rho; d12 = d12 * inv_rho;
         double gam = sqrt(2.0 * fabs(d11 * d11 + d22 * d22 + 2.0 * (d12 * d12)));
         double nu_a;
-        if (gam < P.gamma_min)
+        if (P.rheology == 2) {
+            if (gam < P.gamma_min) {
+                nu_a = sqrt(P.mu0) + sqrt(P.m_pap * P.tau_y);
+            } else {
+                double c = sqrt(P.mu0) + sqrt(P.tau_y / gam) * (1.0 - exp(-sqrt(P.m_pap * gam)));
+                nu_a = c * c;
+            }
+        } else if (gam < P.gamma_min)
             nu_a = P.mu0 + P.m_pap * P.tau_y;
         else
             nu_a = P.mu0 + (P.tau_y / gam) * (1.0 - exp(-(P.m_pap * gam)));
@@ -703,7 +714,7 @@ __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho
     h[8] = h[8] + (t68 - vu);
     om_out = om;
     yielded = 0;
-    if (!OUT && P.rheology != 1) return;
+    if (!OUT && P.rheology == 0) return;
     // Chai strain rate (F6:903-924).  The strain moments f1 = C (s * n)
     // enter d11, d22, d12 only through three column sums of C, which leave
     //   d11 = -(e0 + e1/4 + 3 e7/4), d22 = -(e0 + e1/4 - 3 e7/4), d12 = -3 e8/2
@@ -720,13 +731,21 @@ __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho
         float nu = P.fnu;
         stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
     }
-    if (P.rheology == 1) {
+    if (P.rheology != 0) {
         float inv_rho = frcp(rho);
         d11 *= inv_rho; d22 *= inv_rho; d12 *= inv_rho;
         float gam = fsqrt(2.0f * fabsf(d11 * d11 + d22 * d22 + 2.0f * (d12 * d12)));
         float mu0 = P.fmu0, ty = P.ftau_y, mp = P.fm_pap;
         float nu_a;
-        if (gam < P.fgamma_min)
+        if (P.rheology == 2) {
+            // Papanastasiou-Casson (F6:941, F6:944)
+            if (gam < P.fgamma_min) {
+                nu_a = P.fcas_lim;
+            } else {
+                float c = P.fsqrt_mu0 + fsqrt(ty * frcp(gam)) * (1.0f - fexp(-fsqrt(mp * gam)));
+                nu_a = c * c;
+            }
+        } else if (gam < P.fgamma_min)
             nu_a = mu0 + mp * ty;
         else
             nu_a = mu0 + (ty * frcp(gam)) * (1.0f - fexp(-(mp * gam)));

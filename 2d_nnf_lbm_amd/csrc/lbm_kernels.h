@@ -52,7 +52,7 @@ __global__ void k_collide(DevP P, double *f, const double *rho, const double *ux
     double v[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) v[k] = f[k * P.plane + n];
-    double om = P.rheology == 1 ? obn[n] : P.omega;
+    double om = P.rheology != 0 ? obn[n] : P.omega;
     CollideOut o;
     collide64(P, v, rho[n], ux[n], uy[n], g[n], g[P.plane + n], om, o);
 #pragma unroll
@@ -60,7 +60,7 @@ __global__ void k_collide(DevP P, double *f, const double *rho, const double *ux
     stress[n] = o.stress[0];
     stress[P.plane + n] = o.stress[1];
     stress[2 * P.plane + n] = o.stress[2];
-    if (P.rheology == 1) {
+    if (P.rheology != 0) {
         image[n] = o.yielded ? 0 : 5;
         obn[n] = o.omega_bn;
     }
@@ -639,7 +639,7 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             }
             // density deviation from the same populations
             float rd = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
-            float om = (float)(P.rheology == 1 ? a.cobn[n] : P.omega), om2, str[3];
+            float om = (float)(P.rheology != 0 ? a.cobn[n] : P.omega), om2, str[3];
             if (a.aux || a.slots_yield)
                 collide_dev<true>(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy,
                                   om, om2, str, yielded);
@@ -652,11 +652,11 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             for (int k = 0; k < 9; ++k)
                 St::bst(drs, h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
                         (unsigned)(k * P.plane * (long long)sizeof(typename St::T)), sr_word(nh, k));
-            if (P.rheology == 1) a.rate32[n] = om2;
+            if (P.rheology != 0) a.rate32[n] = om2;
             if (a.aux) {
                 a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
                 a.cstress[2 * P.plane + n] = str[2];
-                if (P.rheology == 1) a.cimage[n] = yielded ? 0 : 5;
+                if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
             }
             if (b >= 0) ((float *)a.erho_dst)[b] = (float)r;
         } else {
@@ -664,16 +664,16 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
 #pragma unroll
             for (int k = 0; k < 9; ++k) f[k] = a.cf[k * P.plane + n];
             CollideOut o;
-            double om = P.rheology == 1 ? a.cobn[n] : P.omega;
+            double om = P.rheology != 0 ? a.cobn[n] : P.omega;
             collide64(P, f, r, ux, uy, gx, gy, om, o);
 #pragma unroll
             for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
-            if (P.rheology == 1) a.cobn[n] = o.omega_bn;
+            if (P.rheology != 0) a.cobn[n] = o.omega_bn;
             yielded = o.yielded;
             if (a.aux) {
                 a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
                 a.cstress[2 * P.plane + n] = o.stress[2];
-                if (P.rheology == 1) a.cimage[n] = o.yielded ? 0 : 5;
+                if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
             }
             if (b >= 0) ((double *)a.erho_dst)[b] = r;
         }
@@ -837,7 +837,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
         if constexpr (DEV) {
             float h[9], rd, rho, ux, uy;
             load_fk_dev<St>(P, src, (const float *)a.erho_src, n, i, j, h);
-            float om = (!FINAL && P.rheology == 1) ? a.rate32[n] : P.fomega;
+            float om = (!FINAL && P.rheology != 0) ? a.rate32[n] : P.fomega;
             // particle test after the population loads are in flight
             const bool any = tile_has_particles_f32(a, tile);
             macros_dev(h, P.frho0, rd, rho, ux, uy);
@@ -855,8 +855,8 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                 }
                 a.crho[n] = mass; a.cux[n] = ucx; a.cuy[n] = ucy;
                 a.cg[n] = gx; a.cg[P.plane + n] = gy;
-                if (P.rheology == 1) a.cobn[n] = a.rate32[n];
-                if (a.aux && P.rheology != 1) a.cimage[n] = image_node(P, a.pin, i, j);
+                if (P.rheology != 0) a.cobn[n] = a.rate32[n];
+                if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
             } else {
                 typename St::T *dst = (typename St::T *)a.dst;
                 float om2, str[3];
@@ -871,11 +871,11 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                     St::bst(drs, h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
                             (unsigned)(k * P.plane * (long long)sizeof(typename St::T)),
                             sr_word(nh, k));
-                if (P.rheology == 1) a.rate32[n] = om2;
+                if (P.rheology != 0) a.rate32[n] = om2;
                 if (a.aux) {
                     a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
                     a.cstress[2 * P.plane + n] = str[2];
-                    if (P.rheology == 1) a.cimage[n] = yielded ? 0 : 5;
+                    if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
                 }
                 if (b >= 0) ((float *)a.erho_dst)[b] = rho;
             }
@@ -894,20 +894,20 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                 for (int k = 0; k < 9; ++k) cfk[k * P.plane + n] = f[k];
                 a.crho[n] = rho; a.cux[n] = ucx; a.cuy[n] = ucy;
                 a.cg[n] = gx; a.cg[P.plane + n] = gy;
-                if (a.aux && P.rheology != 1) a.cimage[n] = image_node(P, a.pin, i, j);
+                if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
             } else {
                 typename St::T *dst = (typename St::T *)a.dst;
                 CollideOut o;
-                double om = P.rheology == 1 ? a.cobn[n] : P.omega;
+                double om = P.rheology != 0 ? a.cobn[n] : P.omega;
                 collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
 #pragma unroll
                 for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
-                if (P.rheology == 1) a.cobn[n] = o.omega_bn;
+                if (P.rheology != 0) a.cobn[n] = o.omega_bn;
                 yielded = o.yielded;
                 if (a.aux) {
                     a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
                     a.cstress[2 * P.plane + n] = o.stress[2];
-                    if (P.rheology == 1) a.cimage[n] = o.yielded ? 0 : 5;
+                    if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
                 }
                 if (b >= 0) ((double *)a.erho_dst)[b] = rho;
             }

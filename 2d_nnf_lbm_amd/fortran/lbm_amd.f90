@@ -9,7 +9,7 @@ MODULE LBM_AMD
   IMPLICIT NONE
 
   INTEGER, PARAMETER :: LBM_PF_COUNT = 22, LBM_H_PCOUNT = 12
-  INTEGER, PARAMETER :: LBM_NEWTONIAN = 0, LBM_BINGHAM = 1
+  INTEGER, PARAMETER :: LBM_NEWTONIAN = 0, LBM_BINGHAM = 1, LBM_CASSON = 2
   INTEGER, PARAMETER :: LBM_RATE_LAGGED = 0, LBM_RATE_LOCAL = 1
   INTEGER, PARAMETER :: LBM_MOTION_FREE = 0, LBM_MOTION_OSC_X = 1
   INTEGER, PARAMETER :: LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16 = 2, LBM_BF16 = 3

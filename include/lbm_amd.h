@@ -38,6 +38,7 @@ extern "C" {
 
 #define LBM_NEWTONIAN 0
 #define LBM_BINGHAM 1            /* Papanastasiou-regularised Bingham (F6:939-945) */
+#define LBM_CASSON 2             /* Papanastasiou-regularised Casson (F6:941, F6:944; F3:986-989) */
 
 #define LBM_MOTION_FREE 0        /* Newton-Euler update of the particles (F6:299-312) */
 #define LBM_MOTION_OSC_X 1       /* prescribed x-oscillation V_x = -U cos(2 pi KK / T) (F4:221-226) */
@@ -70,7 +71,7 @@ enum {
  * become run-time values. */
 typedef struct lbm_params {
     int32_t nx, ny;            /* F6:8-9 */
-    int32_t rheology;          /* LBM_NEWTONIAN / LBM_BINGHAM (F6:4) */
+    int32_t rheology;          /* LBM_NEWTONIAN / LBM_BINGHAM / LBM_CASSON (F6:4) */
     int32_t rate_mode;         /* LBM_RATE_*; the GPU path implements LOCAL */
     int32_t q_mode;            /* 1: s4=s6=8(2-w)/(8-w) (F6:1009); 0: sq */
     int32_t update_angle;      /* 1: integrate the angle (F6:312); 0: F3:327 */
@@ -157,7 +158,8 @@ int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes);
 int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes);
 
 /* Drop-in per-subroutine entries (FP64 state, reference semantics). */
-/* Collision_MRT, F6:867-962.  With LBM_BINGHAM and LBM_RATE_LAGGED it keeps
+/* Collision_MRT, F6:867-962.  With a yield-stress rheology (LBM_BINGHAM,
+ * LBM_CASSON) and LBM_RATE_LAGGED it keeps
  * the reference's module-global rate (each node collides with the rate the
  * node before it set, F6:958), which is a sequential chain over the grid:
  * one GPU thread walks it, so use it for drop-in fidelity on small grids. */

@@ -23,6 +23,10 @@ Patches (each is a parameter or syntax edit, none changes the algorithm):
     lines (F6:1009-1011) are commented out and the commented "Mohammad MRT"
     set (F6:1017-1018) is enabled, the rate set of the one-disk Newtonian
     workload (also F1:1044-1045).
+  * Optional ("casson" rheology): in Collision_MRT the two Papanastasiou-
+    Bingham viscosity lines (F6:940, F6:943) are commented out and the two
+    Casson lines F6 carries commented next to them (F6:941, F6:944; the same
+    model as F3:986-989) are enabled.
 
 A second recipe builds the N-body contact force alone: MODULE PARAM
 (F3:2-83) and SUBROUTINE ComputeCollisionForce (F3:632-806) cut from F3,
@@ -69,6 +73,8 @@ VARIANTS = {
                         uxn="0.01D0"),
     "f6_newt_mohammad": dict(nx=40, ny=120, non_newtonian=False, time_max=1000,
                              rates="mohammad"),
+    "f6_cas_local": dict(nx=40, ny=120, non_newtonian=True, time_max=1000,
+                         local_rate=True, rheology="casson"),
 }
 # CPU-baseline build: the reference's Bingham settling disk on the 401x1601
 # node grid (NX=400, NY=1600), timed for a bounded number of steps.
@@ -79,10 +85,11 @@ TIMING_VARIANTS = {
 
 
 def patch_source(text, nx, ny, non_newtonian, time_max, local_rate=False,
-                 uxn=None, write_output=True, rates=None):
+                 uxn=None, write_output=True, rates=None, rheology=None):
     lines = text.replace("\r", "").split("\n")
     out = []
     in_collide = False
+    in_rate = False
     in_relax = False
     enable = 0
     for n, line in enumerate(lines, start=1):
@@ -118,6 +125,16 @@ def patch_source(text, nx, ny, non_newtonian, time_max, local_rate=False,
             s = s.replace("UXN=0.0", "UXN=" + uxn, 1)
         if re.match(r"\s*SUBROUTINE Collision_MRT\(", s):
             in_collide = True
+            in_rate = True
+        elif re.match(r"\s*END SUBROUTINE Collision_MRT", s):
+            in_rate = False
+        if rheology == "casson" and in_rate:
+            # the Casson lines F6 carries commented out next to the
+            # Papanastasiou-Bingham ones (F6:939-945)
+            if re.match(r"\s*Nu_A=.*!Bingham\s*$", s):
+                s = "!" + s
+            elif re.match(r"\s*!Nu_A=.*!Casson", s):
+                s = s.replace("!", "", 1)
         out.append(s)
         if in_collide and re.match(r"\s*DO I=0,nX\s*$", s):
             if local_rate:

@@ -142,13 +142,23 @@ void orc_collide(const orc_params *p, orc_state *st)
             str[0] = p->nu * 2.0 * d11;
             str[1] = p->nu * 2.0 * d22;
             str[2] = p->nu * 2.0 * d12;
-            if (p->rheology == 1) {
-                /* Papanastasiou-Bingham, F6:921-958. */
+            if (p->rheology != 0) {
+                /* Papanastasiou-Bingham (F6:921-958) or, rheology 2, the
+                 * Papanastasiou-Casson lines F6 carries commented out
+                 * (F6:941, F6:944; F3:986-989). */
                 double inv_rho = 1.0 / rho;
                 d11 = d11 * inv_rho; d22 = d22 * inv_rho; d12 = d12 * inv_rho;
                 double gam = sqrt(2.0 * fabs(d11 * d11 + d22 * d22 + 2.0 * (d12 * d12)));
                 double nu_a;
-                if (gam < p->gamma_min)
+                if (p->rheology == 2) {
+                    if (gam < p->gamma_min) {
+                        nu_a = sqrt(p->mu0) + sqrt(p->m_pap * p->tau_y);
+                    } else {
+                        double c = sqrt(p->mu0) +
+                                   sqrt(p->tau_y / gam) * (1.0 - exp(-sqrt(p->m_pap * gam)));
+                        nu_a = c * c;
+                    }
+                } else if (gam < p->gamma_min)
                     nu_a = p->mu0 + p->m_pap * p->tau_y;
                 else
                     nu_a = p->mu0 + (p->tau_y / gam) * (1.0 - exp(-(p->m_pap * gam)));

@@ -43,7 +43,8 @@ enum {
 
 typedef struct orc_params {
     int32_t nx, ny;          /* cells; nodes 0..nx, 0..ny (F6:8-9) */
-    int32_t rheology;        /* 0 Newtonian, 1 Papanastasiou-Bingham (F6:4) */
+    int32_t rheology;        /* 0 Newtonian, 1 Papanastasiou-Bingham (F6:4),
+                                2 Papanastasiou-Casson (F6:941, F6:944) */
     int32_t rate_mode;       /* 0 lagged module-global rate (F6:890, F6:958),
                                 1 each node's own previous-step rate */
     int32_t q_mode;          /* 1: s4=s6=8(2-w)/(8-w) (F6:1009); 0: sq */

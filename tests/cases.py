@@ -83,4 +83,6 @@ def golden_params(meta):
                     float(meta["uxn"].replace("D", "e")) if meta.get("uxn") else 0.0)
     if meta.get("rates") == "mohammad":
         cfg.update(MOHAMMAD_RATES)
+    if meta.get("rheology") == "casson":
+        cfg["rheology"] = 2  # F6:941, F6:944
     return cfg

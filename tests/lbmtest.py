@@ -15,7 +15,7 @@ def gpu_solver(cfg, storage="fp64", max_particles=8):
     c = dict(cfg)
     c["storage"] = storage
     c["max_particles"] = max_particles
-    if c.get("rheology") == 1:
+    if c.get("rheology", 0) != 0:
         c["rate_mode"] = 1
     return pkg().D2Q9(c)
 

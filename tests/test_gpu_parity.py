@@ -51,12 +51,14 @@ def _compare_state(g, st, label=""):
     check_field_errors(field_errors(g.f, rho, ux, uy, gg, om, ref), label)
 
 
-@pytest.mark.parametrize("bingham", [False, True])
-def test_dropin_subroutines_match_oracle(orc, bingham):
-    """Per-subroutine entries, called in the reference's loop order."""
+@pytest.mark.parametrize("rheology", [0, 1, 2])
+def test_dropin_subroutines_match_oracle(orc, rheology):
+    """Per-subroutine entries, called in the reference's loop order
+    (Newtonian, Papanastasiou-Bingham, Papanastasiou-Casson)."""
     nx, ny = 40, 120
-    cfg, part = f6_case(nx, ny, bingham)
-    print("case: dropin bingham=%s %dx%d" % (bingham, nx, ny), flush=True)
+    cfg, part = f6_case(nx, ny, rheology != 0)
+    cfg["rheology"] = rheology
+    print("case: dropin rheology=%d %dx%d" % (rheology, nx, ny), flush=True)
     st = oracle_state(orc, cfg, part)
     g = gpu_solver(cfg)
     g.initialize()
@@ -75,7 +77,8 @@ def test_dropin_subroutines_match_oracle(orc, bingham):
     assert abs(g.total_mass() - st.total_mass()) <= 1e-12 * st.total_mass()
 
 
-@pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local", "f6_newt_lid", "f6_newt_mohammad"])
+@pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local", "f6_newt_lid", "f6_newt_mohammad",
+                                  "f6_cas_local"])
 def test_fused_fp64_matches_reference_dump(orc, name):
     """Fused FP64 steps against the reference program's own final dump
     (LOCAL rate for Bingham, which is what the GPU implements)."""

@@ -49,3 +49,25 @@ def test_reduced_storage_settling(orc, fp64_reference, storage):
     # FP64 state is rebuilt from the stored deviations at the end of the run
     rho, ux, uy, gg, om = g.get_fields()
     assert rel_err(rho, st.rho) <= 1e-3
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_reduced_storage_casson(orc, storage):
+    """The Papanastasiou-Casson rate (F6:941, F6:944) in the FP32 collide,
+    against the FP64 oracle: same bounds as the Bingham run."""
+    nx, ny, n = 60, 180, 2000
+    cfg, part = f6_case(nx, ny, True)
+    cfg["rheology"] = 2
+    print("case: reduced storage casson", storage, flush=True)
+    st = oracle_state(orc, cfg, part)
+    ho = st.run(0, n)
+    g = gpu_solver(cfg, storage=storage)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, n)
+    vy_g, vy_o = hg[-1, orc.H_VY], ho[-1, orc.H_VY]
+    e_vy = abs(vy_g - vy_o) / abs(vy_o)
+    print("casson %s: vy %.3e (ref %.6e)" % (storage, e_vy, vy_o), flush=True)
+    assert np.all(np.isfinite(hg))
+    assert e_vy <= VY_TOL[storage]
+    assert rel_err(hg[:, -2], ho[:, -2]) <= 1e-4
