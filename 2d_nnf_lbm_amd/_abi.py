@@ -14,6 +14,7 @@ LBM_ERR_NAN = -5
 LBM_ERR_IO = -6
 
 LBM_NEWTONIAN, LBM_BINGHAM, LBM_CASSON = 0, 1, 2
+LBM_BOUNCE_BACK, LBM_ZOU_HE = 0, 1
 LBM_RATE_LAGGED, LBM_RATE_LOCAL = 0, 1
 LBM_MOTION_FREE, LBM_MOTION_OSC_X = 0, 1
 LBM_FP64, LBM_FP32, LBM_FP16, LBM_BF16 = 0, 1, 2, 3
@@ -35,7 +36,7 @@ class Params(ctypes.Structure):
         ("rheology", ctypes.c_int32), ("rate_mode", ctypes.c_int32),
         ("q_mode", ctypes.c_int32), ("update_angle", ctypes.c_int32),
         ("storage", ctypes.c_int32), ("max_particles", ctypes.c_int32),
-        ("motion", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("motion", ctypes.c_int32), ("boundary", ctypes.c_int32),
         ("omega", ctypes.c_double),
         ("s0", ctypes.c_double), ("s1", ctypes.c_double), ("s2", ctypes.c_double),
         ("s3", ctypes.c_double), ("s5", ctypes.c_double), ("sq", ctypes.c_double),

@@ -122,6 +122,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         return bad_create("relaxation time 1/omega below 0.5 (F6:115-118)");
     if (p->rheology != LBM_NEWTONIAN && p->rheology != LBM_BINGHAM && p->rheology != LBM_CASSON)
         return bad_create("unknown rheology");
+    if (p->boundary != LBM_BOUNCE_BACK && p->boundary != LBM_ZOU_HE)
+        return bad_create("unknown boundary mode");
     if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return bad_create("unknown storage");
     if (p->max_particles < 0 || p->max_particles > 65535)
         return bad_create("max_particles outside 0..65535");
@@ -138,6 +140,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.plane = (P.nn + 255) / 256 * 256;
     P.rheology = p->rheology; P.q_mode = p->q_mode; P.update_angle = p->update_angle;
     P.motion = p->motion; P.osc_u = p->osc_u; P.osc_period = p->osc_period;
+    P.boundary = p->boundary;
     P.np = 0;
     P.maxp = std::max(1, p->max_particles);
     P.nbnd = 2 * P.W + 2 * (P.H - 2);
@@ -1546,6 +1549,8 @@ extern "C" int lbm_adjoint_velocity(lbm_handle *h, int32_t nsteps, int32_t windo
         if (P.wall[k] != 0.0) return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs walls at rest");
     if (P.corner[0] != 0.0 || P.corner[1] != 0.0)
         return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs walls at rest");
+    if (pr.boundary != LBM_BOUNCE_BACK)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs the bounce-back walls");
     int rc = check_fused_mode(h);
     if (rc) return rc;
 

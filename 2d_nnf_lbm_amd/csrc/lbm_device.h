@@ -44,6 +44,7 @@ struct DevP {
     long long plane;     // SoA plane stride in elements
     int rheology, q_mode, update_angle, np;  // rheology: LBM_NEWTONIAN/BINGHAM/CASSON
     int motion;          // 0 free, 1 prescribed x-oscillation
+    int boundary;        // 0 halfway bounce-back, 1 Zou-He edges
     double osc_u, osc_period;
     int maxp;            // particle SoA stride
     int nbnd;            // boundary nodes
@@ -204,6 +205,43 @@ __device__ __forceinline__ void rates64(const DevP &P, double om, double s[9]) {
     s[5] = P.s5; s[6] = q; s[7] = om; s[8] = om;
 }
 
+// Zou-He edge rule (F1:1246-1258 top and bottom, F1:1319-1341 left and
+// right) at edge node (i, j), corners excluded: the three unknown
+// populations from the node's own streamed ones and the wall velocity
+// (P.wall: uxS,uyS, uxR,uyR, uxN,uyN, uxL,uyL), in F1's operation order.
+// v holds f (rho_base = 0) or deviations f - w_i rho0 (rho_base = rho0:
+// the density sums of the rule pick up exactly rho0 from the weights).
+template <class T>
+__device__ __forceinline__ void zou_he_edge(const DevP &P, int i, int j, T rho_base, T v[9]) {
+    const T inv6 = (T)INV6;
+    const double *wl = P.wall;
+    if (j == P.ny) {
+        const T ux = (T)wl[4], uy = (T)wl[5];
+        const T r = (rho_base + (v[0] + v[1] + v[3] + (T)2. * (v[2] + v[6] + v[5]))) / ((T)1.0 + uy);
+        v[4] = v[2] - (T)4. * inv6 * r * uy;
+        v[8] = v[6] - (T)0.5 * (v[1] - v[3]) + (T)0.5 * r * ux - inv6 * r * uy;
+        v[7] = v[5] + (T)0.5 * (v[1] - v[3]) - (T)0.5 * r * ux - inv6 * r * uy;
+    } else if (j == 0) {
+        const T ux = (T)wl[0], uy = (T)wl[1];
+        const T r = (rho_base + (v[0] + v[1] + v[3] + (T)2. * (v[4] + v[7] + v[8]))) / ((T)1.0 - uy);
+        v[2] = v[4] + (T)4.0 * inv6 * r * uy;
+        v[5] = v[7] - (T)0.5 * (v[1] - v[3]) + (T)0.5 * r * ux + inv6 * r * uy;
+        v[6] = v[8] + (T)0.5 * (v[1] - v[3]) - (T)0.5 * r * ux + inv6 * r * uy;
+    } else if (i == 0) {
+        const T ux = (T)wl[6], uy = (T)wl[7];
+        const T r = (rho_base + (v[0] + v[2] + v[4] + (T)2.0 * (v[3] + v[6] + v[7]))) / ((T)1.0 - ux);
+        v[5] = v[7] - (T)0.5 * (v[2] - v[4]) + inv6 * r * ux + (T)0.5 * r * uy;
+        v[8] = v[6] + (T)0.5 * (v[2] - v[4]) + inv6 * r * ux - (T)0.5 * r * uy;
+        v[1] = v[3] + (T)4.0 * inv6 * r * ux;
+    } else {  // i == nx
+        const T ux = (T)wl[2], uy = (T)wl[3];
+        const T r = (rho_base + (v[0] + v[2] + v[4] + (T)2.0 * (v[1] + v[5] + v[8]))) / ((T)1.0 + ux);
+        v[6] = v[8] - (T)0.5 * (v[2] - v[4]) - inv6 * r * ux + (T)0.5 * r * uy;
+        v[7] = v[5] + (T)0.5 * (v[2] - v[4]) - inv6 * r * ux - (T)0.5 * r * uy;
+        v[3] = v[1] - (T)4.0 * inv6 * r * ux;
+    }
+}
+
 // Pull-stream with in-domain sources (F6:1164-1188) and the bounce-back
 // rules (F6:1206-1294, F6:1364-1386) for node (i, j): v receives f_k.
 // src: post-collision populations (SoA planes), erho: previous-step density
@@ -226,6 +264,11 @@ __device__ __forceinline__ void load_fk64(const DevP &P, const typename St::T *s
         v[k] = St::ld(src + k * P.plane + s);
     }
     if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
+        const bool corner = (i == 0 || i == P.nx) && (j == 0 || j == P.ny);
+        if (P.boundary == 1 && !corner) {
+            zou_he_edge<double>(P, i, j, 0.0, v);
+            return;
+        }
         const double clb2 = 1.0;
         double rw = erho[bnd_index(P, i, j)];
         auto fo = [&](int k) { return St::ld(src + k * P.plane + node); };
@@ -580,6 +623,11 @@ __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T 
         h[k] = St::ld(src + k * P.plane + s);
     }
     if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
+        const bool corner = (i == 0 || i == P.nx) && (j == 0 || j == P.ny);
+        if (P.boundary == 1 && !corner) {
+            zou_he_edge<float>(P, i, j, P.frho0, h);
+            return;
+        }
         float rw = erho[bnd_index(P, i, j)];
         const float a = (float)INV6, b = (float)(4.0 * INV6);
         auto fo = [&](int k) { return St::ld(src + k * P.plane + node); };

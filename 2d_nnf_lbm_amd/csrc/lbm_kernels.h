@@ -115,15 +115,26 @@ __device__ __forceinline__ void bnd_node(const DevP &P, int b, int &i, int &j) {
     else { i = P.nx; j = b - 2 * P.W - (P.H - 2) + 1; }
 }
 
-// BOUNDARYCONDITION, F6:1195-1390: one thread per boundary node.  The rules
-// at each node are those of load_fk64 with the streamed values kept for the
-// populations the rules do not touch.
+// BOUNDARYCONDITION, F6:1195-1390 (or the Zou-He edges of F1:1222-1417):
+// one thread per boundary node.  The rules at each node are those of
+// load_fk64 with the streamed values kept for the populations the rules do
+// not touch.
 __global__ void k_boundary(DevP P, double *f, const double *fp, const double *rho) {
     int b = blockIdx.x * NT + threadIdx.x;
     if (b >= P.nbnd) return;
     int i, j;
     bnd_node(P, b, i, j);
     long long n = (long long)j * P.W + i;
+    const bool corner = (i == 0 || i == P.nx) && (j == 0 || j == P.ny);
+    if (P.boundary == 1 && !corner) {
+        double v[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = f[k * P.plane + n];
+        zou_he_edge<double>(P, i, j, 0.0, v);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k * P.plane + n] = v[k];
+        return;
+    }
     const double clb2 = 1.0;
     double rw = rho[n];
     auto fo = [&](int k) { return fp[k * P.plane + n]; };

@@ -84,7 +84,7 @@ PROGRAM LBM_F6_DRIVER
   P%STORAGE = STORAGE
   P%MAX_PARTICLES = 1
   P%MOTION = LBM_MOTION_FREE
-  P%RESERVED = 0
+  P%BOUNDARY = LBM_BOUNCE_BACK
   P%OSC_U = 0D0
   P%OSC_PERIOD = 0D0
   P%OMEGA = OMEGA

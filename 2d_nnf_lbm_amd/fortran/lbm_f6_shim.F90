@@ -74,7 +74,7 @@ CONTAINS
     P%STORAGE = LBM_FP64
     P%MAX_PARTICLES = N_PARTICLE
     P%MOTION = LBM_MOTION_FREE
-    P%RESERVED = 0
+    P%BOUNDARY = LBM_BOUNCE_BACK
     P%OMEGA = OMEGA
     P%NU = NU
     P%MU0 = MU_0

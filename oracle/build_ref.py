@@ -23,6 +23,9 @@ Patches (each is a parameter or syntax edit, none changes the algorithm):
     lines (F6:1009-1011) are commented out and the commented "Mohammad MRT"
     set (F6:1017-1018) is enabled, the rate set of the one-disk Newtonian
     workload (also F1:1044-1045).
+  * Optional ("zouhe" boundary): F6's BOUNDARYCONDITION (F6:1195-1390) is
+    replaced by F1's (F1:1222-1417: Zou-He edges, bounce-back corners) with
+    F6's argument list, and optionally a top-wall velocity in F1:1230.
   * Optional ("casson" rheology): in Collision_MRT the two Papanastasiou-
     Bingham viscosity lines (F6:940, F6:943) are commented out and the two
     Casson lines F6 carries commented next to them (F6:941, F6:944; the same
@@ -57,6 +60,7 @@ import tempfile
 REF_ROOT = "/root/reference"
 F6 = os.path.join(REF_ROOT, "6_FreeFall_YieldStress",
                   "D2Q9_BinghamSediment_Validated_20191127.f90")
+F1 = os.path.join(REF_ROOT, "1_Disk_Newtonian", "D2Q9_1P_Validated_20190530.f90")
 F3 = os.path.join(REF_ROOT, "3_MultiParticle_Sedimentation",
                   "D2Q9_RTParticle_20191003.f90")
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -75,6 +79,8 @@ VARIANTS = {
                              rates="mohammad"),
     "f6_cas_local": dict(nx=40, ny=120, non_newtonian=True, time_max=1000,
                          local_rate=True, rheology="casson"),
+    "f6_zouhe_lid": dict(nx=40, ny=60, non_newtonian=False, time_max=300,
+                         boundary="zouhe", zouhe_uxn="0.01D0"),
 }
 # CPU-baseline build: the reference's Bingham settling disk on the 401x1601
 # node grid (NX=400, NY=1600), timed for a bounded number of steps.
@@ -143,9 +149,34 @@ def patch_source(text, nx, ny, non_newtonian, time_max, local_rate=False,
     return "\n".join(out) + "\n"
 
 
+def splice_zouhe(text, uxn=None):
+    """F6 with its BOUNDARYCONDITION (F6:1195-1390) replaced by F1's
+    (F1:1222-1417: Zou-He edges, bounce-back corners), F6's argument list
+    kept (F1's routine does not use KK).  uxn: top-wall velocity for the
+    Zou-He lid (F1:1230, zero in F1)."""
+    with open(F1, "rb") as fh:
+        f1 = fh.read().decode("ascii", errors="replace")
+    sub = cut(f1, r"\s*SUBROUTINE BOUNDARYCONDITION\(", r"\s*END SUBROUTINE BOUNDARYCONDITION\b")
+    sub[0] = sub[0].replace("(F,F_POST,RHO,UX,UY)", "(F,F_POST,RHO,UX,UY,KK)")
+    for k, line in enumerate(sub):
+        if re.match(r"\s*INTEGER :: I,J\s*$", line):
+            sub[k] = line.replace("I,J", "I,J,KK")
+        if uxn is not None and re.match(r"\s*UXN=0\.0;UYN=0\.0\s*$", line):
+            sub[k] = line.replace("UXN=0.0", "UXN=" + uxn, 1)
+    lines = text.replace("\r", "").split("\n")
+    a = next(k for k, l in enumerate(lines) if re.match(r"\s*SUBROUTINE BOUNDARYCONDITION\(", l))
+    b = next(k for k in range(a, len(lines))
+             if re.match(r"\s*END SUBROUTINE BOUNDARYCONDITION\b", lines[k]))
+    assert b - a == len(sub) - 1, "F1's routine no longer has F6's length (line-numbered patches)"
+    return "\n".join(lines[:a] + sub + lines[b + 1:]) + "\n"
+
+
 def build(name, cfg, flags):
     with open(F6, "rb") as fh:
         text = fh.read().decode("ascii")
+    cfg = dict(cfg)
+    if cfg.pop("boundary", None) == "zouhe":
+        text = splice_zouhe(text, cfg.pop("zouhe_uxn", None))
     src = patch_source(text, **cfg)
     os.makedirs(OUT_DIR, exist_ok=True)
     tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")

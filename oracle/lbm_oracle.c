@@ -192,8 +192,10 @@ void orc_stream(const orc_params *p, orc_state *st)
             }
 }
 
-/* BOUNDARYCONDITION, F6:1195-1390: half-way bounce-back with a moving-wall
- * term; rho is the previous step's density. */
+/* BOUNDARYCONDITION: halfway bounce-back with wall velocities on the edges
+ * (F6:1195-1390) or, boundary 1, the Zou-He edges of F1:1222-1417; the
+ * corners bounce back in both.  The bounce-back rules use the previous
+ * step's density. */
 void orc_boundary(const orc_params *p, orc_state *st, int32_t kk)
 {
     (void)kk;
@@ -205,6 +207,42 @@ void orc_boundary(const orc_params *p, orc_state *st, int32_t kk)
 #define FA(k, i, j) F[9 * NID(p, i, j) + (k)]
 #define FPA(k, i, j) FP[9 * NID(p, i, j) + (k)]
 #define RA(i, j) R[NID(p, i, j)]
+    if (p->boundary == 1) {
+        /* Zou-He edges, F1:1246-1258 (top, bottom) and F1:1319-1341 (left,
+         * right), from the node's own streamed populations. */
+        for (int i = 1; i <= nx - 1; ++i) {
+            double rhon = (FA(0, i, ny) + FA(1, i, ny) + FA(3, i, ny) +
+                           2. * (FA(2, i, ny) + FA(6, i, ny) + FA(5, i, ny))) / (1.0 + uyN);
+            FA(4, i, ny) = FA(2, i, ny) - 4. * inv6 * rhon * uyN;
+            FA(8, i, ny) = FA(6, i, ny) - 0.5 * (FA(1, i, ny) - FA(3, i, ny)) + 0.5 * rhon * uxN -
+                           inv6 * rhon * uyN;
+            FA(7, i, ny) = FA(5, i, ny) + 0.5 * (FA(1, i, ny) - FA(3, i, ny)) - 0.5 * rhon * uxN -
+                           inv6 * rhon * uyN;
+            double rhos = (FA(0, i, 0) + FA(1, i, 0) + FA(3, i, 0) +
+                           2. * (FA(4, i, 0) + FA(7, i, 0) + FA(8, i, 0))) / (1.0 - uyS);
+            FA(2, i, 0) = FA(4, i, 0) + 4.0 * inv6 * rhos * uyS;
+            FA(5, i, 0) = FA(7, i, 0) - 0.5 * (FA(1, i, 0) - FA(3, i, 0)) + 0.5 * rhos * uxS +
+                          inv6 * rhos * uyS;
+            FA(6, i, 0) = FA(8, i, 0) + 0.5 * (FA(1, i, 0) - FA(3, i, 0)) - 0.5 * rhos * uxS +
+                          inv6 * rhos * uyS;
+        }
+        for (int j = 1; j <= ny - 1; ++j) {
+            double rl = (FA(0, 0, j) + FA(2, 0, j) + FA(4, 0, j) +
+                         2.0 * (FA(3, 0, j) + FA(6, 0, j) + FA(7, 0, j))) / (1.0 - uxL);
+            FA(5, 0, j) = FA(7, 0, j) - 0.5 * (FA(2, 0, j) - FA(4, 0, j)) + inv6 * rl * uxL +
+                          0.5 * rl * uyL;
+            FA(8, 0, j) = FA(6, 0, j) + 0.5 * (FA(2, 0, j) - FA(4, 0, j)) + inv6 * rl * uxL -
+                          0.5 * rl * uyL;
+            FA(1, 0, j) = FA(3, 0, j) + 4.0 * inv6 * rl * uxL;
+            double rr = (FA(0, nx, j) + FA(2, nx, j) + FA(4, nx, j) +
+                         2.0 * (FA(1, nx, j) + FA(5, nx, j) + FA(8, nx, j))) / (1.0 + uxR);
+            FA(6, nx, j) = FA(8, nx, j) - 0.5 * (FA(2, nx, j) - FA(4, nx, j)) - inv6 * rr * uxR +
+                           0.5 * rr * uyR;
+            FA(7, nx, j) = FA(5, nx, j) + 0.5 * (FA(2, nx, j) - FA(4, nx, j)) - inv6 * rr * uxR -
+                           0.5 * rr * uyR;
+            FA(3, nx, j) = FA(1, nx, j) - 4.0 * inv6 * rr * uxR;
+        }
+    } else {
     for (int i = 1; i <= nx - 1; ++i) {
         FA(8, i, ny) = FPA(6, i, ny) - inv6 * clb2 * RA(i, ny) * (-uxN + uyN);
         FA(4, i, ny) = FPA(2, i, ny) - 4.0 * inv6 * clb2 * RA(i, ny) * (uyN);
@@ -221,6 +259,9 @@ void orc_boundary(const orc_params *p, orc_state *st, int32_t kk)
         FA(1, 0, j) = FPA(3, 0, j) - 4.0 * inv6 * clb2 * RA(0, j) * (-uxL);
         FA(8, 0, j) = FPA(6, 0, j) - inv6 * clb2 * RA(0, j) * (-uxL + uyL);
     }
+    }
+    /* corners: bounce-back with the corner's density (F6:1364-1386, the
+     * same in F1:1390-1414) */
     double uxw = p->corner[0], uyw = p->corner[1], rw;
     rw = RA(0, 0);
     FA(1, 0, 0) = FPA(3, 0, 0) - 4.0 * inv6 * clb2 * rw * (-uxw);

@@ -38,7 +38,7 @@ class Params(ctypes.Structure):
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
         ("rheology", ctypes.c_int32), ("rate_mode", ctypes.c_int32),
         ("q_mode", ctypes.c_int32), ("update_angle", ctypes.c_int32),
-        ("motion", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("motion", ctypes.c_int32), ("boundary", ctypes.c_int32),
         ("omega", ctypes.c_double),
         ("s0", ctypes.c_double), ("s1", ctypes.c_double), ("s2", ctypes.c_double),
         ("s3", ctypes.c_double), ("s5", ctypes.c_double), ("sq", ctypes.c_double),

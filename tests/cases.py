@@ -85,4 +85,8 @@ def golden_params(meta):
         cfg.update(MOHAMMAD_RATES)
     if meta.get("rheology") == "casson":
         cfg["rheology"] = 2  # F6:941, F6:944
+    if meta.get("boundary") == "zouhe":
+        cfg["boundary"] = 1  # F1:1222-1417
+        if meta.get("zouhe_uxn"):
+            cfg["wall"][4] = float(meta["zouhe_uxn"].replace("D", "e"))
     return cfg

@@ -77,8 +77,34 @@ def test_dropin_subroutines_match_oracle(orc, rheology):
     assert abs(g.total_mass() - st.total_mass()) <= 1e-12 * st.total_mass()
 
 
+def test_dropin_zou_he_matches_oracle(orc):
+    """Per-subroutine entries with the Zou-He edges of F1:1222-1417 (a
+    moving lid and a moving left wall, Bingham)."""
+    nx, ny = 40, 60
+    cfg, part = f6_case(nx, ny, True, uxn=0.01)
+    cfg["boundary"] = 1
+    cfg["wall"][7] = -0.004  # uyL
+    print("case: dropin zou-he %dx%d" % (nx, ny), flush=True)
+    st = oracle_state(orc, cfg, part)
+    g = gpu_solver(cfg)
+    g.initialize()
+    g.set_particles(part)
+    for kk in range(1, 41):
+        st.step_parts(kk)
+        g.step_parts(kk)
+    _compare_state(g, st, label="dropin zou-he")
+    # the same run fused
+    st2 = oracle_state(orc, cfg, part)
+    st2.run(0, 40)
+    g2 = gpu_solver(cfg)
+    g2.initialize()
+    g2.set_particles(part)
+    g2.step(0, 40)
+    _compare_state(g2, st2, label="fused zou-he")
+
+
 @pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local", "f6_newt_lid", "f6_newt_mohammad",
-                                  "f6_cas_local"])
+                                  "f6_cas_local", "f6_zouhe_lid"])
 def test_fused_fp64_matches_reference_dump(orc, name):
     """Fused FP64 steps against the reference program's own final dump
     (LOCAL rate for Bingham, which is what the GPU implements)."""
