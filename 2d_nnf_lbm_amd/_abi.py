@@ -14,7 +14,7 @@ LBM_ERR_NAN = -5
 LBM_ERR_IO = -6
 
 LBM_NEWTONIAN, LBM_BINGHAM, LBM_CASSON = 0, 1, 2
-LBM_BOUNCE_BACK, LBM_ZOU_HE = 0, 1
+LBM_BOUNCE_BACK, LBM_ZOU_HE, LBM_EQUILIBRIUM = 0, 1, 2
 LBM_RATE_LAGGED, LBM_RATE_LOCAL = 0, 1
 LBM_MOTION_FREE, LBM_MOTION_OSC_X = 0, 1
 LBM_FP64, LBM_FP32, LBM_FP16, LBM_BF16 = 0, 1, 2, 3

@@ -122,7 +122,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         return bad_create("relaxation time 1/omega below 0.5 (F6:115-118)");
     if (p->rheology != LBM_NEWTONIAN && p->rheology != LBM_BINGHAM && p->rheology != LBM_CASSON)
         return bad_create("unknown rheology");
-    if (p->boundary != LBM_BOUNCE_BACK && p->boundary != LBM_ZOU_HE)
+    if (p->boundary != LBM_BOUNCE_BACK && p->boundary != LBM_ZOU_HE &&
+        p->boundary != LBM_EQUILIBRIUM)
         return bad_create("unknown boundary mode");
     if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return bad_create("unknown storage");
     if (p->max_particles < 0 || p->max_particles > 65535)
@@ -208,8 +209,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->uy, pl * 8)) || (rc = al((void **)&h->g, 2 * pl * 8)) ||
         (rc = al((void **)&h->obn, pl * 8)) || (rc = al((void **)&h->stress, 3 * pl * 8)) ||
         (rc = al((void **)&h->image, pl * 4)) || (rc = al((void **)&h->staging, 9 * pl * 8)) ||
-        (rc = al((void **)&h->erho[0], (size_t)P.nbnd * 8)) ||
-        (rc = al((void **)&h->erho[1], (size_t)P.nbnd * 8)) ||
+        (rc = al((void **)&h->erho[0], (size_t)P.nbnd * 3 * 8)) ||
+        (rc = al((void **)&h->erho[1], (size_t)P.nbnd * 3 * 8)) ||
         (rc = al((void **)&h->pbuf[0], (size_t)PCOUNT * P.maxp * 8)) ||
         (rc = al((void **)&h->pbuf[1], (size_t)PCOUNT * P.maxp * 8)) ||
         (rc = al((void **)&h->dpw[0], 8)) || (rc = al((void **)&h->dpw[1], 8)) ||
@@ -745,7 +746,7 @@ int lbm_boundary(lbm_handle *h, int32_t kk) {
     (void)kk;
     const DevP &P = h->P;
     hipLaunchKernelGGL(k_boundary, dim3(nblk(P.nbnd)), dim3(NT), 0, h->stream, P, h->f, h->fp,
-                       h->rho);
+                       h->rho, h->ux, h->uy);
     return check_launch(h, "k_boundary");
 }
 

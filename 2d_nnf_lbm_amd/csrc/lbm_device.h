@@ -44,7 +44,7 @@ struct DevP {
     long long plane;     // SoA plane stride in elements
     int rheology, q_mode, update_angle, np;  // rheology: LBM_NEWTONIAN/BINGHAM/CASSON
     int motion;          // 0 free, 1 prescribed x-oscillation
-    int boundary;        // 0 halfway bounce-back, 1 Zou-He edges
+    int boundary;        // 0 halfway bounce-back, 1 Zou-He edges, 2 equilibrium edges
     double osc_u, osc_period;
     int maxp;            // particle SoA stride
     int nbnd;            // boundary nodes
@@ -242,10 +242,71 @@ __device__ __forceinline__ void zou_he_edge(const DevP &P, int i, int j, T rho_b
     }
 }
 
+// COMPUTEFEQNEW (F6:1130-1148, F4:1140-1158): the equilibrium with the
+// float-rounded weights, in the reference's operation order.
+__device__ __forceinline__ void feq64(double u, double v, double r, double fe[9]) {
+    double u2 = u * u + v * v;
+    fe[0] = FEQ_W0 * r * (2.0 - 3.0 * u2);
+    fe[1] = FEQ_W1 * r * (2.0 + 6.0 * u + 9.0 * u * u - 3.0 * u2);
+    fe[2] = FEQ_W1 * r * (2.0 + 6.0 * v + 9.0 * v * v - 3.0 * u2);
+    fe[3] = FEQ_W1 * r * (2.0 - 6.0 * u + 9.0 * u * u - 3.0 * u2);
+    fe[4] = FEQ_W1 * r * (2.0 - 6.0 * v + 9.0 * v * v - 3.0 * u2);
+    fe[5] = FEQ_W2 * r * (1.0 + 3.0 * (u + v) + 9.0 * u * v + 3.0 * u2);
+    fe[6] = FEQ_W2 * r * (1.0 - 3.0 * (u - v) - 9.0 * u * v + 3.0 * u2);
+    fe[7] = FEQ_W2 * r * (1.0 - 3.0 * (u + v) + 9.0 * u * v + 3.0 * u2);
+    fe[8] = FEQ_W2 * r * (1.0 + 3.0 * (u - v) - 9.0 * u * v + 3.0 * u2);
+}
+
+// The same equilibrium as deviations from w_k rho0 in FP32, for density
+// deviation rd = rho - rho0: W_k (rd p_k + rho0 (p_k - b_k)), with p_k the
+// bracket of feq64, b_k its value at rest (2 or 1) and W_k the float-rounded
+// weights (W_k b_k equals the float weight w_k exactly: doubling commutes
+// with rounding), so no term of size rho0 is formed and cancelled.
+__device__ __forceinline__ void feq_dev(float rho0, float u, float v, float rd, float h[9]) {
+    const float W0 = 2.0f / 9.0f, W1 = 1.0f / 18.0f, W2 = 1.0f / 36.0f;
+    const float u2 = u * u + v * v;
+    const float d[9] = {-3.0f * u2,
+                        6.0f * u + 9.0f * u * u - 3.0f * u2,
+                        6.0f * v + 9.0f * v * v - 3.0f * u2,
+                        -6.0f * u + 9.0f * u * u - 3.0f * u2,
+                        -6.0f * v + 9.0f * v * v - 3.0f * u2,
+                        3.0f * (u + v) + 9.0f * u * v + 3.0f * u2,
+                        -3.0f * (u - v) - 9.0f * u * v + 3.0f * u2,
+                        -3.0f * (u + v) + 9.0f * u * v + 3.0f * u2,
+                        3.0f * (u - v) - 9.0f * u * v + 3.0f * u2};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const float Wk = k == 0 ? W0 : (k < 5 ? W1 : W2);
+        const float bk = k < 5 ? 2.0f : 1.0f;
+        h[k] = Wk * (rd * (bk + d[k]) + rho0 * d[k]);
+    }
+}
+
+// Equilibrium edges and corners (F4:1218-1220, F4:1277-1279, F4:1368-1373):
+// every population of boundary node (i, j) is the equilibrium of the
+// previous step's density and corrected velocity at interior node
+// (clamp(i, 1, nx-1), clamp(j, 1, ny-1)).  The fused path carries those
+// three fields per boundary node (edge arrays, SoA [rho | ux | uy] of nbnd
+// entries each); eq_sources calls put(b) for every boundary node b whose
+// source is interior node (i, j): up to three (an interior corner feeds
+// both edges and the domain corner).
+template <class Put>
+__device__ __forceinline__ void eq_sources(const DevP &P, int i, int j, Put put) {
+    const bool l = i == 1, r = i == P.nx - 1, b = j == 1, t = j == P.ny - 1;
+    if (!(l || r || b || t)) return;
+    const int bi[3] = {i, l ? 0 : -1, r ? P.nx : -1};
+    const int bj[3] = {j, b ? 0 : -1, t ? P.ny : -1};
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+            if ((x | y) && bi[x] >= 0 && bj[y] >= 0) put(bnd_index(P, bi[x], bj[y]));
+}
+
 // Pull-stream with in-domain sources (F6:1164-1188) and the bounce-back
 // rules (F6:1206-1294, F6:1364-1386) for node (i, j): v receives f_k.
 // src: post-collision populations (SoA planes), erho: previous-step density
-// on the boundary nodes.
+// on the boundary nodes (equilibrium edges: the edge arrays of eq_sources).
 template <class St>
 __device__ __forceinline__ void load_fk64(const DevP &P, const typename St::T *src,
                                           const double *erho, long long node, int i,
@@ -254,6 +315,11 @@ __device__ __forceinline__ void load_fk64(const DevP &P, const typename St::T *s
 #pragma unroll
         for (int k = 0; k < 9; ++k)
             v[k] = St::ld(src + k * P.plane + node - cxk(k) - (long long)cyk(k) * P.W);
+        return;
+    }
+    if (P.boundary == 2) {  // boundary node: the equilibrium of its interior source
+        const int b = bnd_index(P, i, j);
+        feq64(erho[P.nbnd + b], erho[2 * P.nbnd + b], erho[b], v);
         return;
     }
 #pragma unroll
@@ -613,6 +679,11 @@ __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T 
                                      (long long)sizeof(typename St::T));
             h[k] = St::bld(rs, vo, so);
         }
+        return;
+    }
+    if (P.boundary == 2) {  // boundary node: the equilibrium of its interior source
+        const int b = bnd_index(P, i, j);
+        feq_dev(P.frho0, erho[P.nbnd + b], erho[2 * P.nbnd + b], erho[b], h);
         return;
     }
 #pragma unroll

@@ -24,18 +24,8 @@ __global__ void k_init(DevP P, double *f, double *rho, double *ux, double *uy, d
     long long n = (long long)blockIdx.x * NT + threadIdx.x;
     if (n >= P.nn) return;
     double r = P.rho0, u = 0.0, v = 0.0;
-    // COMPUTEFEQNEW at rest, F6:1134-1146
-    double u2 = u * u + v * v;
     double fe[9];
-    fe[0] = FEQ_W0 * r * (2.0 - 3.0 * u2);
-    fe[1] = FEQ_W1 * r * (2.0 + 6.0 * u + 9.0 * u * u - 3.0 * u2);
-    fe[2] = FEQ_W1 * r * (2.0 + 6.0 * v + 9.0 * v * v - 3.0 * u2);
-    fe[3] = FEQ_W1 * r * (2.0 - 6.0 * u + 9.0 * u * u - 3.0 * u2);
-    fe[4] = FEQ_W1 * r * (2.0 - 6.0 * v + 9.0 * v * v - 3.0 * u2);
-    fe[5] = FEQ_W2 * r * (1.0 + 3.0 * (u + v) + 9.0 * u * v + 3.0 * u2);
-    fe[6] = FEQ_W2 * r * (1.0 - 3.0 * (u - v) - 9.0 * u * v + 3.0 * u2);
-    fe[7] = FEQ_W2 * r * (1.0 - 3.0 * (u + v) + 9.0 * u * v + 3.0 * u2);
-    fe[8] = FEQ_W2 * r * (1.0 + 3.0 * (u - v) - 9.0 * u * v + 3.0 * u2);
+    feq64(u, v, r, fe);  // COMPUTEFEQNEW at rest (INITIALIZE, F6:555-572)
 #pragma unroll
     for (int k = 0; k < 9; ++k) f[k * P.plane + n] = fe[k];
     rho[n] = r; ux[n] = u; uy[n] = v;
@@ -115,16 +105,28 @@ __device__ __forceinline__ void bnd_node(const DevP &P, int b, int &i, int &j) {
     else { i = P.nx; j = b - 2 * P.W - (P.H - 2) + 1; }
 }
 
-// BOUNDARYCONDITION, F6:1195-1390 (or the Zou-He edges of F1:1222-1417):
+// BOUNDARYCONDITION, F6:1195-1390 (or the Zou-He edges of F1:1222-1417, or
+// the equilibrium edges and corners of F4:1205-1402):
 // one thread per boundary node.  The rules at each node are those of
 // load_fk64 with the streamed values kept for the populations the rules do
 // not touch.
-__global__ void k_boundary(DevP P, double *f, const double *fp, const double *rho) {
+__global__ void k_boundary(DevP P, double *f, const double *fp, const double *rho,
+                           const double *ux, const double *uy) {
     int b = blockIdx.x * NT + threadIdx.x;
     if (b >= P.nbnd) return;
     int i, j;
     bnd_node(P, b, i, j);
     long long n = (long long)j * P.W + i;
+    if (P.boundary == 2) {  // F4's equilibrium of the interior neighbour
+        int si = i < 1 ? 1 : (i > P.nx - 1 ? P.nx - 1 : i);
+        int sj = j < 1 ? 1 : (j > P.ny - 1 ? P.ny - 1 : j);
+        long long s = (long long)sj * P.W + si;
+        double v[9];
+        feq64(ux[s], uy[s], rho[s], v);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k * P.plane + n] = v[k];
+        return;
+    }
     const bool corner = (i == 0 || i == P.nx) && (j == 0 || j == P.ny);
     if (P.boundary == 1 && !corner) {
         double v[9];
@@ -379,7 +381,10 @@ struct StepArgs {
     double *crho, *cux, *cuy, *cg, *cobn, *cstress;
     int *cimage;
     float *rate32;                // node rates of the reduced formats
-    const void *erho_src;         // previous-step density on boundary nodes
+    // edge arrays: previous-step density on the boundary nodes or, with
+    // equilibrium edges, density and corrected velocity of each boundary
+    // node's interior source (3 x nbnd)
+    const void *erho_src;
     void *erho_dst;
     const double *pin;            // particle state k
     double *pout;                 // particle state k+1 (final step: same as pin)
@@ -627,6 +632,18 @@ __device__ __forceinline__ void tile_monitors(const StepArgs &a, long long tile,
     }
 }
 
+// Equilibrium edges: interior node (i, j) hands its density (FP64) or
+// density deviation (reduced formats), and corrected velocity, to the edge
+// arrays of the boundary nodes it is the source of, for the next launch.
+template <class T>
+__device__ __forceinline__ void eq_put(const StepArgs &a, const DevP &P, int i, int j, T r,
+                                       T ux, T uy) {
+    T *e = (T *)a.erho_dst;
+    eq_sources(P, i, j, [&](int b) {
+        e[b] = r; e[P.nbnd + b] = ux; e[2 * P.nbnd + b] = uy;
+    });
+}
+
 // Prologue of a fused run: collide the FP64 state of step k (F, RHO, corrected
 // UX/UY, G, OMEGA_BN) into f*_{k+1} in the storage format; particle blocks
 // take the particles from state k to k+1.
@@ -669,7 +686,11 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
                 a.cstress[2 * P.plane + n] = str[2];
                 if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
             }
-            if (b >= 0) ((float *)a.erho_dst)[b] = (float)r;
+            if (P.boundary == 2) {
+                if (b < 0) eq_put(a, P, i, j, (float)(r - P.rho0), (float)ux, (float)uy);
+            } else if (b >= 0) {
+                ((float *)a.erho_dst)[b] = (float)r;
+            }
         } else {
             double f[9];
 #pragma unroll
@@ -686,7 +707,11 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
                 a.cstress[2 * P.plane + n] = o.stress[2];
                 if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
             }
-            if (b >= 0) ((double *)a.erho_dst)[b] = r;
+            if (P.boundary == 2) {
+                if (b < 0) eq_put(a, P, i, j, r, ux, uy);
+            } else if (b >= 0) {
+                ((double *)a.erho_dst)[b] = r;
+            }
         }
     }
     if (a.slots_yield) tile_monitors(a, tile, 0.0, yielded, false, true);
@@ -888,7 +913,11 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                     a.cstress[2 * P.plane + n] = str[2];
                     if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
                 }
-                if (b >= 0) ((float *)a.erho_dst)[b] = rho;
+                if (P.boundary == 2) {
+                    if (b < 0) eq_put(a, P, i, j, rd, ucx, ucy);
+                } else if (b >= 0) {
+                    ((float *)a.erho_dst)[b] = rho;
+                }
             }
         } else {
             double f[9], rho, ux, uy;
@@ -920,7 +949,11 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                     a.cstress[2 * P.plane + n] = o.stress[2];
                     if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
                 }
-                if (b >= 0) ((double *)a.erho_dst)[b] = rho;
+                if (P.boundary == 2) {
+                    if (b < 0) eq_put(a, P, i, j, rho, ucx, ucy);
+                } else if (b >= 0) {
+                    ((double *)a.erho_dst)[b] = rho;
+                }
             }
         }
     }

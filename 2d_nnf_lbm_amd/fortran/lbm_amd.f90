@@ -12,7 +12,7 @@ MODULE LBM_AMD
   INTEGER, PARAMETER :: LBM_NEWTONIAN = 0, LBM_BINGHAM = 1, LBM_CASSON = 2
   INTEGER, PARAMETER :: LBM_RATE_LAGGED = 0, LBM_RATE_LOCAL = 1
   INTEGER, PARAMETER :: LBM_MOTION_FREE = 0, LBM_MOTION_OSC_X = 1
-  INTEGER, PARAMETER :: LBM_BOUNCE_BACK = 0, LBM_ZOU_HE = 1
+  INTEGER, PARAMETER :: LBM_BOUNCE_BACK = 0, LBM_ZOU_HE = 1, LBM_EQUILIBRIUM = 2
   INTEGER, PARAMETER :: LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16 = 2, LBM_BF16 = 3
   ! particle record fields, 1-based (LBM_PF_* + 1)
   INTEGER, PARAMETER :: PF_X = 1, PF_Y = 2, PF_ANG = 3, PF_VX = 4, PF_VY = 5, PF_W = 6

@@ -45,6 +45,7 @@ extern "C" {
 
 #define LBM_BOUNCE_BACK 0        /* halfway bounce-back with moving walls on every edge (F6:1195-1390) */
 #define LBM_ZOU_HE 1             /* Zou-He velocity edges, bounce-back corners (F1:1222-1417) */
+#define LBM_EQUILIBRIUM 2        /* equilibrium of the interior neighbour on edges and corners (F4:1205-1402) */
 
 #define LBM_RATE_LAGGED 0        /* module-global rate set by the previous node (F6:958) */
 #define LBM_RATE_LOCAL 1         /* each node's own previous-step rate */
@@ -81,7 +82,7 @@ typedef struct lbm_params {
     int32_t storage;           /* LBM_FP64 / LBM_FP32 / LBM_FP16 / LBM_BF16 */
     int32_t max_particles;     /* capacity of the particle arrays */
     int32_t motion;            /* LBM_MOTION_FREE / LBM_MOTION_OSC_X */
-    int32_t boundary;          /* LBM_BOUNCE_BACK / LBM_ZOU_HE (edges; corners bounce back) */
+    int32_t boundary;          /* LBM_BOUNCE_BACK / LBM_ZOU_HE (edges; corners bounce back) / LBM_EQUILIBRIUM */
     double omega;              /* OMEGA = 1/(3 nu + 0.5) (F6:101) */
     double s0, s1, s2, s3, s5, sq;  /* other MRT rates (F6:1009-1011) */
     double nu;                 /* NU, Newtonian stress factor (F6:918) */

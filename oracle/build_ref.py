@@ -26,6 +26,10 @@ Patches (each is a parameter or syntax edit, none changes the algorithm):
   * Optional ("zouhe" boundary): F6's BOUNDARYCONDITION (F6:1195-1390) is
     replaced by F1's (F1:1222-1417: Zou-He edges, bounce-back corners) with
     F6's argument list, and optionally a top-wall velocity in F1:1230.
+  * Optional ("equilibrium" boundary): the same with F4's routine
+    (F4:1205-1402: every edge and corner node set to the equilibrium of its
+    interior neighbour); its first two comment-only lines (F4:1216-1217)
+    are dropped so the routine keeps F6's length.
   * Optional ("casson" rheology): in Collision_MRT the two Papanastasiou-
     Bingham viscosity lines (F6:940, F6:943) are commented out and the two
     Casson lines F6 carries commented next to them (F6:941, F6:944; the same
@@ -63,6 +67,8 @@ F6 = os.path.join(REF_ROOT, "6_FreeFall_YieldStress",
 F1 = os.path.join(REF_ROOT, "1_Disk_Newtonian", "D2Q9_1P_Validated_20190530.f90")
 F3 = os.path.join(REF_ROOT, "3_MultiParticle_Sedimentation",
                   "D2Q9_RTParticle_20191003.f90")
+F4 = os.path.join(REF_ROOT, "4_Oscillating_Particle",
+                  "D2Q9_V7_OscillatingCylinder_Validated.f90")
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref")
 FLANG = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
@@ -81,6 +87,8 @@ VARIANTS = {
                          local_rate=True, rheology="casson"),
     "f6_zouhe_lid": dict(nx=40, ny=60, non_newtonian=False, time_max=300,
                          boundary="zouhe", zouhe_uxn="0.01D0"),
+    "f6_eq_bing_local": dict(nx=40, ny=120, non_newtonian=True, time_max=1000,
+                             local_rate=True, boundary="equilibrium"),
 }
 # CPU-baseline build: the reference's Bingham settling disk on the 401x1601
 # node grid (NX=400, NY=1600), timed for a bounded number of steps.
@@ -149,14 +157,17 @@ def patch_source(text, nx, ny, non_newtonian, time_max, local_rate=False,
     return "\n".join(out) + "\n"
 
 
-def splice_zouhe(text, uxn=None):
-    """F6 with its BOUNDARYCONDITION (F6:1195-1390) replaced by F1's
-    (F1:1222-1417: Zou-He edges, bounce-back corners), F6's argument list
-    kept (F1's routine does not use KK).  uxn: top-wall velocity for the
-    Zou-He lid (F1:1230, zero in F1)."""
-    with open(F1, "rb") as fh:
-        f1 = fh.read().decode("ascii", errors="replace")
-    sub = cut(f1, r"\s*SUBROUTINE BOUNDARYCONDITION\(", r"\s*END SUBROUTINE BOUNDARYCONDITION\b")
+def splice_boundary(text, src, uxn=None):
+    """F6 with its BOUNDARYCONDITION (F6:1195-1390) replaced by the one of
+    reference program src: F1's (F1:1222-1417: Zou-He edges, bounce-back
+    corners) or F4's (F4:1205-1402: equilibrium edges and corners), with
+    F6's argument list (neither routine uses KK).  A longer routine loses
+    comment-only lines (the first ones after its declarations) so the
+    line-numbered patches after it still apply.  uxn: top-wall velocity for
+    F1's Zou-He lid (F1:1230, zero in F1)."""
+    with open(src, "rb") as fh:
+        other = fh.read().decode("ascii", errors="replace").replace("\r", "")
+    sub = cut(other, r"\s*SUBROUTINE BOUNDARYCONDITION\(", r"\s*END SUBROUTINE BOUNDARYCONDITION\b")
     sub[0] = sub[0].replace("(F,F_POST,RHO,UX,UY)", "(F,F_POST,RHO,UX,UY,KK)")
     for k, line in enumerate(sub):
         if re.match(r"\s*INTEGER :: I,J\s*$", line):
@@ -167,7 +178,12 @@ def splice_zouhe(text, uxn=None):
     a = next(k for k, l in enumerate(lines) if re.match(r"\s*SUBROUTINE BOUNDARYCONDITION\(", l))
     b = next(k for k in range(a, len(lines))
              if re.match(r"\s*END SUBROUTINE BOUNDARYCONDITION\b", lines[k]))
-    assert b - a == len(sub) - 1, "F1's routine no longer has F6's length (line-numbered patches)"
+    extra = len(sub) - (b - a + 1)
+    assert extra >= 0, "the spliced routine is shorter than F6's (line-numbered patches)"
+    while extra:
+        k = next(k for k in range(1, len(sub)) if re.match(r"\s*!", sub[k]))
+        del sub[k]
+        extra -= 1
     return "\n".join(lines[:a] + sub + lines[b + 1:]) + "\n"
 
 
@@ -175,8 +191,11 @@ def build(name, cfg, flags):
     with open(F6, "rb") as fh:
         text = fh.read().decode("ascii")
     cfg = dict(cfg)
-    if cfg.pop("boundary", None) == "zouhe":
-        text = splice_zouhe(text, cfg.pop("zouhe_uxn", None))
+    bnd = cfg.pop("boundary", None)
+    if bnd == "zouhe":
+        text = splice_boundary(text, F1, cfg.pop("zouhe_uxn", None))
+    elif bnd == "equilibrium":
+        text = splice_boundary(text, F4)
     src = patch_source(text, **cfg)
     os.makedirs(OUT_DIR, exist_ok=True)
     tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")

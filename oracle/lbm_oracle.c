@@ -195,13 +195,37 @@ void orc_stream(const orc_params *p, orc_state *st)
 /* BOUNDARYCONDITION: halfway bounce-back with wall velocities on the edges
  * (F6:1195-1390) or, boundary 1, the Zou-He edges of F1:1222-1417; the
  * corners bounce back in both.  The bounce-back rules use the previous
- * step's density. */
+ * step's density.  Boundary 2 is F4's rule (F4:1205-1402): every edge and
+ * corner node takes the equilibrium of the previous step's RHO, UX, UY at
+ * its interior neighbour, the diagonal one at the corners. */
 void orc_boundary(const orc_params *p, orc_state *st, int32_t kk)
 {
     (void)kk;
     const double inv6 = F32(1.0f / 6.0f), clb2 = 1.0;
     const int nx = p->nx, ny = p->ny;
     double *F = st->f, *FP = st->fpost, *R = st->rho;
+    if (p->boundary == 2) {
+        /* F4:1218-1219 bottom and top, F4:1277-1278 left and right, then
+         * the corners F4:1368-1373; every read is of an interior node, so
+         * the order of the writes does not matter */
+        const double *U = st->ux, *V = st->uy;
+#define EQ(i, j, si, sj) \
+    orc_feq(U[NID(p, si, sj)], V[NID(p, si, sj)], R[NID(p, si, sj)], F + 9 * NID(p, i, j))
+        for (int i = 1; i <= nx - 1; ++i) {
+            EQ(i, 0, i, 1);
+            EQ(i, ny, i, ny - 1);
+        }
+        for (int j = 1; j <= ny - 1; ++j) {
+            EQ(0, j, 1, j);
+            EQ(nx, j, nx - 1, j);
+        }
+        EQ(0, 0, 1, 1);
+        EQ(0, ny, 1, ny - 1);
+        EQ(nx, ny, nx - 1, ny - 1);
+        EQ(nx, 0, nx - 1, 1);
+#undef EQ
+        return;
+    }
     double uxS = p->wall[0], uyS = p->wall[1], uxR = p->wall[2], uyR = p->wall[3];
     double uxN = p->wall[4], uyN = p->wall[5], uxL = p->wall[6], uyL = p->wall[7];
 #define FA(k, i, j) F[9 * NID(p, i, j) + (k)]

@@ -50,7 +50,8 @@ typedef struct orc_params {
     int32_t q_mode;          /* 1: s4=s6=8(2-w)/(8-w) (F6:1009); 0: sq */
     int32_t update_angle;    /* 1: F6:312 integrates the angle; 0: F3:327 */
     int32_t motion;          /* 0 free (F6:299-312); 1 prescribed x-oscillation (F4:221-226) */
-    int32_t boundary;        /* 0 halfway bounce-back (F6:1195-1390), 1 Zou-He edges (F1:1222-1417) */
+    int32_t boundary;        /* 0 halfway bounce-back (F6:1195-1390), 1 Zou-He edges (F1:1222-1417),
+                                2 equilibrium edges and corners (F4:1205-1402) */
     double omega;            /* OMEGA = 1/(3 nu + 0.5) (F6:101) */
     double s0, s1, s2, s3, s5, sq;  /* remaining MRT rates (F6:1009-1011) */
     double nu;               /* NU, Newtonian stress factor (F6:918) */

@@ -89,4 +89,6 @@ def golden_params(meta):
         cfg["boundary"] = 1  # F1:1222-1417
         if meta.get("zouhe_uxn"):
             cfg["wall"][4] = float(meta["zouhe_uxn"].replace("D", "e"))
+    if meta.get("boundary") == "equilibrium":
+        cfg["boundary"] = 2  # F4:1205-1402
     return cfg

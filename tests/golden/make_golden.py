@@ -15,7 +15,7 @@ and f3_contact.npz from oracle/_ref/f3_contact (the reference's N-body
 ComputeCollisionForce, F3:632-806, driven by oracle/f3_contact_driver.f90):
 disk positions and diameters in, F_COL of the second and third call out.
 
-Usage: python tests/golden/make_golden.py   (needs oracle/_ref/ built here)
+Usage: python tests/golden/make_golden.py [variant ...]   (needs oracle/_ref/ built here)
 """
 import json
 import os
@@ -74,10 +74,15 @@ def make_contact():
     print("wrote", os.path.relpath(path, REPO), os.path.getsize(path), "bytes")
 
 
-def main():
+def main(names):
+    """names: the variants to write (default: every variant and the contact
+    fixture)."""
     os.makedirs(HERE, exist_ok=True)
-    make_contact()
+    if not names:
+        make_contact()
     for name, cfg in build_ref.VARIANTS.items():
+        if names and name not in names:
+            continue
         exe = os.path.join(build_ref.OUT_DIR, name)
         if not os.path.exists(exe):
             raise SystemExit("missing %s: run python oracle/build_ref.py" % exe)
@@ -93,4 +98,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

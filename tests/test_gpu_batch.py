@@ -72,6 +72,31 @@ def test_batch_equals_single_simulations(storage):
     b.close()
 
 
+@pytest.mark.parametrize("storage", ["fp64", "fp16"])
+def test_batch_mixed_boundaries_equal_single_simulations(storage):
+    """One batch whose simulations use the three boundary rules (bounce-back,
+    Zou-He, equilibrium): each equals its run alone, bit for bit."""
+    cases = small_batch(storage)
+    for (cfg, _), bnd in zip(cases, (0, 1, 2, 2)):
+        cfg["boundary"] = bnd
+    n = 30
+    print("case: mixed-boundary batch, %s, %d steps" % (storage, n), flush=True)
+    b = make_batch(cases)
+    hb = b.step(0, n)
+    for i, (cfg, part) in enumerate(cases):
+        s = pkg().D2Q9(cfg)
+        s.initialize()
+        s.set_particles(part)
+        hs = s.step(0, n)
+        assert np.array_equal(hb[i], hs), i
+        fb, pb = state(b.sims[i])
+        fs, ps = state(s)
+        assert np.array_equal(fb, fs), i
+        assert np.array_equal(pb, ps), i
+        s.close()
+    b.close()
+
+
 def test_batch_graph_replays_equal_eager_launches():
     cases = small_batch("fp16")
     eager = make_batch(cases)

@@ -103,8 +103,37 @@ def test_dropin_zou_he_matches_oracle(orc):
     _compare_state(g2, st2, label="fused zou-he")
 
 
+def test_dropin_equilibrium_matches_oracle(orc):
+    """Per-subroutine entries and fused steps with the equilibrium edges and
+    corners of F4:1205-1402 (Bingham), the disk next to the left wall so
+    that its force sums and contact cover edge nodes."""
+    nx, ny = 40, 60
+    cfg, part = f6_case(nx, ny, True)
+    cfg["boundary"] = 2
+    part = part.copy()
+    part[0, 0] = 0.5 * part[6, 0] + 0.7  # x: 0.7 lattice units from the wall
+    part[1, 0] = 0.5 * ny
+    print("case: dropin equilibrium %dx%d" % (nx, ny), flush=True)
+    st = oracle_state(orc, cfg, part)
+    g = gpu_solver(cfg)
+    g.initialize()
+    g.set_particles(part)
+    for kk in range(1, 41):
+        st.step_parts(kk)
+        g.step_parts(kk)
+    _compare_state(g, st, label="dropin equilibrium")
+    st2 = oracle_state(orc, cfg, part)
+    st2.run(0, 60)
+    g2 = gpu_solver(cfg)
+    g2.initialize()
+    g2.set_particles(part)
+    g2.step(0, 25)
+    g2.step(25, 35)  # a second call restarts the edge arrays from the fields
+    _compare_state(g2, st2, label="fused equilibrium")
+
+
 @pytest.mark.parametrize("name", ["f6_newt", "f6_bing_local", "f6_newt_lid", "f6_newt_mohammad",
-                                  "f6_cas_local", "f6_zouhe_lid"])
+                                  "f6_cas_local", "f6_zouhe_lid", "f6_eq_bing_local"])
 def test_fused_fp64_matches_reference_dump(orc, name):
     """Fused FP64 steps against the reference program's own final dump
     (LOCAL rate for Bingham, which is what the GPU implements)."""

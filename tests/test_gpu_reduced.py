@@ -91,3 +91,28 @@ def test_reduced_storage_zou_he(orc):
     # Zou-He edges do not conserve mass exactly: the FP32 density estimate
     # at the edges lets the total drift apart by ~1e-5 over the run
     assert rel_err(rho, st.rho) <= 1e-4
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_reduced_storage_equilibrium(orc, storage):
+    """Equilibrium edges and corners (F4:1205-1402) in the deviation form of
+    the reduced formats: the Bingham settling run against the FP64 oracle,
+    same bounds as with bounce-back walls."""
+    nx, ny, n = 60, 180, 2000
+    cfg, part = f6_case(nx, ny, True)
+    cfg["boundary"] = 2
+    print("case: reduced storage equilibrium", storage, flush=True)
+    st = oracle_state(orc, cfg, part)
+    ho = st.run(0, n)
+    g = gpu_solver(cfg, storage=storage)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, n)
+    vy_g, vy_o = hg[-1, orc.H_VY], ho[-1, orc.H_VY]
+    e_vy = abs(vy_g - vy_o) / abs(vy_o)
+    print("equilibrium %s: vy %.3e (ref %.6e)" % (storage, e_vy, vy_o), flush=True)
+    assert np.all(np.isfinite(hg))
+    assert e_vy <= VY_TOL[storage]
+    assert rel_err(hg[:, -2], ho[:, -2]) <= 1e-4
+    rho, ux, uy, gg, om = g.get_fields()
+    assert rel_err(rho, st.rho) <= 1e-3

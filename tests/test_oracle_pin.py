@@ -12,7 +12,7 @@ import pytest
 import cases
 
 VARIANTS = ["f6_newt", "f6_bing_lagged", "f6_bing_local", "f6_newt_lid", "f6_newt_mohammad",
-            "f6_cas_local", "f6_zouhe_lid"]
+            "f6_cas_local", "f6_zouhe_lid", "f6_eq_bing_local"]
 
 
 def _run_oracle(orc, meta):
