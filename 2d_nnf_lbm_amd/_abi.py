@@ -15,6 +15,7 @@ LBM_ERR_IO = -6
 
 LBM_NEWTONIAN, LBM_BINGHAM, LBM_CASSON = 0, 1, 2
 LBM_BOUNCE_BACK, LBM_ZOU_HE, LBM_EQUILIBRIUM = 0, 1, 2
+LBM_CONTACT_GLOWINSKI, LBM_CONTACT_WAN_TUREK, LBM_CONTACT_LJ = 0, 1, 2
 LBM_RATE_LAGGED, LBM_RATE_LOCAL = 0, 1
 LBM_MOTION_FREE, LBM_MOTION_OSC_X = 0, 1
 LBM_FP64, LBM_FP32, LBM_FP16, LBM_BF16 = 0, 1, 2, 3
@@ -49,6 +50,8 @@ class Params(ctypes.Structure):
         ("eps_p", ctypes.c_double), ("eps_w", ctypes.c_double),
         ("e_p", ctypes.c_double), ("e_w", ctypes.c_double),
         ("osc_u", ctypes.c_double), ("osc_period", ctypes.c_double),
+        ("contact", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("wall_edge", ctypes.c_double * 4),
     ]
 
 

@@ -125,6 +125,9 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     if (p->boundary != LBM_BOUNCE_BACK && p->boundary != LBM_ZOU_HE &&
         p->boundary != LBM_EQUILIBRIUM)
         return bad_create("unknown boundary mode");
+    if (p->contact != LBM_CONTACT_GLOWINSKI && p->contact != LBM_CONTACT_WAN_TUREK &&
+        p->contact != LBM_CONTACT_LJ)
+        return bad_create("unknown contact model");
     if (p->storage < LBM_FP64 || p->storage > LBM_BF16) return bad_create("unknown storage");
     if (p->max_particles < 0 || p->max_particles > 65535)
         return bad_create("max_particles outside 0..65535");
@@ -152,6 +155,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.corner[0] = p->corner[0]; P.corner[1] = p->corner[1];
     P.zeta_c = p->zeta_c; P.cij = p->cij; P.eps_p = p->eps_p; P.eps_w = p->eps_w;
     P.e_p = p->e_p; P.e_w = p->e_w;
+    P.contact = p->contact;
+    for (int k = 0; k < 4; ++k) P.wall_edge[k] = p->wall_edge[k];
     P.fomega = (float)P.omega; P.fs0 = (float)P.s0; P.fs1 = (float)P.s1; P.fs2 = (float)P.s2;
     P.fs3 = (float)P.s3; P.fs5 = (float)P.s5; P.fsq = (float)P.sq; P.fnu = (float)P.nu;
     P.fmu0 = (float)P.mu0; P.ftau_y = (float)P.tau_y; P.fm_pap = (float)P.m_pap;
@@ -1552,6 +1557,8 @@ extern "C" int lbm_adjoint_velocity(lbm_handle *h, int32_t nsteps, int32_t windo
         return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs walls at rest");
     if (pr.boundary != LBM_BOUNCE_BACK)
         return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs the bounce-back walls");
+    if (pr.contact != LBM_CONTACT_GLOWINSKI)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the adjoint needs the Glowinski contact model");
     int rc = check_fused_mode(h);
     if (rc) return rc;
 

@@ -29,6 +29,7 @@ constexpr double PI_F = (double)3.14159265359f;     // F6:3
 constexpr double FEQ_W0 = (double)(2.0f / 9.0f);    // F6:1135
 constexpr double FEQ_W1 = (double)(1.0f / 18.0f);   // F6:1136
 constexpr double FEQ_W2 = (double)(1.0f / 36.0f);   // F6:1137
+constexpr double LJ_A = (double)2.4f;               // F4:595, default-REAL literal
 
 // Lattice weights of the rest state used by the deviation form.
 constexpr float WF0 = 4.0f / 9.0f, WF1 = 1.0f / 9.0f, WF2 = 1.0f / 36.0f;
@@ -51,6 +52,8 @@ struct DevP {
     double omega, s0, s1, s2, s3, s5, sq, nu, mu0, tau_y, m_pap, gamma_min, rho0, gy;
     double wall[8], corner[2];
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;
+    int contact;         // 0 Glowinski, 1 Wan-Turek, 2 Lennard-Jones
+    double wall_edge[4]; // Wan-Turek walls: bottom y, right x, top y, left x
     unsigned w_magic;    // row = node / W by multiply-high (w_shift >= 1)
     int w_shift;
     // FP32 copies of the parameters used by the reduced-precision path
@@ -898,9 +901,27 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*NT 
     __syncthreads();
 }
 
-// Glowinski pair force of the pair (ip, jp), ip < jp (F3:746-770), with the
-// stale wall distance dpw choosing the branch (F3:760).  Returns false (and a
-// zero force) when the pair is out of range.
+// The reference's (..)**8 and (..)**14 (F4:595-596, F4:616-617) with the
+// roundings of its compiled form: **8 as seven multiplications left to
+// right, **14 by the binary method ((x^2 x^4) x^8).
+__device__ __forceinline__ double pow8_lj(double x) {
+    return ((((((x * x) * x) * x) * x) * x) * x) * x;
+}
+__device__ __forceinline__ double pow14_lj(double x) {
+    double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
+    return (x2 * x4) * x8;
+}
+
+// Pair force of the pair (ip, jp), ip < jp, for the contact model P.contact.
+// Returns false (and a zero force) when the pair is out of range
+// d > r_i + r_j + zeta.
+//   0 Glowinski (F3:746-770): the stale wall distance dpw chooses the branch
+//     (F3:760).
+//   1 Wan-Turek (F4:632-645, F1:654-667): eps_p (x_i - x_j) (r_i + r_j - d)
+//     in contact, eps_p (x_i - x_j) (r_i + r_j + zeta - d)^2 in the gap.
+//   2 Lennard-Jones (F4:590-599, F1:612-621): 2.4 R_P^2 (2 q^14 - q^8)
+//     (x_i - x_j) / (4 R_P^2), q = 2 R_P / d, R_P the radius of particle 1
+//     (F4:588).
 __device__ __forceinline__ bool pair_force(const DevP &P, double dpw, const double *X,
                                            const double *Y, const double *D, int ip, int jp,
                                            double &f0, double &f1) {
@@ -912,6 +933,25 @@ __device__ __forceinline__ bool pair_force(const DevP &P, double dpw, const doub
     double rsum = 0.5 * di + 0.5 * dj;
     f0 = 0.0; f1 = 0.0;
     if (dpp > rsum + zeta) return false;
+    if (P.contact == 1) {
+        if (dpp < rsum) {
+            f0 = P.eps_p * (xi - xj) * (rsum - dpp);
+            f1 = P.eps_p * (yi - yj) * (rsum - dpp);
+        } else {
+            double t = rsum + zeta - dpp;
+            f0 = P.eps_p * (xi - xj) * (t * t);
+            f1 = P.eps_p * (yi - yj) * (t * t);
+        }
+        return true;
+    }
+    if (P.contact == 2) {
+        double rp = 0.5 * D[0];
+        double q = 2.0 * rp / dpp;
+        double a = LJ_A * (rp * rp) * (2.0 * pow14_lj(q) - pow8_lj(q));
+        f0 = a * (xi - xj) / (4.0 * rp * rp);
+        f1 = a * (yi - yj) / (4.0 * rp * rp);
+        return true;
+    }
     double t = (dpp - 0.5 * di - 0.5 * dj - zeta) / zeta;
     if (dpw > rsum) {
         f0 = P.eps_p * cij * (t * t) * (xi - xj) / dpp;
@@ -923,25 +963,59 @@ __device__ __forceinline__ bool pair_force(const DevP &P, double dpw, const doub
     return true;
 }
 
-// Force of wall iw (1 bottom, 2 right, 3 top, 4 left; F3:773-804) on a disk
-// at (x, y), diameter dm; returns the wall distance.
-__device__ __forceinline__ double wall_one(const DevP &P, double x, double y, double dm, int iw,
-                                           double &f0, double &f1) {
+// Force of wall iw (1 bottom, 2 right, 3 top, 4 left) on a disk at (x, y),
+// diameter dm, from the wall image R_W; returns the distance to it.  rp1:
+// radius of particle 1 (Lennard-Jones).
+//   0 Glowinski (F3:773-804): images half a diameter outside the domain.
+//   1 Wan-Turek (F4:648-676, F1:670-698): mirror images about the wall edges
+//     P.wall_edge (WALL%EDGE, F4:182-185); eps_w (c - R_W) (dm - d) inside,
+//     eps_w (c - R_W) (dm + zeta - d)^2 within zeta.
+//   2 Lennard-Jones (F4:600-621, F1:622-643): images (x, 0), (nx, y),
+//     (x, 2 ny - y), (-x, y); 2.4 R_P^2 (2 q^14 - q^8) (c - R_W) / R_P^2,
+//     q = R_P / d.
+__device__ __forceinline__ double wall_one(const DevP &P, double x, double y, double dm,
+                                           double rp1, int iw, double &f0, double &f1) {
     const double zeta = P.zeta_c, cij = P.cij;
     double rw0, rw1;
-    if (iw == 1) { rw0 = x; rw1 = -(0.5 * dm); }
-    else if (iw == 2) { rw0 = (double)P.nx + 0.5 * dm; rw1 = y; }
-    else if (iw == 3) { rw0 = x; rw1 = (double)P.ny + 0.5 * dm; }
-    else { rw0 = -(0.5 * dm); rw1 = y; }
+    if (P.contact == 1) {
+        const double *E = P.wall_edge;
+        if (iw == 1) { rw0 = x; rw1 = 2.0 * E[0] - y; }
+        else if (iw == 2) { rw0 = 2.0 * E[1] - x; rw1 = y; }
+        else if (iw == 3) { rw0 = x; rw1 = 2.0 * E[2] - y; }
+        else { rw0 = 2.0 * E[3] - x; rw1 = y; }
+    } else if (P.contact == 2) {
+        if (iw == 1) { rw0 = x; rw1 = 0.0; }
+        else if (iw == 2) { rw0 = (double)P.nx; rw1 = y; }
+        else if (iw == 3) { rw0 = x; rw1 = 2.0 * (double)P.ny - y; }
+        else { rw0 = -x; rw1 = y; }
+    } else {
+        if (iw == 1) { rw0 = x; rw1 = -(0.5 * dm); }
+        else if (iw == 2) { rw0 = (double)P.nx + 0.5 * dm; rw1 = y; }
+        else if (iw == 3) { rw0 = x; rw1 = (double)P.ny + 0.5 * dm; }
+        else { rw0 = -(0.5 * dm); rw1 = y; }
+    }
     const double d = sqrt((x - rw0) * (x - rw0) + (y - rw1) * (y - rw1));
     f0 = 0.0; f1 = 0.0;
-    if (d > dm + zeta) {
-        f0 = 0.0; f1 = 0.0;
-    } else if (d <= dm + zeta && d > dm) {
+    if (d > dm + zeta) return d;
+    if (P.contact == 1) {
+        if (d < dm) {
+            f0 = P.eps_w * (x - rw0) * (dm - d);
+            f1 = P.eps_w * (y - rw1) * (dm - d);
+        } else {
+            double t = dm + zeta - d;
+            f0 = P.eps_w * (x - rw0) * (t * t);
+            f1 = P.eps_w * (y - rw1) * (t * t);
+        }
+    } else if (P.contact == 2) {
+        double q = rp1 / d;
+        double a = LJ_A * (rp1 * rp1) * (2.0 * pow14_lj(q) - pow8_lj(q));
+        f0 = a * (x - rw0) / (rp1 * rp1);
+        f1 = a * (y - rw1) / (rp1 * rp1);
+    } else if (d > dm) {
         double t = (d - dm - zeta) / zeta;
         f0 = P.eps_w * cij * (t * t) * (x - rw0) / d;
         f1 = P.eps_w * cij * (t * t) * (y - rw1) / d;
-    } else if (d <= dm) {
+    } else {
         double t = (d - dm - zeta) / zeta;
         f0 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (x - rw0) / d;
         f1 = (P.eps_w * cij * (t * t) + P.e_w * cij * (dm - d) / zeta) * (y - rw1) / d;
@@ -953,17 +1027,17 @@ __device__ __forceinline__ double wall_one(const DevP &P, double x, double y, do
 // F3:773-804 (bottom, right, top, left); returns the last wall distance,
 // which becomes the next call's stale D_PW.
 __device__ __forceinline__ double wall_forces(const DevP &P, double x, double y, double dm,
-                                              double &fcx, double &fcy) {
+                                              double rp1, double &fcx, double &fcy) {
     double d = 0.0;
     for (int iw = 1; iw <= 4; ++iw) {
         double f0, f1;
-        d = wall_one(P, x, y, dm, iw, f0, f1);
+        d = wall_one(P, x, y, dm, rp1, iw, f0, f1);
         fcx = fcx + f0; fcy = fcy + f1;
     }
     return d;
 }
 
-// Contact force on particle p (Glowinski model; N-body pair order of
+// Contact force on particle p (model P.contact; N-body pair order of
 // F3:746-770, walls of F3:773-804), evaluated by one thread.  The pair branch
 // reads dpw, the D_PW left by the previous call (F3:760).  Writes in-range
 // partners j>p into nb[0..MAXNB) and returns their count; *dpw_last receives
@@ -989,7 +1063,7 @@ __device__ __forceinline__ int contact_one(const DevP &P, const double *X, const
             ++cnt;
         }
     }
-    *dpw_last = wall_forces(P, X[p], Y[p], D[p], fcx, fcy);
+    *dpw_last = wall_forces(P, X[p], Y[p], D[p], 0.5 * D[0], fcx, fcy);
     return cnt;
 }
 
@@ -1051,7 +1125,7 @@ __device__ __forceinline__ int contact_block(const DevP &P, const double *X, con
                 ++cnt;
             }
         }
-        *dpw_last = wall_forces(P, X[p], Y[p], D[p], fcx, fcy);
+        *dpw_last = wall_forces(P, X[p], Y[p], D[p], 0.5 * D[0], fcx, fcy);
     }
     return cnt;
 }
@@ -1072,7 +1146,7 @@ __device__ __forceinline__ int contact_wave(const DevP &P, const double *X, cons
                          : pair_force(P, dpw, X, Y, D, p, lane, f0, f1);
     unsigned long long m = __ballot(hit);
     double w0 = 0.0, w1 = 0.0, wd = 0.0;
-    if (lane < 4) wd = wall_one(P, X[p], Y[p], D[p], lane + 1, w0, w1);
+    if (lane < 4) wd = wall_one(P, X[p], Y[p], D[p], 0.5 * D[0], lane + 1, w0, w1);
     fcx = 0.0; fcy = 0.0;
     int cnt = 0;
     while (m) {

@@ -13,6 +13,7 @@ MODULE LBM_AMD
   INTEGER, PARAMETER :: LBM_RATE_LAGGED = 0, LBM_RATE_LOCAL = 1
   INTEGER, PARAMETER :: LBM_MOTION_FREE = 0, LBM_MOTION_OSC_X = 1
   INTEGER, PARAMETER :: LBM_BOUNCE_BACK = 0, LBM_ZOU_HE = 1, LBM_EQUILIBRIUM = 2
+  INTEGER, PARAMETER :: LBM_CONTACT_GLOWINSKI = 0, LBM_CONTACT_WAN_TUREK = 1, LBM_CONTACT_LJ = 2
   INTEGER, PARAMETER :: LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16 = 2, LBM_BF16 = 3
   ! particle record fields, 1-based (LBM_PF_* + 1)
   INTEGER, PARAMETER :: PF_X = 1, PF_Y = 2, PF_ANG = 3, PF_VX = 4, PF_VY = 5, PF_W = 6
@@ -32,6 +33,8 @@ MODULE LBM_AMD
     REAL(C_DOUBLE) :: WALL(8), CORNER(2)
     REAL(C_DOUBLE) :: ZETA_C, CIJ, EPS_P, EPS_W, E_P, E_W
     REAL(C_DOUBLE) :: OSC_U, OSC_PERIOD
+    INTEGER(C_INT32_T) :: CONTACT, RESERVED
+    REAL(C_DOUBLE) :: WALL_EDGE(4)
   END TYPE LBM_PARAMS
 
   INTERFACE

@@ -87,6 +87,9 @@ PROGRAM LBM_F6_DRIVER
   P%BOUNDARY = LBM_BOUNCE_BACK
   P%OSC_U = 0D0
   P%OSC_PERIOD = 0D0
+  P%CONTACT = LBM_CONTACT_GLOWINSKI
+  P%RESERVED = 0
+  P%WALL_EDGE = 0D0
   P%OMEGA = OMEGA
   P%S0 = 0.0                                                     ! F6:1009-1011
   P%S1 = 1.1

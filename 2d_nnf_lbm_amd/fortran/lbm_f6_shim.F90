@@ -94,6 +94,9 @@ CONTAINS
     P%E_W = P%E_P
     P%OSC_U = 0D0
     P%OSC_PERIOD = 0D0
+    P%CONTACT = LBM_CONTACT_GLOWINSKI
+    P%RESERVED = 0
+    P%WALL_EDGE = 0D0
     RC = lbm_create(P, SH)
     CALL LBM_CHECK(SH, RC, 'lbm_create')
     RC = lbm_initialize(SH)

@@ -27,7 +27,7 @@ def make_params(cfg):
         if name not in cfg:
             continue
         v = cfg[name]
-        if name in ("wall", "corner"):
+        if name in ("wall", "corner", "wall_edge"):
             arr = getattr(p, name)
             for i, x in enumerate(v):
                 arr[i] = float(x)

@@ -46,6 +46,9 @@ extern "C" {
 #define LBM_BOUNCE_BACK 0        /* halfway bounce-back with moving walls on every edge (F6:1195-1390) */
 #define LBM_ZOU_HE 1             /* Zou-He velocity edges, bounce-back corners (F1:1222-1417) */
 #define LBM_EQUILIBRIUM 2        /* equilibrium of the interior neighbour on edges and corners (F4:1205-1402) */
+#define LBM_CONTACT_GLOWINSKI 0  /* Glowinski springs, N-body (F3:739-804; F6:690-760) */
+#define LBM_CONTACT_WAN_TUREK 1  /* Wan-Turek, mirror walls about wall_edge (F4:624-677; F1:646-699) */
+#define LBM_CONTACT_LJ 2         /* Lennard-Jones (F4:587-622; F1:609-644) */
 
 #define LBM_RATE_LAGGED 0        /* module-global rate set by the previous node (F6:958) */
 #define LBM_RATE_LOCAL 1         /* each node's own previous-step rate */
@@ -93,6 +96,11 @@ typedef struct lbm_params {
     double corner[2];          /* ux_w, uy_w at the four corners (F6:1365) */
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;  /* contact (F6:691-693, F3:739-742) */
     double osc_u, osc_period;  /* U_C, T_C of LBM_MOTION_OSC_X (F4:224) */
+    int32_t contact;           /* LBM_CONTACT_*; zeta_c and eps_p, eps_w are the model's
+                                  ZITA, EPS_P, EPS_W (Wan-Turek: F4:625, F1:647; L-J: F4:588) */
+    int32_t reserved;
+    double wall_edge[4];       /* Wan-Turek walls WALL%EDGE(1:4): bottom y, right x, top y,
+                                  left x (F4:182-185); the domain box is 0, nx, ny, 0 */
 } lbm_params;
 
 typedef struct lbm_handle lbm_handle;

@@ -38,6 +38,12 @@ Patches (each is a parameter or syntax edit, none changes the algorithm):
 A second recipe builds the N-body contact force alone: MODULE PARAM
 (F3:2-83) and SUBROUTINE ComputeCollisionForce (F3:632-806) cut from F3,
 compiled with oracle/f3_contact_driver.f90 into oracle/_ref/f3_contact.
+A further recipe builds the Wan-Turek and Lennard-Jones contact models:
+MODULE PARAM and ComputeCollisionForce(DISK,WALL) cut from F4 (F4:2-89,
+F4:576-743) or F1 (F1:2-93, F1:596-769), the routine's model switches
+(F4:587, F4:624, F4:680; F1:609, F1:646, F1:701) set to the model under test,
+compiled with oracle/contact_model_driver.f90 into oracle/_ref/f4_contact_wt
+(F4 as it ships: Wan-Turek, zeta 1), f4_contact_lj and f1_contact_wt (zeta 2).
 A third recipe builds the reference's own MAIN program on the GPU: F6
 (patched as its variant) with the nine time-step subroutines that
 2d_nnf_lbm_amd/fortran/lbm_f6_shim.F90 replaces cut out, compiled with that
@@ -247,6 +253,57 @@ def build_f3_contact(flags):
     return exe
 
 
+# contact-model builds: name -> (source program, model switched on)
+CONTACT_MODELS = {
+    "f4_contact_wt": ("F4", "W-T"),
+    "f4_contact_lj": ("F4", "L-J"),
+    "f1_contact_wt": ("F1", "W-T"),
+}
+
+
+def set_contact_model(lines, model):
+    """ComputeCollisionForce with the IF(0)/IF(1) switch that follows each
+    model's banner comment ("L-J MODEL", "W-T MODEL", "GLOWINSKY MODEL")
+    set to .TRUE. for `model` and .FALSE. for the others."""
+    out, pending = [], None
+    for line in lines:
+        m = re.match(r"\s*!\*+(L-J|W-T|GLOWINSKY) MODEL", line)
+        if m:
+            pending = m.group(1)
+        elif pending and re.match(r"\s*IF\((0|1)\)THEN", line):
+            line = re.sub(r"IF\((0|1)\)THEN", "IF(.%s.)THEN" % ("TRUE" if pending == model else "FALSE"),
+                          line, count=1)
+            pending = None
+        out.append(line)
+    return out
+
+
+def build_contact_model(name, flags):
+    """MODULE PARAM and ComputeCollisionForce(DISK,WALL) of F4 or F1 with one
+    contact model switched on, compiled with oracle/contact_model_driver.f90
+    into oracle/_ref/<name>."""
+    prog, model = CONTACT_MODELS[name]
+    with open({"F4": F4, "F1": F1}[prog], "rb") as fh:
+        text = fh.read().decode("ascii", errors="replace")
+    src = cut(text, r"\s*MODULE PARAM\b", r"\s*END MODULE PARAM\b")
+    routine = cut(text, r"\s*SUBROUTINE ComputeCollisionForce\(",
+                  r"\s*END SUBROUTINE ComputeCollisionForce\b")
+    src += set_contact_model(routine, model)
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")
+    try:
+        path = os.path.join(tmpdir, name + "_ref.f90")
+        with open(path, "w") as fh:
+            fh.write("\n".join(src) + "\n")
+        exe = os.path.join(OUT_DIR, name)
+        cmd = [FLANG] + flags + [path, os.path.join(HERE, "contact_model_driver.f90"), "-o", exe]
+        subprocess.run(cmd, check=True, cwd=tmpdir,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    return exe
+
+
 SHIM_ROUTINES = ["Collision_MRT", "STREAMING", "BOUNDARYCONDITION", "MACROSCOPICVARIABLES",
                  "ComputeConcenterationFunction", "ComputeForceField_SPM",
                  "ComputeForce_HYDRO", "VelocityCorrector", "ComputeCollisionForce"]
@@ -312,6 +369,9 @@ def main(argv):
         built.append(build(name, cfg, parity_flags))
     if os.path.exists(F3):
         built.append(build_f3_contact(parity_flags + ["-fno-automatic"]))
+    if os.path.exists(F4) and os.path.exists(F1):
+        for name in CONTACT_MODELS:
+            built.append(build_contact_model(name, parity_flags))
     if os.path.exists(os.path.join(PKG, "liblbm_amd.so")):
         for name, defines in SHIM_VARIANTS.items():
             built.append(build_shim(name, VARIANTS[name], defines, parity_flags))

@@ -418,18 +418,148 @@ void orc_correct(const orc_params *p, orc_state *st)
     }
 }
 
-/* ComputeCollisionForce, Glowinski model: N-body pair loop of F3:739-770
- * (accumulating, IP ascending then JP ascending) and the mirror-image walls
- * of F3:773-804 / F6:726-759.  The pair branch reads D_PW as left by the
- * previous call's last wall evaluation (F3:760, F6:711); dpw_stale carries it.
- * Returns the number of pairs with d <= r_i + r_j + zeta (written i<j,
- * ascending, into pairs[2*n] when there is room). */
+/* The reference's (..)**8 and (..)**14 (F4:595-596, F4:616-617) with the
+ * roundings of its compiled form: **8 as seven multiplications left to
+ * right, **14 by the binary method (x^2, x^4, x^8 by squaring, then
+ * (x^2 x^4) x^8).  Of the orders tried (binary, left to right, from x^7
+ * squared, reversed products) only this pair reproduces every F_COL of the
+ * compiled Lennard-Jones routine. */
+static double pow8(double x)
+{
+    return ((((((x * x) * x) * x) * x) * x) * x) * x;
+}
+static double pow14(double x)
+{
+    double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
+    return (x2 * x4) * x8;
+}
+
+/* Pair term of the pair (ip, jp), ip < jp, with the stale wall distance dpw
+ * for the Glowinski branch; returns 0 (and zero force) when the pair is out
+ * of range d > r_i + r_j + zeta.
+ *   contact 0, Glowinski: F3:746-770 (F6:700-720 for one pair).
+ *   contact 1, Wan-Turek: F4:632-645 / F1:654-667: eps_p (x_i - x_j) times
+ *     (r_i + r_j - d) inside contact, (r_i + r_j + zeta - d)^2 in the gap.
+ *   contact 2, Lennard-Jones: F4:590-599 / F1:612-621 with R_P the radius of
+ *     particle 1 (F4:588): 2.4 R_P^2 (2 q^14 - q^8) (x_i - x_j) / (4 R_P^2),
+ *     q = 2 R_P / d. */
+static int pair_term(const orc_params *p, const orc_state *st, int ip, int jp, double dpw,
+                     double *f0, double *f1)
+{
+    const double zeta = p->zeta_c, cij = p->cij, eps_p = p->eps_p, e_p = p->e_p;
+    double xi = PF(st, ORC_PF_X, ip), yi = PF(st, ORC_PF_Y, ip);
+    double xj = PF(st, ORC_PF_X, jp), yj = PF(st, ORC_PF_Y, jp);
+    double di = PF(st, ORC_PF_DIAM, ip), dj = PF(st, ORC_PF_DIAM, jp);
+    double dpp = sqrt((xi - xj) * (xi - xj) + (yi - yj) * (yi - yj));
+    double rsum = 0.5 * di + 0.5 * dj;
+    *f0 = 0.0; *f1 = 0.0;
+    if (dpp > rsum + zeta) return 0;
+    if (p->contact == 1) {
+        if (dpp < rsum) {
+            *f0 = eps_p * (xi - xj) * (rsum - dpp);
+            *f1 = eps_p * (yi - yj) * (rsum - dpp);
+        } else {
+            double t = rsum + zeta - dpp;
+            *f0 = eps_p * (xi - xj) * (t * t);
+            *f1 = eps_p * (yi - yj) * (t * t);
+        }
+        return 1;
+    }
+    if (p->contact == 2) {
+        double rp = 0.5 * PF(st, ORC_PF_DIAM, 0);
+        double q = 2.0 * rp / dpp;
+        double a = F32(2.4f) * (rp * rp) * (2.0 * pow14(q) - pow8(q));
+        *f0 = a * (xi - xj) / (4.0 * rp * rp);
+        *f1 = a * (yi - yj) / (4.0 * rp * rp);
+        return 1;
+    }
+    double t = (dpp - 0.5 * di - 0.5 * dj - zeta) / zeta;
+    if (dpw > rsum) {
+        *f0 = eps_p * cij * (t * t) * (xi - xj) / dpp;
+        *f1 = eps_p * cij * (t * t) * (yi - yj) / dpp;
+    } else {
+        *f0 = (eps_p * cij * (t * t) + e_p * cij * (rsum - dpp) / zeta) * (xi - xj) / dpp;
+        *f1 = (eps_p * cij * (t * t) + e_p * cij * (rsum - dpp) / zeta) * (yi - yj) / dpp;
+    }
+    return 1;
+}
+
+/* Force of wall iw (1 bottom, 2 right, 3 top, 4 left) on a disk at (x, y),
+ * diameter dm, from the wall image R_W; returns the distance to it.
+ *   contact 0, Glowinski: images at half a diameter outside the domain
+ *     (F3:773-804 / F6:726-759).
+ *   contact 1, Wan-Turek: mirror images about the wall edges E
+ *     (F4:648-676 / F1:670-698): eps_w (c - R_W) (dm - d) inside,
+ *     (dm + zeta - d)^2 within zeta.
+ *   contact 2, Lennard-Jones: the images of F4:600-621 / F1:622-643 on the
+ *     nx x ny box ((x, 0), (nx, y), (x, 2 ny - y), (-x, y)) and
+ *     2.4 R_P^2 (2 q^14 - q^8) (c - R_W) / R_P^2, q = R_P / d. */
+static double wall_term(const orc_params *p, const orc_state *st, double x, double y, double dm,
+                        int iw, double *f0, double *f1)
+{
+    const double zeta = p->zeta_c, cij = p->cij, eps_w = p->eps_w, e_w = p->e_w;
+    const double *E = p->wall_edge;
+    double rw0, rw1;
+    if (p->contact == 1) {
+        if (iw == 1) { rw0 = x; rw1 = 2.0 * E[0] - y; }
+        else if (iw == 2) { rw0 = 2.0 * E[1] - x; rw1 = y; }
+        else if (iw == 3) { rw0 = x; rw1 = 2.0 * E[2] - y; }
+        else { rw0 = 2.0 * E[3] - x; rw1 = y; }
+    } else if (p->contact == 2) {
+        if (iw == 1) { rw0 = x; rw1 = 0.0; }
+        else if (iw == 2) { rw0 = (double)p->nx; rw1 = y; }
+        else if (iw == 3) { rw0 = x; rw1 = 2.0 * (double)p->ny - y; }
+        else { rw0 = -x; rw1 = y; }
+    } else {
+        if (iw == 1) { rw0 = x; rw1 = -(0.5 * dm); }
+        else if (iw == 2) { rw0 = (double)p->nx + 0.5 * dm; rw1 = y; }
+        else if (iw == 3) { rw0 = x; rw1 = (double)p->ny + 0.5 * dm; }
+        else { rw0 = -(0.5 * dm); rw1 = y; }
+    }
+    double d = sqrt((x - rw0) * (x - rw0) + (y - rw1) * (y - rw1));
+    *f0 = 0.0; *f1 = 0.0;
+    if (d > dm + zeta) return d;
+    if (p->contact == 1) {
+        if (d < dm) {
+            *f0 = eps_w * (x - rw0) * (dm - d);
+            *f1 = eps_w * (y - rw1) * (dm - d);
+        } else {
+            double t = dm + zeta - d;
+            *f0 = eps_w * (x - rw0) * (t * t);
+            *f1 = eps_w * (y - rw1) * (t * t);
+        }
+    } else if (p->contact == 2) {
+        double rp = 0.5 * PF(st, ORC_PF_DIAM, 0);
+        double q = rp / d;
+        double a = F32(2.4f) * (rp * rp) * (2.0 * pow14(q) - pow8(q));
+        *f0 = a * (x - rw0) / (rp * rp);
+        *f1 = a * (y - rw1) / (rp * rp);
+    } else if (d > dm) {
+        double t = (d - dm - zeta) / zeta;
+        *f0 = eps_w * cij * (t * t) * (x - rw0) / d;
+        *f1 = eps_w * cij * (t * t) * (y - rw1) / d;
+    } else {
+        double t = (d - dm - zeta) / zeta;
+        *f0 = (eps_w * cij * (t * t) + e_w * cij * (dm - d) / zeta) * (x - rw0) / d;
+        *f1 = (eps_w * cij * (t * t) + e_w * cij * (dm - d) / zeta) * (y - rw1) / d;
+    }
+    return d;
+}
+
+/* ComputeCollisionForce: N-body pair loop of F3:739-770 (accumulating, IP
+ * ascending then JP ascending) and the four walls in the order bottom,
+ * right, top, left, with the pair and wall terms of the selected model
+ * (pair_term, wall_term).  The reference's Wan-Turek and Lennard-Jones
+ * blocks handle the one pair (1, 2) by assignment (F4:632-645); for two
+ * disks the accumulation gives the same bits, and for more it sums every
+ * pair as F3's loop does.  The Glowinski pair branch reads D_PW as left by
+ * the previous call's last wall evaluation (F3:760, F6:711); dpw_stale
+ * carries it.  Returns the number of pairs with d <= r_i + r_j + zeta
+ * (written i<j, ascending, into pairs[2*n] when there is room). */
 int32_t orc_contact(const orc_params *p, orc_state *st, int32_t *pairs,
                     int32_t max_pairs)
 {
     const int np = st->np;
-    const double zeta = p->zeta_c, cij = p->cij;
-    const double eps_p = p->eps_p, eps_w = p->eps_w, e_p = p->e_p, e_w = p->e_w;
     double dpw = st->dpw_stale;
     int32_t npairs = 0;
     for (int ip = 0; ip < np; ++ip) {
@@ -438,28 +568,13 @@ int32_t orc_contact(const orc_params *p, orc_state *st, int32_t *pairs,
     }
     for (int ip = 0; ip < np - 1; ++ip) {
         for (int jp = ip + 1; jp < np; ++jp) {
-            double xi = PF(st, ORC_PF_X, ip), yi = PF(st, ORC_PF_Y, ip);
-            double xj = PF(st, ORC_PF_X, jp), yj = PF(st, ORC_PF_Y, jp);
-            double di = PF(st, ORC_PF_DIAM, ip), dj = PF(st, ORC_PF_DIAM, jp);
-            double dpp = sqrt((xi - xj) * (xi - xj) + (yi - yj) * (yi - yj));
             double fpp0, fpp1;
-            double rsum = 0.5 * di + 0.5 * dj;
-            if (dpp > rsum + zeta) {
-                fpp0 = 0.0; fpp1 = 0.0;
-            } else {
+            if (pair_term(p, st, ip, jp, dpw, &fpp0, &fpp1)) {
                 if (pairs && npairs < max_pairs) {
                     pairs[2 * npairs] = ip;
                     pairs[2 * npairs + 1] = jp;
                 }
                 ++npairs;
-                double t = (dpp - 0.5 * di - 0.5 * dj - zeta) / zeta;
-                if (dpw > rsum) {
-                    fpp0 = eps_p * cij * (t * t) * (xi - xj) / dpp;
-                    fpp1 = eps_p * cij * (t * t) * (yi - yj) / dpp;
-                } else {
-                    fpp0 = (eps_p * cij * (t * t) + e_p * cij * (rsum - dpp) / zeta) * (xi - xj) / dpp;
-                    fpp1 = (eps_p * cij * (t * t) + e_p * cij * (rsum - dpp) / zeta) * (yi - yj) / dpp;
-                }
             }
             PF(st, ORC_PF_FCX, ip) = PF(st, ORC_PF_FCX, ip) + fpp0;
             PF(st, ORC_PF_FCY, ip) = PF(st, ORC_PF_FCY, ip) + fpp1;
@@ -470,24 +585,8 @@ int32_t orc_contact(const orc_params *p, orc_state *st, int32_t *pairs,
     for (int ip = 0; ip < np; ++ip) {
         double x = PF(st, ORC_PF_X, ip), y = PF(st, ORC_PF_Y, ip), dm = PF(st, ORC_PF_DIAM, ip);
         for (int iw = 1; iw <= 4; ++iw) {
-            double rw0, rw1;
-            if (iw == 1) { rw0 = x; rw1 = -(0.5 * dm); }
-            else if (iw == 2) { rw0 = (double)p->nx + 0.5 * dm; rw1 = y; }
-            else if (iw == 3) { rw0 = x; rw1 = (double)p->ny + 0.5 * dm; }
-            else { rw0 = -(0.5 * dm); rw1 = y; }
-            dpw = sqrt((x - rw0) * (x - rw0) + (y - rw1) * (y - rw1));
-            double fpw0 = 0.0, fpw1 = 0.0;
-            if (dpw > dm + zeta) {
-                fpw0 = 0.0; fpw1 = 0.0;
-            } else if (dpw <= dm + zeta && dpw > dm) {
-                double t = (dpw - dm - zeta) / zeta;
-                fpw0 = eps_w * cij * (t * t) * (x - rw0) / dpw;
-                fpw1 = eps_w * cij * (t * t) * (y - rw1) / dpw;
-            } else if (dpw <= dm) {
-                double t = (dpw - dm - zeta) / zeta;
-                fpw0 = (eps_w * cij * (t * t) + e_w * cij * (dm - dpw) / zeta) * (x - rw0) / dpw;
-                fpw1 = (eps_w * cij * (t * t) + e_w * cij * (dm - dpw) / zeta) * (y - rw1) / dpw;
-            }
+            double fpw0, fpw1;
+            dpw = wall_term(p, st, x, y, dm, iw, &fpw0, &fpw1);
             PF(st, ORC_PF_FCX, ip) = PF(st, ORC_PF_FCX, ip) + fpw0;
             PF(st, ORC_PF_FCY, ip) = PF(st, ORC_PF_FCY, ip) + fpw1;
         }

@@ -62,6 +62,11 @@ typedef struct orc_params {
     double corner[2];        /* ux_w, uy_w at the four corners (F6:1365) */
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;  /* contact (F6:691-693) */
     double osc_u, osc_period;  /* U_C and T_C of the prescribed velocity (F4:224) */
+    int32_t contact;         /* 0 Glowinski (F3:739-804), 1 Wan-Turek (F4:624-677),
+                                2 Lennard-Jones (F4:587-622) */
+    int32_t pad_;
+    double wall_edge[4];     /* Wan-Turek walls WALL%EDGE(1:4): bottom y, right x, top y,
+                                left x (F4:182-185) */
 } orc_params;
 
 typedef struct orc_state {

@@ -51,6 +51,8 @@ class Params(ctypes.Structure):
         ("eps_p", ctypes.c_double), ("eps_w", ctypes.c_double),
         ("e_p", ctypes.c_double), ("e_w", ctypes.c_double),
         ("osc_u", ctypes.c_double), ("osc_period", ctypes.c_double),
+        ("contact", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("wall_edge", ctypes.c_double * 4),
     ]
 
 
@@ -103,8 +105,8 @@ def make_params(cfg):
     """Params from a plain dict (keys as in the struct; wall/corner sequences)."""
     p = Params()
     for name, ctype in Params._fields_:
-        if name in ("wall", "corner"):
-            vals = cfg.get(name, [0.0] * (8 if name == "wall" else 2))
+        if name in ("wall", "corner", "wall_edge"):
+            vals = cfg.get(name, [0.0] * {"wall": 8, "corner": 2, "wall_edge": 4}[name])
             arr = getattr(p, name)
             for i, v in enumerate(vals):
                 arr[i] = float(v)

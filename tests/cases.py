@@ -92,3 +92,24 @@ def golden_params(meta):
     if meta.get("boundary") == "equilibrium":
         cfg["boundary"] = 2  # F4:1205-1402
     return cfg
+
+
+# Contact-model builds of the reference routine (oracle/build_ref.py):
+# contact model and the constants each block sets (F4:625, F1:647, F4:588).
+CONTACT_MODELS = {
+    "f4_contact_wt": dict(contact=1, zeta_c=1.0, eps_p=1.0, eps_w=2.0),
+    "f4_contact_lj": dict(contact=2, zeta_c=1.0),
+    "f1_contact_wt": dict(contact=1, zeta_c=2.0, eps_p=1.0, eps_w=2.0),
+}
+
+
+def contact_model_cases(name):
+    """(nx, ny, [(wall_edge, disks (n, 3: x y diam), expected F_COL (n, 2))])
+    of one build in the contact_models fixture."""
+    z = np.load(os.path.join(GOLDEN, "contact_models.npz"))
+    nx, ny = (int(v) for v in z[name + "_nx"])
+    out, off = [], 0
+    for edges, n in zip(z[name + "_edges"], z[name + "_n"]):
+        out.append((list(edges), z[name + "_disks"][off:off + n], z[name + "_fcol"][off:off + n]))
+        off += n
+    return nx, ny, out

@@ -11,6 +11,9 @@ parsed doubles are exact), and stores them as compressed .npz fixtures:
   per-step particle history (F6:315-318) and mean density (F6:348-349);
   run constants printed at start-up (F6:104-111): TAU_LB, U_C, TAU_Y, A_G.
 
+contact_models.npz from oracle/_ref/f4_contact_wt, f4_contact_lj and
+f1_contact_wt (the reference's Wan-Turek and Lennard-Jones contact forces,
+driven by oracle/contact_model_driver.f90): wall edges and disks in, F_COL out;
 and f3_contact.npz from oracle/_ref/f3_contact (the reference's N-body
 ComputeCollisionForce, F3:632-806, driven by oracle/f3_contact_driver.f90):
 disk positions and diameters in, F_COL of the second and third call out.
@@ -74,10 +77,98 @@ def make_contact():
     print("wrote", os.path.relpath(path, REPO), os.path.getsize(path), "bytes")
 
 
+def contact_model_cases(name, nx, ny):
+    """Inputs for the Wan-Turek / Lennard-Jones builds: per case the wall
+    edges and two disks (Wan-Turek builds also get one-disk cases).  Pair
+    distances sweep from inside r_i + r_j to beyond r_i + r_j + 2 (every
+    branch for zeta 1 and 2, plus d = r_i + r_j exactly); wall cases put one
+    disk near one or two walls of the model's wall geometry (Wan-Turek:
+    mirror images about the edges; Lennard-Jones: the images of F4:600-621
+    on the module's NX x NY box) with the other disk out of range.
+    Coordinates are multiples of 1/64 so their decimal text is exact."""
+    rng = np.random.default_rng(20190926 + len(name))
+    q = lambda v: float(np.round(v * 64) / 64)
+    lj = name.endswith("_lj")
+    boxes = [(0.0, 100.0, 80.0, 0.0), (2.5, 97.25, 77.5, 3.75)]
+    cases = []
+    for k in range(40):  # pairs near (50, 40), out of every wall's range
+        d1, d2 = (float(rng.choice([8.0, 10.0, 12.0, 16.0])) for _ in range(2))
+        s = 0.5 * d1 + 0.5 * d2
+        dist = rng.uniform(0.7 * s, s + 2.6)
+        a = rng.uniform(0, 2 * np.pi)
+        x1, y1 = q(50 + rng.uniform(-2, 2)), q(40 + rng.uniform(-2, 2))
+        x2, y2 = q(x1 + dist * np.cos(a)), q(y1 + dist * np.sin(a))
+        cases.append((boxes[k % 2], [(x1, y1, d1), (x2, y2, d2)]))
+    cases.append((boxes[0], [(40.0, 40.0, 10.0), (50.0, 40.0, 10.0)]))  # d = r_i + r_j
+    cases.append((boxes[0], [(40.0, 40.0, 10.0), (40.0, 51.0, 12.0)]))  # the same, along y
+    far = (50.0, 40.0, 8.0)
+    for k in range(48):  # one disk near a wall (two near the corners)
+        dm = float(rng.choice([8.0, 10.0, 16.0]))
+        box = boxes[k % 2]
+        w = k % 4
+        if lj:
+            # wall distances of the L-J images: y, NX - x, 2 (NY - y), 2 x
+            g = rng.uniform(0.5 * dm, dm + 1.5)
+            x, y = q(0.5 * nx), q(0.5 * ny)
+            if w == 0: y = q(g)
+            elif w == 1: x = q(nx - g)
+            elif w == 2: y = q(ny - 0.5 * g)
+            else: x = q(0.5 * g)
+            if k >= 40:
+                x, y = q(0.5 * g), q(g)
+            cases.append((box, [(x, y, dm), (q(0.5 * nx + 100.0), q(0.5 * ny), 8.0)]))
+            continue
+        gap = rng.uniform(0.25 * dm, 0.5 * dm + 1.3)  # mirror distance 2 gap
+        x, y = q(0.5 * (box[1] + box[3])), q(0.5 * (box[0] + box[2]))
+        if w == 0: y = q(box[0] + gap)
+        elif w == 1: x = q(box[1] - gap)
+        elif w == 2: y = q(box[2] - gap)
+        else: x = q(box[3] + gap)
+        if k >= 40:
+            x, y = q(box[3] + gap), q(box[0] + gap)
+        far_disk = far if abs(x - far[0]) + abs(y - far[1]) > 30 else (q(x + 40.0), q(y + 20.0), 8.0)
+        cases.append((box, [(x, y, dm), far_disk]))
+    if not lj:
+        cases.append((boxes[0], [(5.0, 40.0, 10.0)]))   # mirror distance = D exactly
+        cases.append((boxes[0], [(4.75, 3.0, 10.0)]))   # one disk, two walls
+        cases.append((boxes[1], [(50.0, 74.0, 8.0)]))
+    return cases
+
+
+def make_contact_models():
+    out = {}
+    for name in build_ref.CONTACT_MODELS:
+        exe = os.path.join(build_ref.OUT_DIR, name)
+        if not os.path.exists(exe):
+            raise SystemExit("missing %s: run python oracle/build_ref.py" % exe)
+        nx, ny = (int(v) for v in subprocess.run([exe], input="0\n", stdout=subprocess.PIPE,
+                                                  check=True, text=True).stdout.split())
+        cases = contact_model_cases(name, nx, ny)
+        text = "%d\n" % len(cases)
+        for box, disks in cases:
+            text += "%r %r %r %r\n%d\n" % (box + (len(disks),))
+            text += "".join("%r %r %r\n" % d for d in disks)
+        r = subprocess.run([exe], input=text, stdout=subprocess.PIPE, check=True, text=True)
+        rows = [ln.split() for ln in r.stdout.strip().split("\n")][1:]
+        out[name + "_nx"] = np.array([nx, ny])
+        out[name + "_edges"] = np.array([box for box, _ in cases])
+        out[name + "_n"] = np.array([len(d) for _, d in cases])
+        out[name + "_disks"] = np.array([d for _, ds in cases for d in ds])
+        out[name + "_fcol"] = np.array([[float(v) for v in row] for row in rows])
+        assert len(out[name + "_fcol"]) == len(out[name + "_disks"])
+        nz = np.count_nonzero(np.any(out[name + "_fcol"] != 0, axis=1))
+        print(name, len(cases), "cases,", nz, "nonzero forces")
+    path = os.path.join(HERE, "contact_models.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", os.path.relpath(path, REPO), os.path.getsize(path), "bytes")
+
+
 def main(names):
     """names: the variants to write (default: every variant and the contact
     fixture)."""
     os.makedirs(HERE, exist_ok=True)
+    if not names or "contact_models" in names:
+        make_contact_models()
     if not names:
         make_contact()
     for name, cfg in build_ref.VARIANTS.items():

@@ -3,6 +3,7 @@ header declares, and rejects bad parameters before touching a device."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -26,10 +27,24 @@ def test_library_exports_every_declared_symbol():
     assert sorted(abi.exported_names()) == names
 
 
-def test_params_struct_matches_header_size():
+def test_params_struct_matches_header_layout(tmp_path):
+    """sizeof(lbm_params) and every field offset, as the C compiler lays out
+    the header's struct, equal the ctypes mirror's."""
     abi = pkg()._abi
-    # 10 int32 + 14 doubles + wall[8] + corner[2] + 6 contact + 2 oscillation doubles
-    assert ctypes.sizeof(abi.Params) == 10 * 4 + (1 + 6 + 1 + 4 + 2 + 8 + 2 + 6 + 2) * 8
+    fields = [n for n, _ in abi.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "lbm_amd.h"\n'
+                   "int main(void) {\n"
+                   '  printf("%zu\\n", sizeof(lbm_params));\n'
+                   + "".join('  printf("%%zu\\n", offsetof(lbm_params, %s));\n' % n for n in fields)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, check=True,
+                                          text=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(abi.Params)
+    assert out[1:] == [getattr(abi.Params, n).offset for n in fields]
 
 
 @pytest.mark.parametrize("bad", [
@@ -38,6 +53,8 @@ def test_params_struct_matches_header_size():
     dict(rheology=7),
     dict(storage=9),
     dict(max_particles=-1),
+    dict(boundary=3),
+    dict(contact=3),
 ])
 def test_create_rejects_bad_parameters(bad):
     abi = pkg()._abi

@@ -290,6 +290,65 @@ def test_contact_force_matches_reference_routine(orc):
         assert np.array_equal(got, z[key]), call
 
 
+@pytest.mark.parametrize("name", sorted(cases.CONTACT_MODELS))
+def test_contact_models_match_reference_routine(orc, name):
+    """Wan-Turek and Lennard-Jones contact forces through lbm_contact_force
+    against the reference routine (F4:587-677, F1:609-699, compiled from the
+    reference with that model switched on; fixture contact_models): every
+    F_COL bit-exact.  One handle per wall-edge set (wall_edge is a creation
+    parameter), on the module's NX x NY grid."""
+    nx, ny, cs = cases.contact_model_cases(name)
+    cfg = dict(cases.f6_params(nx, ny, False), **cases.CONTACT_MODELS[name])
+    print("case: contact model", name, len(cs), flush=True)
+    handles = {}
+    for c, (edges, disks, want) in enumerate(cs):
+        key = tuple(edges)
+        if key not in handles:
+            handles[key] = gpu_solver(dict(cfg, wall_edge=edges), max_particles=2)
+            handles[key].initialize()
+        g = handles[key]
+        recs = np.stack([disk_record(x, y, d, 1.5) for x, y, d in disks], axis=1)
+        g.set_particles(recs)
+        g.compute_collision_force()
+        pg = g.get_particles()
+        got = np.stack([pg[orc.PF_FCX], pg[orc.PF_FCY]], axis=1)
+        assert np.array_equal(got, want), (c, got, want)
+    for g in handles.values():
+        g.close()
+
+
+@pytest.mark.parametrize("model", ["wan_turek", "lennard_jones"])
+def test_fused_contact_models_match_oracle(orc, model):
+    """Fused FP64 steps with the Wan-Turek / Lennard-Jones contact force: two
+    disks starting inside each other's range, the first within range of the
+    left wall, against the oracle."""
+    nx, ny = 48, 96
+    cfg, part = f6_case(nx, ny, False)
+    if model == "wan_turek":
+        cfg.update(contact=1, zeta_c=1.0, eps_p=1.0, eps_w=2.0, wall_edge=[0.0, nx, ny, 0.0])
+    else:
+        cfg.update(contact=2, zeta_c=1.0)
+    d = part[6, 0]
+    a, b = part[:, 0].copy(), part[:, 0].copy()
+    a[0], a[1] = 0.5 * d + 0.3, 0.5 * ny
+    b[0], b[1] = a[0] + d + 0.5, 0.5 * ny + 0.25
+    two = np.stack([a, b], axis=1)
+    print("case: fused contact", model, flush=True)
+    st = oracle_state(orc, cfg, two)
+    ho = st.run(0, 60)
+    g = gpu_solver(cfg, max_particles=2)
+    g.initialize()
+    g.set_particles(two)
+    hg = g.step(0, 60)
+    _compare_state(g, st, label="fused " + model)
+    assert rel_err(hg[:, :-2], ho[:, :-2]) <= TRAJ_TOL
+    pg, po = g.get_particles(), st.part
+    for k in (orc.PF_FCX, orc.PF_FCY):
+        assert rel_err(pg[k], po[k]) <= TRAJ_TOL
+    # the contact force acted: the disks moved apart along x
+    assert pg[orc.PF_X, 1] - pg[orc.PF_X, 0] > b[0] - a[0]
+
+
 def test_dropin_lagged_rate_matches_oracle(orc):
     """The unmodified reference's lagged module-global rate (F6:958) through
     the per-subroutine entries: the sequential collide keeps the oracle's

@@ -128,6 +128,78 @@ def test_oracle_contact_vs_reference_routine(orc):
     assert [tuple(p) for p in pairs] == brute
 
 
+def _branches(name, nx, ny, cases_):
+    """Which branches of the model the fixture's cases reach, computed from
+    the inputs alone: pair overlap / gap, wall overlap / gap."""
+    zeta = cases.CONTACT_MODELS[name]["zeta_c"]
+    seen = set()
+    for edges, disks, _ in cases_:
+        if len(disks) == 2:
+            (x1, y1, d1), (x2, y2, d2) = disks
+            d, s = np.hypot(x1 - x2, y1 - y2), 0.5 * d1 + 0.5 * d2
+            if d <= s + zeta:
+                seen.add("pair overlap" if d < s else "pair gap")
+        for x, y, dm in disks:
+            if name.endswith("_lj"):
+                dw = [y, nx - x, 2 * (ny - y), 2 * x]
+            else:
+                dw = [2 * abs(y - edges[0]), 2 * abs(edges[1] - x), 2 * abs(edges[2] - y),
+                      2 * abs(x - edges[3])]
+            for v in dw:
+                if v <= dm + zeta:
+                    seen.add("wall overlap" if v < dm else "wall gap")
+    return seen
+
+
+@pytest.mark.parametrize("name", sorted(cases.CONTACT_MODELS))
+def test_oracle_contact_models_vs_reference_routine(orc, name):
+    """Wan-Turek (F4:624-677 with zeta 1; F1:646-699 with zeta 2) and
+    Lennard-Jones (F4:587-622) contact forces against the reference routine
+    itself, compiled from F4 / F1 with that model switched on and driven by
+    oracle/contact_model_driver.f90: every F_COL bit for bit, over cases that
+    reach every branch of the model."""
+    nx, ny, cs = cases.contact_model_cases(name)
+    cfg = dict(cases.f6_params(nx, ny, False), **cases.CONTACT_MODELS[name])
+    states = {}
+    for c, (edges, disks, want) in enumerate(cs):
+        n = len(disks)
+        if n not in states:
+            states[n] = orc.State(orc.make_params(cfg), n)
+        st = states[n]
+        for k in range(4):
+            st.p.wall_edge[k] = edges[k]
+        part = np.zeros((orc.PF_COUNT, n))
+        part[orc.PF_X], part[orc.PF_Y], part[orc.PF_DIAM] = disks.T
+        st.set_particles(part)
+        st.contact()
+        got = np.stack([st.part[orc.PF_FCX, :n], st.part[orc.PF_FCY, :n]], axis=1)
+        assert np.array_equal(got, want), (c, got, want)
+    seen = _branches(name, nx, ny, cs)
+    want_b = {"pair overlap", "pair gap", "wall overlap", "wall gap"}
+    if name.endswith("_lj"):  # one formula inside the cut-off
+        want_b = {"pair overlap", "wall overlap"}
+    assert want_b <= seen, seen
+
+
+def test_oracle_contact_model_bites(orc):
+    """The fixture discriminates between models: the Glowinski terms
+    (contact 0) on the Wan-Turek cases' inputs miss most of them."""
+    nx, ny, cs = cases.contact_model_cases("f4_contact_wt")
+    cfg = dict(cases.f6_params(nx, ny, False), **cases.CONTACT_MODELS["f4_contact_wt"])
+    cfg["contact"] = 0  # Glowinski terms on the same inputs
+    differ = 0
+    for edges, disks, want in cs:
+        n = len(disks)
+        st = orc.State(orc.make_params(dict(cfg, wall_edge=edges)), n)
+        part = np.zeros((orc.PF_COUNT, n))
+        part[orc.PF_X], part[orc.PF_Y], part[orc.PF_DIAM] = disks.T
+        st.set_particles(part)
+        st.contact()
+        got = np.stack([st.part[orc.PF_FCX, :n], st.part[orc.PF_FCY, :n]], axis=1)
+        differ += not np.array_equal(got, want)
+    assert differ > len(cs) // 2
+
+
 def test_oracle_prescribed_oscillation_closed_form(orc):
     """Prescribed motion of F4:221-226: V_x(KK) = -U cos(2 pi KK / T), centre
     advanced by the mean of the old and new velocity, and the added-mass
