@@ -32,6 +32,7 @@ struct lbm_handle {
     void *sA = nullptr, *sB = nullptr;
     float *rate32 = nullptr;
     void *erho[2] = {nullptr, nullptr};
+    void *wtab = nullptr;  // wall velocities per edge class: 10 doubles then 10 floats
     // particles
     double *pbuf[2] = {nullptr, nullptr};
     int pcur = 0;
@@ -228,9 +229,28 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->pforce, (size_t)P.maxp * MAXSUB * 3 * 8)) ||
         (rc = al((void **)&h->pcount, (size_t)P.maxp * 4)) ||
         (rc = al((void **)&h->spf[0], (size_t)P.maxp * SPF * 4)) ||
-        (rc = al((void **)&h->spf[1], (size_t)P.maxp * SPF * 4))) {
+        (rc = al((void **)&h->spf[1], (size_t)P.maxp * SPF * 4)) ||
+        (rc = al(&h->wtab, 128))) {
         *out = h;
         return rc;
+    }
+    {
+        // (u_x, u_y) per edge class WT_CORNER, WT_TOP, WT_BOTTOM, WT_RIGHT,
+        // WT_LEFT; the FP32 copies are the same values rounded once
+        const double *w = p->wall;
+        const double wd[10] = {p->corner[0], p->corner[1], w[4], w[5], w[0], w[1],
+                               w[2], w[3], w[6], w[7]};
+        float wf[10];
+        for (int k = 0; k < 10; ++k) wf[k] = (float)wd[k];
+        char blob[128] = {0};
+        std::memcpy(blob, wd, sizeof wd);
+        std::memcpy(blob + 80, wf, sizeof wf);
+        if (hipMemcpy(h->wtab, blob, sizeof blob, hipMemcpyHostToDevice) != hipSuccess) {
+            *out = h;
+            return fail(h, LBM_ERR_HIP, "hipMemcpy of the wall table failed");
+        }
+        h->P.wtab = (const double *)h->wtab;
+        h->P.wtabf = (const float *)((const char *)h->wtab + 80);
     }
     if (p->max_particles > NP_LOOP) {
         size_t nt = (size_t)nblk(P.nn);
@@ -268,7 +288,7 @@ int lbm_destroy(lbm_handle *h) {
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
-                    h->pforce, h->pcount, h->spf[0], h->spf[1], h->lag};
+                    h->pforce, h->pcount, h->spf[0], h->spf[1], h->lag, h->wtab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -972,6 +992,18 @@ int check_lists(lbm_handle *h) {
     return LBM_OK;
 }
 
+// The fused step kernel of one boundary mode (each instantiation carries only
+// its own edge rule).
+template <class St, bool DEV, bool FINAL>
+void launch_step(int boundary, dim3 grid, dim3 block, hipStream_t stream, const StepArgs &a) {
+    if (boundary == LBM_ZOU_HE)
+        hipLaunchKernelGGL((k_step<St, DEV, FINAL, LBM_ZOU_HE>), grid, block, 0, stream, a);
+    else if (boundary == LBM_EQUILIBRIUM)
+        hipLaunchKernelGGL((k_step<St, DEV, FINAL, LBM_EQUILIBRIUM>), grid, block, 0, stream, a);
+    else
+        hipLaunchKernelGGL((k_step<St, DEV, FINAL, LBM_BOUNCE_BACK>), grid, block, 0, stream, a);
+}
+
 template <class St, bool DEV>
 int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed,
               double *ms_total, double *ms_main) {
@@ -994,9 +1026,9 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         if (timed && s == 1 && !fin) (void)hipEventRecord(h->evA, h->stream);
         if (timed && fin && s > 1) (void)hipEventRecord(h->evB, h->stream);
         if (fin)
-            hipLaunchKernelGGL((k_step<St, DEV, true>), grid, block, 0, h->stream, a);
+            launch_step<St, DEV, true>(P.boundary, grid, block, h->stream, a);
         else
-            hipLaunchKernelGGL((k_step<St, DEV, false>), grid, block, 0, h->stream, a);
+            launch_step<St, DEV, false>(P.boundary, grid, block, h->stream, a);
         rc = check_launch(h, "k_step");
         if (rc) return rc;
     }
@@ -1127,6 +1159,7 @@ struct lbm_batch {
     hipGraphExec_t gexec = nullptr;
     int gexec_chunk = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool all_bounce_back = true;          // every simulation has LBM_BOUNCE_BACK walls
 };
 
 namespace {
@@ -1201,11 +1234,20 @@ int batch_launches(lbm_batch *b, int32_t nsteps) {
     // per-step fields, then the launch that reads them
     auto launch = [&](StepArgs *A, const int *sdev, int s_const, int ns, bool fin) {
         hipLaunchKernelGGL(k_batch_fields, fgrid, fblock, 0, b->stream, A, n, sdev, s_const, ns);
-        if (fin)
-            hipLaunchKernelGGL((k_batch_step<St, DEV, true>), grid, block, 0, b->stream,
+        // every simulation with bounce-back walls: the kernel without the
+        // other edge rules; otherwise each simulation's mode is read at run time
+        const bool bb = b->all_bounce_back;
+        if (fin && bb)
+            hipLaunchKernelGGL((k_batch_step<St, DEV, true, LBM_BOUNCE_BACK>), grid, block, 0,
+                               b->stream, (const StepArgs *)A, (const int2 *)b->bmap);
+        else if (fin)
+            hipLaunchKernelGGL((k_batch_step<St, DEV, true, -1>), grid, block, 0, b->stream,
                                (const StepArgs *)A, (const int2 *)b->bmap);
+        else if (bb)
+            hipLaunchKernelGGL((k_batch_step<St, DEV, false, LBM_BOUNCE_BACK>), grid, block, 0,
+                               b->stream, (const StepArgs *)A, (const int2 *)b->bmap);
         else
-            hipLaunchKernelGGL((k_batch_step<St, DEV, false>), grid, block, 0, b->stream,
+            hipLaunchKernelGGL((k_batch_step<St, DEV, false, -1>), grid, block, 0, b->stream,
                                (const StepArgs *)A, (const int2 *)b->bmap);
     };
     hipLaunchKernelGGL(k_batch_fields, fgrid, fblock, 0, b->stream, A0, n, (const int *)nullptr, 0,
@@ -1352,6 +1394,7 @@ int lbm_batch_create(int32_t n, const lbm_params *params, lbm_batch **out) {
             return rc;
         }
         b->sims.push_back(h);
+        if (params[i].boundary != LBM_BOUNCE_BACK) b->all_bounce_back = false;
     }
     b->tpb.assign(n, 1);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||

@@ -51,6 +51,8 @@ struct DevP {
     int nbnd;            // boundary nodes
     double omega, s0, s1, s2, s3, s5, sq, nu, mu0, tau_y, m_pap, gamma_min, rho0, gy;
     double wall[8], corner[2];
+    const double *wtab;  // device table of the wall velocities (WT_* pairs), FP64
+    const float *wtabf;  // the same in FP32
     double zeta_c, cij, eps_p, eps_w, e_p, e_w;
     int contact;         // 0 Glowinski, 1 Wan-Turek, 2 Lennard-Jones
     double wall_edge[4]; // Wan-Turek walls: bottom y, right x, top y, left x
@@ -208,41 +210,80 @@ __device__ __forceinline__ void rates64(const DevP &P, double om, double s[9]) {
     s[5] = P.s5; s[6] = q; s[7] = om; s[8] = om;
 }
 
+// Wall velocity of a boundary node from the handle's device table (pairs
+// (u_x, u_y) of the corners, top, bottom, right and left walls, in FP64 and
+// FP32).  The edge rules read it only on boundary nodes: a choice among the
+// wall fields of the parameter block would either be a run-time index into
+// it (which the compiler copies to scratch memory) or eight loads that it
+// hoists to the kernel entry, where they occupy scalar registers for the
+// whole kernel and push the collide's parameters into spills.
+template <class T> __device__ __forceinline__ const T *wall_table(const DevP &P);
+template <> __device__ __forceinline__ const double *wall_table<double>(const DevP &P) {
+    return P.wtab;
+}
+template <> __device__ __forceinline__ const float *wall_table<float>(const DevP &P) {
+    return P.wtabf;
+}
+enum { WT_CORNER = 0, WT_TOP, WT_BOTTOM, WT_RIGHT, WT_LEFT };
+
 // Zou-He edge rule (F1:1246-1258 top and bottom, F1:1319-1341 left and
 // right) at edge node (i, j), corners excluded: the three unknown
 // populations from the node's own streamed ones and the wall velocity
-// (P.wall: uxS,uyS, uxR,uyR, uxN,uyN, uxL,uyL), in F1's operation order.
+// (wall_table: P.wall's uxS,uyS, uxR,uyR, uxN,uyN, uxL,uyL), in F1's
+// operation order.
 // v holds f (rho_base = 0) or deviations f - w_i rho0 (rho_base = rho0:
 // the density sums of the rule pick up exactly rho0 from the weights).
+//
+// The four edges share one form, evaluated once with per-edge operands
+// selected by value (a branch per edge, each writing its own three
+// elements, would make the compiler merge those stores into one with a
+// run-time index and keep the population array in scratch memory):
+//   r     = (rho_base + ((v0 + a1) + a2 + 2 ((b1 + b2) + b3))) / (1 + s_n u_n)
+//   axis  = b1 - s_n ((4/6) r) u_n
+//   diag1 = ((c1 - D) + s11 T1) + s12 T2,  diag2 = ((c2 + D) + s21 T1) + s22 T2
+// with D = 0.5 (a1 - a2), T1 = (k1 r) ux, T2 = (k2 r) uy; x + (-y) rounds as
+// x - y, so every edge keeps F1's expression bit for bit.
+//   edge    a1,a2  b1,b2,b3  s_n u_n  c1->k  c2->k  k1,k2      s11 s12 s21 s22
+//   top     v1,v3  v2,v6,v5  +  uy    v6->8  v5->7  1/2, 1/6   +   -   -   -
+//   bottom  v1,v3  v4,v7,v8  -  uy    v7->5  v8->6  1/2, 1/6   +   +   -   +
+//   left    v2,v4  v3,v6,v7  -  ux    v7->5  v6->8  1/6, 1/2   +   +   +   -
+//   right   v2,v4  v1,v5,v8  +  ux    v8->6  v5->7  1/6, 1/2   -   +   -   -
+// (axis -> 4, 2, 1, 3 for top, bottom, left, right.)
 template <class T>
 __device__ __forceinline__ void zou_he_edge(const DevP &P, int i, int j, T rho_base, T v[9]) {
-    const T inv6 = (T)INV6;
-    const double *wl = P.wall;
-    if (j == P.ny) {
-        const T ux = (T)wl[4], uy = (T)wl[5];
-        const T r = (rho_base + (v[0] + v[1] + v[3] + (T)2. * (v[2] + v[6] + v[5]))) / ((T)1.0 + uy);
-        v[4] = v[2] - (T)4. * inv6 * r * uy;
-        v[8] = v[6] - (T)0.5 * (v[1] - v[3]) + (T)0.5 * r * ux - inv6 * r * uy;
-        v[7] = v[5] + (T)0.5 * (v[1] - v[3]) - (T)0.5 * r * ux - inv6 * r * uy;
-    } else if (j == 0) {
-        const T ux = (T)wl[0], uy = (T)wl[1];
-        const T r = (rho_base + (v[0] + v[1] + v[3] + (T)2. * (v[4] + v[7] + v[8]))) / ((T)1.0 - uy);
-        v[2] = v[4] + (T)4.0 * inv6 * r * uy;
-        v[5] = v[7] - (T)0.5 * (v[1] - v[3]) + (T)0.5 * r * ux + inv6 * r * uy;
-        v[6] = v[8] + (T)0.5 * (v[1] - v[3]) - (T)0.5 * r * ux + inv6 * r * uy;
-    } else if (i == 0) {
-        const T ux = (T)wl[6], uy = (T)wl[7];
-        const T r = (rho_base + (v[0] + v[2] + v[4] + (T)2.0 * (v[3] + v[6] + v[7]))) / ((T)1.0 - ux);
-        v[5] = v[7] - (T)0.5 * (v[2] - v[4]) + inv6 * r * ux + (T)0.5 * r * uy;
-        v[8] = v[6] + (T)0.5 * (v[2] - v[4]) + inv6 * r * ux - (T)0.5 * r * uy;
-        v[1] = v[3] + (T)4.0 * inv6 * r * ux;
-    } else {  // i == nx
-        const T ux = (T)wl[2], uy = (T)wl[3];
-        const T r = (rho_base + (v[0] + v[2] + v[4] + (T)2.0 * (v[1] + v[5] + v[8]))) / ((T)1.0 + ux);
-        v[6] = v[8] - (T)0.5 * (v[2] - v[4]) - inv6 * r * ux + (T)0.5 * r * uy;
-        v[7] = v[5] + (T)0.5 * (v[2] - v[4]) - inv6 * r * ux - (T)0.5 * r * uy;
-        v[3] = v[1] - (T)4.0 * inv6 * r * ux;
-    }
+    const T inv6 = (T)INV6, half = (T)0.5;
+    const bool top = j == P.ny, bot = !top && j == 0, lft = !top && !bot && i == 0;
+    const bool rgt = !top && !bot && !lft;
+    const bool vert = top || bot;  // top or bottom edge: normal along y
+    const int wc = top ? WT_TOP : bot ? WT_BOTTOM : lft ? WT_LEFT : WT_RIGHT;
+    const T ux = wall_table<T>(P)[2 * wc], uy = wall_table<T>(P)[2 * wc + 1];
+    const T a1 = vert ? v[1] : v[2], a2 = vert ? v[3] : v[4];
+    const T b1 = top ? v[2] : bot ? v[4] : lft ? v[3] : v[1];
+    const T b2 = top ? v[6] : bot ? v[7] : lft ? v[6] : v[5];
+    const T b3 = top ? v[5] : bot ? v[8] : lft ? v[7] : v[8];
+    const T c1 = top ? v[6] : (bot || lft) ? v[7] : v[8];
+    const T c2 = top ? v[5] : bot ? v[8] : lft ? v[6] : v[5];
+    const T un = vert ? uy : ux;
+    const bool pos = top || rgt;  // s_n = +1
+    const T sn_un = pos ? un : -un;
+    const T r = (rho_base + (v[0] + a1 + a2 + (T)2. * (b1 + b2 + b3))) / ((T)1.0 + sn_un);
+    const T ka = (T)4. * inv6 * r * un;
+    const T axis = pos ? b1 - ka : b1 + ka;
+    const T d = half * (a1 - a2);
+    const T t1 = (vert ? half : inv6) * r * ux;
+    const T t2 = (vert ? inv6 : half) * r * uy;
+    const T s11 = rgt ? -t1 : t1, s12 = top ? -t2 : t2;
+    const T s21 = lft ? t1 : -t1, s22 = (bot ? t2 : -t2);
+    const T diag1 = c1 - d + s11 + s12;
+    const T diag2 = c2 + d + s21 + s22;
+    v[1] = lft ? axis : v[1];
+    v[2] = bot ? axis : v[2];
+    v[3] = rgt ? axis : v[3];
+    v[4] = top ? axis : v[4];
+    v[5] = (bot || lft) ? diag1 : v[5];
+    v[6] = bot ? diag2 : (rgt ? diag1 : v[6]);
+    v[7] = (top || rgt) ? diag2 : v[7];
+    v[8] = top ? diag1 : (lft ? diag2 : v[8]);
 }
 
 // COMPUTEFEQNEW (F6:1130-1148, F4:1140-1158): the equilibrium with the
@@ -306,82 +347,99 @@ __device__ __forceinline__ void eq_sources(const DevP &P, int i, int j, Put put)
             if ((x | y) && bi[x] >= 0 && bj[y] >= 0) put(bnd_index(P, bi[x], bj[y]));
 }
 
-// Pull-stream with in-domain sources (F6:1164-1188) and the bounce-back
-// rules (F6:1206-1294, F6:1364-1386) for node (i, j): v receives f_k.
-// src: post-collision populations (SoA planes), erho: previous-step density
-// on the boundary nodes (equilibrium edges: the edge arrays of eq_sources).
-template <class St>
+// Population k of boundary node (i, j) (l, r, b, t: on the left, right,
+// bottom, top edge) is replaced by the bounce-back rule (F6:1206-1294 edges,
+// F6:1364-1386 corners): its source lies outside the domain and, at a
+// corner, it points into the domain (the two tangential diagonals keep the
+// node's own value).  opp(k): the population of opposite velocity.
+__device__ __forceinline__ bool bb_replaced(int k, bool l, bool r, bool b, bool t) {
+    switch (k) {
+    case 1: return l;
+    case 2: return b;
+    case 3: return r;
+    case 4: return t;
+    case 5: return (b && !r) || (l && !t);
+    case 6: return (b && !l) || (r && !t);
+    case 7: return (t && !l) || (r && !b);
+    case 8: return (t && !r) || (l && !b);
+    default: return false;
+    }
+}
+__device__ __forceinline__ constexpr int opp_k(int k) {
+    return k == 0 ? 0 : (k <= 4 ? (k + 1) % 4 + 1 : (k - 3) % 4 + 5);
+}
+
+// Halfway bounce-back with moving walls at boundary node (i, j), in one
+// branch-free form: each replaced population k (bb_replaced) becomes
+// fo_opp(k) - c rw (c_opp(k) . u_w), with c rw precomputed as
+// a46 = ((4/6) clb^2) rw along the axes and a6 = ((1/6) clb^2) rw on the
+// diagonals, and u_w the node's wall velocity (the corner velocity at the
+// four corners).  On entry v[k] already holds fo_opp(k), the node's own
+// post-collision population opp(k), for every replaced k.  For every edge
+// and corner the operands and their order are those of the reference's
+// expressions ((-x) - y, x - y, x + y, (-x) + y), so the result is bitwise
+// the same.  (A branch per edge would write different elements of v in each
+// branch, which the compiler merges into one store at a run-time index and
+// so keeps v in scratch memory.)
+template <class T>
+__device__ __forceinline__ void bounce_back_node(const DevP &P, int i, int j, T a6, T a46,
+                                                 T v[9]) {
+    const bool l = i == 0, r = i == P.nx, b = j == 0, t = j == P.ny;
+    const bool cor = (l || r) && (b || t);
+    const int wc = cor ? WT_CORNER : t ? WT_TOP : b ? WT_BOTTOM : r ? WT_RIGHT : WT_LEFT;
+    const T uw = wall_table<T>(P)[2 * wc], vw = wall_table<T>(P)[2 * wc + 1];
+    v[1] = l ? v[1] - a46 * (-uw) : v[1];
+    v[2] = b ? v[2] - a46 * (-vw) : v[2];
+    v[3] = r ? v[3] - a46 * (uw) : v[3];
+    v[4] = t ? v[4] - a46 * (vw) : v[4];
+    v[5] = bb_replaced(5, l, r, b, t) ? v[5] - a6 * (-uw - vw) : v[5];
+    v[6] = bb_replaced(6, l, r, b, t) ? v[6] - a6 * (uw - vw) : v[6];
+    v[7] = bb_replaced(7, l, r, b, t) ? v[7] - a6 * (uw + vw) : v[7];
+    v[8] = bb_replaced(8, l, r, b, t) ? v[8] - a6 * (-uw + vw) : v[8];
+}
+
+// Pull-stream with in-domain sources (F6:1164-1188) and the boundary rules
+// for node (i, j): v receives f_k.  src: post-collision populations (SoA
+// planes), erho: previous-step density on the boundary nodes (equilibrium
+// edges: the edge arrays of eq_sources).  BND: the boundary mode when it is
+// known at compile time (the fused kernels are instantiated per mode, so a
+// kernel carries only its own edge rule), -1 to read P.boundary.
+template <class St, int BND = -1>
 __device__ __forceinline__ void load_fk64(const DevP &P, const typename St::T *src,
                                           const double *erho, long long node, int i,
                                           int j, double v[9]) {
+    const int bnd = BND >= 0 ? BND : P.boundary;
     if (i > 0 && j > 0 && i < P.nx && j < P.ny) {  // interior: every source in the domain
 #pragma unroll
         for (int k = 0; k < 9; ++k)
             v[k] = St::ld(src + k * P.plane + node - cxk(k) - (long long)cyk(k) * P.W);
         return;
     }
-    if (P.boundary == 2) {  // boundary node: the equilibrium of its interior source
+    if (bnd == 2) {  // boundary node: the equilibrium of its interior source
         const int b = bnd_index(P, i, j);
         feq64(erho[P.nbnd + b], erho[2 * P.nbnd + b], erho[b], v);
         return;
     }
+    // a source outside the domain reads the node itself: population opp(k)
+    // where the bounce-back rule replaces k (bb_replaced), k itself otherwise
+    const bool el = i == 0, er = i == P.nx, eb = j == 0, et = j == P.ny;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         int si = i - cxk(k), sj = j - cyk(k);
         bool in = (unsigned)si <= (unsigned)P.nx && (unsigned)sj <= (unsigned)P.ny;
         long long s = in ? node - cxk(k) - (long long)cyk(k) * P.W : node;
-        v[k] = St::ld(src + k * P.plane + s);
+        const int kk = (!in && bb_replaced(k, el, er, eb, et)) ? opp_k(k) : k;
+        v[k] = St::ld(src + kk * P.plane + s);
     }
     if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
         const bool corner = (i == 0 || i == P.nx) && (j == 0 || j == P.ny);
-        if (P.boundary == 1 && !corner) {
+        if (bnd == 1 && !corner) {
             zou_he_edge<double>(P, i, j, 0.0, v);
             return;
         }
         const double clb2 = 1.0;
-        double rw = erho[bnd_index(P, i, j)];
-        auto fo = [&](int k) { return St::ld(src + k * P.plane + node); };
-        const double *wl = P.wall;
-        if (j == P.ny && i >= 1 && i <= P.nx - 1) {
-            double uxN = wl[4], uyN = wl[5];
-            v[8] = fo(6) - INV6 * clb2 * rw * (-uxN + uyN);
-            v[4] = fo(2) - 4.0 * INV6 * clb2 * rw * (uyN);
-            v[7] = fo(5) - INV6 * clb2 * rw * (uxN + uyN);
-        } else if (j == 0 && i >= 1 && i <= P.nx - 1) {
-            double uxS = wl[0], uyS = wl[1];
-            v[5] = fo(7) - INV6 * clb2 * rw * (-uxS - uyS);
-            v[2] = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyS);
-            v[6] = fo(8) - INV6 * clb2 * rw * (uxS - uyS);
-        } else if (i == P.nx && j >= 1 && j <= P.ny - 1) {
-            double uxR = wl[2], uyR = wl[3];
-            v[6] = fo(8) - INV6 * clb2 * rw * (uxR - uyR);
-            v[3] = fo(1) - 4.0 * INV6 * clb2 * rw * (uxR);
-            v[7] = fo(5) - INV6 * clb2 * rw * (uxR + uyR);
-        } else if (i == 0 && j >= 1 && j <= P.ny - 1) {
-            double uxL = wl[6], uyL = wl[7];
-            v[5] = fo(7) - INV6 * clb2 * rw * (-uxL - uyL);
-            v[1] = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxL);
-            v[8] = fo(6) - INV6 * clb2 * rw * (-uxL + uyL);
-        } else {
-            double uxw = P.corner[0], uyw = P.corner[1];
-            if (i == 0 && j == 0) {
-                v[1] = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxw);
-                v[5] = fo(7) - INV6 * clb2 * rw * (-uxw - uyw);
-                v[2] = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyw);
-            } else if (i == 0) {  // (0, ny)
-                v[1] = fo(3) - 4.0 * INV6 * clb2 * rw * (-uxw);
-                v[8] = fo(6) - INV6 * clb2 * rw * (-uxw + uyw);
-                v[4] = fo(2) - 4.0 * INV6 * clb2 * rw * (uyw);
-            } else if (j == P.ny) {  // (nx, ny)
-                v[3] = fo(1) - 4.0 * INV6 * clb2 * rw * (uxw);
-                v[7] = fo(5) - INV6 * clb2 * rw * (uxw + uyw);
-                v[4] = fo(2) - 4.0 * INV6 * clb2 * rw * (uyw);
-            } else {  // (nx, 0)
-                v[2] = fo(4) - 4.0 * INV6 * clb2 * rw * (-uyw);
-                v[6] = fo(8) - INV6 * clb2 * rw * (uxw - uyw);
-                v[3] = fo(1) - 4.0 * INV6 * clb2 * rw * (uxw);
-            }
-        }
+        const double rw = erho[bnd_index(P, i, j)];
+        bounce_back_node<double>(P, i, j, INV6 * clb2 * rw, 4.0 * INV6 * clb2 * rw, v);
     }
 }
 
@@ -668,10 +726,11 @@ __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho
 // ------------------------------------------------ FP32 deviation pipeline
 // Pull-stream and bounce-back on deviations h = f - w_i*rho0.  Opposite
 // directions share a weight, so the bounce-back rule keeps its form.
-template <class St>
+template <class St, int BND = -1>
 __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T *src,
                                             const float *erho, long long node, int i, int j,
                                             float h[9]) {
+    const int bnd = BND >= 0 ? BND : P.boundary;
     if (i > 0 && j > 0 && i < P.nx && j < P.ny) {  // interior: every source in the domain
         // one vector offset for the node, a uniform offset per population
         const auto rs = make_rsrc(src, P.buf_bytes);
@@ -684,68 +743,31 @@ __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T 
         }
         return;
     }
-    if (P.boundary == 2) {  // boundary node: the equilibrium of its interior source
+    if (bnd == 2) {  // boundary node: the equilibrium of its interior source
         const int b = bnd_index(P, i, j);
         feq_dev(P.frho0, erho[P.nbnd + b], erho[2 * P.nbnd + b], erho[b], h);
         return;
     }
+    // a source outside the domain reads the node itself: population opp(k)
+    // where the bounce-back rule replaces k (bb_replaced), k itself otherwise
+    const bool el = i == 0, er = i == P.nx, eb = j == 0, et = j == P.ny;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         int si = i - cxk(k), sj = j - cyk(k);
         bool in = (unsigned)si <= (unsigned)P.nx && (unsigned)sj <= (unsigned)P.ny;
         long long s = in ? node - cxk(k) - (long long)cyk(k) * P.W : node;
-        h[k] = St::ld(src + k * P.plane + s);
+        const int kk = (!in && bb_replaced(k, el, er, eb, et)) ? opp_k(k) : k;
+        h[k] = St::ld(src + kk * P.plane + s);
     }
     if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
         const bool corner = (i == 0 || i == P.nx) && (j == 0 || j == P.ny);
-        if (P.boundary == 1 && !corner) {
+        if (bnd == 1 && !corner) {
             zou_he_edge<float>(P, i, j, P.frho0, h);
             return;
         }
-        float rw = erho[bnd_index(P, i, j)];
+        const float rw = erho[bnd_index(P, i, j)];
         const float a = (float)INV6, b = (float)(4.0 * INV6);
-        auto fo = [&](int k) { return St::ld(src + k * P.plane + node); };
-        const double *wl = P.wall;
-        if (j == P.ny && i >= 1 && i <= P.nx - 1) {
-            float uxN = (float)wl[4], uyN = (float)wl[5];
-            h[8] = fo(6) - a * rw * (-uxN + uyN);
-            h[4] = fo(2) - b * rw * (uyN);
-            h[7] = fo(5) - a * rw * (uxN + uyN);
-        } else if (j == 0 && i >= 1 && i <= P.nx - 1) {
-            float uxS = (float)wl[0], uyS = (float)wl[1];
-            h[5] = fo(7) - a * rw * (-uxS - uyS);
-            h[2] = fo(4) - b * rw * (-uyS);
-            h[6] = fo(8) - a * rw * (uxS - uyS);
-        } else if (i == P.nx && j >= 1 && j <= P.ny - 1) {
-            float uxR = (float)wl[2], uyR = (float)wl[3];
-            h[6] = fo(8) - a * rw * (uxR - uyR);
-            h[3] = fo(1) - b * rw * (uxR);
-            h[7] = fo(5) - a * rw * (uxR + uyR);
-        } else if (i == 0 && j >= 1 && j <= P.ny - 1) {
-            float uxL = (float)wl[6], uyL = (float)wl[7];
-            h[5] = fo(7) - a * rw * (-uxL - uyL);
-            h[1] = fo(3) - b * rw * (-uxL);
-            h[8] = fo(6) - a * rw * (-uxL + uyL);
-        } else {
-            float uxw = (float)P.corner[0], uyw = (float)P.corner[1];
-            if (i == 0 && j == 0) {
-                h[1] = fo(3) - b * rw * (-uxw);
-                h[5] = fo(7) - a * rw * (-uxw - uyw);
-                h[2] = fo(4) - b * rw * (-uyw);
-            } else if (i == 0) {
-                h[1] = fo(3) - b * rw * (-uxw);
-                h[8] = fo(6) - a * rw * (-uxw + uyw);
-                h[4] = fo(2) - b * rw * (uyw);
-            } else if (j == P.ny) {
-                h[3] = fo(1) - b * rw * (uxw);
-                h[7] = fo(5) - a * rw * (uxw + uyw);
-                h[4] = fo(2) - b * rw * (uyw);
-            } else {
-                h[2] = fo(4) - b * rw * (-uyw);
-                h[6] = fo(8) - a * rw * (uxw - uyw);
-                h[3] = fo(1) - b * rw * (uxw);
-            }
-        }
+        bounce_back_node<float>(P, i, j, a * rw, b * rw, h);
     }
 }
 

@@ -550,7 +550,7 @@ __device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
 // rebuilt by the same pull-stream and bounce-back as the grid pass; a
 // fixed-order block reduction keeps it deterministic.  Block `sub` of `nsub`
 // takes every nsub-th group of NT nodes of the reach box.
-template <class St, bool DEV>
+template <class St, bool DEV, int BND>
 __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, double (&F)[3]) {
     __shared__ double lds[3 * NT];
     const DevP &P = a.P;
@@ -577,7 +577,7 @@ __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, doub
                 float phf = phi_f32(s, i, j, dx, dy);
                 if (!(phf > 0.0f)) continue;
                 float h[9], rd, rf, uf, vf;
-                load_fk_dev<St>(P, src, (const float *)a.erho_src, node, i, j, h);
+                load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, node, i, j, h);
                 macros_dev(h, P.frho0, rd, rf, uf, vf);
                 float uxpf = s[SVX] - s[SW] * dy, uypf = s[SVY] + s[SW] * dx;
                 double ph = phf;
@@ -591,7 +591,7 @@ __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, doub
                 double ph = phi_at(xc, yc, dm, zt, i, j);
                 if (!(ph > 0.0)) continue;
                 double f[9];
-                load_fk64<St>(P, src, (const double *)a.erho_src, node, i, j, f);
+                load_fk64<St, BND>(P, src, (const double *)a.erho_src, node, i, j, f);
                 macros64(f, r, u, v);
                 double uxp = vx - w * ((double)j - yc);
                 double uyp = vy + w * ((double)i - xc);
@@ -753,7 +753,7 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
 // k+1 stored as f*_{k+1}, or (FINAL = true) the FP64 state of step k.
 // Particle blocks compute the hydrodynamic force of step k and advance the
 // particles to state k+1.
-template <class St, bool DEV, bool FINAL>
+template <class St, bool DEV, bool FINAL, int BND>
 __device__ void particle_block(const StepArgs &a, int pb) {
     const DevP &P = a.P;
     {
@@ -779,7 +779,7 @@ __device__ void particle_block(const StepArgs &a, int pb) {
             if (t == 0) sdpw = *a.dpw_in;
         }
         double F[3];
-        particle_force<St, DEV>(a, p, sub, a.nsub, F);  // ends with block barriers
+        particle_force<St, DEV, BND>(a, p, sub, a.nsub, F);  // ends with block barriers
         if (a.nsub > 1) {
             // partial of this block; the last block of the particle adds all
             // partials in sub order and carries on with the update.
@@ -860,7 +860,7 @@ __device__ void particle_block(const StepArgs &a, int pb) {
     }
 }
 
-template <class St, bool DEV, bool FINAL>
+template <class St, bool DEV, bool FINAL, int BND>
 __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
     const DevP &P = a.P;
     long long n = tile * NT + threadIdx.x;
@@ -872,7 +872,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
         int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
         if constexpr (DEV) {
             float h[9], rd, rho, ux, uy;
-            load_fk_dev<St>(P, src, (const float *)a.erho_src, n, i, j, h);
+            load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, n, i, j, h);
             float om = (!FINAL && P.rheology != 0) ? a.rate32[n] : P.fomega;
             // particle test after the population loads are in flight
             const bool any = tile_has_particles_f32(a, tile);
@@ -913,7 +913,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                     a.cstress[2 * P.plane + n] = str[2];
                     if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
                 }
-                if (P.boundary == 2) {
+                if ((BND >= 0 ? BND : P.boundary) == 2) {
                     if (b < 0) eq_put(a, P, i, j, rd, ucx, ucy);
                 } else if (b >= 0) {
                     ((float *)a.erho_dst)[b] = rho;
@@ -921,7 +921,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
             }
         } else {
             double f[9], rho, ux, uy;
-            load_fk64<St>(P, src, (const double *)a.erho_src, n, i, j, f);
+            load_fk64<St, BND>(P, src, (const double *)a.erho_src, n, i, j, f);
             const bool any = tile_has_particles(a, tile);
             macros64(f, rho, ux, uy);
             double gx, gy;
@@ -949,7 +949,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                     a.cstress[2 * P.plane + n] = o.stress[2];
                     if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
                 }
-                if (P.boundary == 2) {
+                if ((BND >= 0 ? BND : P.boundary) == 2) {
                     if (b < 0) eq_put(a, P, i, j, rho, ucx, ucy);
                 } else if (b >= 0) {
                     ((double *)a.erho_dst)[b] = rho;
@@ -962,13 +962,13 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
                       !FINAL && a.slots_yield != nullptr);
 }
 
-template <class St, bool DEV, bool FINAL>
+template <class St, bool DEV, bool FINAL, int BND>
 __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     if ((int)blockIdx.x < a.n_pblocks) {
-        particle_block<St, DEV, FINAL>(a, (int)blockIdx.x);
+        particle_block<St, DEV, FINAL, BND>(a, (int)blockIdx.x);
         return;
     }
-    step_tile<St, DEV, FINAL>(a, xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles));
+    step_tile<St, DEV, FINAL, BND>(a, xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles));
 }
 
 // ================================================================= batch
@@ -1001,19 +1001,19 @@ __global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *__restric
     }
 }
 
-template <class St, bool DEV, bool FINAL>
+template <class St, bool DEV, bool FINAL, int BND>
 __global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ args,
                                                    const int2 *__restrict__ bmap) {
     const int2 m = bmap[blockIdx.x];
     const StepArgs &a = args[m.x];
     if (m.y < a.n_pblocks) {
-        particle_block<St, DEV, FINAL>(a, m.y);
+        particle_block<St, DEV, FINAL, BND>(a, m.y);
         return;
     }
     const long long tb = m.y - a.n_pblocks;
     for (int r = 0; r < a.tpb; ++r) {
         const long long t = tb * a.tpb + r;
-        if (t < a.ntiles) step_tile<St, DEV, FINAL>(a, t);
+        if (t < a.ntiles) step_tile<St, DEV, FINAL, BND>(a, t);
     }
 }
 
