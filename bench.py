@@ -16,6 +16,7 @@ import importlib
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -157,6 +158,17 @@ def accuracy(pkg, configs, steps):
     )
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline():
     """The reference program (oracle/_ref/f6_bing_400x1600: the settling-disk
     Bingham program on the same 401 x 1601 node grid, 40 steps) timed on one
@@ -173,10 +185,28 @@ def cpu_baseline():
             wall = time.perf_counter() - t0
         m = re.search(r"RUNTIME=\s*([0-9.Ee+-]+)", r.stdout)
         cpu_t = float(m.group(1)) if m else wall
-        return dict(value=nodes * steps / wall / 1e6, unit="MLUPS", cores=1, kind="reference",
-                    sample="reference Bingham settling-disk program (F6, patched grid "
-                           "NX=400 NY=1600, one disk, per-step history I/O to tmpfs), "
-                           "%d steps, wall %.2f s, CPU_TIME %.2f s" % (steps, wall, cpu_t))
+        out = dict(value=nodes * steps / wall / 1e6, unit="MLUPS", cores=1, kind="reference",
+                   sample="reference Bingham settling-disk program (F6, patched grid "
+                          "NX=400 NY=1600, one disk, per-step history I/O to a temporary "
+                          "directory), %d steps, wall %.2f s, CPU_TIME %.2f s" % (steps, wall, cpu_t))
+        # the reference has no threads: independent copies, one per core of
+        # this process's share of the host (at most 16), run side by side
+        copies = max(1, min(16, len(os.sched_getaffinity(0))))
+        dirs = [tempfile.mkdtemp(prefix="refrun_") for _ in range(copies)]
+        try:
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([exe], cwd=d, stdout=subprocess.DEVNULL,
+                                      stderr=subprocess.DEVNULL) for d in dirs]
+            codes = [p.wait(timeout=900) for p in procs]
+            wall_n = time.perf_counter() - t0
+        finally:
+            for d in dirs:
+                shutil.rmtree(d, ignore_errors=True)
+        if all(c == 0 for c in codes):
+            out["copies"] = dict(copies=copies, value=copies * nodes * steps / wall_n / 1e6,
+                                 unit="MLUPS", wall_s=round(wall_n, 2))
+        out["cpu_model"] = cpu_model()
+        return out
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
     import numpy as np

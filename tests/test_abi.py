@@ -3,6 +3,7 @@ header declares, and rejects bad parameters before touching a device."""
 import ctypes
 import os
 import re
+import shutil
 import subprocess
 
 import pytest
@@ -45,6 +46,41 @@ def test_params_struct_matches_header_layout(tmp_path):
                                           text=True).stdout.split()]
     assert out[0] == ctypes.sizeof(abi.Params)
     assert out[1:] == [getattr(abi.Params, n).offset for n in fields]
+
+
+def test_fortran_params_type_matches_header(tmp_path):
+    """The Fortran binding's TYPE, BIND(C) :: LBM_PARAMS has the C struct's
+    size and the offsets of its last fields (flang's C_SIZEOF and LOC
+    differences), so a Fortran caller passes the same block."""
+    flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    if not os.path.exists(flang):
+        pytest.skip("flang not present")
+    abi = pkg()._abi
+    src = tmp_path / "layout.f90"
+    src.write_text(
+        "PROGRAM LAYOUT\n"
+        "  USE LBM_AMD\n"
+        "  USE ISO_C_BINDING\n"
+        "  IMPLICIT NONE\n"
+        "  TYPE(LBM_PARAMS), TARGET :: P\n"
+        "  INTEGER(C_INTPTR_T) :: B\n"
+        "  B = TRANSFER(C_LOC(P), B)\n"
+        "  PRINT *, C_SIZEOF(P)\n"
+        "  PRINT *, TRANSFER(C_LOC(P%OSC_PERIOD), B) - B\n"
+        "  PRINT *, TRANSFER(C_LOC(P%CONTACT), B) - B\n"
+        "  PRINT *, TRANSFER(C_LOC(P%WALL_EDGE), B) - B\n"
+        "END PROGRAM LAYOUT\n")
+    exe = tmp_path / "layout"
+    pkgdir = os.path.join(REPO, "2d_nnf_lbm_amd")
+    mod = os.path.join(pkgdir, "fortran", "lbm_amd.f90")
+    abi.load()  # the library must be built (the module's helpers reference it)
+    subprocess.run([flang, "-J", str(tmp_path), mod, str(src), "-o", str(exe), "-L" + pkgdir,
+                    "-llbm_amd", "-Wl,-rpath," + pkgdir], check=True, cwd=str(tmp_path),
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    out = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, check=True,
+                                          text=True).stdout.split()]
+    P = abi.Params
+    assert out == [ctypes.sizeof(P), P.osc_period.offset, P.contact.offset, P.wall_edge.offset]
 
 
 @pytest.mark.parametrize("bad", [
