@@ -33,6 +33,7 @@ struct lbm_handle {
     float *rate32 = nullptr;
     void *erho[2] = {nullptr, nullptr};
     void *wtab = nullptr;  // wall velocities per edge class: 10 doubles then 10 floats
+    unsigned long long *mpart[2] = {nullptr, nullptr};  // monitor partials by launch parity
     // particles
     double *pbuf[2] = {nullptr, nullptr};
     int pcur = 0;
@@ -230,7 +231,9 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->pcount, (size_t)P.maxp * 4)) ||
         (rc = al((void **)&h->spf[0], (size_t)P.maxp * SPF * 4)) ||
         (rc = al((void **)&h->spf[1], (size_t)P.maxp * SPF * 4)) ||
-        (rc = al(&h->wtab, 128))) {
+        (rc = al(&h->wtab, 128)) ||
+        (rc = al((void **)&h->mpart[0], (size_t)nblk(P.nn) * 2 * 8)) ||
+        (rc = al((void **)&h->mpart[1], (size_t)nblk(P.nn) * 2 * 8))) {
         *out = h;
         return rc;
     }
@@ -288,7 +291,8 @@ int lbm_destroy(lbm_handle *h) {
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
-                    h->pforce, h->pcount, h->spf[0], h->spf[1], h->lag, h->wtab};
+                    h->pforce, h->pcount, h->spf[0], h->spf[1], h->lag, h->wtab,
+                    h->mpart[0], h->mpart[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -926,6 +930,8 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
     a.kk0 = kk0;
     a.g0 = h->gen_base;
     a.tpb = 1;
+    a.mpart[0] = h->mpart[0]; a.mpart[1] = h->mpart[1];
+    a.nred = (record && s > 0) ? 1 : 0;  // a reducer block when there is something to reduce
     const unsigned g = h->gen_base + (unsigned)s;
     a.tl_hdr_in = h->tl_hdr[g & 1]; a.tl_ids_in = h->tl_ids[g & 1];
     a.tl_hdr_out = h->tl_hdr[(g + 1) & 1]; a.tl_ids_out = h->tl_ids[(g + 1) & 1];
@@ -1010,7 +1016,9 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
     const DevP &P = h->P;
     const long long ntiles = (P.nn + NT - 1) / NT;
     const int n_pblocks = P.np > 0 ? P.np * h->nsub : 0;
+    // the step launches of a recording call carry one reducer block
     dim3 grid((unsigned)(n_pblocks + ntiles)), block(NT);
+    dim3 sgrid((unsigned)(n_pblocks + (record ? 1 : 0) + ntiles));
 
     if (timed) (void)hipEventRecord(h->ev0, h->stream);
     StepArgs a = make_args<DEV>(h, kk0, record, 0, nsteps == 0);
@@ -1026,10 +1034,16 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         if (timed && s == 1 && !fin) (void)hipEventRecord(h->evA, h->stream);
         if (timed && fin && s > 1) (void)hipEventRecord(h->evB, h->stream);
         if (fin)
-            launch_step<St, DEV, true>(P.boundary, grid, block, h->stream, a);
+            launch_step<St, DEV, true>(P.boundary, sgrid, block, h->stream, a);
         else
-            launch_step<St, DEV, false>(P.boundary, grid, block, h->stream, a);
+            launch_step<St, DEV, false>(P.boundary, sgrid, block, h->stream, a);
         rc = check_launch(h, "k_step");
+        if (rc) return rc;
+    }
+    if (record) {
+        // the last launch's monitor partials (the prologue's when nsteps is 0)
+        hipLaunchKernelGGL(k_monitor_final, dim3(1), block, 0, h->stream, a);
+        rc = check_launch(h, "k_monitor_final");
         if (rc) return rc;
     }
     if (timed) (void)hipEventRecord(h->ev1, h->stream);
@@ -1202,7 +1216,9 @@ int build_map(lbm_batch *b, const std::vector<int> &which) {
         const int npb = np > 0 ? np * h->nsub : 0;
         const long long ntiles = (h->P.nn + NT - 1) / NT;
         const long long tb = (ntiles + b->tpb[i] - 1) / b->tpb[i];
-        for (long long k = 0; k < tb; ++k) tiles.push_back(make_int2(i, npb + (int)k));
+        // block npb of each simulation is its monitor reducer (idle unless recording)
+        tiles.push_back(make_int2(i, npb));
+        for (long long k = 0; k < tb; ++k) tiles.push_back(make_int2(i, npb + 1 + (int)k));
     }
     const long long T = (long long)tiles.size();
     const size_t base = map.size();
@@ -1279,6 +1295,9 @@ int batch_launches(lbm_batch *b, int32_t nsteps) {
     }
     for (; s < nsteps; ++s) launch(step_args(s), nullptr, s, nsteps, false);
     launch(Afin, nullptr, nsteps, nsteps, true);
+    // the last launch's monitor partials (a no-op per simulation unless recording)
+    hipLaunchKernelGGL(k_batch_monitor_final, dim3((unsigned)n), block, 0, b->stream,
+                       (const StepArgs *)Afin);
     e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, LBM_ERR_HIP, std::string("k_batch_step: ") + hipGetErrorString(e));
     return LBM_OK;
@@ -1313,6 +1332,7 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
             StepArgs a = dev ? make_args<true>(h, kk0, record, s, fin)
                              : make_args<false>(h, kk0, record, s, fin);
             a.tpb = b->tpb[i];
+            a.nred = 1;  // the block map always holds the reducer block
             return a;
         };
         args[i] = mk(0, false);

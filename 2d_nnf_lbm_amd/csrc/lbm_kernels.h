@@ -421,6 +421,15 @@ struct StepArgs {
     int kk0;                      // step index before the call
     unsigned g0;                  // tile-list generation at the start of the call
     int tpb;                      // 256-node tiles per block (batched launches)
+    // monitor partials: each tile block stores its fixed-point mass and its
+    // yielded count (plain stores, [mass | yield] x ntiles, two buffers by
+    // launch parity); a reducer block of the next launch sums the previous
+    // launch's partials into that launch's monitor slots
+    unsigned long long *mpart[2];
+    unsigned long long *mpart_out;        // this launch's partials, or null
+    const unsigned long long *mpart_in;   // the previous launch's, or null
+    unsigned long long *red_mass, *red_yield;  // where their sums go (null: none)
+    int nred;                     // reducer block slots in the block layout (0 or 1)
 };
 
 // Per-step fields of launch s of a call of nsteps fused steps (s = 0: the
@@ -444,6 +453,12 @@ __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nst
         a.slots_yield = (rec && !fin) ? a.slots_base + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
         a.aux = (s == nsteps - 1) || fin;
     }
+    // the previous launch (s - 1) wrote mass partials from its second launch
+    // on and yield partials unless it was the last (it never is here)
+    a.mpart_out = rec ? a.mpart[s & 1] : nullptr;
+    a.mpart_in = (rec && s > 0) ? a.mpart[(s - 1) & 1] : nullptr;
+    a.red_mass = (rec && s > 1) ? a.slots_base + (size_t)(s - 2) * 2 * NSLOT : nullptr;
+    a.red_yield = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * 2 * NSLOT + NSLOT : nullptr;
 }
 
 // FP32 variant of tile_has_particles for the reduced-precision path.
@@ -616,20 +631,64 @@ __device__ __forceinline__ void hist_write(const StepArgs &a, int p, const doubl
 }
 
 // Block totals of the monitors: FP64 mass as a fixed-point integer (so the
-// per-step total is independent of the order the blocks finish in) and the
-// yielded-node count.
+// per-step total is independent of the order in which it is summed) and the
+// yielded-node count, stored as this tile's partials.  (Atomic adds of every
+// block into a few shared counters serialised at the coherence point and
+// doubled the FP16 step time of a 642k-node grid.)
 __device__ __forceinline__ void tile_monitors(const StepArgs &a, long long tile, double mass,
                                               int yielded, bool want_mass, bool want_yield) {
     __shared__ double lds[2 * NT];
     double v[2] = {mass, (double)yielded};
     block_sum<2>(v, lds);
     if (threadIdx.x == 0) {
-        int s = (int)(tile % NSLOT);
         if (want_mass)
-            atomicAdd(a.slots_mass + s, (unsigned long long)(long long)llrint(v[0] * MASS_SCALE));
-        if (want_yield)
-            atomicAdd(a.slots_yield + s, (unsigned long long)(long long)v[1]);
+            a.mpart_out[tile] = (unsigned long long)(long long)llrint(v[0] * MASS_SCALE);
+        if (want_yield) a.mpart_out[a.ntiles + tile] = (unsigned long long)(long long)v[1];
     }
+}
+
+// Sum of n partials by one block: wrap-around integer addition, so any order
+// gives the same total.
+__device__ __forceinline__ unsigned long long block_sum_u64(const unsigned long long *x,
+                                                            long long n) {
+    __shared__ unsigned long long red[NT];
+    unsigned long long acc = 0;
+    for (long long t = threadIdx.x; t < n; t += NT) acc += x[t];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const unsigned long long r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Reducer block: the previous launch's monitor partials into its slots (the
+// first slot of each; the others stay zero).
+__device__ __forceinline__ void monitor_reduce(const unsigned long long *part, long long ntiles,
+                                               unsigned long long *mass_slot,
+                                               unsigned long long *yield_slot) {
+    if (!part) return;
+    if (mass_slot) {
+        const unsigned long long m = block_sum_u64(part, ntiles);
+        if (threadIdx.x == 0) *mass_slot = m;
+    }
+    if (yield_slot) {
+        const unsigned long long y = block_sum_u64(part + ntiles, ntiles);
+        if (threadIdx.x == 0) *yield_slot = y;
+    }
+}
+
+// The partials of a call's last launch, reduced after it: one grid
+// (arguments by value), or every simulation of a batch (block b: args[b]).
+__global__ __launch_bounds__(NT) void k_monitor_final(StepArgs a) {
+    monitor_reduce(a.mpart_out, a.ntiles, a.slots_mass, a.slots_yield);
+}
+__global__ __launch_bounds__(NT) void k_batch_monitor_final(const StepArgs *__restrict__ args) {
+    const StepArgs &a = args[blockIdx.x];
+    monitor_reduce(a.mpart_out, a.ntiles, a.slots_mass, a.slots_yield);
 }
 
 // Equilibrium edges: interior node (i, j) hands its density (FP64) or
@@ -723,7 +782,8 @@ __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
         if ((int)blockIdx.x % a.nsub == 0) particle_advance(a, (int)blockIdx.x / a.nsub);
         return;
     }
-    prologue_tile<St, DEV>(a, xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles));
+    if ((int)blockIdx.x < a.n_pblocks + a.nred) return;  // nothing before the prologue
+    prologue_tile<St, DEV>(a, xcd_tile(blockIdx.x - a.n_pblocks - a.nred, a.ntiles));
 }
 
 // DOMAIN_IMAGE from the particles alone (F6:438, F6:460-467).
@@ -968,7 +1028,11 @@ __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
         particle_block<St, DEV, FINAL, BND>(a, (int)blockIdx.x);
         return;
     }
-    step_tile<St, DEV, FINAL, BND>(a, xcd_tile(blockIdx.x - a.n_pblocks, a.ntiles));
+    if ((int)blockIdx.x < a.n_pblocks + a.nred) {
+        monitor_reduce(a.mpart_in, a.ntiles, a.red_mass, a.red_yield);
+        return;
+    }
+    step_tile<St, DEV, FINAL, BND>(a, xcd_tile(blockIdx.x - a.n_pblocks - a.nred, a.ntiles));
 }
 
 // ================================================================= batch
@@ -994,7 +1058,8 @@ __global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *__restric
         if (m.y % a.nsub == 0) particle_advance(a, m.y / a.nsub);
         return;
     }
-    const long long tb = m.y - a.n_pblocks;
+    if (m.y < a.n_pblocks + a.nred) return;  // nothing before the prologue
+    const long long tb = m.y - a.n_pblocks - a.nred;
     for (int r = 0; r < a.tpb; ++r) {
         const long long t = tb * a.tpb + r;
         if (t < a.ntiles) prologue_tile<St, DEV>(a, t);
@@ -1010,7 +1075,11 @@ __global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ 
         particle_block<St, DEV, FINAL, BND>(a, m.y);
         return;
     }
-    const long long tb = m.y - a.n_pblocks;
+    if (m.y < a.n_pblocks + a.nred) {
+        monitor_reduce(a.mpart_in, a.ntiles, a.red_mass, a.red_yield);
+        return;
+    }
+    const long long tb = m.y - a.n_pblocks - a.nred;
     for (int r = 0; r < a.tpb; ++r) {
         const long long t = tb * a.tpb + r;
         if (t < a.ntiles) step_tile<St, DEV, FINAL, BND>(a, t);
