@@ -65,6 +65,15 @@ def measure(pkg, configs, storage, steps, warmup, workload=None):
     n = configs.nodes(cfg)
     bpn = configs.bytes_per_node(storage, cfg["rheology"] == 1)
     per_launch = ms_main / (steps - 1)  # ms per launch of the fused step kernel
+    # the same steps with the per-step history (particle rows, mass and
+    # yielded count: what the reference writes every step), by wall clock
+    rec_steps = min(steps, 500)
+    s.step(warmup + steps, rec_steps, record=True)  # sizes the history buffers
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(warmup + steps + rec_steps, rec_steps, record=True)
+    s.sync()
+    ms_rec = (time.perf_counter() - t0) * 1e3 / rec_steps
     out = dict(
         storage=storage,
         ms_per_step=ms_total / steps,
@@ -73,6 +82,8 @@ def measure(pkg, configs, storage, steps, warmup, workload=None):
         mlups=n * steps / (ms_total * 1e-3) / 1e6,
         bytes_per_node=bpn,
         achieved_gbs=bpn * n / (per_launch * 1e-3) / 1e9,
+        ms_per_step_recording=ms_rec,
+        mlups_recording=n / (ms_rec * 1e-3) / 1e6,
     )
     s.close()
     return out
@@ -332,7 +343,13 @@ def main():
             "traffic": pmc_traffic("fp64"),
             "algorithmic_bytes_per_node": r64["bytes_per_node"],
             "kernel_us_per_launch": r64["ms_main_kernel"] * 1e3,
+            "recording": {"ms_per_step": r64["ms_per_step_recording"],
+                          "mlups": r64["mlups_recording"]},
         },
+        "fp16_recording": {"ms_per_step": r16["ms_per_step_recording"],
+                           "mlups": r16["mlups_recording"],
+                           "note": "the same FP16 steps with the per-step history "
+                                   "(particle rows, mass, yielded count), wall clock"},
         "fp16_over_fp64": r16["mlups"] / r64["mlups"],
         "dense_sedimentation_bf16": dense,
         "cavity_ensemble_fp16": ensemble,
