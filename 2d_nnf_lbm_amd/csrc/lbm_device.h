@@ -905,32 +905,46 @@ __device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho
 
 // --------------------------------------------------------- block helpers
 // Fixed-order tree reduction of NV doubles over the block (deterministic):
-// at stride s = 128, 64, ..., 1 element t < s adds element t + s.  The two
-// strides that cross wavefronts go through LDS; the last six run inside
-// wavefront 0 with shuffles (lane t reads lane t + s), the same additions in
-// the same order without a block barrier each.
-template <int NV>
+// at stride s = 128, 64, ..., 1 element t < s adds element t + s.  With
+// WAVE_TAIL the two strides that cross wavefronts go through LDS and the
+// last six run inside wavefront 0 with shuffles (lane t reads lane t + s):
+// the same additions in the same order with four block barriers instead of
+// nine.  Which form a kernel uses is chosen per instantiation by
+// measurement: the shuffles changed the register allocation of the whole
+// fused kernel, for the better in FP64 (-4 % per step) and for the worse in
+// the reduced formats (+3 %).
+template <int NV, bool WAVE_TAIL = false>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*NT */) {
     static_assert(NT == 256, "the tree below is written for 256 threads");
     const int t = threadIdx.x;
 #pragma unroll
     for (int a = 0; a < NV; ++a) lds[a * NT + t] = v[a];
     __syncthreads();
-    if (t < 128) {
+    if constexpr (WAVE_TAIL) {
+        if (t < 128) {
 #pragma unroll
-        for (int a = 0; a < NV; ++a) lds[a * NT + t] += lds[a * NT + t + 128];
-    }
-    __syncthreads();
-    if (t < 64) {
+            for (int a = 0; a < NV; ++a) lds[a * NT + t] += lds[a * NT + t + 128];
+        }
+        __syncthreads();
+        if (t < 64) {
 #pragma unroll
-        for (int a = 0; a < NV; ++a) {
-            double x = lds[a * NT + t] + lds[a * NT + t + 64];
+            for (int a = 0; a < NV; ++a) {
+                double x = lds[a * NT + t] + lds[a * NT + t + 64];
 #pragma unroll
-            for (int s = 32; s > 0; s >>= 1) x += __shfl_down(x, s, 64);
-            if (t == 0) lds[a * NT] = x;
+                for (int s = 32; s > 0; s >>= 1) x += __shfl_down(x, s, 64);
+                if (t == 0) lds[a * NT] = x;
+            }
+        }
+        __syncthreads();
+    } else {
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if (t < s) {
+#pragma unroll
+                for (int a = 0; a < NV; ++a) lds[a * NT + t] += lds[a * NT + t + s];
+            }
+            __syncthreads();
         }
     }
-    __syncthreads();
 #pragma unroll
     for (int a = 0; a < NV; ++a) v[a] = lds[a * NT];
     __syncthreads();

@@ -616,7 +616,7 @@ __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, doub
             }
         }
     }
-    block_sum<3>(acc, lds);
+    block_sum<3, !DEV>(acc, lds);
     F[0] = acc[0]; F[1] = acc[1]; F[2] = acc[2];
 }
 
@@ -635,11 +635,12 @@ __device__ __forceinline__ void hist_write(const StepArgs &a, int p, const doubl
 // yielded-node count, stored as this tile's partials.  (Atomic adds of every
 // block into a few shared counters serialised at the coherence point and
 // doubled the FP16 step time of a 642k-node grid.)
+template <bool DEV>
 __device__ __forceinline__ void tile_monitors(const StepArgs &a, long long tile, double mass,
                                               int yielded, bool want_mass, bool want_yield) {
     __shared__ double lds[2 * NT];
     double v[2] = {mass, (double)yielded};
-    block_sum<2>(v, lds);
+    block_sum<2, !DEV>(v, lds);
     if (threadIdx.x == 0) {
         if (want_mass)
             a.mpart_out[tile] = (unsigned long long)(long long)llrint(v[0] * MASS_SCALE);
@@ -773,7 +774,7 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             }
         }
     }
-    if (a.slots_yield) tile_monitors(a, tile, 0.0, yielded, false, true);
+    if (a.slots_yield) tile_monitors<DEV>(a, tile, 0.0, yielded, false, true);
 }
 
 template <class St, bool DEV>
@@ -1018,7 +1019,7 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
         }
     }
     if (a.slots_mass || (!FINAL && a.slots_yield))
-        tile_monitors(a, tile, mass, yielded, a.slots_mass != nullptr,
+        tile_monitors<DEV>(a, tile, mass, yielded, a.slots_mass != nullptr,
                       !FINAL && a.slots_yield != nullptr);
 }
 
