@@ -97,6 +97,44 @@ def test_batch_mixed_boundaries_equal_single_simulations(storage):
     b.close()
 
 
+def test_batch_mixed_contact_models_equal_single_simulations():
+    """One FP64 batch whose simulations use the three contact models
+    (Glowinski, Wan-Turek with the cavity as wall box, Lennard-Jones), with
+    a second disk next to the first so the pair term acts: each equals its
+    run alone, bit for bit."""
+    cases = small_batch("fp64")
+    models = (0, 1, 2, 1)
+    for (cfg, part), model in zip(cases, models):
+        side = cfg["nx"]
+        cfg["contact"] = model
+        cfg["max_particles"] = 2
+        if model == 1:
+            cfg.update(zeta_c=1.0, eps_p=1.0, eps_w=2.0, wall_edge=[0.0, side, side, 0.0])
+        elif model == 2:
+            cfg["zeta_c"] = 1.0
+    two = []
+    for cfg, part in cases:
+        b = part[:, 0].copy()
+        b[0] = part[0, 0] + part[6, 0] + 0.5  # just outside contact, inside range
+        two.append((cfg, np.stack([part[:, 0], b], axis=1)))
+    n = 25
+    print("case: mixed-contact batch, %d steps" % n, flush=True)
+    bt = make_batch(two)
+    hb = bt.step(0, n)
+    for i, (cfg, part) in enumerate(two):
+        s = pkg().D2Q9(cfg)
+        s.initialize()
+        s.set_particles(part)
+        hs = s.step(0, n)
+        assert np.array_equal(hb[i], hs), i
+        fb, pb = state(bt.sims[i])
+        fs, ps = state(s)
+        assert np.array_equal(fb, fs), i
+        assert np.array_equal(pb, ps), i
+        s.close()
+    bt.close()
+
+
 def test_batch_graph_replays_equal_eager_launches():
     cases = small_batch("fp16")
     eager = make_batch(cases)

@@ -73,24 +73,28 @@ def test_reduced_storage_casson(orc, storage):
     assert rel_err(hg[:, -2], ho[:, -2]) <= 1e-4
 
 
-def test_reduced_storage_zou_he(orc):
-    """Zou-He edges (F1:1222-1417) in the deviation form of the FP32 path:
-    a lid-driven cavity against the FP64 oracle."""
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_reduced_storage_zou_he(orc, storage):
+    """Zou-He edges (F1:1222-1417) in the deviation form of the reduced
+    formats: a lid-driven cavity against the FP64 oracle (FP16 within its
+    storage bound)."""
     nx, ny, n = 60, 60, 1500
     cfg, part = f6_case(nx, ny, False, uxn=0.02)
     cfg["boundary"] = 1
+    print("case: reduced storage zou-he", storage, flush=True)
     st = oracle_state(orc, cfg, part)
     st.run(0, n)
-    g = gpu_solver(cfg, storage="fp32")
+    g = gpu_solver(cfg, storage=storage)
     g.initialize()
     g.set_particles(part)
     g.step(0, n)
     rho, ux, uy, gg, om = g.get_fields()
-    print("zou-he fp32: ux %.3e" % rel_err(ux, st.ux), flush=True)
-    assert rel_err(ux, st.ux) <= 1e-4 and rel_err(uy, st.uy) <= 1e-4
-    # Zou-He edges do not conserve mass exactly: the FP32 density estimate
-    # at the edges lets the total drift apart by ~1e-5 over the run
-    assert rel_err(rho, st.rho) <= 1e-4
+    tol = VY_TOL[storage]
+    print("zou-he %s: ux %.3e" % (storage, rel_err(ux, st.ux)), flush=True)
+    assert rel_err(ux, st.ux) <= tol and rel_err(uy, st.uy) <= tol
+    # Zou-He edges do not conserve mass exactly: the reduced-format density
+    # estimate at the edges lets the total drift apart by ~1e-5 over the run
+    assert rel_err(rho, st.rho) <= max(1e-4, tol / 5)
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
