@@ -20,7 +20,13 @@ PROGRAM LBM_F6_DRIVER
   INTEGER :: NX = 80, NY = 300, TIME_MAX = 1000, STORAGE = 0
   LOGICAL :: NON_NEWTONIAN = .TRUE., WRITE_OUTPUT = .TRUE., FUSED = .TRUE., READ_INPUT = .FALSE.
   DOUBLE PRECISION :: UXN = 0D0
-  NAMELIST /RUN/ NX, NY, TIME_MAX, NON_NEWTONIAN, WRITE_OUTPUT, FUSED, STORAGE, UXN, READ_INPUT
+  ! variants: RHEOLOGY (-1: from NON_NEWTONIAN; LBM_CASSON for the
+  ! Papanastasiou-Casson lines F6:941, F6:944), BOUNDARY (LBM_BOUNCE_BACK,
+  ! LBM_ZOU_HE of F1, LBM_EQUILIBRIUM of F4), CONTACT (LBM_CONTACT_*, with the
+  ! constants of F4's blocks and the domain as the Wan-Turek wall box)
+  INTEGER :: RHEOLOGY = -1, BOUNDARY = 0, CONTACT = 0
+  NAMELIST /RUN/ NX, NY, TIME_MAX, NON_NEWTONIAN, WRITE_OUTPUT, FUSED, STORAGE, UXN, READ_INPUT, &
+                 RHEOLOGY, BOUNDARY, CONTACT
 
   DOUBLE PRECISION :: PI, RE_S, BN_S, Y_G, RHO_S, RHO_F, NU, L_C, A_G, U_C, TAU_Y
   DOUBLE PRECISION :: MU_0, M_PAP, RHO0, OMEGA, AVGDEN, MASS, M_P, I_P, D
@@ -78,18 +84,19 @@ PROGRAM LBM_F6_DRIVER
   P%NX = NX
   P%NY = NY
   P%RHEOLOGY = MERGE(LBM_BINGHAM, LBM_NEWTONIAN, NON_NEWTONIAN)
+  IF (RHEOLOGY >= 0) P%RHEOLOGY = RHEOLOGY
   P%RATE_MODE = LBM_RATE_LOCAL
   P%Q_MODE = 1
   P%UPDATE_ANGLE = 1
   P%STORAGE = STORAGE
   P%MAX_PARTICLES = 1
   P%MOTION = LBM_MOTION_FREE
-  P%BOUNDARY = LBM_BOUNCE_BACK
+  P%BOUNDARY = BOUNDARY
   P%OSC_U = 0D0
   P%OSC_PERIOD = 0D0
-  P%CONTACT = LBM_CONTACT_GLOWINSKI
+  P%CONTACT = CONTACT
   P%RESERVED = 0
-  P%WALL_EDGE = 0D0
+  P%WALL_EDGE = (/ 0D0, DBLE(NX), DBLE(NY), 0D0 /)
   P%OMEGA = OMEGA
   P%S0 = 0.0                                                     ! F6:1009-1011
   P%S1 = 1.1
@@ -113,6 +120,13 @@ PROGRAM LBM_F6_DRIVER
   P%EPS_W = P%EPS_P
   P%E_P = 10.0
   P%E_W = P%E_P
+  IF (CONTACT == LBM_CONTACT_WAN_TUREK) THEN                    ! F4:625
+    P%ZETA_C = 1.0
+    P%EPS_P = 1.0
+    P%EPS_W = 2.0 * P%EPS_P
+  ELSE IF (CONTACT == LBM_CONTACT_LJ) THEN                       ! F4:588
+    P%ZETA_C = 1.0
+  END IF
 
   RC = lbm_create(P, H)
   CALL LBM_CHECK(H, RC, 'lbm_create')

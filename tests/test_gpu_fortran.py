@@ -20,20 +20,33 @@ DRIVER = os.path.join(REPO, "2d_nnf_lbm_amd", "fortran", "lbm_f6_driver")
 
 
 @pytest.mark.parametrize("name,fused", [("f6_bing_local", True), ("f6_bing_local", False),
-                                        ("f6_newt", True), ("f6_newt_lid", False)])
+                                        ("f6_newt", True), ("f6_newt_lid", False),
+                                        ("f6_cas_local", True), ("f6_zouhe_lid", True),
+                                        ("f6_eq_bing_local", True), ("f6_eq_bing_local", False)])
 def test_fortran_driver_matches_reference_files(name, fused):
+    """The driver's output files against the reference program's for the same
+    case, including the variants selected by the RHEOLOGY and BOUNDARY
+    namelist switches (Casson; F1's Zou-He and F4's equilibrium edges,
+    against builds of F6 with those routines)."""
     assert os.path.exists(DRIVER), "build the Fortran driver first"
     meta, ref = cases.load_golden(name)
     nx, ny, n = meta["nx"], meta["ny"], meta["time_max"]
-    uxn = meta.get("uxn") or "0D0"
+    uxn = meta.get("uxn") or meta.get("zouhe_uxn") or "0D0"
+    extra = ""
+    if meta.get("rheology") == "casson":
+        extra += ", RHEOLOGY=2"
+    if meta.get("boundary") == "zouhe":
+        extra += ", BOUNDARY=1"
+    elif meta.get("boundary") == "equilibrium":
+        extra += ", BOUNDARY=2"
     print("case: fortran driver", name, "fused" if fused else "subroutines", flush=True)
     with tempfile.TemporaryDirectory(prefix="fdrv_") as d:
         nml = os.path.join(d, "run.nml")
         with open(nml, "w") as fh:
             fh.write("&RUN NX=%d, NY=%d, TIME_MAX=%d, NON_NEWTONIAN=%s, WRITE_OUTPUT=.TRUE., "
-                     "FUSED=%s, STORAGE=0, UXN=%s /\n"
+                     "FUSED=%s, STORAGE=0, UXN=%s%s /\n"
                      % (nx, ny, n, ".TRUE." if meta["non_newtonian"] else ".FALSE.",
-                        ".TRUE." if fused else ".FALSE.", uxn))
+                        ".TRUE." if fused else ".FALSE.", uxn, extra))
         r = subprocess.run([DRIVER, nml], cwd=d, stdout=subprocess.PIPE,
                            stderr=subprocess.STDOUT, text=True, timeout=300)
         print(r.stdout[-2000:], flush=True)
