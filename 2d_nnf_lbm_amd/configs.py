@@ -284,3 +284,17 @@ def settling_metrics(hist, np_, cfg, p=0):
     fy_prev = hist[-2][base + H["fy"]] if len(hist) > 1 else 0.0
     cd = (fy_prev + row[base + H["fhy"]]) / (cfg["nu"] * -vy) if vy != 0 else math.nan
     return vy, cd
+
+
+def mean_drag_coefficient(hist, cfg, p=0, window=1000):
+    """C_d of F6:317 averaged over the last `window` steps of the history
+    (each step's value uses the force of the step before, as the reference's
+    history line does)."""
+    H = _abi.H
+    base = p * _abi.H_PCOUNT
+    w = min(window, len(hist) - 1)
+    vals = []
+    for k in range(len(hist) - w, len(hist)):
+        vy = hist[k][base + H["vy"]]
+        vals.append((hist[k - 1][base + H["fy"]] + hist[k][base + H["fhy"]]) / (cfg["nu"] * -vy))
+    return float(sum(vals) / len(vals))

@@ -155,8 +155,9 @@ def accuracy(pkg, configs, steps):
         hist = s.step(0, steps)
         vy0, cd0 = configs.settling_metrics(hist, 2, cfg, 0)
         vy1, cd1 = configs.settling_metrics(hist, 2, cfg, 1)
+        cdm = [configs.mean_drag_coefficient(hist, cfg, p) for p in (0, 1)]
         yielded = hist[-1][-1] / configs.nodes(cfg)
-        res[storage] = dict(vy=[vy0, vy1], cd=[cd0, cd1], yielded=yielded)
+        res[storage] = dict(vy=[vy0, vy1], cd=[cd0, cd1], cdm=cdm, yielded=yielded)
         s.close()
     a, b = res["fp64"], res["fp16"]
     rel = lambda x, y: abs(x - y) / max(abs(y), 1e-300)
@@ -164,6 +165,8 @@ def accuracy(pkg, configs, steps):
         steps=steps,
         terminal_velocity_rel_err=max(rel(b["vy"][i], a["vy"][i]) for i in range(2)),
         drag_coefficient_rel_err=max(rel(b["cd"][i], a["cd"][i]) for i in range(2)),
+        drag_coefficient_mean_last_1000_rel_err=max(rel(b["cdm"][i], a["cdm"][i])
+                                                    for i in range(2)),
         yielded_fraction_rel_err=rel(b["yielded"], a["yielded"]),
         fp64_vy=a["vy"], fp16_vy=b["vy"], fp64_yielded_fraction=a["yielded"],
     )
@@ -242,7 +245,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--accuracy-steps", type=int, default=3000)
+    ap.add_argument("--accuracy-steps", type=int, default=50000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dense-steps", type=int, default=200,
                     help="timed steps of the 504-disk BF16 cavity (0 skips it)")
