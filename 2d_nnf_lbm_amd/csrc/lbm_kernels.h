@@ -571,12 +571,30 @@ __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, doub
     const DevP &P = a.P;
     const int m = P.maxp;
     const double *pin = a.pin;
-    double xc = pin[PX * m + p], yc = pin[PY * m + p];
-    double vx = pin[PVX * m + p], vy = pin[PVY * m + p], w = pin[PW * m + p];
-    double dm = pin[PDIAM * m + p], zt = pin[PZETA * m + p];
-    double reach = 0.5 * dm + 0.5 * zt + 1.0;
-    int i0 = max(0, (int)ceil(xc - reach)), i1 = min(P.nx, (int)floor(xc + reach));
-    int j0 = max(0, (int)ceil(yc - reach)), j1 = min(P.ny, (int)floor(yc + reach));
+    double xc = 0.0, yc = 0.0, vx = 0.0, vy = 0.0, w = 0.0, dm = 0.0, zt = 0.0;
+    float sv[SPF];
+    int i0, i1, j0, j1;
+    if constexpr (DEV) {
+        // The FP32 profile record alone (one 48-byte load) gives the reach
+        // box and everything the loop needs.  A node with phi > 0 lies more
+        // than one lattice unit inside the box, so the FP32 rounding of its
+        // edges cannot drop a contributing node.
+        const float *s = a.spf_in + p * SPF;
+#pragma unroll
+        for (int k = 0; k < SPF; ++k) sv[k] = s[k];
+        const int X = (int)sv[SX], Y = (int)sv[SY];
+        i0 = max(0, X + (int)ceilf(sv[SXF] - sv[SREACH]));
+        i1 = min(P.nx, X + (int)floorf(sv[SXF] + sv[SREACH]));
+        j0 = max(0, Y + (int)ceilf(sv[SYF] - sv[SREACH]));
+        j1 = min(P.ny, Y + (int)floorf(sv[SYF] + sv[SREACH]));
+    } else {
+        xc = pin[PX * m + p]; yc = pin[PY * m + p];
+        vx = pin[PVX * m + p]; vy = pin[PVY * m + p]; w = pin[PW * m + p];
+        dm = pin[PDIAM * m + p]; zt = pin[PZETA * m + p];
+        double reach = 0.5 * dm + 0.5 * zt + 1.0;
+        i0 = max(0, (int)ceil(xc - reach)); i1 = min(P.nx, (int)floor(xc + reach));
+        j0 = max(0, (int)ceil(yc - reach)); j1 = min(P.ny, (int)floor(yc + reach));
+    }
     int bw = i1 - i0 + 1, bh = j1 - j0 + 1;
     double acc[3] = {0.0, 0.0, 0.0};
     if (bw > 0 && bh > 0) {
@@ -588,13 +606,12 @@ __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, doub
             if constexpr (DEV) {
                 // same FP32 profile and rigid-body velocity as the grid pass
                 float dx, dy;
-                const float *s = a.spf_in + p * SPF;
-                float phf = phi_f32(s, i, j, dx, dy);
+                float phf = phi_f32(sv, i, j, dx, dy);
                 if (!(phf > 0.0f)) continue;
                 float h[9], rd, rf, uf, vf;
                 load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, node, i, j, h);
                 macros_dev(h, P.frho0, rd, rf, uf, vf);
-                float uxpf = s[SVX] - s[SW] * dy, uypf = s[SVY] + s[SW] * dx;
+                float uxpf = sv[SVX] - sv[SW] * dy, uypf = sv[SVY] + sv[SW] * dx;
                 double ph = phf;
                 r = rf;
                 double du = (double)(uf - uxpf), dv = (double)(vf - uypf);
@@ -820,27 +837,50 @@ __device__ void particle_block(const StepArgs &a, int pb) {
     {
         int p = pb / a.nsub, sub = pb - p * a.nsub;
         const int m = P.maxp;
-        // Stage the inputs of the update in LDS while the force gather runs:
-        // this particle's record, the contact inputs of all particles and the
-        // stale wall distance (the thread that finishes the update then reads
-        // LDS instead of waiting on a chain of dependent global loads).
+        // Stage the inputs of the update in LDS: this particle's record, the
+        // contact inputs of all particles (NP_LDS <= NT: one per thread) and
+        // the stale wall distance.  In the reduced-precision steps they are
+        // loaded into registers before the force gather and written to LDS
+        // after it, so their round trip overlaps the gather's instead of
+        // preceding it (a load feeding an LDS store is waited for at the
+        // store); the thread that finishes the update reads LDS instead of
+        // global memory.
+        static_assert(NP_LDS <= NT, "one staged particle per thread");
         __shared__ double srec[PCOUNT];
         __shared__ double scon[3 * NP_LDS];
         __shared__ double sdpw;
         const bool staged = !FINAL && P.np <= NP_LDS;
+        double r_rec = 0.0, r_x = 0.0, r_y = 0.0, r_d = 0.0, r_dpw = 0.0;
         if (!FINAL) {
             const int t = threadIdx.x;
-            if (t < PCOUNT) srec[t] = a.pin[t * m + p];
-            if (staged)
-                for (int q = t; q < P.np; q += NT) {
-                    scon[q] = a.pin[PX * m + q];
-                    scon[NP_LDS + q] = a.pin[PY * m + q];
-                    scon[2 * NP_LDS + q] = a.pin[PDIAM * m + q];
-                }
-            if (t == 0) sdpw = *a.dpw_in;
+            if (t < PCOUNT) r_rec = a.pin[t * m + p];
+            if (staged && t < P.np) {
+                r_x = a.pin[PX * m + t]; r_y = a.pin[PY * m + t]; r_d = a.pin[PDIAM * m + t];
+            }
+            if (t == 0) r_dpw = *a.dpw_in;
         }
+        auto stage = [&]() {
+            const int t = threadIdx.x;
+            if (t < PCOUNT) srec[t] = r_rec;
+            if (staged && t < P.np) {
+                scon[t] = r_x; scon[NP_LDS + t] = r_y; scon[2 * NP_LDS + t] = r_d;
+            }
+            if (t == 0) sdpw = r_dpw;
+        };
+        // The FP64 step is bound by its tiles' HBM traffic, not by this
+        // chain: it stages before the gather (the barriers of the gather's
+        // block sum publish the LDS) and keeps the registers for the gather
+        // (staging after it raised its allocation from 98 to 106 VGPRs and
+        // its step time by 3 %).
+        if constexpr (!DEV)
+            if (!FINAL) stage();
         double F[3];
         particle_force<St, DEV, BND>(a, p, sub, a.nsub, F);  // ends with block barriers
+        if constexpr (DEV)
+            if (!FINAL) {
+                stage();
+                if (a.nsub <= 1) __syncthreads();  // else the barrier after the partial sum
+            }
         if (a.nsub > 1) {
             // partial of this block; the last block of the particle adds all
             // partials in sub order and carries on with the update.
