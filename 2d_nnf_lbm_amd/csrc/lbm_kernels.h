@@ -637,14 +637,18 @@ __device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, doub
     F[0] = acc[0]; F[1] = acc[1]; F[2] = acc[2];
 }
 
-__device__ __forceinline__ void hist_write(const StepArgs &a, int p, const double F[3]) {
+// rec: this particle's record staged in LDS (field f at rec[f]), or null to
+// read it from global memory (a round trip inside the particle chain).
+__device__ __forceinline__ void hist_write(const StepArgs &a, int p, const double F[3],
+                                           const double *rec) {
     if (!a.hist_part || threadIdx.x != 0) return;
     const int m = a.P.maxp;
+    auto fld = [&](int f) { return rec ? rec[f] : a.pin[f * m + p]; };
     double *h = a.hist_part + (size_t)p * HPC;
-    h[0] = a.pin[PX * m + p]; h[1] = a.pin[PY * m + p]; h[2] = a.pin[PANG * m + p];
-    h[3] = a.pin[PVX * m + p]; h[4] = a.pin[PVY * m + p]; h[5] = a.pin[PW * m + p];
+    h[0] = fld(PX); h[1] = fld(PY); h[2] = fld(PANG);
+    h[3] = fld(PVX); h[4] = fld(PVY); h[5] = fld(PW);
     h[6] = F[0]; h[7] = F[1]; h[8] = F[2];
-    h[9] = a.pin[PFHX * m + p]; h[10] = a.pin[PFHY * m + p]; h[11] = a.pin[PFHW * m + p];
+    h[9] = fld(PFHX); h[10] = fld(PFHY); h[11] = fld(PFHW);
 }
 
 // Block totals of the monitors: FP64 mass as a fixed-point integer (so the
@@ -913,7 +917,7 @@ __device__ void particle_block(const StepArgs &a, int pb) {
             __syncthreads();
             if (!last) return;
         }
-        hist_write(a, p, F);
+        hist_write(a, p, F, FINAL ? nullptr : srec);  // srec is staged and published here
         if (threadIdx.x == 0) {
             if (FINAL) {
                 a.pout[PFX * m + p] = F[0]; a.pout[PFY * m + p] = F[1]; a.pout[PTQ * m + p] = F[2];
