@@ -57,13 +57,17 @@ struct lbm_handle {
     int *tl_ids[2] = {nullptr, nullptr};
     int *err_flag = nullptr;
     double *lag = nullptr;  // module-global rate of the lagged collide
-    // split force reduction: blocks per particle, partials and counters
-    int nsub = 1;
-    double *pforce = nullptr;
-    int *pcount = nullptr;
-    // FP32 smoothed-profile records, same parity as pbuf
+    // fused path: reach boxes and FP32 profile records of the particle
+    // states (same parity as pbuf), tagged force partials, slots per particle
+    int *pbox[2] = {nullptr, nullptr};
     float *spf[2] = {nullptr, nullptr};
-    unsigned gen_base = 1;
+    unsigned long long *fslot = nullptr;
+    int rcap = 0;
+    // launch tags never repeat on a handle (tile-list generations and
+    // tagged words); sr_base is the stochastic-rounding sequence position,
+    // restarted by lbm_initialize and carried by snapshots
+    unsigned tag_base = 1;
+    unsigned sr_base = 1;
 };
 
 namespace {
@@ -173,6 +177,9 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     P.cD5a = (float)(-P.s5 / 6.0); P.cD5b = (float)((1.0 - 0.5 * P.s5) / 6.0);
     P.cDq1 = (float)(-P.sq / 12.0); P.cDq2 = (float)(-(1.0 - 0.5 * P.sq) / 12.0);
     P.cS1 = (float)(-0.25 * P.s1); P.cS2 = (float)(-0.75 * P.s1);
+    P.cC3 = (float)(0.5 * P.s3 / 6.0 + (1.0 - 0.5 * P.s3) / 6.0);
+    P.cC5 = (float)(0.5 * P.s5 / 6.0 + (1.0 - 0.5 * P.s5) / 6.0);
+    P.fgamma_min2 = (float)(P.gamma_min * P.gamma_min);
     {
         unsigned long long bb = 9ULL * (unsigned long long)P.plane *
                                 (p->storage == LBM_FP64 ? 8ULL : p->storage == LBM_FP32 ? 4ULL : 2ULL);
@@ -227,8 +234,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->partial, (size_t)nblk(P.nn) * 8)) ||
         (rc = al((void **)&h->scalar, 8)) ||
         (rc = al((void **)&h->err_flag, 4)) || (rc = al((void **)&h->lag, 8)) ||
-        (rc = al((void **)&h->pforce, (size_t)P.maxp * MAXSUB * 3 * 8)) ||
-        (rc = al((void **)&h->pcount, (size_t)P.maxp * 4)) ||
+        (rc = al((void **)&h->pbox[0], (size_t)P.maxp * 4 * 4)) ||
+        (rc = al((void **)&h->pbox[1], (size_t)P.maxp * 4 * 4)) ||
         (rc = al((void **)&h->spf[0], (size_t)P.maxp * SPF * 4)) ||
         (rc = al((void **)&h->spf[1], (size_t)P.maxp * SPF * 4)) ||
         (rc = al(&h->wtab, 128)) ||
@@ -291,7 +298,7 @@ int lbm_destroy(lbm_handle *h) {
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
-                    h->pforce, h->pcount, h->spf[0], h->spf[1], h->lag, h->wtab,
+                    h->pbox[0], h->pbox[1], h->spf[0], h->spf[1], h->fslot, h->lag, h->wtab,
                     h->mpart[0], h->mpart[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -317,10 +324,10 @@ int lbm_initialize(lbm_handle *h) {
                        h->ux, h->uy, h->g, h->obn);
     int rc = check_launch(h, "k_init");
     if (rc) return rc;
-    // run state back to its start: the stochastic-rounding sequence and the
-    // tile-list generations restart (headers cleared so no stale generation
-    // matches), and the stale wall distance of the contact force is unset
-    h->gen_base = 1;
+    // run state back to its start: the stochastic-rounding sequence restarts
+    // (launch tags go on: they only have to be unique), and the stale wall
+    // distance of the contact force is unset
+    h->sr_base = 1;
     for (int b = 0; b < 2; ++b)
         if (h->tl_hdr[b])
             HIPCHECK(h, hipMemsetAsync(h->tl_hdr[b], 0, (size_t)nblk(P.nn) * 8, h->stream));
@@ -430,9 +437,22 @@ int lbm_set_particles(lbm_handle *h, int32_t n, const double *part) {
     HIPCHECK(h, hipMemcpyAsync(h->pbuf[h->pcur], buf.data(), buf.size() * 8,
                                hipMemcpyHostToDevice, h->stream));
     int PS = (int)std::ceil(2.0 * reach) + 2;
-    // blocks per particle for the force sum: one reach-box node per thread,
-    // so no thread waits on two dependent gathers in a row
-    h->nsub = std::max(1, std::min(MAXSUB, (PS * PS + NT - 1) / NT));
+    // force partial slots per particle: the tiles of a reach box (rows x
+    // the tiles one row segment can touch)
+    {
+        const long long side = 2LL * (long long)std::ceil(reach) + 1;
+        const long long rows = std::min<long long>(side, P.H);
+        const long long per_row = std::min<long long>(side, P.W) / NT + 2;
+        const int rcap = (int)std::min<long long>(rows * per_row, (P.nn + NT - 1) / NT);
+        if (n > 0 && rcap > h->rcap) {
+            if (h->fslot) (void)hipFree(h->fslot);
+            h->fslot = nullptr;
+            const size_t bytes = (size_t)P.maxp * rcap * SLOTW * 8;
+            HIPCHECK(h, hipMalloc((void **)&h->fslot, bytes));
+            HIPCHECK(h, hipMemsetAsync(h->fslot, 0, bytes, h->stream));
+            h->rcap = rcap;
+        }
+    }
     if (n > 0 && PS > h->PS) {
         if (h->phi) (void)hipFree(h->phi);
         h->phi = nullptr;
@@ -682,7 +702,7 @@ int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes) {
     SnapHeader hd;
     std::memset(&hd, 0, sizeof(hd));
     std::memcpy(hd.magic, SNAP_MAGIC, 8);
-    hd.version = 1; hd.gen_base = h->gen_base;
+    hd.version = 1; hd.gen_base = h->sr_base;
     hd.nx = P.nx; hd.ny = P.ny; hd.maxp = P.maxp; hd.np = P.np;
     hd.storage = h->prm.storage; hd.rheology = h->prm.rheology; hd.bytes = need;
     char *o = (char *)buf;
@@ -738,7 +758,8 @@ int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes) {
     HIPCHECK(h, hipMemcpyAsync(h->lag, tail + 1, 8, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     // the stochastic-rounding sequence continues where the saved run was
-    h->gen_base = hd.gen_base;
+    // (launch tags go on)
+    h->sr_base = hd.gen_base;
     return LBM_OK;
 }
 
@@ -918,23 +939,25 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
     a.cf = h->f; a.cfp = h->fp; a.crho = h->rho; a.cux = h->ux; a.cuy = h->uy; a.cg = h->g;
     a.cobn = h->obn; a.cstress = h->stress; a.cimage = h->image; a.rate32 = h->rate32;
     a.pairs = h->pairs; a.paircnt = h->paircnt;
-    a.nsub = np > 0 ? h->nsub : 1;
-    a.pforce = h->pforce;
-    a.pcount = h->pcount;
-    a.n_pblocks = np * a.nsub;
+    a.n_pblocks = np;
     a.ntiles = (P.nn + NT - 1) / NT;
     a.err = h->err_flag;
     a.hist_base = record ? h->hist_part : nullptr;
     a.slots_base = record ? h->hist_slots : nullptr;
     a.rowp = np * HPC;
     a.kk0 = kk0;
-    a.g0 = h->gen_base;
+    a.tag0 = h->tag_base;
+    a.sr0 = h->sr_base;
     a.tpb = 1;
+    a.rcap = h->rcap;
     a.mpart[0] = h->mpart[0]; a.mpart[1] = h->mpart[1];
     a.nred = (record && s > 0) ? 1 : 0;  // a reducer block when there is something to reduce
-    const unsigned g = h->gen_base + (unsigned)s;
+    // tile lists: read by this launch's tiles (generation = its tag), written
+    // for the next launch
+    const unsigned g = h->tag_base + (unsigned)s;
     a.tl_hdr_in = h->tl_hdr[g & 1]; a.tl_ids_in = h->tl_ids[g & 1];
     a.tl_hdr_out = h->tl_hdr[(g + 1) & 1]; a.tl_ids_out = h->tl_ids[(g + 1) & 1];
+    a.fslot = h->fslot;
     // storage buffers: FP64 runs ping-pong between F and F_POST
     void *buf[2];
     if (DEV) { buf[0] = h->sA; buf[1] = h->sB; }
@@ -948,6 +971,7 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
         a.pin = h->pbuf[h->pcur]; a.pout = h->pbuf[h->pcur ^ 1];
         a.spf_in = nullptr;
         a.spf_out = DEV ? h->spf[h->pcur ^ 1] : nullptr;
+        a.pbox_in = h->pbox[h->pcur]; a.pbox_out = h->pbox[h->pcur ^ 1];
         a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
         return a;
     }
@@ -962,6 +986,8 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
     a.pout = fin ? h->pbuf[pc] : h->pbuf[pc ^ 1];
     a.spf_in = DEV ? h->spf[pc] : nullptr;
     a.spf_out = (DEV && !fin) ? h->spf[pc ^ 1] : nullptr;
+    a.pbox_in = h->pbox[pc];
+    a.pbox_out = h->pbox[pc ^ 1];
     const int dc = h->dcur ^ flip;
     a.dpw_in = h->dpw[dc]; a.dpw_out = h->dpw[dc ^ 1];
     return a;
@@ -982,20 +1008,25 @@ void finish_fused(lbm_handle *h, int32_t nsteps) {
         h->pcur ^= nsteps & 1;
         h->dcur ^= nsteps & 1;
     }
-    h->gen_base += (unsigned)nsteps + 2u;
+    h->tag_base += (unsigned)nsteps + 1u;
+    h->sr_base += (unsigned)nsteps + 2u;
 }
 
-// Tile-list overflow reported by the kernels of the last call.
-int check_lists(lbm_handle *h) {
-    if (h->P.np <= NP_LOOP) return LBM_OK;
+// Error bits reported by the kernels of the last call (cleared when read).
+int check_flags(lbm_handle *h) {
     int ef = 0;
     HIPCHECK(h, hipMemcpyAsync(&ef, h->err_flag, 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    if (ef) {
-        (void)hipMemsetAsync(h->err_flag, 0, 4, h->stream);
+    if (!ef) return LBM_OK;
+    (void)hipMemsetAsync(h->err_flag, 0, 4, h->stream);
+    (void)hipStreamSynchronize(h->stream);
+    if (ef & LBM_EF_NAN)
+        return fail(h, LBM_ERR_NAN, "non-finite density or particle state: the run diverged");
+    if (ef & LBM_EF_LIST)
         return fail(h, LBM_ERR_CAPACITY, "more particles reach one grid tile than its candidate list holds");
-    }
-    return LBM_OK;
+    if (ef & LBM_EF_SLOT)
+        return fail(h, LBM_ERR_CAPACITY, "a reach box covers more tiles than its force partial slots");
+    return fail(h, LBM_ERR_HIP, "force partials of a particle did not appear within the wait bound");
 }
 
 // The fused step kernel of one boundary mode (each instantiation carries only
@@ -1015,7 +1046,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
               double *ms_total, double *ms_main) {
     const DevP &P = h->P;
     const long long ntiles = (P.nn + NT - 1) / NT;
-    const int n_pblocks = P.np > 0 ? P.np * h->nsub : 0;
+    const int n_pblocks = P.np;
     // the step launches of a recording call carry one reducer block
     dim3 grid((unsigned)(n_pblocks + ntiles)), block(NT);
     dim3 sgrid((unsigned)(n_pblocks + (record ? 1 : 0) + ntiles));
@@ -1048,7 +1079,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
     }
     if (timed) (void)hipEventRecord(h->ev1, h->stream);
     finish_fused<DEV>(h, nsteps);
-    if ((rc = check_lists(h))) return rc;
+    if ((rc = check_flags(h))) return rc;
     if (timed) {
         HIPCHECK(h, hipEventSynchronize(h->ev1));
         float t = 0.0f;
@@ -1174,6 +1205,7 @@ struct lbm_batch {
     int gexec_chunk = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool all_bounce_back = true;          // every simulation has LBM_BOUNCE_BACK walls
+    long long max_tiles = 0;              // 256-node tiles of the largest grid
 };
 
 namespace {
@@ -1206,14 +1238,14 @@ int build_map(lbm_batch *b, const std::vector<int> &which) {
     for (int i : which) {
         const lbm_handle *h = b->sims[i];
         const int np = h->P.np;
-        const int npb = np > 0 ? np * h->nsub : 0;
+        const int npb = np;
         for (int k = 0; k < npb; ++k) map.push_back(make_int2(i, k));
     }
     std::vector<int2> tiles;
     for (int i : which) {
         const lbm_handle *h = b->sims[i];
         const int np = h->P.np;
-        const int npb = np > 0 ? np * h->nsub : 0;
+        const int npb = np;
         const long long ntiles = (h->P.nn + NT - 1) / NT;
         const long long tb = (ntiles + b->tpb[i] - 1) / b->tpb[i];
         // block npb of each simulation is its monitor reducer (idle unless recording)
@@ -1345,8 +1377,7 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
     // the map depends on each simulation's particle count and tiling
     std::vector<long long> key;
     for (size_t i = 0; i < n; ++i)
-        key.push_back(((long long)b->sims[i]->P.np << 24) ^ ((long long)b->sims[i]->nsub << 8) ^
-                      b->tpb[i]);
+        key.push_back(((long long)b->sims[i]->P.np << 24) ^ b->tpb[i]);
     if (b->map_dirty || !all || key != b->map_key) {
         int rc = build_map(b, which);
         if (rc) return rc;
@@ -1364,12 +1395,22 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
     if (rc) return rc;
     if (timed) (void)hipEventRecord(b->ev1, b->stream);
     BHIPCHECK(b, hipStreamSynchronize(b->stream));
+    // every simulation's error bits are read (and cleared); the first error
+    // is reported
+    int first = LBM_OK;
     for (int i : which) {
         lbm_handle *h = b->sims[i];
         if (b->storage == LBM_FP64) finish_fused<false>(h, nsteps);
         else finish_fused<true>(h, nsteps);
-        if ((rc = check_lists(h))) return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
+        if ((rc = check_flags(h)) && first == LBM_OK)
+            first = bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
     }
+    if (timed) {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, b->ev0, b->ev1);
+        *ms_total = t;
+    }
+    if (first) return first;
     if (record) {
         double *out = hist;
         for (int i : which) {
@@ -1378,11 +1419,6 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
                 return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
             out += (size_t)nsteps * ((size_t)h->P.np * HPC + 2);
         }
-    }
-    if (timed) {
-        float t = 0.0f;
-        (void)hipEventElapsedTime(&t, b->ev0, b->ev1);
-        *ms_total = t;
     }
     return LBM_OK;
 }
@@ -1415,6 +1451,7 @@ int lbm_batch_create(int32_t n, const lbm_params *params, lbm_batch **out) {
         }
         b->sims.push_back(h);
         if (params[i].boundary != LBM_BOUNCE_BACK) b->all_bounce_back = false;
+        b->max_tiles = std::max(b->max_tiles, (h->P.nn + NT - 1) / NT);
     }
     b->tpb.assign(n, 1);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1520,10 +1557,12 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps, int32_t *tiles_per_block) {
         for (int t : cand) {
             for (int i : which) b->tpb[i] = t;
             double ms = 0.0;
-            // one untimed call to settle, then the timed one
+            // one untimed call to settle, then the timed one; the runs only
+            // time the tilings, so a state that diverges under them (the
+            // tuner may run on an uninitialised batch) is not an error here
             int rc = batch_run(b, 0, 2, nullptr, false, nullptr, which);
-            if (!rc) rc = batch_run(b, 0, steps, nullptr, true, &ms, which);
-            if (rc) return rc;
+            if (!rc || rc == LBM_ERR_NAN) rc = batch_run(b, 0, steps, nullptr, true, &ms, which);
+            if (rc && rc != LBM_ERR_NAN) return rc;
             if (ms < best) { best = ms; best_t = t; }
         }
         for (int i : which) b->tpb[i] = best_t;
@@ -1578,14 +1617,14 @@ int save_state(lbm_handle *h, double *dst, Checkpoint &c) {
     HIPCHECK(h, hipMemcpyAsync(d, h->obn, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
     HIPCHECK(h, hipMemcpyAsync(d, h->pbuf[h->pcur], pc * 8, hipMemcpyDeviceToDevice, h->stream)); d += pc;
     HIPCHECK(h, hipMemcpyAsync(d, h->dpw[h->dcur], 8, hipMemcpyDeviceToDevice, h->stream));
-    c.pcur = h->pcur; c.dcur = h->dcur; c.gen = h->gen_base;
+    c.pcur = h->pcur; c.dcur = h->dcur; c.gen = h->sr_base;
     return LBM_OK;
 }
 
 int restore_state(lbm_handle *h, const double *src, const Checkpoint &c) {
     const size_t pl = (size_t)h->P.plane;
     const size_t pc = (size_t)PCOUNT * h->P.maxp;
-    h->pcur = c.pcur; h->dcur = c.dcur; h->gen_base = c.gen;
+    h->pcur = c.pcur; h->dcur = c.dcur; h->sr_base = c.gen;
     const double *d = src;
     HIPCHECK(h, hipMemcpyAsync(h->f, d, 9 * pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += 9 * pl;
     HIPCHECK(h, hipMemcpyAsync(h->rho, d, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;

@@ -64,6 +64,10 @@ struct DevP {
     // for the uniform rates (forcing-moment constants c_k folded in), the
     // fixed-q factors, and the strain-rate factors -s_1/4 and -0.75 s_1
     float cD1a, cD1b, cD2a, cD2b, cD3a, cD3b, cD5a, cD5b, cDq1, cDq2, cS1, cS2;
+    // forcing factors of D_3 / N_3 and D_5 / N_5 per unit FX, FY (with
+    // n3 = -FX/2): -cD3a/2 + cD3b and -cD5a/2 + cD5b
+    float cC3, cC5;
+    float fgamma_min2;  // gamma_min^2: the small-rate branch compares gamma^2
     float fsqrt_mu0, fcas_lim;  // Casson: sqrt(mu0), sqrt(mu0) + sqrt(m tau_y)
     unsigned buf_bytes;  // bytes of one 9-plane storage buffer (< 2^32)
 };
@@ -127,6 +131,12 @@ __device__ __forceinline__ uint32_t sr_word(uint32_t node_hash, int k) {
 // half passes through from the input register itself (cvt_sr writes one half
 // of its destination, which is tied to that input), so no register is
 // cleared for it.
+//
+// The reduced formats exchange values in storage units: S h with the
+// format's scale S (2^10 for FP16, 1 otherwise).  The deviation-form collide
+// is linear in h, so it runs in those units and the scale is folded into a
+// few per-node factors instead of one multiply per population on each load
+// and store.
 struct StF64 {
     using T = double;
     __device__ static double ld(const T *p) { return *p; }
@@ -134,6 +144,7 @@ struct StF64 {
 };
 struct StF32 {
     using T = float;
+    static constexpr float S = 1.0f, invS = 1.0f;
     __device__ static float ld(const T *p) { return *p; }
     __device__ static void st(T *p, float v, uint32_t) { *p = v; }
     __device__ static float bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
@@ -147,24 +158,23 @@ struct StF32 {
 };
 struct StF16 {
     using T = uint16_t;
-    __device__ static float ld(const T *p) {
-        return (float)__builtin_bit_cast(_Float16, *p) * (1.0f / 1024.0f);
-    }
+    // h is stored as 2^10 h, clear of the FP16 subnormals
+    static constexpr float S = 1024.0f, invS = 1.0f / 1024.0f;
+    __device__ static float ld(const T *p) { return (float)__builtin_bit_cast(_Float16, *p); }
     // stochastic rounding (gfx950 v_cvt_sr_f16_f32): in the normal FP16 range
     // the top 13 bits of r are added below the kept mantissa bits before
     // truncation, which makes the stored value unbiased
     __device__ static void st(T *p, float v, uint32_t r) {
-        h2 w = __builtin_amdgcn_cvt_sr_f16_f32((h2){}, v * 1024.0f, r, false);
+        h2 w = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, v), v, r, false);
         *p = __builtin_bit_cast(uint16_t, w[0]);
     }
     __device__ static float bld(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so) {
         unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)vo, (int)so, 0);
-        return (float)__builtin_bit_cast(_Float16, b) * (1.0f / 1024.0f);
+        return (float)__builtin_bit_cast(_Float16, b);
     }
     __device__ static void bst(__amdgpu_buffer_rsrc_t rs, float v, unsigned vo, unsigned so,
                                uint32_t r) {
-        const float sv = v * 1024.0f;
-        h2 w = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, sv), sv, r, false);
+        h2 w = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, v), v, r, false);
         __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, w[0]), rs,
                                               (int)vo, (int)so, 0);
     }
@@ -176,6 +186,7 @@ struct StF16 {
 // the stored value unbiased.
 struct StBF16 {
     using T = uint16_t;
+    static constexpr float S = 1.0f, invS = 1.0f;
     __device__ static float ld(const T *p) {
         return __builtin_bit_cast(float, (uint32_t)(*p) << 16);
     }
@@ -524,79 +535,137 @@ __device__ __forceinline__ bool spm_node_cand(const double *part, int maxp, cons
     return true;
 }
 
-// ------------------------------------------ FP32 smoothed-profile records
-// The reduced-precision path evaluates phi and G in FP32 from a compact
-// per-particle record.  Centres are split into an integer and a fractional
-// part so node offsets (i - X) stay exact to FP32 rounding of the fraction.
-enum { SX = 0, SXF, SY, SYF, SVX, SVY, SW, SRP, SHZ, SPZ, SREACH, SPAD, SPF };
+// ------------------------------------------------ particle state in a launch
+// Reach box of a disk: the nodes that can have phi > 0 (distance below
+// r + zeta/2, plus one lattice unit that covers rounding), clipped to the
+// grid; empty when i0 > i1 or j0 > j1.
+struct Box {
+    int i0, i1, j0, j1;
+};
+__device__ __forceinline__ Box reach_box(const DevP &P, double x, double y, double diam,
+                                         double zeta) {
+    const double reach = 0.5 * diam + 0.5 * zeta + 1.0;
+    Box b;
+    b.i0 = max(0, (int)ceil(x - reach)); b.i1 = min(P.nx, (int)floor(x + reach));
+    b.j0 = max(0, (int)ceil(y - reach)); b.j1 = min(P.ny, (int)floor(y + reach));
+    return b;
+}
 
+// The 256-node tiles holding nodes of a box, in ascending order, each once:
+// row j of the box lies in tiles [(jW + i0)/NT, (jW + i1)/NT]; a tile already
+// met in row j-1 is not counted again (rows only move forward).  The force
+// on a particle is summed over these tiles in this order, so the tile that
+// writes a partial and the block that sums them agree on its slot.
+// box_tile_rank: position of tile t in that sequence, or -1 when t holds no
+// node of the box.
+__device__ __forceinline__ int box_tile_rank(const DevP &P, const Box &b, long long t) {
+    int cnt = 0;
+    long long last = -1;
+    for (int j = b.j0; j <= b.j1; ++j) {
+        long long lo = ((long long)j * P.W + b.i0) / NT, hi = ((long long)j * P.W + b.i1) / NT;
+        if (lo <= last) lo = last + 1;
+        if (lo > t) return -1;
+        if (hi >= lo) {
+            if (t <= hi) return cnt + (int)(t - lo);
+            cnt += (int)(hi - lo + 1);
+            last = hi;
+        }
+    }
+    return -1;
+}
+__device__ __forceinline__ int box_tile_count(const DevP &P, const Box &b) {
+    int cnt = 0;
+    long long last = -1;
+    for (int j = b.j0; j <= b.j1; ++j) {
+        long long lo = ((long long)j * P.W + b.i0) / NT, hi = ((long long)j * P.W + b.i1) / NT;
+        if (lo <= last) lo = last + 1;
+        if (hi >= lo) {
+            cnt += (int)(hi - lo + 1);
+            last = hi;
+        }
+    }
+    return cnt;
+}
+
+// Does tile t hold a node of the box [i0, i1] x [j0, j1] (bounds may lie
+// outside the grid)?  Uniform over the block.
+__device__ __forceinline__ bool tile_meets_box(const DevP &P, long long t, int i0, int i1,
+                                               int j0, int j1) {
+    const long long n0 = t * NT, n1 = min(n0 + NT - 1, P.nn - 1);
+    const int jl = (int)(n0 / P.W), jh = (int)(n1 / P.W);
+    for (int j = max(jl, j0); j <= min(jh, j1); ++j) {
+        const long long r0 = (long long)j * P.W;
+        const long long a = max(n0 - r0, (long long)i0), b = min(n1 - r0, (long long)i1);
+        if (a <= b) return true;
+    }
+    return false;
+}
+
+// Force partials cross between blocks of one launch as generation-tagged
+// 64-bit words: (tag << 32) | 32-bit payload, written and read with
+// agent-scope atomic accesses.  A reader polls until each word it needs
+// carries the launch's tag; a word is a single atomic location holding both
+// the tag and its value, so a word with the right tag holds the right value
+// and no ordering between different words is needed.  Tags never repeat on
+// a handle.  A partial (force x, y, torque) is 6 words: hi and lo halves.
+constexpr int SLOTW = 6;
+constexpr int TAG_SPIN = 1 << 16;  // bounded wait for tagged words (tens of ms)
+
+__device__ __forceinline__ void put_tagged_f64(unsigned long long *w, unsigned tag, double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const unsigned long long t = (unsigned long long)tag << 32;
+    __hip_atomic_store(w, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w + 1, t | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// FP32 profile record of a disk for the reduced-precision grid pass: the
+// centre split into an integer and a fractional part (node offsets i - X
+// stay exact up to the rounding of the fraction), velocity, spin, radius,
+// half width, pi/zeta.
+enum { SX = 0, SXF, SY, SYF, SVX, SVY, SW, SRP, SHZ, SPZ, SPAD0, SPAD1, SPF };
 __device__ __forceinline__ void make_spf(const double *rec /* PCOUNT */, float *out) {
-    double x = rec[PX], y = rec[PY];
-    double xi = floor(x), yi = floor(y);
+    const double x = rec[PX], y = rec[PY];
+    const double xi = floor(x), yi = floor(y);
     out[SX] = (float)xi; out[SXF] = (float)(x - xi);
     out[SY] = (float)yi; out[SYF] = (float)(y - yi);
     out[SVX] = (float)rec[PVX]; out[SVY] = (float)rec[PVY]; out[SW] = (float)rec[PW];
     out[SRP] = (float)(0.5 * rec[PDIAM]);
     out[SHZ] = (float)(0.5 * rec[PZETA]);
     out[SPZ] = (float)(PI_F / rec[PZETA]);
-    out[SREACH] = (float)(0.5 * rec[PDIAM] + 0.5 * rec[PZETA] + 1.0);
-    out[SPAD] = 0.0f;
+    out[SPAD0] = 0.0f; out[SPAD1] = 0.0f;
 }
+struct Prof {
+    float x, xf, y, yf, vx, vy, w, rp, hz, pz;
+};
 
-// phi of one particle at node (i, j) in FP32 (same piecewise profile as
-// phi_at); dx, dy receive the node offset from the centre.
-__device__ __forceinline__ float phi_f32(const float *s, int i, int j, float &dx, float &dy) {
-    dx = ((float)i - s[SX]) - s[SXF];
-    dy = ((float)j - s[SY]) - s[SYF];
-    if (fabsf(dx) > s[SREACH] || fabsf(dy) > s[SREACH]) return 0.0f;
-    float di = s[SRP] - __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
-    if (di <= -s[SHZ]) return 0.0f;
-    if (fabsf(di) < s[SHZ]) return 0.5f * (__sinf(s[SPZ] * di) + 1.0f);
+// phi of one particle at node (i, j) in FP32 (the piecewise profile of
+// phi_at, F6:451-459) from the profile fields; dx, dy receive the node
+// offset from the centre.
+__device__ __forceinline__ float phi_prof(const Prof &s, int i, int j, float &dx, float &dy) {
+    dx = ((float)i - s.x) - s.xf;
+    dy = ((float)j - s.y) - s.yf;
+    const float di = s.rp - __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+    if (di <= -s.hz) return 0.0f;
+    if (fabsf(di) < s.hz) return 0.5f * (__sinf(s.pz * di) + 1.0f);
     return 1.0f;
-}
-
-// G at a node in FP32 (F6:485-495): the largest index with phi > 0 wins.
-// ids == nullptr: test particles 0..cnt-1; else the listed ones.
-__device__ __forceinline__ void spm_node_f32(const float *spf, const int *ids, int cnt, int i,
-                                             int j, float ux, float uy, float &gx, float &gy) {
-    gx = 0.0f;
-    gy = 0.0f;
-    int best = -1;
-    float bphi = 0.0f, bdx = 0.0f, bdy = 0.0f;
-    for (int c = 0; c < cnt; ++c) {
-        int p = ids ? ids[c] : c;
-        if (p <= best) continue;
-        float dx, dy;
-        float ph = phi_f32(spf + p * SPF, i, j, dx, dy);
-        if (ph > 0.0f) { best = p; bphi = ph; bdx = dx; bdy = dy; }
-    }
-    if (best < 0) return;
-    const float *s = spf + best * SPF;
-    float uxp = s[SVX] - s[SW] * bdy;
-    float uyp = s[SVY] + s[SW] * bdx;
-    gx = bphi * (uxp - ux);
-    gy = bphi * (uyp - uy);
 }
 
 constexpr int NP_LOOP = 16;    // above this many particles, tiles use candidate lists
 constexpr int MAXC = 64;       // candidate particles per tile
-constexpr int MAXSUB = 16;     // blocks summing the force on one particle
 constexpr int NP_LDS = 128;    // up to this many particles, contact inputs are staged in LDS
 
-// Registers particle p (record rec, stride 1) in the candidate list of every
-// grid tile its reach box touches, for generation gen.  Each tile header packs
+// Registers particle p in the candidate list of every grid tile that its
+// reach box b touches, for generation gen (the tag of the launch whose
+// tiles read the lists).  Each tile header packs
 // (generation << 32 | count); the first registration of a new generation
 // resets the count (lock-free CAS loop, one contender per touching particle).
 // Called by all NT threads of a block: thread r takes box rows j0+r, j0+r+NT,
 // ...; a tile that row j shares with row j-1 is left to row j-1, so each
 // tile is registered once per particle.
-__device__ __forceinline__ void register_tiles(const DevP &P, const double *rec, int p,
-                                               unsigned gen, unsigned long long *hdr,
-                                               int *ids, int *err) {
-    double reach = 0.5 * rec[PDIAM] + 0.5 * rec[PZETA] + 1.0;
-    int i0 = max(0, (int)ceil(rec[PX] - reach)), i1 = min(P.nx, (int)floor(rec[PX] + reach));
-    int j0 = max(0, (int)ceil(rec[PY] - reach)), j1 = min(P.ny, (int)floor(rec[PY] + reach));
-    if (i0 > i1 || j0 > j1) return;
+__device__ __forceinline__ void register_tiles(const DevP &P, Box b, int p, unsigned gen,
+                                               unsigned long long *hdr, int *ids, int *err) {
+    if (b.i0 > b.i1 || b.j0 > b.j1) return;
+    const int i0 = b.i0, i1 = b.i1, j0 = b.j0, j1 = b.j1;
     for (int j = j0 + (int)threadIdx.x; j <= j1; j += NT) {
         long long tlo = ((long long)j * P.W + i0) / NT, thi = ((long long)j * P.W + i1) / NT;
         if (j > j0) {
@@ -724,28 +793,43 @@ __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho
 }
 
 // ------------------------------------------------ FP32 deviation pipeline
-// Pull-stream and bounce-back on deviations h = f - w_i*rho0.  Opposite
-// directions share a weight, so the bounce-back rule keeps its form.
+// Populations are handled as H_k = S h_k = S (f_k - w_k rho0) in the storage
+// units of the format (scale St::S).
+
+// Raw buffer loads of the nine pulled populations of an interior node (every
+// source in the domain): one vector offset for the node, a uniform offset per
+// population.
+template <class St>
+__device__ __forceinline__ void load_interior_dev(const DevP &P, const typename St::T *src,
+                                                  long long node, float H[9]) {
+    const auto rs = make_rsrc(src, P.buf_bytes);
+    const unsigned vo = (unsigned)node * (unsigned)sizeof(typename St::T);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const unsigned so = (unsigned)((k * P.plane - cxk(k) - (long long)cyk(k) * P.W) *
+                                       (long long)sizeof(typename St::T));
+        H[k] = St::bld(rs, vo, so);
+    }
+}
+
+// Pull-stream and the boundary rules (F6:1164-1188 and F6:1206-1294, or the
+// Zou-He / equilibrium edges) on deviations in storage units.  Opposite
+// directions share a weight, so the bounce-back rule keeps its form; every
+// edge rule is linear in (rho0, h), so it is applied in storage units with
+// S rho0 and S times the wall terms.
 template <class St, int BND = -1>
 __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T *src,
                                             const float *erho, long long node, int i, int j,
-                                            float h[9]) {
+                                            float H[9]) {
     const int bnd = BND >= 0 ? BND : P.boundary;
+    const float S = St::S;
     if (i > 0 && j > 0 && i < P.nx && j < P.ny) {  // interior: every source in the domain
-        // one vector offset for the node, a uniform offset per population
-        const auto rs = make_rsrc(src, P.buf_bytes);
-        const unsigned vo = (unsigned)node * (unsigned)sizeof(typename St::T);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            unsigned so = (unsigned)((k * P.plane - cxk(k) - (long long)cyk(k) * P.W) *
-                                     (long long)sizeof(typename St::T));
-            h[k] = St::bld(rs, vo, so);
-        }
+        load_interior_dev<St>(P, src, node, H);
         return;
     }
     if (bnd == 2) {  // boundary node: the equilibrium of its interior source
         const int b = bnd_index(P, i, j);
-        feq_dev(P.frho0, erho[P.nbnd + b], erho[2 * P.nbnd + b], erho[b], h);
+        feq_dev(S * P.frho0, erho[P.nbnd + b], erho[2 * P.nbnd + b], S * erho[b], H);
         return;
     }
     // a source outside the domain reads the node itself: population opp(k)
@@ -757,149 +841,174 @@ __device__ __forceinline__ void load_fk_dev(const DevP &P, const typename St::T 
         bool in = (unsigned)si <= (unsigned)P.nx && (unsigned)sj <= (unsigned)P.ny;
         long long s = in ? node - cxk(k) - (long long)cyk(k) * P.W : node;
         const int kk = (!in && bb_replaced(k, el, er, eb, et)) ? opp_k(k) : k;
-        h[k] = St::ld(src + kk * P.plane + s);
+        H[k] = St::ld(src + kk * P.plane + s);
     }
     if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
         const bool corner = (i == 0 || i == P.nx) && (j == 0 || j == P.ny);
         if (bnd == 1 && !corner) {
-            zou_he_edge<float>(P, i, j, P.frho0, h);
+            zou_he_edge<float>(P, i, j, S * P.frho0, H);
             return;
         }
         const float rw = erho[bnd_index(P, i, j)];
-        const float a = (float)INV6, b = (float)(4.0 * INV6);
-        bounce_back_node<float>(P, i, j, a * rw, b * rw, h);
+        const float a = (float)INV6 * S, b = (float)(4.0 * INV6) * S;
+        bounce_back_node<float>(P, i, j, a * rw, b * rw, H);
     }
 }
 
-// Density deviation and velocity from deviations (sum of c_i w_i is zero).
-__device__ __forceinline__ void macros_dev(const float h[9], float rho0, float &rho_dev,
-                                           float &rho, float &ux, float &uy) {
-    rho_dev = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
-    rho = rho0 + rho_dev;
-    float rinv = __builtin_amdgcn_rcpf(rho);
-    ux = rinv * ((h[1] + h[5] + h[8]) - (h[6] + h[3] + h[7]));
-    uy = rinv * ((h[5] + h[2] + h[6]) - (h[7] + h[4] + h[8]));
+// Moments of the deviations in storage units (F6:878-886): density deviation
+// rd and momentum jx, jy (MACROSCOPICVARIABLES, F6:1405-1408; the sums of
+// c_k w_k vanish), and the non-conserved moments the collide needs (m3 = jx
+// and m5 = jy).  Computed once for the macroscopic fields and the collide.
+struct DevMom {
+    float rd, jx, jy, m1, m2, m4, m6, m7, m8;
+};
+__device__ __forceinline__ DevMom dev_moments(const float H[9]) {
+    DevMom m;
+    const float s13 = H[1] + H[3], s24 = H[2] + H[4];
+    const float s14 = s13 + s24;
+    const float c56 = H[5] + H[6], c78 = H[7] + H[8], s58 = c56 + c78;
+    const float a13 = H[1] - H[3], a24 = H[2] - H[4];
+    const float b56 = H[5] - H[6], b78 = H[7] - H[8];
+    const float d1 = b56 - b78, d2 = c56 - c78;
+    m.rd = (H[0] + s14) + s58;
+    m.jx = a13 + d1;
+    m.jy = a24 + d2;
+    m.m1 = (-4.0f * H[0] - s14) + 2.0f * s58;
+    m.m2 = (4.0f * H[0] - 2.0f * s14) + s58;
+    m.m4 = -2.0f * a13 + d1;
+    m.m6 = -2.0f * a24 + d2;
+    m.m7 = s13 - s24;
+    m.m8 = b56 + b78;
+    return m;
 }
 
 // Hardware reciprocal, square root and base-2 exponential (about 1 ulp):
 // the FP32 path trades IEEE-exact division for issue rate.
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fexp(float x) {
     return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
 }
 
-// Collision_MRT in deviation form and FP32 (same physics as collide64).
-// OUT: also produce the stress tensor and the yield flag (output only); the
-// Newtonian rate needs neither, the Papanastasiou rate needs the strain rate.
+// Collision_MRT in deviation form, FP32, storage units (same physics as
+// collide64, F6:867-958).  rho: density, rinvS = 1 / (S rho); ucx, ucy: the
+// corrected velocity (F6:1118-1124); gx, gy: the SPM forcing G (FORCED
+// only; zero otherwise); om: the node's rate.  OUT: also the stress tensor
+// and the yield flag (F6:945-954), output only.
 //
 // Written as a change of the populations: with n_k = m_k - meq_k the
 // post-collision moments differ from m_k by D_k = -s_k n_k + (1 - s_k/2) mf_k
 // (D_0 = 0: n_0 = mf_0 = 0), and since the rows of the moment matrix M are
 // orthogonal, M^-1 = M^T diag(1/N) with N = (9, 36, 36, 6, 12, 6, 12, 4, 4),
 // so h* = h + M^T (D / N).  The uniform factors s_k / N_k and
-// (1 - s_k/2) / N_k come precomputed in DevP (cD*).
-template <bool OUT>
-__device__ __forceinline__ void collide_dev(const DevP &P, float h[9], float rho_dev, float rho,
-                                            float ux, float uy, float gx, float gy, float om,
+// (1 - s_k/2) / N_k come precomputed in DevP (cD*).  The momentum S rho u_c
+// of the corrected velocity is jx + FX/2 with FX = S rho gx, so n3 = -FX/2
+// and n5 = -FY/2 (zero without forcing) are formed without cancellation.
+template <bool FORCED, bool OUT>
+__device__ __forceinline__ void collide_dev(const DevP &P, float H[9], const DevMom &mo,
+                                            float S, float rho, float rinvS, float ucx,
+                                            float ucy, float gx, float gy, float om,
                                             float &om_out, float stress[3], int &yielded) {
 #pragma clang fp contract(on)  // fused within one expression: the same in every kernel
-    // moments of h (F6:873-881); m_0 = rho_dev
-    const float s14 = (h[1] + h[2]) + (h[3] + h[4]), s58 = (h[5] + h[6]) + (h[7] + h[8]);
-    const float a13 = h[1] - h[3], a24 = h[2] - h[4];
-    const float b56 = h[5] - h[6], b78 = h[7] - h[8], c56 = h[5] + h[6], c78 = h[7] + h[8];
-    const float m1 = -4.0f * h[0] - s14 + 2.0f * s58;
-    const float m2 = 4.0f * h[0] - 2.0f * s14 + s58;
-    const float m3 = a13 + (b56 - b78), m4 = -2.0f * a13 + (b56 - b78);
-    const float m5 = a24 + (c56 - c78), m6 = -2.0f * a24 + (c56 - c78);
-    const float m7 = (h[1] - h[2]) + (h[3] - h[4]), m8 = b56 + b78;
-    // non-equilibrium parts n_k = m_k - meq_k (equilibrium relative to the
-    // rest state) and the forcing moments (F6:883-900)
-    const float fx = rho * gx, fy = rho * gy;
-    const float rux = rho * ux, ruy = rho * uy;
-    const float ru2 = rux * ux + ruy * uy;
-    const float fu = fx * ux + fy * uy;
-    const float n1 = m1 - (3.0f * ru2 - 2.0f * rho_dev);
-    const float n2 = m2 - (rho_dev - 3.0f * ru2);
-    const float n3 = m3 - rux, n4 = m4 + rux, n5 = m5 - ruy, n6 = m6 + ruy;
-    const float n7 = m7 - (rux * ux - ruy * uy), n8 = m8 - rux * uy;
-    const float mf7 = fx * ux - fy * uy, mf8 = fy * ux + fx * uy;
+    float FX = 0.0f, FY = 0.0f, rux = mo.jx, ruy = mo.jy;
+    if constexpr (FORCED) {
+        const float rS = S * rho;
+        FX = rS * gx;
+        FY = rS * gy;
+        rux = mo.jx + 0.5f * FX;
+        ruy = mo.jy + 0.5f * FY;
+    }
+    // non-equilibrium parts (equilibrium relative to the rest state)
+    const float ru2 = rux * ucx + ruy * ucy;
+    const float n1 = (mo.m1 + 2.0f * mo.rd) - 3.0f * ru2;
+    const float n2 = (mo.m2 - mo.rd) + 3.0f * ru2;
+    const float n4 = mo.m4 + rux, n6 = mo.m6 + ruy;
+    const float n7 = mo.m7 - (rux * ucx - ruy * ucy), n8 = mo.m8 - rux * ucy;
     // D_k / N_k; the q and om factors vary per node
-    float qa, qb;
+    float qa, qb = 0.0f;
     if (P.q_mode) {
         const float q = (8.0f * (2.0f - om)) * frcp(8.0f - om);
         qa = q * (-1.0f / 12.0f);
-        qb = q * (1.0f / 24.0f) - (1.0f / 12.0f);
+        if constexpr (FORCED) qb = q * (1.0f / 24.0f) - (1.0f / 12.0f);
     } else {
-        qa = P.cDq1; qb = P.cDq2;
+        qa = P.cDq1;
+        if constexpr (FORCED) qb = P.cDq2;
     }
     const float oa = om * -0.25f;
-    const float A = n1 * P.cD1a + fu * P.cD1b;
-    const float B = n2 * P.cD2a + fu * P.cD2b;
-    const float C = n3 * P.cD3a + fx * P.cD3b;
-    const float E = n5 * P.cD5a + fy * P.cD5b;
-    const float Q = n4 * qa + fx * qb;
-    const float G = n6 * qa + fy * qb;
-    const float R7 = n7 * oa + mf7 * (0.5f - 0.25f * om);
-    const float R8 = n8 * oa + mf8 * (0.25f - 0.125f * om);
+    float A = n1 * P.cD1a, B = n2 * P.cD2a, C = 0.0f, E = 0.0f;
+    float Q = n4 * qa, G = n6 * qa, R7 = n7 * oa, R8 = n8 * oa;
+    if constexpr (FORCED) {
+        const float fu = FX * ucx + FY * ucy;
+        const float mf7 = FX * ucx - FY * ucy, mf8 = FY * ucx + FX * ucy;
+        A = A + fu * P.cD1b;
+        B = B + fu * P.cD2b;
+        C = FX * P.cC3;
+        E = FY * P.cC5;
+        Q = Q + FX * qb;
+        G = G + FY * qb;
+        R7 = R7 + mf7 * (0.5f - 0.25f * om);
+        R8 = R8 + mf8 * (0.25f - 0.125f * om);
+    }
     // h* = h + M^T (D / N)
     const float ab1 = -A - 2.0f * B, ab5 = 2.0f * A + B;
     const float x = C - 2.0f * Q, y = E - 2.0f * G, u = C + Q, v = E + G;
     const float t13 = ab1 + R7, t24 = ab1 - R7, t57 = ab5 + R8, t68 = ab5 - R8;
     const float uv = u + v, vu = v - u;
-    h[0] = h[0] + 4.0f * (B - A);
-    h[1] = h[1] + (t13 + x);
-    h[3] = h[3] + (t13 - x);
-    h[2] = h[2] + (t24 + y);
-    h[4] = h[4] + (t24 - y);
-    h[5] = h[5] + (t57 + uv);
-    h[7] = h[7] + (t57 - uv);
-    h[6] = h[6] + (t68 + vu);
-    h[8] = h[8] + (t68 - vu);
+    H[0] = H[0] + 4.0f * (B - A);
+    H[1] = H[1] + (t13 + x);
+    H[3] = H[3] + (t13 - x);
+    H[2] = H[2] + (t24 + y);
+    H[4] = H[4] + (t24 - y);
+    H[5] = H[5] + (t57 + uv);
+    H[7] = H[7] + (t57 - uv);
+    H[6] = H[6] + (t68 + vu);
+    H[8] = H[8] + (t68 - vu);
     om_out = om;
     yielded = 0;
     if (!OUT && P.rheology == 0) return;
     // Chai strain rate (F6:903-924).  The strain moments f1 = C (s * n)
     // enter d11, d22, d12 only through three column sums of C, which leave
     //   d11 = -(e0 + e1/4 + 3 e7/4), d22 = -(e0 + e1/4 - 3 e7/4), d12 = -3 e8/2
-    // (e_k = s_k n_k; e0 = 0) plus the forcing terms: the same quantities as
-    // the reference's sums, in a few operations instead of about 50.
-    const float gu = gx * ux + gy * uy;
+    // (e_k = s_k n_k; e0 = 0) plus the forcing terms (which the reference
+    // adds before its 1/rho), here in storage units.
     const float base = n1 * P.cS1;  // -e1 / 4
     const float e7 = om * n7;
-    const float gs = (0.75f * om + P.cS2) * gu;  // 0.75 (s7 - s1) gu
-    float d11 = (base - 0.75f * e7) + (gs - 1.5f * (om * (gx * ux)));
-    float d22 = (base + 0.75f * e7) + (gs - 1.5f * (om * (gy * uy)));
-    float d12 = -1.5f * (om * n8) - 0.75f * (om * (gx * uy + gy * ux));
-    if (OUT) {
-        float nu = P.fnu;
-        stress[0] = nu * 2.0f * d11; stress[1] = nu * 2.0f * d22; stress[2] = nu * 2.0f * d12;
+    float D11 = base - 0.75f * e7, D22 = base + 0.75f * e7, D12 = -1.5f * (om * n8);
+    if constexpr (FORCED) {
+        const float gxS = S * gx, gyS = S * gy;
+        const float gs = (0.75f * om + P.cS2) * (gxS * ucx + gyS * ucy);  // 0.75 (s7 - s1) gu
+        D11 = D11 + (gs - 1.5f * (om * (gxS * ucx)));
+        D22 = D22 + (gs - 1.5f * (om * (gyS * ucy)));
+        D12 = D12 - 0.75f * (om * (gxS * ucy + gyS * ucx));
+    }
+    if (OUT && P.rheology == 0) {
+        const float c = 2.0f * P.fnu * (1.0f / S);
+        stress[0] = c * D11; stress[1] = c * D22; stress[2] = c * D12;
     }
     if (P.rheology != 0) {
-        float inv_rho = frcp(rho);
-        d11 *= inv_rho; d22 *= inv_rho; d12 *= inv_rho;
-        float gam = fsqrt(2.0f * fabsf(d11 * d11 + d22 * d22 + 2.0f * (d12 * d12)));
-        float mu0 = P.fmu0, ty = P.ftau_y, mp = P.fm_pap;
+        // gamma^2 = 2 |d:d| / rho^2 with d = D / S; one reciprocal square root
+        // gives both gamma and 1/gamma
+        const float g2 = (2.0f * (D11 * D11 + D22 * D22 + 2.0f * (D12 * D12))) * (rinvS * rinvS);
+        const float rg = frsq(g2), gam = g2 * rg;
+        const bool small = !(g2 >= P.fgamma_min2);
+        const float mu0 = P.fmu0, ty = P.ftau_y, mp = P.fm_pap;
         float nu_a;
         if (P.rheology == 2) {
             // Papanastasiou-Casson (F6:941, F6:944)
-            if (gam < P.fgamma_min) {
-                nu_a = P.fcas_lim;
-            } else {
-                float c = P.fsqrt_mu0 + fsqrt(ty * frcp(gam)) * (1.0f - fexp(-fsqrt(mp * gam)));
-                nu_a = c * c;
-            }
-        } else if (gam < P.fgamma_min)
-            nu_a = mu0 + mp * ty;
-        else
-            nu_a = mu0 + (ty * frcp(gam)) * (1.0f - fexp(-(mp * gam)));
-        if (OUT) {
-            float t11 = nu_a * 2.0f * d11, t22 = nu_a * 2.0f * d22, t12 = nu_a * 2.0f * d12;
-            stress[0] = t11; stress[1] = t22; stress[2] = t12;
-            float tmag = fsqrt(0.5f * fabsf(t11 * t11 + t22 * t22 + 2.0f * (t12 * t12)));
-            yielded = (tmag <= ty) ? 0 : 1;
+            const float c = P.fsqrt_mu0 + fsqrt(ty * rg) * (1.0f - fexp(-fsqrt(mp * gam)));
+            nu_a = small ? P.fcas_lim : c * c;
+        } else {
+            nu_a = small ? mu0 + mp * ty : mu0 + (ty * rg) * (1.0f - fexp(-(mp * gam)));
         }
-        om_out = frcp(3.0f * (nu_a * inv_rho) + 0.5f);
+        const float rinv = S * rinvS;
+        if (OUT) {
+            // tau = 2 nu_a d / rho, |tau| = sqrt(tau:tau / 2) = nu_a gamma
+            const float c = 2.0f * nu_a * rinvS;
+            stress[0] = c * D11; stress[1] = c * D22; stress[2] = c * D12;
+            yielded = (small ? 0.0f : nu_a * gam) <= ty ? 0 : 1;
+        }
+        om_out = frcp(3.0f * (nu_a * rinv) + 0.5f);
     }
 }
 

@@ -373,6 +373,22 @@ __global__ void k_soa_to_aos(long long nn, int nc, const double *soa, double *ao
 }
 
 // ================================================================= fused
+// One launch per time step.  Launch s (s = 1..n of a call of n steps) runs
+// step k = kk0 + s:
+//  * tile blocks pull-stream f*_k into f_k with the boundary rules, form
+//    rho and u, the SPM forcing G from particle state k (complete in memory
+//    since the launch before), correct u and collide into f*_{k+1} (or,
+//    FINAL, write the FP64 state); the tiles that hold nodes of a particle's
+//    reach box also write their share of its hydrodynamic force and torque
+//    (F6:542-556) as tagged words, and they are dispatched first;
+//  * particle blocks (one per particle, dispatched before the tiles)
+//    compute the contact force on state k (F3:739-804) at once, wait for
+//    their particle's partials, add them in a fixed order into FORCE_k, and
+//    run the Newton-Euler update to state k+1 (F6:299-313), which the next
+//    launch reads; the final launch only writes FORCE_k into the record.
+// The prologue (s = 0) collides the canonical FP64 state into the storage
+// format while its particle blocks advance state kk0 to kk0+1 with the
+// record's FORCE.  The particle state is double-buffered by launch parity.
 struct StepArgs {
     DevP P;
     const void *src;              // post-collision populations f*_k (storage format)
@@ -386,8 +402,14 @@ struct StepArgs {
     // node's interior source (3 x nbnd)
     const void *erho_src;
     void *erho_dst;
-    const double *pin;            // particle state k
-    double *pout;                 // particle state k+1 (final step: same as pin)
+    const double *pin;            // particle state k (SoA records)
+    double *pout;                 // particle state k+1 (final launch: same as pin)
+    const float *spf_in;          // FP32 profile records of state k (reduced formats)
+    float *spf_out;               // those of state k+1
+    const int *pbox_in;           // reach boxes of state k [maxp][4]: i0, i1, j0, j1
+    int *pbox_out;                // those of state k+1
+    unsigned long long *fslot;    // tagged force partials [maxp][rcap][SLOTW]
+    int rcap;                     // partial slots per particle
     const double *dpw_in;
     double *dpw_out;
     int *pairs, *paircnt;
@@ -398,28 +420,22 @@ struct StepArgs {
     int kk_upd;                   // loop iteration of the particle update in this launch
     long long ntiles;
     int aux;                      // write STRESS / DOMAIN_IMAGE
-    // per-tile candidate lists (more than NP_LOOP particles): read for the
-    // state the tiles use (generation gen_in), written for the next state
+    // per-tile candidate lists (more than NP_LOOP particles): read for this
+    // launch (generation = its tag), written for the next one
     const unsigned long long *tl_hdr_in;
     const int *tl_ids_in;
     unsigned long long *tl_hdr_out;
     int *tl_ids_out;
-    unsigned gen_in, gen_out;
-    int *err;                     // bit 0: a tile list overflowed
-    // the force on one particle is summed by nsub blocks; the last of them
-    // (counter pcount) adds the partials pforce in order and advances it
-    int nsub;
-    double *pforce;               // [maxp][nsub][3]
-    int *pcount;                  // [maxp]
-    // FP32 smoothed-profile records (reduced-precision runs; null for FP64)
-    const float *spf_in;          // state k
-    float *spf_out;               // state k+1
+    unsigned tag;                 // this launch's tag (list generation, partial words)
+    unsigned sr;                  // stochastic-rounding sequence position
+    int *err;                     // LBM_EF_* bits
     // per-call bases from which step_fields() derives the per-step fields
     double *hist_base;            // history rows of the call, or null
     unsigned long long *slots_base;  // monitor slots of the call, or null
     int rowp;                     // history doubles per step (particles)
     int kk0;                      // step index before the call
-    unsigned g0;                  // tile-list generation at the start of the call
+    unsigned tag0;                // tag of the call's prologue
+    unsigned sr0;                 // rounding sequence at the prologue
     int tpb;                      // 256-node tiles per block (batched launches)
     // monitor partials: each tile block stores its fixed-point mass and its
     // yielded count (plain stores, [mass | yield] x ntiles, two buffers by
@@ -432,15 +448,23 @@ struct StepArgs {
     int nred;                     // reducer block slots in the block layout (0 or 1)
 };
 
+// error bits of the fused path
+enum {
+    LBM_EF_LIST = 1,      // a tile's candidate list overflowed
+    LBM_EF_NAN = 2,       // a non-finite density or particle state
+    LBM_EF_WAIT = 4,      // force partials did not appear within the wait bound
+    LBM_EF_SLOT = 16,     // a reach box holds more tiles than its partial slots
+};
+
 // Per-step fields of launch s of a call of nsteps fused steps (s = 0: the
-// prologue; s = nsteps: the final step): tile-list generations, the loop
-// iteration of the particle update, the history row and monitor slots, and
-// whether the launch writes the output-only fields.
+// prologue; s = nsteps: the final step): the tag, the loop iteration of the
+// particle update, the history row and monitor slots, and whether the launch
+// writes the output-only fields.
 __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nsteps) {
     const bool rec = a.slots_base != nullptr;
     const bool fin = s == nsteps;
-    a.gen_in = a.g0 + (unsigned)s;
-    a.gen_out = a.gen_in + 1u;
+    a.tag = a.tag0 + (unsigned)s;
+    a.sr = a.sr0 + (unsigned)s + 1u;
     a.kk_upd = a.kk0 + s + 1;
     if (s == 0) {
         a.hist_part = nullptr;
@@ -448,7 +472,7 @@ __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nst
         a.slots_yield = rec ? a.slots_base + NSLOT : nullptr;
         a.aux = nsteps == 1;
     } else {
-        a.hist_part = rec ? a.hist_base + (size_t)(s - 1) * (size_t)a.rowp : nullptr;
+        a.hist_part = a.hist_base ? a.hist_base + (size_t)(s - 1) * (size_t)a.rowp : nullptr;
         a.slots_mass = rec ? a.slots_base + (size_t)(s - 1) * 2 * NSLOT : nullptr;
         a.slots_yield = (rec && !fin) ? a.slots_base + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
         a.aux = (s == nsteps - 1) || fin;
@@ -461,194 +485,218 @@ __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nst
     a.red_yield = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * 2 * NSLOT + NSLOT : nullptr;
 }
 
-// FP32 variant of tile_has_particles for the reduced-precision path.
-__device__ __forceinline__ bool tile_has_particles_f32(const StepArgs &a, long long tile) {
-    const DevP &P = a.P;
-    if (P.np == 0) return false;
-    if (P.np > NP_LOOP) return true;
-    long long n0 = tile * NT, n1 = min(n0 + NT - 1, P.nn - 1);
-    float jlo = (float)row_of(P, (unsigned)n0), jhi = (float)row_of(P, (unsigned)n1);
-    for (int p = 0; p < P.np; ++p) {
-        const float *s = a.spf_in + p * SPF;
-        float y = s[SY] + s[SYF], r = s[SREACH];
-        if (y + r >= jlo && y - r <= jhi) return true;
-    }
-    return false;
+__device__ __forceinline__ void set_err(const StepArgs &a, int bits) { atomicOr(a.err, bits); }
+
+// history row of one particle: record fields of the state, its FORCE, F_H_IN
+__device__ __forceinline__ void hist_row(double *h, const double *rec, const double F[3]) {
+    h[0] = rec[PX]; h[1] = rec[PY]; h[2] = rec[PANG];
+    h[3] = rec[PVX]; h[4] = rec[PVY]; h[5] = rec[PW];
+    h[6] = F[0]; h[7] = F[1]; h[8] = F[2];
+    h[9] = rec[PFHX]; h[10] = rec[PFHY]; h[11] = rec[PFHW];
 }
 
-// G at a node of a tile, FP32 (reduced-precision path).
-__device__ __forceinline__ void spm_tile_node_f32(const StepArgs &a, long long tile, bool any,
-                                                  int i, int j, float ux, float uy, float &gx,
-                                                  float &gy) {
-    const DevP &P = a.P;
-    gx = 0.0f;
-    gy = 0.0f;
-    if (!any) return;
-    if (P.np <= NP_LOOP) {
-        spm_node_f32(a.spf_in, nullptr, P.np, i, j, ux, uy, gx, gy);
-        return;
-    }
-    unsigned long long h = a.tl_hdr_in[tile];
-    if ((unsigned)(h >> 32) != a.gen_in) return;
-    int cnt = min((int)(h & 0xffffffffu), MAXC);
-    spm_node_f32(a.spf_in, a.tl_ids_in + tile * MAXC, cnt, i, j, ux, uy, gx, gy);
+// The reach box of a state stored for the tiles of the next launch
+// (i0, i1, j0, j1; empty when i0 > i1).
+__device__ __forceinline__ void put_box(int *dst, const Box &b) {
+    dst[0] = b.i0; dst[1] = b.i1; dst[2] = b.j0; dst[3] = b.j1;
 }
 
-// Can any particle reach the rows of this tile?  (Uniform over the block;
-// with few particles it lets most tiles skip the smoothed-profile work.)
-__device__ __forceinline__ bool tile_has_particles(const StepArgs &a, long long tile) {
+// FORCE and TORQUE on particle p (ComputeForce_HYDRO, F6:542-556) from the
+// tagged partials of the cnt tiles of its reach box, by one wavefront: lane
+// l takes slots l, l + 64, ... in order, waiting (bounded) until their words
+// carry this launch's tag; a fixed shuffle tree then adds the lanes.  Valid
+// in every lane; false when the wait ran out.
+__device__ __forceinline__ bool sum_force_slots(const StepArgs &a, int p, int cnt, double F[3]) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long *q = a.fslot + (size_t)p * a.rcap * SLOTW;
+    double acc[3] = {0.0, 0.0, 0.0};
+    bool ok = true;
+    for (int r = lane; r < cnt; r += 64) {
+        const unsigned long long *w = q + (size_t)r * SLOTW;
+        unsigned long long v[SLOTW];
+        int it = 0;
+        for (;;) {
+            bool ready = true;
+#pragma unroll
+            for (int k = 0; k < SLOTW; ++k) {
+                v[k] = __hip_atomic_load(w + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ready = ready && (unsigned)(v[k] >> 32) == a.tag;
+            }
+            if (ready) break;
+            if (++it >= TAG_SPIN) { ok = false; break; }
+            __builtin_amdgcn_s_sleep(2);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            acc[c] = acc[c] + __builtin_bit_cast(double, (v[2 * c] << 32) | (v[2 * c + 1] & 0xffffffffull));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double x = acc[c];
+#pragma unroll
+        for (int sft = 32; sft > 0; sft >>= 1) x = x + __shfl_xor(x, sft, 64);
+        F[c] = x;
+    }
+    return __all(ok);
+}
+
+// Particle block of launch s >= 1 (all NT threads): contact force on state
+// k (F3:739-804) first, FORCE_k from the tiles' partials, the history row of
+// step k, then the update to state k+1 (the final launch writes FORCE_k into
+// the record instead).  Every input is loaded into registers before any is
+// written to LDS: a load that feeds an LDS store inside a branch is waited
+// for there.
+template <bool DEV, bool FINAL>
+__device__ void particle_block(const StepArgs &a, int p) {
     const DevP &P = a.P;
-    if (P.np == 0) return false;
-    if (P.np > NP_LOOP) return true;
-    long long n0 = tile * NT, n1 = min(n0 + NT - 1, P.nn - 1);
-    double jlo = (double)row_of(P, (unsigned)n0), jhi = (double)row_of(P, (unsigned)n1);
     const int m = P.maxp;
-    for (int p = 0; p < P.np; ++p) {
-        double reach = 0.5 * a.pin[PDIAM * m + p] + 0.5 * a.pin[PZETA * m + p] + 1.0;
-        double y = a.pin[PY * m + p];
-        if (y + reach >= jlo && y - reach <= jhi) return true;
+    static_assert(NP_LDS <= NT, "one staged particle per thread");
+    __shared__ double srec[PCOUNT];
+    __shared__ double scon[3 * NP_LDS];
+    __shared__ double sdpw;
+    __shared__ ContactLDS cl;
+    __shared__ int scnt;
+    const int t = threadIdx.x;
+    const bool staged = P.np <= NP_LDS;
+    const int tr = min(t, PCOUNT - 1), tc = min(t, max(P.np - 1, 0));
+    const double r_rec = a.pin[tr * m + p];
+    const double r_x = a.pin[PX * m + tc], r_y = a.pin[PY * m + tc], r_d = a.pin[PDIAM * m + tc];
+    const double r_dpw = *a.dpw_in;
+    if (t < PCOUNT) srec[t] = r_rec;
+    if (staged && t < P.np) {
+        scon[t] = r_x; scon[NP_LDS + t] = r_y; scon[2 * NP_LDS + t] = r_d;
     }
-    return false;
-}
-
-// G at a node of a tile: all particles tested directly when there are few,
-// else the tile's candidate list for the current generation.
-__device__ __forceinline__ void spm_tile_node(const StepArgs &a, long long tile, bool any, int i,
-                                              int j, double ux, double uy, double &gx,
-                                              double &gy) {
-    const DevP &P = a.P;
-    gx = 0.0;
-    gy = 0.0;
-    if (!any) return;
-    if (P.np <= NP_LOOP) {
-        spm_node(a.pin, P.maxp, P.np, i, j, ux, uy, gx, gy);
+    if (t == 0) sdpw = r_dpw;
+    __syncthreads();
+    // contact force on state k: it does not depend on FORCE_k, so it runs
+    // while the tiles are still producing the partials
+    double fcx = 0.0, fcy = 0.0, dl = 0.0;
+    int cnt = 0;
+    if (!FINAL) {
+        if (P.np <= 64) {
+            if (t < 64)
+                cnt = contact_wave(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, fcx,
+                                   fcy, a.pairs + p * MAXNB, &dl);
+        } else {
+            cnt = staged
+                ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl, fcx,
+                                fcy, a.pairs + p * MAXNB, &dl)
+                : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
+                                sdpw, cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
+        }
+    }
+    if (t == 0) {
+        const Box b = reach_box(P, srec[PX], srec[PY], srec[PDIAM], srec[PZETA]);
+        scnt = (b.i0 > b.i1 || b.j0 > b.j1) ? 0 : min(box_tile_count(P, b), a.rcap);
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    double F[3];
+    if (!sum_force_slots(a, p, scnt, F) && t == 0) set_err(a, LBM_EF_WAIT);
+    if (a.hist_part && t == 0) hist_row(a.hist_part + (size_t)p * HPC, srec, F);
+    if (FINAL) {
+        if (t == 0) {
+            a.pout[PFX * m + p] = F[0]; a.pout[PFY * m + p] = F[1]; a.pout[PTQ * m + p] = F[2];
+            if (!__builtin_isfinite(F[0] + F[1] + F[2])) set_err(a, LBM_EF_NAN);
+        }
         return;
     }
-    unsigned long long h = a.tl_hdr_in[tile];
-    if ((unsigned)(h >> 32) != a.gen_in) return;
-    int cnt = min((int)(h & 0xffffffffu), MAXC);
-    spm_node_cand(a.pin, P.maxp, a.tl_ids_in + tile * MAXC, cnt, i, j, ux, uy, gx, gy);
+    // the update by wavefront 0; the contact force is final in lane 0
+    __shared__ double snew[PCOUNT];
+    const double val = update_wave(P, srec, F[0], F[1], F[2], __shfl(fcx, 0), __shfl(fcy, 0),
+                                   a.kk_upd);
+    if (t < PCOUNT) {
+        a.pout[t * m + p] = val;
+        snew[t] = val;
+    }
+    if (t == 0) {
+        a.paircnt[p] = cnt;
+        if (p == P.np - 1) *a.dpw_out = dl;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (t == 0) {
+        const Box nb = reach_box(P, snew[PX], snew[PY], snew[PDIAM], snew[PZETA]);
+        put_box(a.pbox_out + 4 * p, nb);
+        if (a.spf_out) make_spf(snew, a.spf_out + p * SPF);
+        bool bad = false;
+        for (int f = 0; f < PCOUNT; ++f) bad = bad || !__builtin_isfinite(snew[f]);
+        if (bad) set_err(a, LBM_EF_NAN);
+    }
 }
 
-// Particle block (all NT threads): contact force of particle p over all
-// partners, then thread 0 runs the Newton-Euler update from state k (pin) into
-// state k+1 (pout); with many particles the block then registers p in the
-// candidate lists of the tiles it reaches.
-__device__ __forceinline__ void particle_advance(const StepArgs &a, int p) {
+// Tile registration of the new state of particle p for the next launch
+// (many particles), by all NT threads of its block after particle_block.
+__device__ __forceinline__ void particle_register(const StepArgs &a, int p) {
+    const DevP &P = a.P;
+    if (P.np <= NP_LOOP) return;
+    __syncthreads();
+    const int m = P.maxp;
+    const Box nb = reach_box(P, a.pout[PX * m + p], a.pout[PY * m + p], a.pout[PDIAM * m + p],
+                             a.pout[PZETA * m + p]);
+    register_tiles(P, nb, p, a.tag + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
+}
+
+// Prologue particle block: contact force on state kk0, the update with the
+// record's FORCE (the reference's next loop iteration, F6:299-313), the new
+// state's box, profile record and tile registration.
+template <bool DEV>
+__device__ void particle_advance(const StepArgs &a, int p) {
     const DevP &P = a.P;
     const int m = P.maxp;
     __shared__ double srec[PCOUNT];
+    __shared__ double scon[3 * NP_LDS];
     __shared__ double sdpw;
     __shared__ ContactLDS cl;
-    if (threadIdx.x < PCOUNT) srec[threadIdx.x] = a.pin[threadIdx.x * m + p];
-    if (threadIdx.x == 0) sdpw = *a.dpw_in;
+    const int t = threadIdx.x;
+    const bool staged = P.np <= NP_LDS;
+    const int tr = min(t, PCOUNT - 1), tc = min(t, max(P.np - 1, 0));
+    const double r_rec = a.pin[tr * m + p];
+    const double r_x = a.pin[PX * m + tc], r_y = a.pin[PY * m + tc], r_d = a.pin[PDIAM * m + tc];
+    const double r_dpw = *a.dpw_in;
+    if (t < PCOUNT) srec[t] = r_rec;
+    if (staged && t < P.np) {
+        scon[t] = r_x; scon[NP_LDS + t] = r_y; scon[2 * NP_LDS + t] = r_d;
+    }
+    if (t == 0) sdpw = r_dpw;
     __syncthreads();
     double fcx = 0.0, fcy = 0.0, dl = 0.0;
-    int cnt = contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p, sdpw,
-                            cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
-    if (threadIdx.x == 0) {
-        double rec[PCOUNT];
-        for (int f = 0; f < PCOUNT; ++f) rec[f] = srec[f];
-        a.paircnt[p] = cnt;
-        rec[PFCX] = fcx; rec[PFCY] = fcy;
-        update_one(P, rec, a.kk_upd);
-        for (int f = 0; f < PCOUNT; ++f) { a.pout[f * m + p] = rec[f]; srec[f] = rec[f]; }
-        if (a.spf_out) make_spf(rec, a.spf_out + p * SPF);
-        if (p == P.np - 1) *a.dpw_out = dl;
-    }
-    __syncthreads();
-    if (P.np > NP_LOOP) register_tiles(P, srec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
-}
-
-// Hydrodynamic force and torque on particle p at step k (ComputeForce_HYDRO,
-// F6:542-556) over the nodes that can have phi > 0, from populations
-// rebuilt by the same pull-stream and bounce-back as the grid pass; a
-// fixed-order block reduction keeps it deterministic.  Block `sub` of `nsub`
-// takes every nsub-th group of NT nodes of the reach box.
-template <class St, bool DEV, int BND>
-__device__ void particle_force(const StepArgs &a, int p, int sub, int nsub, double (&F)[3]) {
-    __shared__ double lds[3 * NT];
-    const DevP &P = a.P;
-    const int m = P.maxp;
-    const double *pin = a.pin;
-    double xc = 0.0, yc = 0.0, vx = 0.0, vy = 0.0, w = 0.0, dm = 0.0, zt = 0.0;
-    float sv[SPF];
-    int i0, i1, j0, j1;
-    if constexpr (DEV) {
-        // The FP32 profile record alone (one 48-byte load) gives the reach
-        // box and everything the loop needs.  A node with phi > 0 lies more
-        // than one lattice unit inside the box, so the FP32 rounding of its
-        // edges cannot drop a contributing node.
-        const float *s = a.spf_in + p * SPF;
-#pragma unroll
-        for (int k = 0; k < SPF; ++k) sv[k] = s[k];
-        const int X = (int)sv[SX], Y = (int)sv[SY];
-        i0 = max(0, X + (int)ceilf(sv[SXF] - sv[SREACH]));
-        i1 = min(P.nx, X + (int)floorf(sv[SXF] + sv[SREACH]));
-        j0 = max(0, Y + (int)ceilf(sv[SYF] - sv[SREACH]));
-        j1 = min(P.ny, Y + (int)floorf(sv[SYF] + sv[SREACH]));
+    int cnt = 0;
+    if (P.np <= 64) {
+        if (t < 64)
+            cnt = contact_wave(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, fcx, fcy,
+                               a.pairs + p * MAXNB, &dl);
     } else {
-        xc = pin[PX * m + p]; yc = pin[PY * m + p];
-        vx = pin[PVX * m + p]; vy = pin[PVY * m + p]; w = pin[PW * m + p];
-        dm = pin[PDIAM * m + p]; zt = pin[PZETA * m + p];
-        double reach = 0.5 * dm + 0.5 * zt + 1.0;
-        i0 = max(0, (int)ceil(xc - reach)); i1 = min(P.nx, (int)floor(xc + reach));
-        j0 = max(0, (int)ceil(yc - reach)); j1 = min(P.ny, (int)floor(yc + reach));
+        cnt = staged
+            ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl, fcx, fcy,
+                            a.pairs + p * MAXNB, &dl)
+            : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p, sdpw,
+                            cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
     }
-    int bw = i1 - i0 + 1, bh = j1 - j0 + 1;
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (bw > 0 && bh > 0) {
-        const typename St::T *src = (const typename St::T *)a.src;
-        for (int t = sub * NT + threadIdx.x; t < bw * bh; t += nsub * NT) {
-            int i = i0 + t % bw, j = j0 + t / bw;
-            long long node = (long long)j * P.W + i;
-            double r, u, v;
-            if constexpr (DEV) {
-                // same FP32 profile and rigid-body velocity as the grid pass
-                float dx, dy;
-                float phf = phi_f32(sv, i, j, dx, dy);
-                if (!(phf > 0.0f)) continue;
-                float h[9], rd, rf, uf, vf;
-                load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, node, i, j, h);
-                macros_dev(h, P.frho0, rd, rf, uf, vf);
-                float uxpf = sv[SVX] - sv[SW] * dy, uypf = sv[SVY] + sv[SW] * dx;
-                double ph = phf;
-                r = rf;
-                double du = (double)(uf - uxpf), dv = (double)(vf - uypf);
-                acc[0] = acc[0] + r * ph * du;
-                acc[1] = acc[1] + r * ph * dv;
-                acc[2] = acc[2] + r * ph * ((double)dx * dv - (double)dy * du);
-                continue;
-            } else {
-                double ph = phi_at(xc, yc, dm, zt, i, j);
-                if (!(ph > 0.0)) continue;
-                double f[9];
-                load_fk64<St, BND>(P, src, (const double *)a.erho_src, node, i, j, f);
-                macros64(f, r, u, v);
-                double uxp = vx - w * ((double)j - yc);
-                double uyp = vy + w * ((double)i - xc);
-                acc[0] = acc[0] + r * ph * (u - uxp);
-                acc[1] = acc[1] + r * ph * (v - uyp);
-                acc[2] = acc[2] + r * ph * (((double)i - xc) * (v - uyp) - ((double)j - yc) * (u - uxp));
-            }
+    __shared__ double snew[PCOUNT];
+    if (t < 64) {
+        const double val = update_wave(P, srec, srec[PFX], srec[PFY], srec[PTQ], __shfl(fcx, 0),
+                                       __shfl(fcy, 0), a.kk_upd);
+        if (t < PCOUNT) {
+            a.pout[t * m + p] = val;
+            snew[t] = val;
+        }
+        if (t == 0) {
+            a.paircnt[p] = cnt;
+            if (p == P.np - 1) *a.dpw_out = dl;
         }
     }
-    block_sum<3, !DEV>(acc, lds);
-    F[0] = acc[0]; F[1] = acc[1]; F[2] = acc[2];
-}
-
-// rec: this particle's record staged in LDS (field f at rec[f]), or null to
-// read it from global memory (a round trip inside the particle chain).
-__device__ __forceinline__ void hist_write(const StepArgs &a, int p, const double F[3],
-                                           const double *rec) {
-    if (!a.hist_part || threadIdx.x != 0) return;
-    const int m = a.P.maxp;
-    auto fld = [&](int f) { return rec ? rec[f] : a.pin[f * m + p]; };
-    double *h = a.hist_part + (size_t)p * HPC;
-    h[0] = fld(PX); h[1] = fld(PY); h[2] = fld(PANG);
-    h[3] = fld(PVX); h[4] = fld(PVY); h[5] = fld(PW);
-    h[6] = F[0]; h[7] = F[1]; h[8] = F[2];
-    h[9] = fld(PFHX); h[10] = fld(PFHY); h[11] = fld(PFHW);
+    __syncthreads();
+    const Box nb = reach_box(P, snew[PX], snew[PY], snew[PDIAM], snew[PZETA]);
+    if (t == 0) {
+        put_box(a.pbox_out + 4 * p, nb);
+        if (a.spf_out) make_spf(snew, a.spf_out + p * SPF);
+        bool bad = false;
+        for (int f = 0; f < PCOUNT; ++f) bad = bad || !__builtin_isfinite(snew[f]);
+        if (bad) set_err(a, LBM_EF_NAN);
+    }
+    if (P.np > NP_LOOP) register_tiles(P, nb, p, a.tag + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
 // Block totals of the monitors: FP64 mass as a fixed-point integer (so the
@@ -725,9 +773,26 @@ __device__ __forceinline__ void eq_put(const StepArgs &a, const DevP &P, int i, 
     });
 }
 
+// Stochastic-rounding words of node n: one 32-bit hash of the node and the
+// launch, spread over the nine populations by distinct odd 24-bit multipliers
+// (24 x 24-bit products issue at full rate).
+__device__ __forceinline__ uint32_t node_hash(unsigned n, unsigned sr) {
+    return mix32(n ^ mix32(sr));
+}
+
+// Store of the nine post-collision values of node n in storage units.
+template <class St>
+__device__ __forceinline__ void store_dev(const DevP &P, void *dst, unsigned n, const float H[9],
+                                          uint32_t nh) {
+    const auto drs = make_rsrc(dst, P.buf_bytes);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        St::bst(drs, H[k], n * (unsigned)sizeof(typename St::T),
+                (unsigned)(k * P.plane * (long long)sizeof(typename St::T)), sr_word(nh, k));
+}
+
 // Prologue of a fused run: collide the FP64 state of step k (F, RHO, corrected
-// UX/UY, G, OMEGA_BN) into f*_{k+1} in the storage format; particle blocks
-// take the particles from state k to k+1.
+// UX/UY, G, OMEGA_BN) into f*_{k+1} in the storage format.
 template <class St, bool DEV>
 __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile) {
     const DevP &P = a.P;
@@ -735,32 +800,28 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
     int yielded = 0;
     if (n < P.nn) {
         int j = row_of(P, (unsigned)n), i = (int)(n - (long long)j * P.W);
-        typename St::T *dst = (typename St::T *)a.dst;
         double r = a.crho[n], ux = a.cux[n], uy = a.cuy[n];
         double gx = a.cg[n], gy = a.cg[P.plane + n];
         int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
         if constexpr (DEV) {
-            float h[9];
+            const float S = St::S;
+            float H[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 double fk = a.cf[k * P.plane + n];
-                h[k] = (float)(fk - (double)wdev(k) * P.rho0);
+                H[k] = (float)((fk - (double)wdev(k) * P.rho0) * (double)S);
             }
-            // density deviation from the same populations
-            float rd = ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7])) + h[8];
+            const DevMom mo = dev_moments(H);
             float om = (float)(P.rheology != 0 ? a.cobn[n] : P.omega), om2, str[3];
+            const float rhof = (float)r;
+            const float rinvS = frcp(rhof) * St::invS;
             if (a.aux || a.slots_yield)
-                collide_dev<true>(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy,
-                                  om, om2, str, yielded);
+                collide_dev<true, true>(P, H, mo, S, rhof, rinvS, (float)ux, (float)uy, (float)gx,
+                                        (float)gy, om, om2, str, yielded);
             else
-                collide_dev<false>(P, h, rd, (float)r, (float)ux, (float)uy, (float)gx, (float)gy,
-                                   om, om2, str, yielded);
-            uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
-            const auto drs = make_rsrc(dst, P.buf_bytes);
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                St::bst(drs, h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
-                        (unsigned)(k * P.plane * (long long)sizeof(typename St::T)), sr_word(nh, k));
+                collide_dev<true, false>(P, H, mo, S, rhof, rinvS, (float)ux, (float)uy,
+                                         (float)gx, (float)gy, om, om2, str, yielded);
+            store_dev<St>(P, a.dst, (unsigned)n, H, node_hash((unsigned)n, a.sr));
             if (P.rheology != 0) a.rate32[n] = om2;
             if (a.aux) {
                 a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
@@ -773,6 +834,7 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
                 ((float *)a.erho_dst)[b] = (float)r;
             }
         } else {
+            typename St::T *dst = (typename St::T *)a.dst;
             double f[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) f[k] = a.cf[k * P.plane + n];
@@ -801,7 +863,7 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
 template <class St, bool DEV>
 __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
     if ((int)blockIdx.x < a.n_pblocks) {
-        if ((int)blockIdx.x % a.nsub == 0) particle_advance(a, (int)blockIdx.x / a.nsub);
+        particle_advance<DEV>(a, (int)blockIdx.x);
         return;
     }
     if ((int)blockIdx.x < a.n_pblocks + a.nred) return;  // nothing before the prologue
@@ -829,255 +891,346 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
     return img;
 }
 
-// One fused time step k over the grid: pull-stream + bounce-back (f_k),
-// density and predicted velocity, smoothed-profile forcing G_k, velocity
-// correction, then (FINAL = false) the MRT / Papanastasiou collision of step
-// k+1 stored as f*_{k+1}, or (FINAL = true) the FP64 state of step k.
-// Particle blocks compute the hydrodynamic force of step k and advance the
-// particles to state k+1.
-template <class St, bool DEV, bool FINAL, int BND>
-__device__ void particle_block(const StepArgs &a, int pb) {
+// ---------------------------------------------------------------- tiles
+// Which particles can reach tile t in this launch?  Up to NP_LOOP particles:
+// a bit per particle whose reach box of state k meets the tile; beyond: the
+// tile's candidate list of this launch.  Uniform over the block.
+__device__ __forceinline__ int tile_candidates(const StepArgs &a, long long t, unsigned &mask,
+                                               const int *&ids) {
     const DevP &P = a.P;
-    {
-        int p = pb / a.nsub, sub = pb - p * a.nsub;
-        const int m = P.maxp;
-        // Stage the inputs of the update in LDS: this particle's record, the
-        // contact inputs of all particles (NP_LDS <= NT: one per thread) and
-        // the stale wall distance.  In the reduced-precision steps they are
-        // loaded into registers before the force gather and written to LDS
-        // after it, so their round trip overlaps the gather's instead of
-        // preceding it (a load feeding an LDS store is waited for at the
-        // store); the thread that finishes the update reads LDS instead of
-        // global memory.
-        static_assert(NP_LDS <= NT, "one staged particle per thread");
-        __shared__ double srec[PCOUNT];
-        __shared__ double scon[3 * NP_LDS];
-        __shared__ double sdpw;
-        const bool staged = !FINAL && P.np <= NP_LDS;
-        double r_rec = 0.0, r_x = 0.0, r_y = 0.0, r_d = 0.0, r_dpw = 0.0;
-        if (!FINAL) {
-            const int t = threadIdx.x;
-            if (t < PCOUNT) r_rec = a.pin[t * m + p];
-            if (staged && t < P.np) {
-                r_x = a.pin[PX * m + t]; r_y = a.pin[PY * m + t]; r_d = a.pin[PDIAM * m + t];
-            }
-            if (t == 0) r_dpw = *a.dpw_in;
+    mask = 0u;
+    ids = nullptr;
+    if (P.np == 0) return 0;
+    if (P.np <= NP_LOOP) {
+        for (int p = 0; p < P.np; ++p) {
+            const int *w = a.pbox_in + 4 * p;
+            const int i0 = __builtin_amdgcn_readfirstlane(w[0]), i1 = __builtin_amdgcn_readfirstlane(w[1]);
+            const int j0 = __builtin_amdgcn_readfirstlane(w[2]), j1 = __builtin_amdgcn_readfirstlane(w[3]);
+            if (i0 <= i1 && j0 <= j1 && tile_meets_box(P, t, i0, i1, j0, j1)) mask |= 1u << p;
         }
-        auto stage = [&]() {
-            const int t = threadIdx.x;
-            if (t < PCOUNT) srec[t] = r_rec;
-            if (staged && t < P.np) {
-                scon[t] = r_x; scon[NP_LDS + t] = r_y; scon[2 * NP_LDS + t] = r_d;
-            }
-            if (t == 0) sdpw = r_dpw;
-        };
-        // The FP64 step is bound by its tiles' HBM traffic, not by this
-        // chain: it stages before the gather (the barriers of the gather's
-        // block sum publish the LDS) and keeps the registers for the gather
-        // (staging after it raised its allocation from 98 to 106 VGPRs and
-        // its step time by 3 %).
-        if constexpr (!DEV)
-            if (!FINAL) stage();
-        double F[3];
-        particle_force<St, DEV, BND>(a, p, sub, a.nsub, F);  // ends with block barriers
-        if constexpr (DEV)
-            if (!FINAL) {
-                stage();
-                if (a.nsub <= 1) __syncthreads();  // else the barrier after the partial sum
-            }
-        if (a.nsub > 1) {
-            // partial of this block; the last block of the particle adds all
-            // partials in sub order and carries on with the update.
-            // The partials are written and read with agent-scope atomic
-            // accesses, which bypass the per-XCD L2, and the counter add is
-            // issued only after those stores have completed (vmcnt drained).
-            // A __threadfence() here would instead write back every dirty
-            // line of this XCD's L2 (the tile stores of the whole launch) on
-            // every particle block; that cost ~40 us per step at 504 disks.
-            __shared__ int last;
-            if (threadIdx.x == 0) {
-                double *pf = a.pforce + ((size_t)p * a.nsub + sub) * 3;
-                for (int c = 0; c < 3; ++c)
-                    __hip_atomic_store(pf + c, F[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                __builtin_amdgcn_s_waitcnt(0);
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                last = __hip_atomic_fetch_add(a.pcount + p, 1, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT) == a.nsub - 1;
-                if (last) {
-                    const double *q = a.pforce + (size_t)p * a.nsub * 3;
-                    F[0] = 0.0; F[1] = 0.0; F[2] = 0.0;
-                    for (int s = 0; s < a.nsub; ++s)
-                        for (int c = 0; c < 3; ++c)
-                            F[c] += __hip_atomic_load(q + 3 * s + c, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(a.pcount + p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_popcount(mask);
+    }
+    const unsigned long long h = a.tl_hdr_in[t];
+    if ((unsigned)(h >> 32) != a.tag) return 0;
+    ids = a.tl_ids_in + t * MAXC;
+    return min((int)(h & 0xffffffffu), MAXC);
+}
+
+// A value every lane holds alike, as a wavefront-uniform one.
+__device__ __forceinline__ int uniform_i32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uniform_f32(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double uniform_f64(double x) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Smoothed-profile forcing and force partials of a tile (all NT threads;
+// block barriers).  For each candidate particle p (ascending when tested
+// directly) whose reach box the tile holds nodes of: phi at the thread's
+// node from state k, the highest index with phi > 0 sets G (F6:485-495), and
+// the node's share of the hydrodynamic force and torque (F6:542-556) goes
+// into the tile's partial for p: each wavefront adds its lanes with a fixed
+// shuffle tree, thread 0 adds the four in order and writes the tagged words
+// of slot rank (the tile's position in the box's tile order).  r, u:
+// density and uncorrected velocity of the node (F6:333-340); valid: the
+// thread holds a node.
+template <bool DEV, class Real>
+__device__ void tile_spm(const StepArgs &a, long long t, int cnt, unsigned mask, const int *ids,
+                         int i, int j, bool valid, Real r, Real u, Real v, Real &gx, Real &gy) {
+    const DevP &P = a.P;
+    const int m = P.maxp;
+    __shared__ double red[4 * 3];
+    gx = (Real)0;
+    gy = (Real)0;
+    int best = -1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = 0; c < cnt; ++c) {
+        int p;
+        if (ids) p = uniform_i32(ids[c]);
+        else { p = __builtin_ctz(mask); mask &= mask - 1u; }
+        Box b;
+        b.i0 = uniform_i32(a.pbox_in[4 * p]); b.i1 = uniform_i32(a.pbox_in[4 * p + 1]);
+        b.j0 = uniform_i32(a.pbox_in[4 * p + 2]); b.j1 = uniform_i32(a.pbox_in[4 * p + 3]);
+        const int rank = (b.i0 > b.i1 || b.j0 > b.j1) ? -1 : box_tile_rank(P, b, t);
+        if (rank < 0) continue;  // the tile holds no node of the box
+        double acc[3] = {0.0, 0.0, 0.0};
+        if constexpr (DEV) {
+            const float *q = a.spf_in + p * SPF;
+            Prof s;
+            s.x = uniform_f32(q[SX]); s.xf = uniform_f32(q[SXF]);
+            s.y = uniform_f32(q[SY]); s.yf = uniform_f32(q[SYF]);
+            s.vx = uniform_f32(q[SVX]); s.vy = uniform_f32(q[SVY]);
+            s.w = uniform_f32(q[SW]); s.rp = uniform_f32(q[SRP]);
+            s.hz = uniform_f32(q[SHZ]); s.pz = uniform_f32(q[SPZ]);
+            float dx, dy;
+            const float ph = valid ? phi_prof(s, i, j, dx, dy) : 0.0f;
+            if (ph > 0.0f) {
+                const float uxp = s.vx - s.w * dy, uyp = s.vy + s.w * dx;
+                if (p > best) {
+                    gx = ph * (uxp - u);
+                    gy = ph * (uyp - v);
+                    best = p;
                 }
+                const double rp = (double)r * (double)ph;
+                const double du = (double)(u - uxp), dv = (double)(v - uyp);
+                acc[0] = rp * du;
+                acc[1] = rp * dv;
+                acc[2] = rp * ((double)dx * dv - (double)dy * du);
             }
-            __syncthreads();
-            if (!last) return;
+        } else {
+            const double xc = uniform_f64(a.pin[PX * m + p]), yc = uniform_f64(a.pin[PY * m + p]);
+            const double vx = uniform_f64(a.pin[PVX * m + p]), vy = uniform_f64(a.pin[PVY * m + p]);
+            const double w = uniform_f64(a.pin[PW * m + p]);
+            const double dm = uniform_f64(a.pin[PDIAM * m + p]), zt = uniform_f64(a.pin[PZETA * m + p]);
+            const double ph = valid ? phi_at(xc, yc, dm, zt, i, j) : 0.0;
+            if (ph > 0.0) {
+                const double uxp = vx - w * ((double)j - yc);
+                const double uyp = vy + w * ((double)i - xc);
+                if (p > best) {
+                    gx = ph * (uxp - (double)u);
+                    gy = ph * (uyp - (double)v);
+                    best = p;
+                }
+                acc[0] = r * ph * (u - uxp);
+                acc[1] = r * ph * (v - uyp);
+                acc[2] = r * ph * (((double)i - xc) * (v - uyp) - ((double)j - yc) * (u - uxp));
+            }
         }
-        hist_write(a, p, F, FINAL ? nullptr : srec);  // srec is staged and published here
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double x = acc[k];
+#pragma unroll
+            for (int sft = 32; sft > 0; sft >>= 1) x = x + __shfl_down(x, sft, 64);
+            if (lane == 0) red[wv * 3 + k] = x;
+        }
+        __syncthreads();
         if (threadIdx.x == 0) {
-            if (FINAL) {
-                a.pout[PFX * m + p] = F[0]; a.pout[PFY * m + p] = F[1]; a.pout[PTQ * m + p] = F[2];
-            }
-        }
-        if (!FINAL) {
-            // contact force over all partners (one wavefront up to 64
-            // disks, the whole block beyond), then the update, which uses
-            // FORCE_k: written into the outgoing record first
-            __shared__ ContactLDS cl;
-            double fcx = 0.0, fcy = 0.0, dl = 0.0;
-            int cnt = 0;
-            if (P.np <= 64) {
-                // one wavefront suffices (the staged copy holds every disk)
-                if (threadIdx.x < 64)
-                    cnt = contact_wave(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw,
-                                       fcx, fcy, a.pairs + p * MAXNB, &dl);
+            if (rank < a.rcap) {
+                unsigned long long *o = a.fslot + ((size_t)p * a.rcap + rank) * SLOTW;
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    put_tagged_f64(o + 2 * k, a.tag, ((red[k] + red[3 + k]) + red[6 + k]) + red[9 + k]);
             } else {
-                cnt = staged
-                    ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl,
-                                    fcx, fcy, a.pairs + p * MAXNB, &dl)
-                    : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np,
-                                    p, sdpw, cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
+                set_err(a, LBM_EF_SLOT);
             }
-            if (threadIdx.x < 64) {
-                // the update by wavefront 0; FORCE_k and the contact force
-                // are final in thread 0 (lane 0): broadcast them
-                const double val =
-                    update_wave(P, srec, __shfl(F[0], 0), __shfl(F[1], 0), __shfl(F[2], 0),
-                                __shfl(fcx, 0), __shfl(fcy, 0), a.kk_upd);
-                if (threadIdx.x < PCOUNT) {
-                    a.pout[threadIdx.x * m + p] = val;
-                    srec[threadIdx.x] = val;
-                }
-                if (threadIdx.x == 0) {
-                    a.paircnt[p] = cnt;
-                    if (p == P.np - 1) *a.dpw_out = dl;
-                }
-            }
-            __syncthreads();
-            if (threadIdx.x == 0 && a.spf_out) make_spf(srec, a.spf_out + p * SPF);
-            if (P.np > NP_LOOP)
-                register_tiles(P, srec, p, a.gen_out, a.tl_hdr_out, a.tl_ids_out, a.err);
         }
+        __syncthreads();
     }
 }
 
-template <class St, bool DEV, bool FINAL, int BND>
-__device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
+// Is every node of this thread's wavefront an interior node away from the
+// edge rules (for equilibrium edges, also away from the nodes that feed the
+// edge arrays)?  Uniform over the wavefront.
+template <int BND>
+__device__ __forceinline__ bool wave_interior(const DevP &P, long long tile, int &jw) {
+    const long long w0 = tile * NT + (long long)__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+    const long long w1 = w0 + 63;
+    jw = 0;
+    if (w1 >= P.nn) return false;
+    const int bnd = BND >= 0 ? BND : P.boundary;
+    const int e = bnd == 2 ? 2 : 1;
+    const int j0 = row_of(P, (unsigned)w0), j1 = row_of(P, (unsigned)w1);
+    const int i0 = (int)(w0 - (long long)j0 * P.W), i1 = (int)(w1 - (long long)j1 * P.W);
+    jw = j0;
+    return j0 == j1 && j0 >= e && j0 <= P.ny - e && i0 >= e && i1 <= P.nx - e;
+}
+
+// One fused time step k over a tile, reduced-precision storage (FP32
+// collide): pull-stream + boundary rules (f_k), density and velocity, SPM
+// forcing G_k and the force partials, velocity correction, then (FINAL =
+// false) the collision of step k+1 stored as f*_{k+1}, or (FINAL = true) the
+// FP64 state of step k.  FORCED: a particle can reach the tile.
+template <class St, bool FINAL, int BND, bool FORCED>
+__device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile, int cnt,
+                                         unsigned mask, const int *ids) {
     const DevP &P = a.P;
-    long long n = tile * NT + threadIdx.x;
-    double mass = 0.0;
+    const float S = St::S, invS = St::invS;
+    const long long n = tile * NT + threadIdx.x;
+    const bool valid = n < P.nn;
+    const typename St::T *src = (const typename St::T *)a.src;
+    int jw;
+    const bool lean = wave_interior<BND>(P, tile, jw);
+    int i, j;
+    float H[9];
+    if (lean) {
+        j = jw;
+        i = (int)(n - (long long)jw * P.W);
+        load_interior_dev<St>(P, src, n, H);
+    } else {
+        const long long nc = valid ? n : P.nn - 1;
+        j = row_of(P, (unsigned)nc);
+        i = (int)(nc - (long long)j * P.W);
+        load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, nc, i, j, H);
+    }
+    const float om = (!FINAL && P.rheology != 0) ? a.rate32[valid ? n : P.nn - 1] : P.fomega;
+    const DevMom mo = dev_moments(H);
+    const float rho = P.frho0 + mo.rd * invS;
+    const float rinv = frcp(rho), rinvS = rinv * invS;
+    const float ux = mo.jx * rinvS, uy = mo.jy * rinvS;
+    if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
+    float gx = 0.0f, gy = 0.0f, ucx = ux, ucy = uy;
+    if constexpr (FORCED) {
+        tile_spm<true, float>(a, tile, cnt, mask, ids, i, j, valid, rho, ux, uy, gx, gy);
+        ucx = ux + 0.5f * gx;
+        ucy = uy + 0.5f * gy;
+    }
     int yielded = 0;
-    if (n < P.nn) {
-        int j = row_of(P, (unsigned)n), i = (int)(n - (long long)j * P.W);
-        const typename St::T *src = (const typename St::T *)a.src;
-        int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
-        if constexpr (DEV) {
-            float h[9], rd, rho, ux, uy;
-            load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, n, i, j, h);
-            float om = (!FINAL && P.rheology != 0) ? a.rate32[n] : P.fomega;
-            // particle test after the population loads are in flight
-            const bool any = tile_has_particles_f32(a, tile);
-            macros_dev(h, P.frho0, rd, rho, ux, uy);
-            float gx, gy;
-            spm_tile_node_f32(a, tile, any, i, j, ux, uy, gx, gy);
-            float ucx = ux + 0.5f * gx, ucy = uy + 0.5f * gy;
-            mass = (double)P.rho0 + (double)rd;
-            if (FINAL) {
-                const typename St::T *s0 = src;
+    if (valid) {
+        if (FINAL) {
+            const typename St::T *s0 = src;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    double w0 = (double)wdev(k) * P.rho0;
-                    a.cf[k * P.plane + n] = w0 + (double)h[k];
-                    a.cfp[k * P.plane + n] = w0 + (double)St::ld(s0 + k * P.plane + n);
-                }
-                a.crho[n] = mass; a.cux[n] = ucx; a.cuy[n] = ucy;
-                a.cg[n] = gx; a.cg[P.plane + n] = gy;
-                if (P.rheology != 0) a.cobn[n] = a.rate32[n];
-                if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
-            } else {
-                typename St::T *dst = (typename St::T *)a.dst;
-                float om2, str[3];
-                if (a.aux || a.slots_yield)
-                    collide_dev<true>(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
-                else
-                    collide_dev<false>(P, h, rd, rho, ucx, ucy, gx, gy, om, om2, str, yielded);
-                uint32_t nh = mix32((uint32_t)n ^ mix32(a.gen_out));
-                const auto drs = make_rsrc(dst, P.buf_bytes);
-#pragma unroll
-                for (int k = 0; k < 9; ++k)
-                    St::bst(drs, h[k], (unsigned)n * (unsigned)sizeof(typename St::T),
-                            (unsigned)(k * P.plane * (long long)sizeof(typename St::T)),
-                            sr_word(nh, k));
-                if (P.rheology != 0) a.rate32[n] = om2;
-                if (a.aux) {
-                    a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
-                    a.cstress[2 * P.plane + n] = str[2];
-                    if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
-                }
+            for (int k = 0; k < 9; ++k) {
+                double w0 = (double)wdev(k) * P.rho0;
+                a.cf[k * P.plane + n] = w0 + (double)H[k] * (double)invS;
+                a.cfp[k * P.plane + n] = w0 + (double)St::ld(s0 + k * P.plane + n) * (double)invS;
+            }
+            a.crho[n] = (double)P.rho0 + (double)mo.rd * (double)invS;
+            a.cux[n] = ucx; a.cuy[n] = ucy;
+            a.cg[n] = gx; a.cg[P.plane + n] = gy;
+            if (P.rheology != 0) a.cobn[n] = a.rate32[n];
+            if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
+        } else {
+            float om2, str[3];
+            if (a.aux || a.slots_yield)
+                collide_dev<FORCED, true>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2, str,
+                                          yielded);
+            else
+                collide_dev<FORCED, false>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2,
+                                           str, yielded);
+            store_dev<St>(P, a.dst, (unsigned)n, H, node_hash((unsigned)n, a.sr));
+            if (P.rheology != 0) a.rate32[n] = om2;
+            if (a.aux) {
+                a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
+                a.cstress[2 * P.plane + n] = str[2];
+                if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
+            }
+            if (!lean) {
                 if ((BND >= 0 ? BND : P.boundary) == 2) {
-                    if (b < 0) eq_put(a, P, i, j, rd, ucx, ucy);
-                } else if (b >= 0) {
-                    ((float *)a.erho_dst)[b] = rho;
+                    if (!(i == 0 || j == 0 || i == P.nx || j == P.ny))
+                        eq_put(a, P, i, j, mo.rd * invS, ucx, ucy);
+                } else if (i == 0 || j == 0 || i == P.nx || j == P.ny) {
+                    ((float *)a.erho_dst)[bnd_index(P, i, j)] = rho;
                 }
             }
+        }
+    }
+    if (a.slots_mass || (!FINAL && a.slots_yield)) {
+        const double mass = valid ? (double)P.rho0 + (double)mo.rd * (double)invS : 0.0;
+        tile_monitors<true>(a, tile, mass, yielded, a.slots_mass != nullptr,
+                            !FINAL && a.slots_yield != nullptr);
+    }
+}
+
+// The same step with FP64 storage, in the reference's operation order.
+template <class St, bool FINAL, int BND, bool FORCED>
+__device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile, int cnt,
+                                         unsigned mask, const int *ids) {
+    const DevP &P = a.P;
+    const long long n = tile * NT + threadIdx.x;
+    const bool valid = n < P.nn;
+    const long long nc = valid ? n : P.nn - 1;
+    const int j = row_of(P, (unsigned)nc), i = (int)(nc - (long long)j * P.W);
+    const typename St::T *src = (const typename St::T *)a.src;
+    double f[9], rho, ux, uy;
+    load_fk64<St, BND>(P, src, (const double *)a.erho_src, nc, i, j, f);
+    macros64(f, rho, ux, uy);
+    if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
+    double gx = 0.0, gy = 0.0;
+    if constexpr (FORCED) tile_spm<false, double>(a, tile, cnt, mask, ids, i, j, valid, rho, ux, uy, gx, gy);
+    const double ucx = ux + 0.5 * 1.0 * (gx + 0.0), ucy = uy + 0.5 * 1.0 * (gy + 0.0);
+    int yielded = 0;
+    if (valid) {
+        if (FINAL) {
+            double *cfk = (double *)a.dst;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) cfk[k * P.plane + n] = f[k];
+            a.crho[n] = rho; a.cux[n] = ucx; a.cuy[n] = ucy;
+            a.cg[n] = gx; a.cg[P.plane + n] = gy;
+            if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
         } else {
-            double f[9], rho, ux, uy;
-            load_fk64<St, BND>(P, src, (const double *)a.erho_src, n, i, j, f);
-            const bool any = tile_has_particles(a, tile);
-            macros64(f, rho, ux, uy);
-            double gx, gy;
-            spm_tile_node(a, tile, any, i, j, ux, uy, gx, gy);
-            double ucx = ux + 0.5 * 1.0 * (gx + 0.0), ucy = uy + 0.5 * 1.0 * (gy + 0.0);
-            mass = rho;
-            if (FINAL) {
-                double *cfk = (double *)a.dst;
+            typename St::T *dst = (typename St::T *)a.dst;
+            CollideOut o;
+            double om = P.rheology != 0 ? a.cobn[n] : P.omega;
+            collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) cfk[k * P.plane + n] = f[k];
-                a.crho[n] = rho; a.cux[n] = ucx; a.cuy[n] = ucy;
-                a.cg[n] = gx; a.cg[P.plane + n] = gy;
-                if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
-            } else {
-                typename St::T *dst = (typename St::T *)a.dst;
-                CollideOut o;
-                double om = P.rheology != 0 ? a.cobn[n] : P.omega;
-                collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
-                if (P.rheology != 0) a.cobn[n] = o.omega_bn;
-                yielded = o.yielded;
-                if (a.aux) {
-                    a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
-                    a.cstress[2 * P.plane + n] = o.stress[2];
-                    if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
-                }
-                if ((BND >= 0 ? BND : P.boundary) == 2) {
-                    if (b < 0) eq_put(a, P, i, j, rho, ucx, ucy);
-                } else if (b >= 0) {
-                    ((double *)a.erho_dst)[b] = rho;
-                }
+            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
+            if (P.rheology != 0) a.cobn[n] = o.omega_bn;
+            yielded = o.yielded;
+            if (a.aux) {
+                a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
+                a.cstress[2 * P.plane + n] = o.stress[2];
+                if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
+            }
+            const int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
+            if ((BND >= 0 ? BND : P.boundary) == 2) {
+                if (b < 0) eq_put(a, P, i, j, rho, ucx, ucy);
+            } else if (b >= 0) {
+                ((double *)a.erho_dst)[b] = rho;
             }
         }
     }
     if (a.slots_mass || (!FINAL && a.slots_yield))
-        tile_monitors<DEV>(a, tile, mass, yielded, a.slots_mass != nullptr,
-                      !FINAL && a.slots_yield != nullptr);
+        tile_monitors<false>(a, tile, valid ? rho : 0.0, yielded, a.slots_mass != nullptr,
+                             !FINAL && a.slots_yield != nullptr);
+}
+
+template <class St, bool DEV, bool FINAL, int BND>
+__device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
+    unsigned mask;
+    const int *ids;
+    const int cnt = tile_candidates(a, tile, mask, ids);
+    if constexpr (DEV) {
+        if (cnt > 0) tile_dev<St, FINAL, BND, true>(a, tile, cnt, mask, ids);
+        else tile_dev<St, FINAL, BND, false>(a, tile, 0, 0u, nullptr);
+    } else {
+        if (cnt > 0) tile_f64<St, FINAL, BND, true>(a, tile, cnt, mask, ids);
+        else tile_f64<St, FINAL, BND, false>(a, tile, 0, 0u, nullptr);
+    }
+}
+
+// Block b -> tile of a step launch.  With up to NP_LOOP particles, the
+// tiles of the range their reach boxes span come first: they carry the
+// particle work (the particle blocks wait for their partials), start early,
+// and the round-robin dispatch spreads them over all XCDs; the other tiles
+// follow in one contiguous range per XCD.  A range over more than a quarter
+// of the grid (particles spread out) leaves the plain per-XCD order.  A
+// bijection of [0, ntiles), uniform over the block.
+__device__ __forceinline__ long long step_tile_of(const StepArgs &a, long long b) {
+    const DevP &P = a.P;
+    const long long T = a.ntiles;
+    if (P.np == 0 || P.np > NP_LOOP) return xcd_tile(b, T);
+    long long lo = T, hi = -1;
+    for (int p = 0; p < P.np; ++p) {
+        const int *w = a.pbox_in + 4 * p;
+        const int i0 = __builtin_amdgcn_readfirstlane(w[0]), i1 = __builtin_amdgcn_readfirstlane(w[1]);
+        const int j0 = __builtin_amdgcn_readfirstlane(w[2]), j1 = __builtin_amdgcn_readfirstlane(w[3]);
+        if (i0 > i1 || j0 > j1) continue;
+        lo = min(lo, ((long long)j0 * P.W + i0) / NT);
+        hi = max(hi, ((long long)j1 * P.W + i1) / NT);
+    }
+    const long long R = hi - lo + 1;
+    if (R <= 0 || 4 * R > T) return xcd_tile(b, T);
+    if (b < R) return lo + b;
+    const long long q = xcd_tile(b - R, T - R);
+    return q < lo ? q : q + R;
 }
 
 template <class St, bool DEV, bool FINAL, int BND>
 __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
     if ((int)blockIdx.x < a.n_pblocks) {
-        particle_block<St, DEV, FINAL, BND>(a, (int)blockIdx.x);
+        particle_block<DEV, FINAL>(a, (int)blockIdx.x);
+        if (!FINAL) particle_register(a, (int)blockIdx.x);
         return;
     }
     if ((int)blockIdx.x < a.n_pblocks + a.nred) {
         monitor_reduce(a.mpart_in, a.ntiles, a.red_mass, a.red_yield);
         return;
     }
-    step_tile<St, DEV, FINAL, BND>(a, xcd_tile(blockIdx.x - a.n_pblocks - a.nred, a.ntiles));
+    step_tile<St, DEV, FINAL, BND>(a, step_tile_of(a, blockIdx.x - a.n_pblocks - a.nred));
 }
 
 // ================================================================= batch
@@ -1100,7 +1253,7 @@ __global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *__restric
     const int2 m = bmap[blockIdx.x];
     const StepArgs &a = args[m.x];
     if (m.y < a.n_pblocks) {
-        if (m.y % a.nsub == 0) particle_advance(a, m.y / a.nsub);
+        particle_advance<DEV>(a, m.y);
         return;
     }
     if (m.y < a.n_pblocks + a.nred) return;  // nothing before the prologue
@@ -1117,7 +1270,8 @@ __global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ 
     const int2 m = bmap[blockIdx.x];
     const StepArgs &a = args[m.x];
     if (m.y < a.n_pblocks) {
-        particle_block<St, DEV, FINAL, BND>(a, m.y);
+        particle_block<DEV, FINAL>(a, m.y);
+        if (!FINAL) particle_register(a, m.y);
         return;
     }
     if (m.y < a.n_pblocks + a.nred) {
