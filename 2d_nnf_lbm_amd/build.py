@@ -4,6 +4,7 @@ The library is compiled with contraction off (-ffp-contract=off) so the FP64
 path keeps the reference's rounding, and with packed FP32 instructions
 disabled.
 """
+import glob
 import os
 import subprocess
 
@@ -11,8 +12,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblbm_amd.so")
 SOURCES = [os.path.join(CSRC, "lbm_amd.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("lbm_device.h", "lbm_kernels.h")] + [
-    os.path.join(os.path.dirname(HERE), "include", "lbm_amd.h")]
+# every source and header the library is built from (a change to any of
+# them makes the built library stale)
+DEPS = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.h")) +
+              glob.glob(os.path.join(os.path.dirname(HERE), "include", "*.h")))
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -61,8 +61,10 @@ struct lbm_handle {
     // states (same parity as pbuf), tagged force partials, slots per particle
     int *pbox[2] = {nullptr, nullptr};
     float *spf[2] = {nullptr, nullptr};
-    unsigned long long *fslot = nullptr;
-    int rcap = 0;
+    unsigned long long *fslot = nullptr;  // chunk force partials [maxp][kpart][SLOTW]
+    size_t fslot_bytes = 0;
+    unsigned *pcount = nullptr;           // chunks done per particle (zero between launches)
+    int kpart = 1;                        // helper blocks per particle
     // launch tags never repeat on a handle (tile-list generations and
     // tagged words); sr_base is the stochastic-rounding sequence position,
     // restarted by lbm_initialize and carried by snapshots
@@ -235,6 +237,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->scalar, 8)) ||
         (rc = al((void **)&h->err_flag, 4)) || (rc = al((void **)&h->lag, 8)) ||
         (rc = al((void **)&h->pbox[0], (size_t)P.maxp * 4 * 4)) ||
+        (rc = al((void **)&h->pcount, (size_t)P.maxp * 4)) ||
         (rc = al((void **)&h->pbox[1], (size_t)P.maxp * 4 * 4)) ||
         (rc = al((void **)&h->spf[0], (size_t)P.maxp * SPF * 4)) ||
         (rc = al((void **)&h->spf[1], (size_t)P.maxp * SPF * 4)) ||
@@ -298,7 +301,7 @@ int lbm_destroy(lbm_handle *h) {
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
-                    h->pbox[0], h->pbox[1], h->spf[0], h->spf[1], h->fslot, h->lag, h->wtab,
+                    h->pbox[0], h->pbox[1], h->spf[0], h->spf[1], h->fslot, h->pcount, h->lag, h->wtab,
                     h->mpart[0], h->mpart[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -437,20 +440,20 @@ int lbm_set_particles(lbm_handle *h, int32_t n, const double *part) {
     HIPCHECK(h, hipMemcpyAsync(h->pbuf[h->pcur], buf.data(), buf.size() * 8,
                                hipMemcpyHostToDevice, h->stream));
     int PS = (int)std::ceil(2.0 * reach) + 2;
-    // force partial slots per particle: the tiles of a reach box (rows x
-    // the tiles one row segment can touch)
+    // helper blocks per particle: NT-node chunks of the largest reach box
+    // (at most `side` nodes across, clipped to the grid)
     {
         const long long side = 2LL * (long long)std::ceil(reach) + 1;
-        const long long rows = std::min<long long>(side, P.H);
-        const long long per_row = std::min<long long>(side, P.W) / NT + 2;
-        const int rcap = (int)std::min<long long>(rows * per_row, (P.nn + NT - 1) / NT);
-        if (n > 0 && rcap > h->rcap) {
+        const long long nodes = std::min<long long>(side, P.W) * std::min<long long>(side, P.H);
+        h->kpart = n > 0 ? (int)((nodes + NT - 1) / NT) : 1;
+        const size_t bytes = (size_t)P.maxp * h->kpart * SLOTW * 8;
+        if (n > 0 && bytes > h->fslot_bytes) {
             if (h->fslot) (void)hipFree(h->fslot);
             h->fslot = nullptr;
-            const size_t bytes = (size_t)P.maxp * rcap * SLOTW * 8;
+            h->fslot_bytes = 0;
             HIPCHECK(h, hipMalloc((void **)&h->fslot, bytes));
             HIPCHECK(h, hipMemsetAsync(h->fslot, 0, bytes, h->stream));
-            h->rcap = rcap;
+            h->fslot_bytes = bytes;
         }
     }
     if (n > 0 && PS > h->PS) {
@@ -939,7 +942,9 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
     a.cf = h->f; a.cfp = h->fp; a.crho = h->rho; a.cux = h->ux; a.cuy = h->uy; a.cg = h->g;
     a.cobn = h->obn; a.cstress = h->stress; a.cimage = h->image; a.rate32 = h->rate32;
     a.pairs = h->pairs; a.paircnt = h->paircnt;
-    a.n_pblocks = np;
+    a.n_pblocks = np * h->kpart;
+    a.kpart = h->kpart;
+    a.pcount = h->pcount;
     a.ntiles = (P.nn + NT - 1) / NT;
     a.err = h->err_flag;
     a.hist_base = record ? h->hist_part : nullptr;
@@ -949,7 +954,6 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
     a.tag0 = h->tag_base;
     a.sr0 = h->sr_base;
     a.tpb = 1;
-    a.rcap = h->rcap;
     a.mpart[0] = h->mpart[0]; a.mpart[1] = h->mpart[1];
     a.nred = (record && s > 0) ? 1 : 0;  // a reducer block when there is something to reduce
     // tile lists: read by this launch's tiles (generation = its tag), written
@@ -1046,7 +1050,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
               double *ms_total, double *ms_main) {
     const DevP &P = h->P;
     const long long ntiles = (P.nn + NT - 1) / NT;
-    const int n_pblocks = P.np;
+    const int n_pblocks = P.np * h->kpart;
     // the step launches of a recording call carry one reducer block
     dim3 grid((unsigned)(n_pblocks + ntiles)), block(NT);
     dim3 sgrid((unsigned)(n_pblocks + (record ? 1 : 0) + ntiles));
@@ -1197,7 +1201,7 @@ struct lbm_batch {
     int2 *bmap = nullptr;                 // block -> (simulation, block within it)
     size_t bmap_cap = 0, nblocks = 0;
     bool map_dirty = true;
-    std::vector<long long> map_key;       // (particles, blocks per particle, tiles per block) the map was built for
+    std::vector<long long> map_key;       // (particles, helper blocks per particle, tiles per block) the map was built for
     StepArgs *dargs = nullptr;            // [4][n]: prologue, odd steps, even steps, final
     int *sdev = nullptr;                  // first step index of a replayed chunk
     int chunk = 0;                        // graph chunk length (even), 0: no graph
@@ -1237,29 +1241,26 @@ int build_map(lbm_batch *b, const std::vector<int> &which) {
     std::vector<int2> map;
     for (int i : which) {
         const lbm_handle *h = b->sims[i];
-        const int np = h->P.np;
-        const int npb = np;
+        const int npb = h->P.np * h->kpart;
         for (int k = 0; k < npb; ++k) map.push_back(make_int2(i, k));
     }
     std::vector<int2> tiles;
     for (int i : which) {
         const lbm_handle *h = b->sims[i];
-        const int np = h->P.np;
-        const int npb = np;
+        const int npb = h->P.np * h->kpart;
         const long long ntiles = (h->P.nn + NT - 1) / NT;
         const long long tb = (ntiles + b->tpb[i] - 1) / b->tpb[i];
         // block npb of each simulation is its monitor reducer (idle unless recording)
         tiles.push_back(make_int2(i, npb));
         for (long long k = 0; k < tb; ++k) tiles.push_back(make_int2(i, npb + 1 + (int)k));
     }
+    // particle helper blocks first, then the tile blocks in one contiguous
+    // range per XCD
     const long long T = (long long)tiles.size();
     const size_t base = map.size();
     map.resize(base + tiles.size());
-    for (long long g = 0; g < T; ++g) {
-        long long x = g & 7, per = T >> 3, rem = T & 7;
-        long long c = x * per + (x < rem ? x : rem) + (g >> 3);
-        map[base + g] = tiles[c];
-    }
+    for (long long g = 0; g < T; ++g)
+        map[base + g] = tiles[xcd_rank((long long)base + g, (long long)base, T)];
     if (map.size() > b->bmap_cap) {
         if (b->bmap) (void)hipFree(b->bmap);
         b->bmap = nullptr;
@@ -1374,10 +1375,13 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
     }
     BHIPCHECK(b, hipMemcpyAsync(b->dargs, args.data(), args.size() * sizeof(StepArgs),
                                 hipMemcpyHostToDevice, b->stream));
-    // the map depends on each simulation's particle count and tiling
+    // the map depends on each simulation's particle count, helper blocks per
+    // particle and tiling
     std::vector<long long> key;
-    for (size_t i = 0; i < n; ++i)
-        key.push_back(((long long)b->sims[i]->P.np << 24) ^ b->tpb[i]);
+    for (size_t i = 0; i < n; ++i) {
+        key.push_back(b->sims[i]->P.np);
+        key.push_back(((long long)b->sims[i]->kpart << 32) | (unsigned)b->tpb[i]);
+    }
     if (b->map_dirty || !all || key != b->map_key) {
         int rc = build_map(b, which);
         if (rc) return rc;

@@ -551,54 +551,24 @@ __device__ __forceinline__ Box reach_box(const DevP &P, double x, double y, doub
     return b;
 }
 
-// The 256-node tiles holding nodes of a box, in ascending order, each once:
-// row j of the box lies in tiles [(jW + i0)/NT, (jW + i1)/NT]; a tile already
-// met in row j-1 is not counted again (rows only move forward).  The force
-// on a particle is summed over these tiles in this order, so the tile that
-// writes a partial and the block that sums them agree on its slot.
-// box_tile_rank: position of tile t in that sequence, or -1 when t holds no
-// node of the box.
-__device__ __forceinline__ int box_tile_rank(const DevP &P, const Box &b, long long t) {
-    int cnt = 0;
-    long long last = -1;
-    for (int j = b.j0; j <= b.j1; ++j) {
-        long long lo = ((long long)j * P.W + b.i0) / NT, hi = ((long long)j * P.W + b.i1) / NT;
-        if (lo <= last) lo = last + 1;
-        if (lo > t) return -1;
-        if (hi >= lo) {
-            if (t <= hi) return cnt + (int)(t - lo);
-            cnt += (int)(hi - lo + 1);
-            last = hi;
-        }
-    }
-    return -1;
-}
-__device__ __forceinline__ int box_tile_count(const DevP &P, const Box &b) {
-    int cnt = 0;
-    long long last = -1;
-    for (int j = b.j0; j <= b.j1; ++j) {
-        long long lo = ((long long)j * P.W + b.i0) / NT, hi = ((long long)j * P.W + b.i1) / NT;
-        if (lo <= last) lo = last + 1;
-        if (hi >= lo) {
-            cnt += (int)(hi - lo + 1);
-            last = hi;
-        }
-    }
-    return cnt;
-}
-
 // Does tile t hold a node of the box [i0, i1] x [j0, j1] (bounds may lie
 // outside the grid)?  Uniform over the block.
 __device__ __forceinline__ bool tile_meets_box(const DevP &P, long long t, int i0, int i1,
                                                int j0, int j1) {
-    const long long n0 = t * NT, n1 = min(n0 + NT - 1, P.nn - 1);
-    const int jl = (int)(n0 / P.W), jh = (int)(n1 / P.W);
-    for (int j = max(jl, j0); j <= min(jh, j1); ++j) {
-        const long long r0 = (long long)j * P.W;
-        const long long a = max(n0 - r0, (long long)i0), b = min(n1 - r0, (long long)i1);
-        if (a <= b) return true;
-    }
-    return false;
+    // the tile holds nodes n0..n1, rows jl..jh: a full row between its first
+    // and last row meets any box column range; otherwise the (at most two)
+    // partial rows in the box's row range are tested
+    const unsigned n0 = (unsigned)(t * NT), n1 = (unsigned)min(t * NT + NT - 1, P.nn - 1);
+    const int jl = row_of(P, n0), jh = row_of(P, n1);
+    const int ja = max(jl, j0), jb = min(jh, j1);
+    if (ja > jb || i0 > i1) return false;
+    if (jb - ja >= 2) return true;
+    auto seg = [&](int j) {
+        const int lo = j == jl ? (int)(n0 - (unsigned)j * (unsigned)P.W) : 0;
+        const int hi = j == jh ? (int)(n1 - (unsigned)j * (unsigned)P.W) : P.W - 1;
+        return max(lo, i0) <= min(hi, i1);
+    };
+    return seg(ja) || seg(jb);
 }
 
 // Force partials cross between blocks of one launch as generation-tagged
@@ -610,6 +580,35 @@ __device__ __forceinline__ bool tile_meets_box(const DevP &P, long long t, int i
 // a handle.  A partial (force x, y, torque) is 6 words: hi and lo halves.
 constexpr int SLOTW = 6;
 constexpr int TAG_SPIN = 1 << 16;  // bounded wait for tagged words (tens of ms)
+
+// Sum of x over the 64 lanes of a wavefront by DPP lane moves, in a fixed
+// order: pairs, quads, half rows, rows (each lane of a group ends with the
+// group's sum), then row 1 += row 0 and row 3 += row 2, then rows 2 and 3
+// += the sum of rows 0 and 1.  The total is valid in lane 63.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double wave_sum_f64(double x) {
+    x = x + dpp_f64<0xB1, 0xF>(x);   // quad_perm [1, 0, 3, 2]
+    x = x + dpp_f64<0x4E, 0xF>(x);   // quad_perm [2, 3, 0, 1]
+    x = x + dpp_f64<0x141, 0xF>(x);  // row_half_mirror
+    x = x + dpp_f64<0x140, 0xF>(x);  // row_mirror
+    x = x + dpp_f64<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+    x = x + dpp_f64<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
+// A load through the constant address space: data that no block writes
+// during the launch, at a wavefront-uniform address, is read with scalar
+// loads that need no wait for the vector memory pipe.
+template <class T>
+__device__ __forceinline__ T cload(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
 
 __device__ __forceinline__ void put_tagged_f64(unsigned long long *w, unsigned tag, double v) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);

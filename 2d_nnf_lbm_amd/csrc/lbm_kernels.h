@@ -10,12 +10,22 @@
 namespace lbm {
 
 // Tiles are handed to the 8 XCDs in contiguous ranges so that rows shared
-// between neighbouring tiles meet in the same L2 (blocks are dispatched to
-// XCDs round-robin by block index).  A bijection of [0, T).
-__device__ __forceinline__ long long xcd_tile(long long b, long long T) {
-    long long x = b & 7, per = T >> 3, rem = T & 7;
-    long long start = x * per + (x < rem ? x : rem);
-    return start + (b >> 3);
+// between neighbouring tiles meet in the same L2.  Blocks are dispatched to
+// the XCDs round-robin by block index (block g runs on XCD g % 8).  For the
+// m blocks base .. base + m - 1 of a launch: the position of block g in the
+// order that gives XCD 0 the first n_0 positions, XCD 1 the next n_1, ...
+// (n_x: how many of those blocks XCD x runs), each XCD's blocks in dispatch
+// order.  A bijection of [0, m).
+__host__ __device__ __forceinline__ long long xcd_rank(long long g, long long base, long long m) {
+    const int x = (int)(g & 7);
+    const int b8 = (int)(base & 7);
+    long long start = 0;
+    for (int y = 0; y < x; ++y) {
+        const int r = (y - b8) & 7;  // first offset from base that XCD y runs
+        start += r < m ? (m - r + 7) >> 3 : 0;
+    }
+    const long long j = g - base;
+    return start + ((j - ((x - b8) & 7)) >> 3);
 }
 
 // =============================================================== drop-in
@@ -408,8 +418,9 @@ struct StepArgs {
     float *spf_out;               // those of state k+1
     const int *pbox_in;           // reach boxes of state k [maxp][4]: i0, i1, j0, j1
     int *pbox_out;                // those of state k+1
-    unsigned long long *fslot;    // tagged force partials [maxp][rcap][SLOTW]
-    int rcap;                     // partial slots per particle
+    unsigned long long *fslot;    // tagged chunk force partials [maxp][kpart][SLOTW]
+    unsigned *pcount;             // per particle: chunks done in this launch (wraps to 0)
+    int kpart;                    // helper blocks (box chunks of NT nodes) per particle
     const double *dpw_in;
     double *dpw_out;
     int *pairs, *paircnt;
@@ -453,7 +464,7 @@ enum {
     LBM_EF_LIST = 1,      // a tile's candidate list overflowed
     LBM_EF_NAN = 2,       // a non-finite density or particle state
     LBM_EF_WAIT = 4,      // force partials did not appear within the wait bound
-    LBM_EF_SLOT = 16,     // a reach box holds more tiles than its partial slots
+    LBM_EF_SLOT = 16,     // a reach box holds more nodes than its helper blocks cover
 };
 
 // Per-step fields of launch s of a call of nsteps fused steps (s = 0: the
@@ -487,6 +498,13 @@ __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nst
 
 __device__ __forceinline__ void set_err(const StepArgs &a, int bits) { atomicOr(a.err, bits); }
 
+// A 64-bit value every lane holds alike, as a wavefront-uniform one.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // history row of one particle: record fields of the state, its FORCE, F_H_IN
 __device__ __forceinline__ void hist_row(double *h, const double *rec, const double F[3]) {
     h[0] = rec[PX]; h[1] = rec[PY]; h[2] = rec[PANG];
@@ -499,143 +517,6 @@ __device__ __forceinline__ void hist_row(double *h, const double *rec, const dou
 // (i0, i1, j0, j1; empty when i0 > i1).
 __device__ __forceinline__ void put_box(int *dst, const Box &b) {
     dst[0] = b.i0; dst[1] = b.i1; dst[2] = b.j0; dst[3] = b.j1;
-}
-
-// FORCE and TORQUE on particle p (ComputeForce_HYDRO, F6:542-556) from the
-// tagged partials of the cnt tiles of its reach box, by one wavefront: lane
-// l takes slots l, l + 64, ... in order, waiting (bounded) until their words
-// carry this launch's tag; a fixed shuffle tree then adds the lanes.  Valid
-// in every lane; false when the wait ran out.
-__device__ __forceinline__ bool sum_force_slots(const StepArgs &a, int p, int cnt, double F[3]) {
-    const int lane = threadIdx.x & 63;
-    const unsigned long long *q = a.fslot + (size_t)p * a.rcap * SLOTW;
-    double acc[3] = {0.0, 0.0, 0.0};
-    bool ok = true;
-    for (int r = lane; r < cnt; r += 64) {
-        const unsigned long long *w = q + (size_t)r * SLOTW;
-        unsigned long long v[SLOTW];
-        int it = 0;
-        for (;;) {
-            bool ready = true;
-#pragma unroll
-            for (int k = 0; k < SLOTW; ++k) {
-                v[k] = __hip_atomic_load(w + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ready = ready && (unsigned)(v[k] >> 32) == a.tag;
-            }
-            if (ready) break;
-            if (++it >= TAG_SPIN) { ok = false; break; }
-            __builtin_amdgcn_s_sleep(2);
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            acc[c] = acc[c] + __builtin_bit_cast(double, (v[2 * c] << 32) | (v[2 * c + 1] & 0xffffffffull));
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double x = acc[c];
-#pragma unroll
-        for (int sft = 32; sft > 0; sft >>= 1) x = x + __shfl_xor(x, sft, 64);
-        F[c] = x;
-    }
-    return __all(ok);
-}
-
-// Particle block of launch s >= 1 (all NT threads): contact force on state
-// k (F3:739-804) first, FORCE_k from the tiles' partials, the history row of
-// step k, then the update to state k+1 (the final launch writes FORCE_k into
-// the record instead).  Every input is loaded into registers before any is
-// written to LDS: a load that feeds an LDS store inside a branch is waited
-// for there.
-template <bool DEV, bool FINAL>
-__device__ void particle_block(const StepArgs &a, int p) {
-    const DevP &P = a.P;
-    const int m = P.maxp;
-    static_assert(NP_LDS <= NT, "one staged particle per thread");
-    __shared__ double srec[PCOUNT];
-    __shared__ double scon[3 * NP_LDS];
-    __shared__ double sdpw;
-    __shared__ ContactLDS cl;
-    __shared__ int scnt;
-    const int t = threadIdx.x;
-    const bool staged = P.np <= NP_LDS;
-    const int tr = min(t, PCOUNT - 1), tc = min(t, max(P.np - 1, 0));
-    const double r_rec = a.pin[tr * m + p];
-    const double r_x = a.pin[PX * m + tc], r_y = a.pin[PY * m + tc], r_d = a.pin[PDIAM * m + tc];
-    const double r_dpw = *a.dpw_in;
-    if (t < PCOUNT) srec[t] = r_rec;
-    if (staged && t < P.np) {
-        scon[t] = r_x; scon[NP_LDS + t] = r_y; scon[2 * NP_LDS + t] = r_d;
-    }
-    if (t == 0) sdpw = r_dpw;
-    __syncthreads();
-    // contact force on state k: it does not depend on FORCE_k, so it runs
-    // while the tiles are still producing the partials
-    double fcx = 0.0, fcy = 0.0, dl = 0.0;
-    int cnt = 0;
-    if (!FINAL) {
-        if (P.np <= 64) {
-            if (t < 64)
-                cnt = contact_wave(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, fcx,
-                                   fcy, a.pairs + p * MAXNB, &dl);
-        } else {
-            cnt = staged
-                ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl, fcx,
-                                fcy, a.pairs + p * MAXNB, &dl)
-                : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
-                                sdpw, cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
-        }
-    }
-    if (t == 0) {
-        const Box b = reach_box(P, srec[PX], srec[PY], srec[PDIAM], srec[PZETA]);
-        scnt = (b.i0 > b.i1 || b.j0 > b.j1) ? 0 : min(box_tile_count(P, b), a.rcap);
-    }
-    __syncthreads();
-    if (t >= 64) return;
-    double F[3];
-    if (!sum_force_slots(a, p, scnt, F) && t == 0) set_err(a, LBM_EF_WAIT);
-    if (a.hist_part && t == 0) hist_row(a.hist_part + (size_t)p * HPC, srec, F);
-    if (FINAL) {
-        if (t == 0) {
-            a.pout[PFX * m + p] = F[0]; a.pout[PFY * m + p] = F[1]; a.pout[PTQ * m + p] = F[2];
-            if (!__builtin_isfinite(F[0] + F[1] + F[2])) set_err(a, LBM_EF_NAN);
-        }
-        return;
-    }
-    // the update by wavefront 0; the contact force is final in lane 0
-    __shared__ double snew[PCOUNT];
-    const double val = update_wave(P, srec, F[0], F[1], F[2], __shfl(fcx, 0), __shfl(fcy, 0),
-                                   a.kk_upd);
-    if (t < PCOUNT) {
-        a.pout[t * m + p] = val;
-        snew[t] = val;
-    }
-    if (t == 0) {
-        a.paircnt[p] = cnt;
-        if (p == P.np - 1) *a.dpw_out = dl;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (t == 0) {
-        const Box nb = reach_box(P, snew[PX], snew[PY], snew[PDIAM], snew[PZETA]);
-        put_box(a.pbox_out + 4 * p, nb);
-        if (a.spf_out) make_spf(snew, a.spf_out + p * SPF);
-        bool bad = false;
-        for (int f = 0; f < PCOUNT; ++f) bad = bad || !__builtin_isfinite(snew[f]);
-        if (bad) set_err(a, LBM_EF_NAN);
-    }
-}
-
-// Tile registration of the new state of particle p for the next launch
-// (many particles), by all NT threads of its block after particle_block.
-__device__ __forceinline__ void particle_register(const StepArgs &a, int p) {
-    const DevP &P = a.P;
-    if (P.np <= NP_LOOP) return;
-    __syncthreads();
-    const int m = P.maxp;
-    const Box nb = reach_box(P, a.pout[PX * m + p], a.pout[PY * m + p], a.pout[PDIAM * m + p],
-                             a.pout[PZETA * m + p]);
-    register_tiles(P, nb, p, a.tag + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
 // Prologue particle block: contact force on state kk0, the update with the
@@ -863,11 +744,12 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
 template <class St, bool DEV>
 __global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
     if ((int)blockIdx.x < a.n_pblocks) {
-        particle_advance<DEV>(a, (int)blockIdx.x);
+        // the first helper slot of each particle advances it
+        if ((int)blockIdx.x % a.kpart == 0) particle_advance<DEV>(a, (int)blockIdx.x / a.kpart);
         return;
     }
     if ((int)blockIdx.x < a.n_pblocks + a.nred) return;  // nothing before the prologue
-    prologue_tile<St, DEV>(a, xcd_tile(blockIdx.x - a.n_pblocks - a.nred, a.ntiles));
+    prologue_tile<St, DEV>(a, xcd_rank(blockIdx.x, a.n_pblocks + a.nred, a.ntiles));
 }
 
 // DOMAIN_IMAGE from the particles alone (F6:438, F6:460-467).
@@ -892,133 +774,128 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
 }
 
 // ---------------------------------------------------------------- tiles
-// Which particles can reach tile t in this launch?  Up to NP_LOOP particles:
-// a bit per particle whose reach box of state k meets the tile; beyond: the
-// tile's candidate list of this launch.  Uniform over the block.
-__device__ __forceinline__ int tile_candidates(const StepArgs &a, long long t, unsigned &mask,
-                                               const int *&ids) {
+// Which particles can reach tile t in this launch?  In two parts, so that
+// the loads they need are in flight together with the tile's population
+// loads: cand_prefetch issues them (up to NP_LOOP particles: lane l loads
+// particle l's reach box of state k; beyond: the tile's candidate list
+// header of this launch), cand_resolve turns them into the candidates (a bit
+// per particle whose box meets the tile, or the list).  Uniform over the
+// block.
+struct CandPre {
+    int4 box;
+    unsigned long long hdr;
+};
+// (The loads are unconditional, from valid addresses, so that no branch
+// makes the wavefront wait for them before the population loads.)
+__device__ __forceinline__ CandPre cand_prefetch(const StepArgs &a, long long t) {
+    CandPre c;
+    const int np = a.P.np, lane = threadIdx.x & 63;
+    c.box = *(const int4 *)(a.pbox_in + 4 * min(lane, max(np - 1, 0)));
+    // a vector load (a scalar one would be waited for with the kernel
+    // argument loads, before the population loads)
+    c.hdr = np > NP_LOOP ? __hip_atomic_load(a.tl_hdr_in + t, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP)
+                         : 0ull;
+    return c;
+}
+__device__ __forceinline__ int cand_resolve(const StepArgs &a, long long t, const CandPre &c,
+                                            unsigned &mask, const int *&ids) {
     const DevP &P = a.P;
     mask = 0u;
     ids = nullptr;
     if (P.np == 0) return 0;
     if (P.np <= NP_LOOP) {
-        for (int p = 0; p < P.np; ++p) {
-            const int *w = a.pbox_in + 4 * p;
-            const int i0 = __builtin_amdgcn_readfirstlane(w[0]), i1 = __builtin_amdgcn_readfirstlane(w[1]);
-            const int j0 = __builtin_amdgcn_readfirstlane(w[2]), j1 = __builtin_amdgcn_readfirstlane(w[3]);
-            if (i0 <= i1 && j0 <= j1 && tile_meets_box(P, t, i0, i1, j0, j1)) mask |= 1u << p;
-        }
+        const bool hit = (int)(threadIdx.x & 63) < P.np && c.box.x <= c.box.y &&
+                         c.box.z <= c.box.w &&
+                         tile_meets_box(P, t, c.box.x, c.box.y, c.box.z, c.box.w);
+        mask = (unsigned)__ballot(hit);
         return __builtin_popcount(mask);
     }
-    const unsigned long long h = a.tl_hdr_in[t];
+    const unsigned long long h = uniform_u64(c.hdr);
     if ((unsigned)(h >> 32) != a.tag) return 0;
     ids = a.tl_ids_in + t * MAXC;
     return min((int)(h & 0xffffffffu), MAXC);
 }
+__device__ __forceinline__ int tile_candidates(const StepArgs &a, long long t, unsigned &mask,
+                                               const int *&ids) {
+    return cand_resolve(a, t, cand_prefetch(a, t), mask, ids);
+}
 
 // A value every lane holds alike, as a wavefront-uniform one.
 __device__ __forceinline__ int uniform_i32(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ float uniform_f32(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
-}
-__device__ __forceinline__ double uniform_f64(double x) {
-    const uint64_t b = __builtin_bit_cast(uint64_t, x);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+
+// Particle inputs of a tile's SPM pass for one candidate particle p: its
+// reach box of state k and, reduced formats, its FP32 profile record, or,
+// FP64, the state fields the profile and the rigid-body velocity need.
+// Read with scalar loads (p is uniform; these buffers are not written during
+// the launch).
+template <bool DEV> struct Cand;
+template <> struct Cand<true> {
+    int p;
+    Box b;
+    Prof s;
+};
+template <> struct Cand<false> {
+    int p;
+    Box b;
+    double xc, yc, vx, vy, w, dm, zt;
+};
+template <bool DEV>
+__device__ __forceinline__ Cand<DEV> load_cand(const StepArgs &a, int p) {
+    Cand<DEV> c;
+    c.p = p;
+    const int *w = a.pbox_in + 4 * p;
+    c.b.i0 = cload(w); c.b.i1 = cload(w + 1); c.b.j0 = cload(w + 2); c.b.j1 = cload(w + 3);
+    if constexpr (DEV) {
+        const float *q = a.spf_in + p * SPF;
+        c.s.x = cload(q + SX); c.s.xf = cload(q + SXF);
+        c.s.y = cload(q + SY); c.s.yf = cload(q + SYF);
+        c.s.vx = cload(q + SVX); c.s.vy = cload(q + SVY);
+        c.s.w = cload(q + SW); c.s.rp = cload(q + SRP);
+        c.s.hz = cload(q + SHZ); c.s.pz = cload(q + SPZ);
+    } else {
+        const int m = a.P.maxp;
+        c.xc = cload(a.pin + PX * m + p); c.yc = cload(a.pin + PY * m + p);
+        c.vx = cload(a.pin + PVX * m + p); c.vy = cload(a.pin + PVY * m + p);
+        c.w = cload(a.pin + PW * m + p);
+        c.dm = cload(a.pin + PDIAM * m + p); c.zt = cload(a.pin + PZETA * m + p);
+    }
+    return c;
 }
 
-// Smoothed-profile forcing and force partials of a tile (all NT threads;
-// block barriers).  For each candidate particle p (ascending when tested
-// directly) whose reach box the tile holds nodes of: phi at the thread's
-// node from state k, the highest index with phi > 0 sets G (F6:485-495), and
-// the node's share of the hydrodynamic force and torque (F6:542-556) goes
-// into the tile's partial for p: each wavefront adds its lanes with a fixed
-// shuffle tree, thread 0 adds the four in order and writes the tagged words
-// of slot rank (the tile's position in the box's tile order).  r, u:
-// density and uncorrected velocity of the node (F6:333-340); valid: the
-// thread holds a node.
+// SPM forcing G at a tile's node (F6:485-495): of the candidate particles
+// whose phi at (i, j) is positive, the one with the highest index sets
+// G = phi (u_p - u), u_p the rigid-body velocity at the node, from state k.
+// u: uncorrected velocity of the node (F6:333-340); valid: the thread holds
+// a node.
 template <bool DEV, class Real>
-__device__ void tile_spm(const StepArgs &a, long long t, int cnt, unsigned mask, const int *ids,
-                         int i, int j, bool valid, Real r, Real u, Real v, Real &gx, Real &gy) {
-    const DevP &P = a.P;
-    const int m = P.maxp;
-    __shared__ double red[4 * 3];
+__device__ __forceinline__ void tile_forcing(const StepArgs &a, int cnt, unsigned mask,
+                                             const int *ids, int i, int j, bool valid, Real u,
+                                             Real v, Real &gx, Real &gy) {
     gx = (Real)0;
     gy = (Real)0;
     int best = -1;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned rest = mask;
     for (int c = 0; c < cnt; ++c) {
-        int p;
-        if (ids) p = uniform_i32(ids[c]);
-        else { p = __builtin_ctz(mask); mask &= mask - 1u; }
-        Box b;
-        b.i0 = uniform_i32(a.pbox_in[4 * p]); b.i1 = uniform_i32(a.pbox_in[4 * p + 1]);
-        b.j0 = uniform_i32(a.pbox_in[4 * p + 2]); b.j1 = uniform_i32(a.pbox_in[4 * p + 3]);
-        const int rank = (b.i0 > b.i1 || b.j0 > b.j1) ? -1 : box_tile_rank(P, b, t);
-        if (rank < 0) continue;  // the tile holds no node of the box
-        double acc[3] = {0.0, 0.0, 0.0};
+        const int p = ids ? cload(ids + c) : __builtin_ctz(rest);
+        rest &= rest - 1u;
+        const Cand<DEV> cd = load_cand<DEV>(a, p);
         if constexpr (DEV) {
-            const float *q = a.spf_in + p * SPF;
-            Prof s;
-            s.x = uniform_f32(q[SX]); s.xf = uniform_f32(q[SXF]);
-            s.y = uniform_f32(q[SY]); s.yf = uniform_f32(q[SYF]);
-            s.vx = uniform_f32(q[SVX]); s.vy = uniform_f32(q[SVY]);
-            s.w = uniform_f32(q[SW]); s.rp = uniform_f32(q[SRP]);
-            s.hz = uniform_f32(q[SHZ]); s.pz = uniform_f32(q[SPZ]);
             float dx, dy;
-            const float ph = valid ? phi_prof(s, i, j, dx, dy) : 0.0f;
-            if (ph > 0.0f) {
-                const float uxp = s.vx - s.w * dy, uyp = s.vy + s.w * dx;
-                if (p > best) {
-                    gx = ph * (uxp - u);
-                    gy = ph * (uyp - v);
-                    best = p;
-                }
-                const double rp = (double)r * (double)ph;
-                const double du = (double)(u - uxp), dv = (double)(v - uyp);
-                acc[0] = rp * du;
-                acc[1] = rp * dv;
-                acc[2] = rp * ((double)dx * dv - (double)dy * du);
+            const float ph = valid ? phi_prof(cd.s, i, j, dx, dy) : 0.0f;
+            if (ph > 0.0f && p > best) {
+                gx = ph * ((cd.s.vx - cd.s.w * dy) - u);
+                gy = ph * ((cd.s.vy + cd.s.w * dx) - v);
+                best = p;
             }
         } else {
-            const double xc = uniform_f64(a.pin[PX * m + p]), yc = uniform_f64(a.pin[PY * m + p]);
-            const double vx = uniform_f64(a.pin[PVX * m + p]), vy = uniform_f64(a.pin[PVY * m + p]);
-            const double w = uniform_f64(a.pin[PW * m + p]);
-            const double dm = uniform_f64(a.pin[PDIAM * m + p]), zt = uniform_f64(a.pin[PZETA * m + p]);
-            const double ph = valid ? phi_at(xc, yc, dm, zt, i, j) : 0.0;
-            if (ph > 0.0) {
-                const double uxp = vx - w * ((double)j - yc);
-                const double uyp = vy + w * ((double)i - xc);
-                if (p > best) {
-                    gx = ph * (uxp - (double)u);
-                    gy = ph * (uyp - (double)v);
-                    best = p;
-                }
-                acc[0] = r * ph * (u - uxp);
-                acc[1] = r * ph * (v - uyp);
-                acc[2] = r * ph * (((double)i - xc) * (v - uyp) - ((double)j - yc) * (u - uxp));
+            const double ph = valid ? phi_at(cd.xc, cd.yc, cd.dm, cd.zt, i, j) : 0.0;
+            if (ph > 0.0 && p > best) {
+                gx = ph * ((cd.vx - cd.w * ((double)j - cd.yc)) - u);
+                gy = ph * ((cd.vy + cd.w * ((double)i - cd.xc)) - v);
+                best = p;
             }
         }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            double x = acc[k];
-#pragma unroll
-            for (int sft = 32; sft > 0; sft >>= 1) x = x + __shfl_down(x, sft, 64);
-            if (lane == 0) red[wv * 3 + k] = x;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            if (rank < a.rcap) {
-                unsigned long long *o = a.fslot + ((size_t)p * a.rcap + rank) * SLOTW;
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    put_tagged_f64(o + 2 * k, a.tag, ((red[k] + red[3 + k]) + red[6 + k]) + red[9 + k]);
-            } else {
-                set_err(a, LBM_EF_SLOT);
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -1039,49 +916,32 @@ __device__ __forceinline__ bool wave_interior(const DevP &P, long long tile, int
     return j0 == j1 && j0 >= e && j0 <= P.ny - e && i0 >= e && i1 <= P.nx - e;
 }
 
-// One fused time step k over a tile, reduced-precision storage (FP32
-// collide): pull-stream + boundary rules (f_k), density and velocity, SPM
-// forcing G_k and the force partials, velocity correction, then (FINAL =
-// false) the collision of step k+1 stored as f*_{k+1}, or (FINAL = true) the
-// FP64 state of step k.  FORCED: a particle can reach the tile.
+// Density and uncorrected velocity of a node from its moments in storage
+// units (reduced formats); the tiles and the particle helper blocks form
+// them alike.
+__device__ __forceinline__ void dev_velocity(const DevP &P, const DevMom &mo, float invS,
+                                             float &rho, float &rinvS, float &ux, float &uy) {
+    rho = P.frho0 + mo.rd * invS;
+    rinvS = frcp(rho) * invS;
+    ux = mo.jx * rinvS;
+    uy = mo.jy * rinvS;
+}
+
+// Outputs of a tile's node, reduced-precision storage: (FINAL = false) the
+// collision of step k+1 stored as f*_{k+1}, the rate, edge arrays and
+// auxiliary fields, or (FINAL = true) the FP64 state of step k; then the
+// monitor partials.  FORCED: G may be non-zero.
 template <class St, bool FINAL, int BND, bool FORCED>
-__device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile, int cnt,
-                                         unsigned mask, const int *ids) {
+__device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, long long n,
+                                             bool valid, bool lean, int i, int j, float (&H)[9],
+                                             const DevMom &mo, float rho, float rinvS, float ucx,
+                                             float ucy, float gx, float gy, float om) {
     const DevP &P = a.P;
     const float S = St::S, invS = St::invS;
-    const long long n = tile * NT + threadIdx.x;
-    const bool valid = n < P.nn;
-    const typename St::T *src = (const typename St::T *)a.src;
-    int jw;
-    const bool lean = wave_interior<BND>(P, tile, jw);
-    int i, j;
-    float H[9];
-    if (lean) {
-        j = jw;
-        i = (int)(n - (long long)jw * P.W);
-        load_interior_dev<St>(P, src, n, H);
-    } else {
-        const long long nc = valid ? n : P.nn - 1;
-        j = row_of(P, (unsigned)nc);
-        i = (int)(nc - (long long)j * P.W);
-        load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, nc, i, j, H);
-    }
-    const float om = (!FINAL && P.rheology != 0) ? a.rate32[valid ? n : P.nn - 1] : P.fomega;
-    const DevMom mo = dev_moments(H);
-    const float rho = P.frho0 + mo.rd * invS;
-    const float rinv = frcp(rho), rinvS = rinv * invS;
-    const float ux = mo.jx * rinvS, uy = mo.jy * rinvS;
-    if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
-    float gx = 0.0f, gy = 0.0f, ucx = ux, ucy = uy;
-    if constexpr (FORCED) {
-        tile_spm<true, float>(a, tile, cnt, mask, ids, i, j, valid, rho, ux, uy, gx, gy);
-        ucx = ux + 0.5f * gx;
-        ucy = uy + 0.5f * gy;
-    }
     int yielded = 0;
     if (valid) {
         if (FINAL) {
-            const typename St::T *s0 = src;
+            const typename St::T *s0 = (const typename St::T *)a.src;
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 double w0 = (double)wdev(k) * P.rho0;
@@ -1125,22 +985,76 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile, int 
     }
 }
 
+// One fused time step k over a tile, reduced-precision storage (FP32
+// collide): pull-stream + boundary rules (f_k), density and velocity, SPM
+// forcing G_k from the particles that can reach the tile, velocity
+// correction, then the outputs (tile_dev_out).  The candidate particles are
+// looked up after the population loads are issued, so that no load waits
+// for another before the tile's stream of populations starts.
+template <class St, bool FINAL, int BND>
+__device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
+    const DevP &P = a.P;
+    const long long n = tile * NT + threadIdx.x;
+    const bool valid = n < P.nn;
+    const typename St::T *src = (const typename St::T *)a.src;
+    // the node rate and the candidates' inputs are requested before the
+    // populations (loads complete in order)
+    const float om = (!FINAL && P.rheology != 0) ? a.rate32[valid ? n : P.nn - 1] : P.fomega;
+    const CandPre cp = cand_prefetch(a, tile);
+    int jw;
+    const bool lean = wave_interior<BND>(P, tile, jw);
+    int i, j;
+    float H[9];
+    if (lean) {
+        j = jw;
+        i = (int)(n - (long long)jw * P.W);
+        load_interior_dev<St>(P, src, n, H);
+    } else {
+        const long long nc = valid ? n : P.nn - 1;
+        j = row_of(P, (unsigned)nc);
+        i = (int)(nc - (long long)j * P.W);
+        load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, nc, i, j, H);
+    }
+    unsigned mask;
+    const int *ids;
+    const int cnt = cand_resolve(a, tile, cp, mask, ids);
+    const DevMom mo = dev_moments(H);
+    float rho, rinvS, ux, uy;
+    dev_velocity(P, mo, St::invS, rho, rinvS, ux, uy);
+    if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
+    if (cnt > 0) {
+        float gx, gy;
+        tile_forcing<true, float>(a, cnt, mask, ids, i, j, valid, ux, uy, gx, gy);
+        tile_dev_out<St, FINAL, BND, true>(a, tile, n, valid, lean, i, j, H, mo, rho, rinvS,
+                                           ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
+    } else {
+        tile_dev_out<St, FINAL, BND, false>(a, tile, n, valid, lean, i, j, H, mo, rho, rinvS, ux,
+                                            uy, 0.0f, 0.0f, om);
+    }
+}
+
 // The same step with FP64 storage, in the reference's operation order.
-template <class St, bool FINAL, int BND, bool FORCED>
-__device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile, int cnt,
-                                         unsigned mask, const int *ids) {
+template <class St, bool FINAL, int BND>
+__device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
     const DevP &P = a.P;
     const long long n = tile * NT + threadIdx.x;
     const bool valid = n < P.nn;
     const long long nc = valid ? n : P.nn - 1;
     const int j = row_of(P, (unsigned)nc), i = (int)(nc - (long long)j * P.W);
     const typename St::T *src = (const typename St::T *)a.src;
+    // the node rate and the candidates' inputs are requested before the
+    // populations (loads complete in order)
+    const double om = (!FINAL && P.rheology != 0) ? a.cobn[nc] : P.omega;
+    const CandPre cp = cand_prefetch(a, tile);
     double f[9], rho, ux, uy;
     load_fk64<St, BND>(P, src, (const double *)a.erho_src, nc, i, j, f);
+    unsigned mask;
+    const int *ids;
+    const int cnt = cand_resolve(a, tile, cp, mask, ids);
     macros64(f, rho, ux, uy);
     if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
     double gx = 0.0, gy = 0.0;
-    if constexpr (FORCED) tile_spm<false, double>(a, tile, cnt, mask, ids, i, j, valid, rho, ux, uy, gx, gy);
+    if (cnt > 0) tile_forcing<false, double>(a, cnt, mask, ids, i, j, valid, ux, uy, gx, gy);
     const double ucx = ux + 0.5 * 1.0 * (gx + 0.0), ucy = uy + 0.5 * 1.0 * (gy + 0.0);
     int yielded = 0;
     if (valid) {
@@ -1154,7 +1068,6 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile, int 
         } else {
             typename St::T *dst = (typename St::T *)a.dst;
             CollideOut o;
-            double om = P.rheology != 0 ? a.cobn[n] : P.omega;
             collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
 #pragma unroll
             for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
@@ -1180,57 +1093,236 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile, int 
 
 template <class St, bool DEV, bool FINAL, int BND>
 __device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
-    unsigned mask;
-    const int *ids;
-    const int cnt = tile_candidates(a, tile, mask, ids);
-    if constexpr (DEV) {
-        if (cnt > 0) tile_dev<St, FINAL, BND, true>(a, tile, cnt, mask, ids);
-        else tile_dev<St, FINAL, BND, false>(a, tile, 0, 0u, nullptr);
-    } else {
-        if (cnt > 0) tile_f64<St, FINAL, BND, true>(a, tile, cnt, mask, ids);
-        else tile_f64<St, FINAL, BND, false>(a, tile, 0, 0u, nullptr);
-    }
+    if constexpr (DEV) tile_dev<St, FINAL, BND>(a, tile);
+    else tile_f64<St, FINAL, BND>(a, tile);
 }
 
-// Block b -> tile of a step launch.  With up to NP_LOOP particles, the
-// tiles of the range their reach boxes span come first: they carry the
-// particle work (the particle blocks wait for their partials), start early,
-// and the round-robin dispatch spreads them over all XCDs; the other tiles
-// follow in one contiguous range per XCD.  A range over more than a quarter
-// of the grid (particles spread out) leaves the plain per-XCD order.  A
-// bijection of [0, ntiles), uniform over the block.
-__device__ __forceinline__ long long step_tile_of(const StepArgs &a, long long b) {
+// ---------------------------------------------------------------- particles
+// Chunk c of particle p's reach box (box nodes c NT .. c NT + NT - 1 in
+// row-major order, one per thread): the chunk's share of the hydrodynamic
+// force and torque on p (ComputeForce_HYDRO, F6:542-556).  At each node with
+// phi > 0, f_k is pulled from f*_k with the boundary rules and the node's
+// density and uncorrected velocity are formed exactly as the tile that owns
+// the node forms them; the node adds rho phi (u - u_p) and its moment arm
+// cross that.  All NT threads; the block total is valid in thread 0 (a fixed
+// DPP tree per wavefront, then the four wavefronts in order).
+template <class St, bool DEV, int BND>
+__device__ __forceinline__ void chunk_force(const StepArgs &a, int p, int c, double F[3]) {
+    static_assert(NT == 256, "four wavefronts per block");
     const DevP &P = a.P;
-    const long long T = a.ntiles;
-    if (P.np == 0 || P.np > NP_LOOP) return xcd_tile(b, T);
-    long long lo = T, hi = -1;
-    for (int p = 0; p < P.np; ++p) {
-        const int *w = a.pbox_in + 4 * p;
-        const int i0 = __builtin_amdgcn_readfirstlane(w[0]), i1 = __builtin_amdgcn_readfirstlane(w[1]);
-        const int j0 = __builtin_amdgcn_readfirstlane(w[2]), j1 = __builtin_amdgcn_readfirstlane(w[3]);
-        if (i0 > i1 || j0 > j1) continue;
-        lo = min(lo, ((long long)j0 * P.W + i0) / NT);
-        hi = max(hi, ((long long)j1 * P.W + i1) / NT);
+    const Cand<DEV> cd = load_cand<DEV>(a, p);
+    const Box &b = cd.b;
+    const int bw = b.i1 - b.i0 + 1, bh = b.j1 - b.j0 + 1;
+    const int q = c * NT + (int)threadIdx.x;
+    const bool valid = bw > 0 && bh > 0 && q < bw * bh;
+    if (c == 0 && threadIdx.x == 0 && bw > 0 && bh > 0 && bw * bh > a.kpart * NT)
+        set_err(a, LBM_EF_SLOT);
+    const int i = valid ? b.i0 + q % bw : 0, j = valid ? b.j0 + q / bw : 0;
+    const long long n = (long long)j * P.W + i;
+    const typename St::T *src = (const typename St::T *)a.src;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if constexpr (DEV) {
+        float dx, dy;
+        const float ph = valid ? phi_prof(cd.s, i, j, dx, dy) : 0.0f;
+        if (ph > 0.0f) {
+            float H[9];
+            load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, n, i, j, H);
+            const DevMom mo = dev_moments(H);
+            float rho, rinvS, ux, uy;
+            dev_velocity(P, mo, St::invS, rho, rinvS, ux, uy);
+            const float uxp = cd.s.vx - cd.s.w * dy, uyp = cd.s.vy + cd.s.w * dx;
+            const double rp = (double)rho * (double)ph;
+            const double du = (double)(ux - uxp), dv = (double)(uy - uyp);
+            acc[0] = rp * du;
+            acc[1] = rp * dv;
+            acc[2] = rp * ((double)dx * dv - (double)dy * du);
+        }
+    } else {
+        const double ph = valid ? phi_at(cd.xc, cd.yc, cd.dm, cd.zt, i, j) : 0.0;
+        if (ph > 0.0) {
+            double f[9], r, u, v;
+            load_fk64<St, BND>(P, src, (const double *)a.erho_src, n, i, j, f);
+            macros64(f, r, u, v);
+            const double uxp = cd.vx - cd.w * ((double)j - cd.yc);
+            const double uyp = cd.vy + cd.w * ((double)i - cd.xc);
+            acc[0] = r * ph * (u - uxp);
+            acc[1] = r * ph * (v - uyp);
+            acc[2] = r * ph * (((double)i - cd.xc) * (v - uyp) - ((double)j - cd.yc) * (u - uxp));
+        }
     }
-    const long long R = hi - lo + 1;
-    if (R <= 0 || 4 * R > T) return xcd_tile(b, T);
-    if (b < R) return lo + b;
-    const long long q = xcd_tile(b - R, T - R);
-    return q < lo ? q : q + R;
+    __shared__ double red[NT / 64][3];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double x = wave_sum_f64(acc[k]);
+        if (lane == 63) red[wv][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) F[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    }
 }
 
+// FORCE and TORQUE on particle p from the tagged chunk partials 0 .. K-1
+// (K = a.kpart), by one wavefront: lane l takes chunks l, l + 64, ... in
+// order, waiting (bounded) until their words carry this launch's tag; a
+// fixed shuffle tree then adds the lanes.  Every chunk's block wrote its
+// words before it counted itself done, so the wait is short.  Valid in every
+// lane; false when the wait ran out.
+__device__ __forceinline__ bool sum_force_slots(const StepArgs &a, int p, double F[3]) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long *q = a.fslot + (size_t)p * a.kpart * SLOTW;
+    double acc[3] = {0.0, 0.0, 0.0};
+    bool ok = true;
+    for (int r = lane; r < a.kpart; r += 64) {
+        const unsigned long long *w = q + (size_t)r * SLOTW;
+        unsigned long long v[SLOTW];
+        int it = 0;
+        for (;;) {
+            bool ready = true;
+#pragma unroll
+            for (int k = 0; k < SLOTW; ++k) {
+                v[k] = __hip_atomic_load(w + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ready = ready && (unsigned)(v[k] >> 32) == a.tag;
+            }
+            if (ready) break;
+            if (++it >= TAG_SPIN) { ok = false; break; }
+            __builtin_amdgcn_s_sleep(2);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            acc[c] = acc[c] + __builtin_bit_cast(double, (v[2 * c] << 32) | (v[2 * c + 1] & 0xffffffffull));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double x = acc[c];
+#pragma unroll
+        for (int sft = 32; sft > 0; sft >>= 1) x = x + __shfl_xor(x, sft, 64);
+        F[c] = x;
+    }
+    return __all(ok);
+}
+
+// Helper block c of particle p in launch s >= 1 (all NT threads): the chunk
+// partial of the force (chunk_force), stored as tagged words in slot (p, c);
+// then the block counts itself done on p's counter (a wrapping increment, so
+// the counter is back at zero after the launch).  The block that completes
+// the count finishes the particle: the contact force on state k (F3:739-804),
+// FORCE_k from the K partials in chunk order, the history row of step k, and
+// the Newton-Euler update to state k+1 (F6:299-313) with its reach box,
+// profile record and, for many particles, tile registration; the final
+// launch only writes FORCE_k into the record.  No block waits for another
+// that has not yet run.  The particle's record and the contact inputs are
+// requested before the chunk's populations, so the finishing block has them
+// at hand.
+template <class St, bool DEV, bool FINAL, int BND>
+__device__ void particle_helper(const StepArgs &a, int p, int c) {
+    const DevP &P = a.P;
+    const int m = P.maxp;
+    static_assert(NP_LDS <= NT, "one staged particle per thread");
+    __builtin_amdgcn_s_setprio(3);  // the particle chain is the launch's critical path
+    const int t = threadIdx.x;
+    const int tr = min(t, PCOUNT - 1), tc = min(t, max(P.np - 1, 0));
+    const double r_rec = a.pin[tr * m + p];
+    double r_x = 0.0, r_y = 0.0, r_d = 0.0, r_dpw = 0.0;
+    if (!FINAL) {
+        r_x = a.pin[PX * m + tc]; r_y = a.pin[PY * m + tc]; r_d = a.pin[PDIAM * m + tc];
+        r_dpw = *a.dpw_in;
+    }
+    double Fc[3];
+    chunk_force<St, DEV, BND>(a, p, c, Fc);
+    __shared__ int slast;
+    if (t == 0) {
+        unsigned long long *o = a.fslot + ((size_t)p * a.kpart + c) * SLOTW;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) put_tagged_f64(o + 2 * k, a.tag, Fc[k]);
+        const unsigned k1 = (unsigned)a.kpart - 1u;
+        slast = atomicInc(a.pcount + p, k1) == k1;
+    }
+    __syncthreads();
+    if (!slast) return;
+    // ---- the particle's last chunk: finish it
+    __shared__ double srec[PCOUNT];
+    __shared__ double scon[3 * NP_LDS];
+    __shared__ double sdpw;
+    __shared__ ContactLDS cl;
+    __shared__ double snew[PCOUNT];
+    const bool staged = P.np <= NP_LDS;
+    if (t < PCOUNT) srec[t] = r_rec;
+    if (!FINAL) {
+        if (staged && t < P.np) {
+            scon[t] = r_x; scon[NP_LDS + t] = r_y; scon[2 * NP_LDS + t] = r_d;
+        }
+        if (t == 0) sdpw = r_dpw;
+    }
+    __syncthreads();
+    double fcx = 0.0, fcy = 0.0, dl = 0.0;
+    int cnt = 0;
+    if (!FINAL) {
+        if (P.np <= 64) {
+            if (t < 64)
+                cnt = contact_wave(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, fcx,
+                                   fcy, a.pairs + p * MAXNB, &dl);
+        } else {
+            cnt = staged
+                ? contact_block(P, scon, scon + NP_LDS, scon + 2 * NP_LDS, P.np, p, sdpw, cl, fcx,
+                                fcy, a.pairs + p * MAXNB, &dl)
+                : contact_block(P, a.pin + PX * m, a.pin + PY * m, a.pin + PDIAM * m, P.np, p,
+                                sdpw, cl, fcx, fcy, a.pairs + p * MAXNB, &dl);
+        }
+    }
+    if (t < 64) {
+        double F[3];
+        if (!sum_force_slots(a, p, F) && t == 0) set_err(a, LBM_EF_WAIT);
+        if (a.hist_part && t == 0) hist_row(a.hist_part + (size_t)p * HPC, srec, F);
+        if (FINAL) {
+            if (t == 0) {
+                a.pout[PFX * m + p] = F[0]; a.pout[PFY * m + p] = F[1]; a.pout[PTQ * m + p] = F[2];
+                if (!__builtin_isfinite(F[0] + F[1] + F[2])) set_err(a, LBM_EF_NAN);
+            }
+        } else {
+            // the update by wavefront 0; the contact force is final in lane 0
+            const double val = update_wave(P, srec, F[0], F[1], F[2], __shfl(fcx, 0),
+                                           __shfl(fcy, 0), a.kk_upd);
+            if (t < PCOUNT) {
+                a.pout[t * m + p] = val;
+                snew[t] = val;
+            }
+            if (t == 0) {
+                a.paircnt[p] = cnt;
+                if (p == P.np - 1) *a.dpw_out = dl;
+            }
+        }
+    }
+    if (FINAL) return;
+    __syncthreads();
+    const Box nb = reach_box(P, snew[PX], snew[PY], snew[PDIAM], snew[PZETA]);
+    if (t == 0) {
+        put_box(a.pbox_out + 4 * p, nb);
+        if (a.spf_out) make_spf(snew, a.spf_out + p * SPF);
+        bool bad = false;
+        for (int f = 0; f < PCOUNT; ++f) bad = bad || !__builtin_isfinite(snew[f]);
+        if (bad) set_err(a, LBM_EF_NAN);
+    }
+    if (P.np > NP_LOOP) register_tiles(P, nb, p, a.tag + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
+}
+
+// Block layout of a step launch: the particle helper blocks (particle-major,
+// K per particle), the reducer slot, then the tiles in one contiguous range
+// per XCD.
 template <class St, bool DEV, bool FINAL, int BND>
 __global__ __launch_bounds__(NT) void k_step(StepArgs a) {
-    if ((int)blockIdx.x < a.n_pblocks) {
-        particle_block<DEV, FINAL>(a, (int)blockIdx.x);
-        if (!FINAL) particle_register(a, (int)blockIdx.x);
+    const long long b = blockIdx.x;
+    if (b < a.n_pblocks) {
+        particle_helper<St, DEV, FINAL, BND>(a, (int)(b / a.kpart), (int)(b % a.kpart));
         return;
     }
-    if ((int)blockIdx.x < a.n_pblocks + a.nred) {
+    if (b < a.n_pblocks + a.nred) {
         monitor_reduce(a.mpart_in, a.ntiles, a.red_mass, a.red_yield);
         return;
     }
-    step_tile<St, DEV, FINAL, BND>(a, step_tile_of(a, blockIdx.x - a.n_pblocks - a.nred));
+    step_tile<St, DEV, FINAL, BND>(a, xcd_rank(b, a.n_pblocks + a.nred, a.ntiles));
 }
 
 // ================================================================= batch
@@ -1253,7 +1345,7 @@ __global__ __launch_bounds__(NT) void k_batch_prologue(const StepArgs *__restric
     const int2 m = bmap[blockIdx.x];
     const StepArgs &a = args[m.x];
     if (m.y < a.n_pblocks) {
-        particle_advance<DEV>(a, m.y);
+        if (m.y % a.kpart == 0) particle_advance<DEV>(a, m.y / a.kpart);
         return;
     }
     if (m.y < a.n_pblocks + a.nred) return;  // nothing before the prologue
@@ -1270,8 +1362,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ 
     const int2 m = bmap[blockIdx.x];
     const StepArgs &a = args[m.x];
     if (m.y < a.n_pblocks) {
-        particle_block<DEV, FINAL>(a, m.y);
-        if (!FINAL) particle_register(a, m.y);
+        particle_helper<St, DEV, FINAL, BND>(a, m.y / a.kpart, m.y % a.kpart);
         return;
     }
     if (m.y < a.n_pblocks + a.nred) {
