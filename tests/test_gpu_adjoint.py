@@ -96,7 +96,7 @@ def test_adjoint_fp32_storage_close_to_fp64():
     _, g32, _ = adjoint(theta, 400, 100, 32, storage="fp32", **SMALL)
     rel = np.abs(g32 - g64) / np.abs(g64)
     print("fp32 vs fp64", g32, g64, rel, flush=True)
-    assert np.all(rel <= 5e-3)
+    assert np.all(rel <= 1e-4)
 
 
 def test_torch_autograd_function():
@@ -123,4 +123,4 @@ def test_adjoint_full_channel_4096_steps():
     _, g32, mem32 = adjoint(theta, n, w, k, storage="fp32")
     rel = np.abs(g32 - g64) / np.abs(g64)
     print("full channel fp32", g32, "rel to fp64", rel, "mem %.3g" % mem32, flush=True)
-    assert np.all(rel <= 1e-2)
+    assert np.all(rel <= 1e-4)

@@ -63,3 +63,73 @@ def test_many_particles_fused_matches_oracle(orc, bingham):
     pairs_g = g.last_pairs()
     print("pairs: ref %d gpu %d" % (nref, len(pairs_g)), flush=True)
     assert np.array_equal(pairs_g, pairs_ref)
+
+
+# Reduced-precision storage on the many-particle path (FP32 candidate lists
+# and profile records, contact forces on staged FP64 inputs) against the FP64
+# oracle: the same 40 disks, Bingham fluid.  Bounds on every disk's settling
+# velocity and position after the run, relative to the largest of them.
+MANY_TOL = {"fp16": 1e-3, "bf16": 2e-3}
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_many_particles_reduced_storage_near_oracle(orc, storage):
+    rng = np.random.default_rng(11)
+    nx, ny, n, steps = 200, 200, 40, 1500
+    cfg = cases.f6_params(nx, ny, True, local_rate=True)
+    cfg["gy"] = -2e-5
+    d = 10.0
+    cfg["cij"] = -((((((1.5 - 1.0) * cfg["gy"]) * cases.PI) * 0.5) * d) * 0.5) * d
+    part = _packed_disks(rng, nx, ny, n, d, 0.3, close_pairs=8)
+    print("case: many particles n=%d %s vs FP64 oracle, %d steps" % (n, storage, steps),
+          flush=True)
+    st = oracle_state(orc, cfg, part)
+    ho = st.run(0, steps)
+    g = gpu_solver(cfg, storage=storage, max_particles=64)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, steps)
+    assert np.all(np.isfinite(hg))
+    H = orc.H_PCOUNT
+    vy = [p * H + orc.H_VY for p in range(n)]
+    ys = [p * H + orc.H_Y for p in range(n)]
+    e_vy = rel_err(hg[-1, vy], ho[-1, vy])
+    e_y = np.max(np.abs(hg[-1, ys] - ho[-1, ys]) / np.abs(ho[-1, ys]))
+    e_mass = rel_err(hg[:, -2], ho[:, -2])
+    print("%s: vy %.3e  y %.3e  mass %.3e  (max |vy| %.3e)"
+          % (storage, e_vy, e_y, e_mass, np.max(np.abs(ho[-1, vy]))), flush=True)
+    assert e_vy <= MANY_TOL[storage]
+    assert e_y <= MANY_TOL[storage]
+    assert e_mass <= 1e-4
+
+
+def test_hundred_particles_fp64_matches_oracle(orc):
+    """100 disks: the contact force of each disk by one block over inputs
+    staged in LDS (65 to 128 disks), FP64 fused path against the oracle."""
+    rng = np.random.default_rng(5)
+    nx, ny, n, steps = 240, 240, 100, 200
+    cfg = cases.f6_params(nx, ny, False)
+    cfg["gy"] = -2e-5
+    d = 10.0
+    cfg["cij"] = -((((((1.5 - 1.0) * cfg["gy"]) * cases.PI) * 0.5) * d) * 0.5) * d
+    part = _packed_disks(rng, nx, ny, n, d, 0.3, close_pairs=20)
+    print("case: many particles n=%d FP64, %d steps" % (n, steps), flush=True)
+    st = oracle_state(orc, cfg, part)
+    ho = st.run(0, steps)
+    g = gpu_solver(cfg, max_particles=128)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, steps)
+    assert rel_err(g.f, st.f) <= 1e-12
+    H = orc.H_PCOUNT
+    for fld in (orc.H_X, orc.H_Y, orc.H_VX, orc.H_VY):
+        cols = [p * H + fld for p in range(n)]
+        e = rel_err(hg[:, cols], ho[:, cols])
+        assert e <= 1e-9, (fld, e)
+    st2 = oracle_state(orc, cfg, part)
+    st2.run(0, steps - 1, record=False)
+    pairs_ref, nref = st2.contact(max_pairs=4096)
+    pairs_g = g.last_pairs()
+    print("pairs: ref %d gpu %d" % (nref, len(pairs_g)), flush=True)
+    assert nref > 0
+    assert np.array_equal(pairs_g, pairs_ref)

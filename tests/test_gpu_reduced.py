@@ -2,9 +2,12 @@
 f - w_i*rho0 in HBM) against the FP64 oracle on a settling-disk Bingham run.
 
 Measured quantities: settling velocity and position of the disk, mean
-density, and the yielded-node count.  Bounds: FP32 storage 1e-4, FP16 5e-3,
-BF16 5e-2 relative on the settling velocity (the FP16 target on the two-disk
-benchmark workload is 1e-3 and is reported by bench.py).
+density, and the yielded-node count.  Bounds: FP32 storage 1e-4, FP16 1e-3,
+BF16 2e-3 relative on the settling velocity (the FP16 bound is the
+terminal-velocity contract of the two-disk workload, which
+test_gpu_workloads.py checks at full size).  Velocity fields of the
+lid-driven Zou-He cavity: 1e-4 (FP32) and 5e-3 (FP16) relative to the
+largest velocity.
 """
 import numpy as np
 import pytest
@@ -14,7 +17,8 @@ from lbmtest import f6_case, gpu_solver, oracle_state, rel_err
 
 pytestmark = pytest.mark.gpu
 
-VY_TOL = {"fp32": 1e-4, "fp16": 5e-3, "bf16": 5e-2}
+VY_TOL = {"fp32": 1e-4, "fp16": 1e-3, "bf16": 2e-3}
+FIELD_TOL = {"fp32": 1e-4, "fp16": 5e-3}
 
 
 @pytest.fixture(scope="module")
@@ -89,7 +93,7 @@ def test_reduced_storage_zou_he(orc, storage):
     g.set_particles(part)
     g.step(0, n)
     rho, ux, uy, gg, om = g.get_fields()
-    tol = VY_TOL[storage]
+    tol = FIELD_TOL[storage]
     print("zou-he %s: ux %.3e" % (storage, rel_err(ux, st.ux)), flush=True)
     assert rel_err(ux, st.ux) <= tol and rel_err(uy, st.uy) <= tol
     # Zou-He edges do not conserve mass exactly: the reduced-format density

@@ -212,3 +212,36 @@ def test_fortran_driver_read_input():
         second = state_files(d, nx, ny)
     for k in first:
         assert np.array_equal(first[k], second[k]), k
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp16"])
+def test_snapshot_reloaded_into_the_same_handle_many_disks(storage):
+    """40 disks (tiles find them through per-tile candidate lists): run,
+    save a snapshot, run on, load the snapshot back into the same handle and
+    run the same steps again; the second continuation equals the first bit
+    for bit (the handle's candidate lists from the later steps are never
+    taken for the reloaded state's)."""
+    from test_gpu_many import _packed_disks
+    rng = np.random.default_rng(11)
+    nx, ny, n = 200, 200, 40
+    cfg = cases.f6_params(nx, ny, True, local_rate=True)
+    cfg["gy"] = -2e-5
+    d = 10.0
+    cfg["cij"] = -((((((1.5 - 1.0) * cfg["gy"]) * cases.PI) * 0.5) * d) * 0.5) * d
+    part = _packed_disks(rng, nx, ny, n, d, 0.3, close_pairs=8)
+    cfg = dict(cfg, storage=storage, max_particles=64)
+    lbm = pkg()
+    print("case: snapshot reload into the same handle, 40 disks", storage, flush=True)
+    a = lbm.D2Q9(cfg)
+    a.initialize()
+    a.set_particles(part)
+    a.step(0, 30)
+    snap = a.snapshot()
+    h1 = a.step(30, 40)
+    f1, p1 = a.f, a.get_particles()
+    a.load_snapshot(snap)
+    h2 = a.step(30, 40)
+    assert np.array_equal(h1, h2)
+    assert np.array_equal(f1, a.f)
+    assert np.array_equal(p1, a.get_particles())
+    a.close()

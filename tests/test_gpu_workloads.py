@@ -122,3 +122,83 @@ def test_dense_sedimentation_504_disks_bf16():
     assert abs(mass[-1] / mass[0] - 1.0) < 2e-5, mass[[0, -1]] / c.nodes(cfg)
     pairs = g1.compute_collision_force(max_pairs=8192)
     assert [tuple(q) for q in pairs] == brute_pairs(g1.get_particles(), pf)
+
+
+def test_two_disk_bingham_full_grid_fp64_matches_oracle(orc):
+    """The two-disk Bingham workload (401 x 1601 nodes, D = 40, Bn = 0.5,
+    tau = 0.8, m = 1e5) at its full size: fused FP64 steps against the
+    oracle, populations <= 1e-12 relative after the run, both disks' x, y
+    and v_y <= 1e-9 relative at every step, yielded-node counts within 2."""
+    cfg, part = configs().two_disk_bingham("fp64")
+    n = 200
+    print("case: two disks Bingham 401x1601 FP64, %d steps" % n, flush=True)
+    st = oracle_state(orc, cfg, part)
+    ho = st.run(0, n)
+    g = gpu_solver(cfg, max_particles=2)
+    g.initialize()
+    g.set_particles(part)
+    hg = g.step(0, n)
+    assert rel_err(g.f, st.f) <= POP_TOL
+    H = orc.H_PCOUNT
+    for p in range(2):
+        for col in (orc.H_X, orc.H_Y, orc.H_VY):
+            e = rel_err(hg[:, p * H + col], ho[:, p * H + col])
+            assert e <= TRAJ_TOL, (p, col, e)
+    assert np.all(np.abs(hg[:, -1] - ho[:, -1]) <= 2)
+    assert rel_err(hg[:, -2], ho[:, -2]) <= 1e-12  # mass monitor
+
+
+def test_two_disk_bingham_fp16_matches_fp64_over_the_run():
+    """FP32 collide / FP16 deviation storage against FP64 (itself pinned to
+    the oracle above) over the workload's full 50,000 steps: terminal
+    velocity of both disks and their drag coefficient averaged over the last
+    1000 steps within 1e-3 relative; the yielded-node counts agree."""
+    c = configs()
+    res = {}
+    for storage in ("fp64", "fp16"):
+        cfg, part = c.two_disk_bingham(storage)
+        print("case: two disks Bingham 401x1601 %s, 50000 steps" % storage, flush=True)
+        g = gpu_solver(cfg, storage=storage, max_particles=2)
+        g.initialize()
+        g.set_particles(part)
+        hist = g.step(0, 50000)
+        assert np.all(np.isfinite(hist))
+        res[storage] = (hist, cfg)
+        g.close()
+    (h64, cfg64), (h16, _) = res["fp64"], res["fp16"]
+    for p in range(2):
+        vy64, _ = c.settling_metrics(h64, 2, cfg64, p)
+        vy16, _ = c.settling_metrics(h16, 2, cfg64, p)
+        cd64 = c.mean_drag_coefficient(h64, cfg64, p)
+        cd16 = c.mean_drag_coefficient(h16, cfg64, p)
+        e_vy, e_cd = abs(vy16 - vy64) / abs(vy64), abs(cd16 - cd64) / abs(cd64)
+        print("disk %d: vy %.6e rel err %.2e, mean Cd %.6e rel err %.2e"
+              % (p, vy64, e_vy, cd64, e_cd), flush=True)
+        assert vy64 < 0
+        assert e_vy <= 1e-3
+        assert e_cd <= 1e-3
+    assert np.array_equal(h16[:, -1], h64[:, -1])  # yielded-node counts
+
+
+def test_lid_cavity_yielded_fraction_fp16_vs_fp64():
+    """Yielded-area fraction where it is not zero: one cavity of the batched
+    ensemble (side 256, Bn = 1, lid and side wall at 0.05, oscillating
+    cylinder) in FP16 storage against FP64 over 4000 steps."""
+    c = configs()
+    out = {}
+    for storage in ("fp64", "fp16"):
+        cfg, part = c.oscillating_cavity(256, 1.0, storage=storage)
+        print("case: lid cavity 257^2 Bn=1 %s, 4000 steps" % storage, flush=True)
+        g = gpu_solver(cfg, storage=storage, max_particles=1)
+        g.initialize()
+        g.set_particles(part)
+        hist = g.step(0, 4000)
+        out[storage] = hist[:, -1] / c.nodes(cfg)
+        g.close()
+    f64, f16 = out["fp64"], out["fp16"]
+    e = abs(f16[-1] - f64[-1]) / f64[-1]
+    e_mean = abs(f16[-1000:].mean() - f64[-1000:].mean()) / f64[-1000:].mean()
+    print("yielded fraction: fp64 %.6f fp16 %.6f rel err %.2e (mean of the last 1000 steps %.2e)"
+          % (f64[-1], f16[-1], e, e_mean), flush=True)
+    assert 0.0 < f64[-1] < 1.0
+    assert e <= 1e-3 and e_mean <= 1e-3
