@@ -87,6 +87,7 @@ SIGNATURES = {
     "lbm_step": [ctypes.c_int32, ctypes.c_int32, _dp],
     "lbm_last_pairs": [_ip, ctypes.c_int32, _ip],
     "lbm_step_timed": [ctypes.c_int32, ctypes.c_int32, _dp, _dp],
+    "lbm_step_hist_static": [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p],
     "lbm_adjoint_velocity": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp],
     # batch entries (first argument: the lbm_batch pointer)
     "lbm_batch_destroy": [],

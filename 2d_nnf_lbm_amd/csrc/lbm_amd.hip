@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,11 @@ struct lbm_handle {
     double *hist_part = nullptr;
     unsigned long long *hist_slots = nullptr;
     size_t hist_rows = 0;
+    // history buffers the next recording fused call writes: the two above,
+    // or regions of the device's static scratch array
+    double *hist_part_use = nullptr;
+    unsigned long long *hist_slots_use = nullptr;
+    int dev = 0;  // device the handle's memory lives on
     // timing events: whole sequence (ev0..ev1), back-to-back main launches (evA..evB)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
     // per-tile candidate lists of many-particle runs
@@ -109,6 +115,55 @@ int check_launch(lbm_handle *h, const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(h, LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     return LBM_OK;
+}
+
+// ------------------------------------------------------------ static scratch
+// Device memory of lbm_step_hist_static: the monitor slots, the particle
+// history rows and the formatted history rows of one recording call, in a
+// file-scope array that each GPU holds until the process ends, so that the
+// call allocates nothing.  In host code the array's name is not a device
+// address: the address of the current device's copy comes from
+// hipGetSymbolAddress, looked up once per device (scratch_setup, from
+// lbm_create with the handle's device current) and kept per device.
+constexpr size_t SCRATCH_WORDS = (size_t)1 << 19;  // 4 MiB of 8-byte words
+constexpr int SCRATCH_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_MAX_DEV] = {};
+hipError_t g_scratch_err[SCRATCH_MAX_DEV] = {};
+bool g_scratch_done[SCRATCH_MAX_DEV] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned long long scratch[SCRATCH_WORDS];
+
+namespace {
+
+void scratch_setup(int dev) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEV) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[dev]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    g_scratch_err[dev] = e;
+    g_scratch_ptr[dev] = e == hipSuccess ? ptr : nullptr;
+    g_scratch_done[dev] = true;
+}
+
+// This device's scratch address (null, with the reason in *why, when the
+// lookup failed or never ran).
+void *scratch_address(int dev, std::string *why) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEV) {
+        *why = "device index " + std::to_string(dev) + " beyond the scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[dev]) {
+        *why = "scratch address never looked up";
+        return nullptr;
+    }
+    if (!g_scratch_ptr[dev])
+        *why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratch_err[dev]);
+    return g_scratch_ptr[dev];
 }
 
 }  // namespace
@@ -220,6 +275,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
     int rc = LBM_OK;
     hipError_t es = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (es != hipSuccess) { *out = h; return fail(h, LBM_ERR_HIP, "hipStreamCreate failed"); }
+    if (hipGetDevice(&h->dev) != hipSuccess) { *out = h; return fail(h, LBM_ERR_HIP, "hipGetDevice failed"); }
+    scratch_setup(h->dev);  // this device's static scratch address, once per device
     if ((rc = al((void **)&h->f, 9 * pl * 8)) || (rc = al((void **)&h->fp, 9 * pl * 8)) ||
         (rc = al((void **)&h->rho, pl * 8)) || (rc = al((void **)&h->ux, pl * 8)) ||
         (rc = al((void **)&h->uy, pl * 8)) || (rc = al((void **)&h->g, 2 * pl * 8)) ||
@@ -947,8 +1004,8 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
     a.pcount = h->pcount;
     a.ntiles = (P.nn + NT - 1) / NT;
     a.err = h->err_flag;
-    a.hist_base = record ? h->hist_part : nullptr;
-    a.slots_base = record ? h->hist_slots : nullptr;
+    a.hist_base = record ? h->hist_part_use : nullptr;
+    a.slots_base = record ? h->hist_slots_use : nullptr;
     a.rowp = np * HPC;
     a.kk0 = kk0;
     a.tag0 = h->tag_base;
@@ -1130,6 +1187,8 @@ int prepare_history(lbm_handle *h, int32_t nsteps) {
         h->hist_rows = rows;
     }
     HIPCHECK(h, hipMemsetAsync(h->hist_slots, 0, (size_t)nsteps * 2 * NSLOT * 8, h->stream));
+    h->hist_part_use = h->hist_part;
+    h->hist_slots_use = h->hist_slots;
     return LBM_OK;
 }
 
@@ -1182,6 +1241,76 @@ int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps, double *ms_total,
     int rc0 = check_fused_mode(h);
     if (rc0) return rc0;
     return fused_dispatch(h, kk0, nsteps, false, 1, ms_total, ms_main);
+}
+
+}  // extern "C"
+
+namespace {
+
+__global__ void k_fill_u64(unsigned long long *x, long long n, unsigned long long v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// History rows in lbm_step's layout: per step the particle rows, then the
+// mass (sum of the NSLOT fixed-point slots / 2^36) and the yielded count.
+__global__ void k_hist_format(const double *part, const unsigned long long *slots, int nsteps,
+                              long long rowp, double *out) {
+    const long long row = rowp + 2;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nsteps * row) return;
+    const long long s = i / row, c = i - s * row;
+    if (c < rowp) {
+        out[i] = part[s * rowp + c];
+        return;
+    }
+    const unsigned long long *q = slots + s * 2 * NSLOT + (c == rowp ? 0 : NSLOT);
+    unsigned long long sum = 0;
+    for (int k = 0; k < NSLOT; ++k) sum += q[k];
+    out[i] = c == rowp ? (double)(long long)sum / MASS_SCALE : (double)(long long)sum;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_step_hist_static(lbm_handle *h, int32_t kk0, int32_t nsteps, double *hist) {
+    if (!h || nsteps < 0 || !hist) return LBM_ERR_PARAM;
+    if (nsteps == 0) return LBM_OK;
+    const size_t rowp = (size_t)h->P.np * HPC, row = rowp + 2;
+    const size_t nslots = (size_t)nsteps * 2 * NSLOT;
+    const size_t words = nslots + (size_t)nsteps * (rowp + row);
+    if (words > SCRATCH_WORDS)
+        return fail(h, LBM_ERR_CAPACITY,
+                    "lbm_step_hist_static: " + std::to_string(nsteps) + " steps of " +
+                        std::to_string(h->P.np) + " particles need " +
+                        std::to_string(words * 8) +
+                        " bytes of scratch (steps x (24 x particles + 66) x 8); the static "
+                        "scratch holds " + std::to_string(SCRATCH_WORDS * 8) + " bytes");
+    int rc = check_fused_mode(h);
+    if (rc) return rc;
+    std::string why;
+    unsigned long long *base = (unsigned long long *)scratch_address(h->dev, &why);
+    if (!base) return fail(h, LBM_ERR_HIP, "lbm_step_hist_static: " + why);
+    unsigned long long *slots = base;
+    double *part = (double *)(base + nslots);
+    double *fmt = part + (size_t)nsteps * rowp;
+    // the slots start at zero (a fill kernel, not a memset)
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk((long long)nslots)), dim3(NT), 0, h->stream, slots,
+                       (long long)nslots, 0ull);
+    if ((rc = check_launch(h, "k_fill_u64"))) return rc;
+    h->hist_part_use = part;
+    h->hist_slots_use = slots;
+    rc = fused_dispatch(h, kk0, nsteps, true, 0, nullptr, nullptr);
+    h->hist_part_use = nullptr;
+    h->hist_slots_use = nullptr;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_hist_format, dim3(nblk((long long)nsteps * (long long)row)), dim3(NT), 0,
+                       h->stream, part, slots, nsteps, (long long)rowp, fmt);
+    if ((rc = check_launch(h, "k_hist_format"))) return rc;
+    HIPCHECK(h, hipMemcpyAsync(hist, fmt, (size_t)nsteps * row * 8, hipMemcpyDefault, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
 }
 
 }  // extern "C"

@@ -202,6 +202,15 @@ MODULE LBM_AMD
       TYPE(C_PTR), VALUE :: hist
       INTEGER(C_INT) :: rc
     END FUNCTION
+    ! lbm_step with the history staged in the library's static device
+    ! array: allocates nothing (for callers that cannot allocate)
+    FUNCTION lbm_step_hist_static(h, kk0, nsteps, hist) BIND(C, NAME='lbm_step_hist_static') RESULT(rc)
+      IMPORT :: C_PTR, C_INT, C_INT32_T
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: kk0, nsteps
+      TYPE(C_PTR), VALUE :: hist
+      INTEGER(C_INT) :: rc
+    END FUNCTION
     FUNCTION lbm_last_pairs(h, pairs, max_pairs, npairs) BIND(C, NAME='lbm_last_pairs') RESULT(rc)
       IMPORT :: C_PTR, C_INT, C_INT32_T
       TYPE(C_PTR), VALUE :: h, pairs

@@ -208,6 +208,18 @@ int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
  * duration of one launch of the main kernel. */
 int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps,
                    double *ms_total, double *ms_main);
+/* lbm_step with the history, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * history rows (layout as lbm_step's hist, which must not be NULL and may be
+ * host or device memory) are staged in a static 4 MiB device array that the
+ * library holds on each GPU until the process ends, instead of the handle's
+ * history buffers, which lbm_step grows with hipMalloc.  A call needs
+ * nsteps * (24 * n_particles + 66) * 8 bytes of it; a larger call fails with
+ * LBM_ERR_CAPACITY before it runs anything.  Calls on one GPU share the
+ * array: calls that overlap in time on the same GPU (other threads, other
+ * handles' streams) race and must be serialised by the caller. */
+int lbm_step_hist_static(lbm_handle *h, int32_t kk0, int32_t nsteps, double *hist);
 
 /* ---------------------------------------------------------------- adjoint
  * Sensitivity of the settling velocity of one disk (the reference has no
