@@ -44,6 +44,13 @@ F4:576-743) or F1 (F1:2-93, F1:596-769), the routine's model switches
 (F4:587, F4:624, F4:680; F1:609, F1:646, F1:701) set to the model under test,
 compiled with oracle/contact_model_driver.f90 into oracle/_ref/f4_contact_wt
 (F4 as it ships: Wan-Turek, zeta 1), f4_contact_lj and f1_contact_wt (zeta 2).
+A fourth recipe builds whole multi-particle programs (patch_nparticle): F3
+(grid, particle count and steps set, the STOP before its time loop at
+F3:202 commented out, two output lines added) into oracle/_ref/f3_multi,
+which reads its disks from 0_Particle_Initial_Position.txt in the run
+directory, and F2, the two-disk program (grid, steps and its two initial
+centres set, the same output lines) into oracle/_ref/f2_dkt.  Both are
+compiled with -fno-automatic (see below).
 A third recipe builds the reference's own MAIN program on the GPU: F6
 (patched as its variant) with the nine time-step subroutines that
 2d_nnf_lbm_amd/fortran/lbm_f6_shim.F90 replaces cut out, compiled with that
@@ -73,6 +80,7 @@ F6 = os.path.join(REF_ROOT, "6_FreeFall_YieldStress",
 F1 = os.path.join(REF_ROOT, "1_Disk_Newtonian", "D2Q9_1P_Validated_20190530.f90")
 F3 = os.path.join(REF_ROOT, "3_MultiParticle_Sedimentation",
                   "D2Q9_RTParticle_20191003.f90")
+F2 = os.path.join(REF_ROOT, "2_DKT_Newtonian", "D2Q9_DKT_Validated_20190926.f90")
 F4 = os.path.join(REF_ROOT, "4_Oscillating_Particle",
                   "D2Q9_V7_OscillatingCylinder_Validated.f90")
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -304,6 +312,97 @@ def build_contact_model(name, flags):
     return exe
 
 
+# Multi-particle builds of whole reference programs: name -> source program,
+# grid, steps and the disks.  F3 (N-particle sedimentation) reads its disks
+# from 0_Particle_Initial_Position.txt, which the fixture generator
+# (tests/golden/make_golden.py) writes into the run directory; F2 (the
+# two-disk DKT program) sets its two centres in the code (F2:161-162), which
+# the build replaces with the values given here.
+NPARTICLE_BUILDS = {
+    "f3_multi": dict(src="F3", nx=160, ny=160, n_particle=12, time_max=300),
+    "f2_dkt": dict(src="F2", nx=120, ny=240, time_max=400,
+                   centers=[("60.0D0", "150.0D0"), ("60.5D0", "129.75D0")]),
+}
+
+
+def patch_nparticle(text, nx, ny, time_max, n_particle=None, centers=None):
+    """F3 or F2 made runnable and observable, without touching the
+    algorithm:
+      * IF(0)/IF(1) to logicals and PAUSE removed, as for F6;
+      * PARAMETERs NX, NY (F3:8-9, F2:7-8), TIME_MAX (F3:37, F2:35) and
+        WRITE_OUTPUT (F3:6, F2:5: the final state dump, F3:425-447); F3's
+        initial N_PARTICLE (F3:68);
+      * F3: the STOP after the initial domain image (F3:202), which ends the
+        program before its time loop, commented out;
+      * F2: the two initial centres (F2:161-162) set to `centers`;
+      * two output lines: the run constants OMEGA, NU, A_G, RHO_S, RHO_F, PI
+        written to ref_constants.txt after OMEGA is set (F3:99, F2:97), and,
+        at the end of every loop iteration (after the mean-density line,
+        F3:378, F2:371), KK and each disk's CENTER, VELOCITY, FORCE and
+        ANGLE written to ref_history.txt."""
+    lines = text.replace("\r", "").split("\n")
+    out = []
+    in_loop = False
+    after_image = False
+    stop_done = False
+    for line in lines:
+        s = line
+        s = re.sub(r"\bIF\(0\)THEN", "IF(.FALSE.)THEN", s)
+        s = re.sub(r"\bIF\(1\)THEN", "IF(.TRUE.)THEN", s)
+        s = re.sub(r";\s*PAUSE\s*$", "", s)
+        s = re.sub(r"(INTEGER,PARAMETER :: NX=)\d+", r"\g<1>%d" % nx, s)
+        s = re.sub(r"(INTEGER,PARAMETER :: NY=)\d+", r"\g<1>%d" % ny, s)
+        s = re.sub(r"(INTEGER,PARAMETER :: TIME_MAX=).*$", r"\g<1>%d" % time_max, s)
+        s = re.sub(r"(LOGICAL,PARAMETER :: WRITE_OUTPUT=)\s*\.\w+\.", r"\g<1> .TRUE.", s)
+        if n_particle is not None:
+            s = re.sub(r"(INTEGER :: N_PARTICLE=)\d+", r"\g<1>%d" % n_particle, s)
+        if centers is not None:
+            m = re.match(r"(\s*)DISK\(([12])\)%CENTER=\(/", s)
+            if m:
+                x, y = centers[int(m.group(2)) - 1]
+                s = "%sDISK(%s)%%CENTER=(/%s,%s/)" % (m.group(1), m.group(2), x, y)
+        if n_particle is not None and after_image and not stop_done and re.match(r"\s*STOP\s*$", s):
+            s = "!" + s
+            stop_done = True
+        if re.match(r"\s*CALL WRITE_DOMAINIMAGE\(DOMAIN_IMAGE,0,10\)", s):
+            after_image = True
+        out.append(s)
+        if re.match(r"\s*OMEGA=1\.0/\(3\.\*NU\+0\.5\)", s):
+            out.append("\tOPEN(96,FILE='ref_constants.txt');"
+                       "WRITE(96,*) OMEGA,NU,A_G,RHO_S,RHO_F,PI;CLOSE(96)")
+            out.append("\tOPEN(97,FILE='ref_history.txt')")
+        if re.match(r"\s*DO WHILE\(KK<TIME_MAX\)", s):
+            in_loop = True
+        if in_loop and re.match(r"\s*WRITE\(8,\*\) KK,AvgDen", s):
+            out.append("\t\tWRITE(97,*) KK,(DISK(IP)%CENTER,DISK(IP)%VELOCITY,"
+                       "DISK(IP)%FORCE,DISK(IP)%ANGLE,IP=1,N_PARTICLE)")
+            in_loop = False
+    assert n_particle is None or stop_done, "F3's STOP before the time loop not found"
+    return "\n".join(out) + "\n"
+
+
+def build_nparticle(name, cfg, flags):
+    cfg = dict(cfg)
+    src_path = {"F3": F3, "F2": F2}[cfg.pop("src")]
+    with open(src_path, "rb") as fh:
+        text = fh.read().decode("ascii", errors="replace")
+    src = patch_nparticle(text, **cfg)
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")
+    try:
+        path = os.path.join(tmpdir, name + ".f90")
+        with open(path, "w") as fh:
+            fh.write(src)
+        exe = os.path.join(OUT_DIR, name)
+        r = subprocess.run([FLANG] + flags + [path, "-o", exe], cwd=tmpdir,
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode:
+            raise RuntimeError("%s build failed:\n%s" % (name, r.stdout[-3000:]))
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    return exe
+
+
 SHIM_ROUTINES = ["Collision_MRT", "STREAMING", "BOUNDARYCONDITION", "MACROSCOPICVARIABLES",
                  "ComputeConcenterationFunction", "ComputeForceField_SPM",
                  "ComputeForce_HYDRO", "VelocityCorrector", "ComputeCollisionForce"]
@@ -369,6 +468,9 @@ def main(argv):
         built.append(build(name, cfg, parity_flags))
     if os.path.exists(F3):
         built.append(build_f3_contact(parity_flags + ["-fno-automatic"]))
+    for name, cfg in NPARTICLE_BUILDS.items():
+        if os.path.exists({"F3": F3, "F2": F2}[cfg["src"]]):
+            built.append(build_nparticle(name, cfg, parity_flags + ["-fno-automatic"]))
     if os.path.exists(F4) and os.path.exists(F1):
         for name in CONTACT_MODELS:
             built.append(build_contact_model(name, parity_flags))

@@ -113,3 +113,41 @@ def contact_model_cases(name):
         out.append((list(edges), z[name + "_disks"][off:off + n], z[name + "_fcol"][off:off + n]))
         off += n
     return nx, ny, out
+
+
+# Contact constants and disk size of the whole-program builds (F3:739-742,
+# F2:710-713; D = L_C, F3:22, F2:21) and whether the program integrates the
+# angle (F2:331; F3:327 has it commented out).
+NPARTICLE_PROGRAMS = {
+    "F3": dict(diam=16.0, eps_p=2.0, eps_w=1.0, e_p=20.0, e_w=10.0, update_angle=0),
+    "F2": dict(diam=20.0, eps_p=1.0, eps_w=1.0, e_p=20.0, e_w=20.0, update_angle=1),
+}
+
+
+def nparticle_setup(meta, pos):
+    """Run parameters and disk records of a whole multi-particle program
+    build (F3's or F2's module PARAM with the grid the build sets): the
+    constants printed by the program itself (OMEGA, NU, A_G, RHO_S, RHO_F,
+    PI), the rate set (the same ComputeRelaxation_MRT as F6), the program's
+    Glowinski contact constants, and its disks (D = L_C, RHO_P = RHO_S,
+    ZITA = 1, mass and inertia in the reference's operation order, F3:151-155,
+    F2:166-172)."""
+    prog = NPARTICLE_PROGRAMS[meta["src"]]
+    d = prog["diam"]
+    cfg = f6_params(meta["nx"], meta["ny"], False)
+    # CIJ = -(RHO_S-RHO_F)*A_G*PI*0.5*D*0.5*D, left to right (F3:739, F2:710)
+    cij = -((((((meta["rho_s"] - meta["rho_f"]) * meta["a_g"]) * meta["pi"]) * 0.5) * d) * 0.5) * d
+    cfg.update(omega=meta["omega"], nu=meta["nu"], mu0=meta["nu"], tau_y=0.0,
+               gy=meta["a_g"], update_angle=prog["update_angle"], zeta_c=1.0, cij=cij,
+               eps_p=prog["eps_p"], eps_w=prog["eps_w"], e_p=prog["e_p"], e_w=prog["e_w"])
+    n = len(pos)
+    part = np.zeros((22, n))
+    rho_p = meta["rho_s"]
+    m_p = ((rho_p * meta["pi"]) * (0.5 * d)) * (0.5 * d)
+    part[0], part[1] = pos[:, 0], pos[:, 1]
+    part[6] = d
+    part[12] = rho_p
+    part[13] = m_p
+    part[14] = ((0.5 * m_p) * (0.5 * d)) * (0.5 * d)
+    part[15] = 1.0
+    return cfg, part

@@ -16,7 +16,11 @@ f1_contact_wt (the reference's Wan-Turek and Lennard-Jones contact forces,
 driven by oracle/contact_model_driver.f90): wall edges and disks in, F_COL out;
 and f3_contact.npz from oracle/_ref/f3_contact (the reference's N-body
 ComputeCollisionForce, F3:632-806, driven by oracle/f3_contact_driver.f90):
-disk positions and diameters in, F_COL of the second and third call out.
+disk positions and diameters in, F_COL of the second and third call out;
+f3_multi.npz and f2_dkt.npz from oracle/_ref/f3_multi and f2_dkt (F3's
+whole N-particle program, 12 disks, 300 steps; F2's two-disk program, 400
+steps): disk centres in, per-step disk states and the final F, UX, UY, RHO,
+G out.
 
 Usage: python tests/golden/make_golden.py [variant ...]   (needs oracle/_ref/ built here)
 """
@@ -163,6 +167,72 @@ def make_contact_models():
     print("wrote", os.path.relpath(path, REPO), os.path.getsize(path), "bytes")
 
 
+def f3_multi_positions():
+    """Disk centres of the F3 multi-particle run (12 disks of F3's D = 16 in
+    a 160 x 160 cavity), multiples of 1/64 so their decimal text is exact:
+      * two pairs and a triple whose surfaces are 0.25-0.75 lu apart, so their
+        smoothed profiles (phi > 0 within r + zeta/2) overlap and the
+        highest-index-wins overwrite of G (F6:487-494) decides nodes, and the
+        pair spring (d <= r_i + r_j + zeta, F3:746-770) acts;
+      * one disk within wall range of the bottom wall (F3:773-804);
+      * the last disk 7.75 lu from the left wall: its left-wall distance 15.5
+        <= r_i + r_j is the stale D_PW the next call's pair branch reads
+        (F3:760), so later steps take the E_P term."""
+    q = lambda v: float(np.round(v * 64) / 64)
+    pts = [(40.0, 120.0), (q(40.0 + 16.25 * np.cos(0.3)), q(120.0 + 16.25 * np.sin(0.3))),
+           (110.0, 125.0), (q(110.0 + 16.75 * np.cos(2.0)), q(125.0 + 16.75 * np.sin(2.0))),
+           (75.0, 80.0), (91.5, 80.0), (q(83.25 + 0.25), q(80.0 + 14.5)),
+           (128.0, 8.375), (30.0, 60.0), (130.0, 60.0), (80.0, 140.0), (7.75, 100.0)]
+    return np.array(pts)
+
+
+def make_nparticle(name):
+    """tests/golden/<name>.npz from oracle/_ref/<name> (a whole
+    multi-particle reference program, oracle/build_ref.py NPARTICLE_BUILDS):
+    the disk centres in, the run constants, the per-step CENTER, VELOCITY,
+    FORCE and ANGLE of every disk and the final state dump out."""
+    cfg = build_ref.NPARTICLE_BUILDS[name]
+    exe = os.path.join(build_ref.OUT_DIR, name)
+    if not os.path.exists(exe):
+        raise SystemExit("missing %s: run python oracle/build_ref.py" % exe)
+    if cfg["src"] == "F3":
+        pos = f3_multi_positions()
+        assert len(pos) == cfg["n_particle"]
+    else:
+        pos = np.array([[float(v.replace("D", "e")) for v in c] for c in cfg["centers"]])
+    n = len(pos)
+    with tempfile.TemporaryDirectory(prefix="golden_", dir="/tmp") as d:
+        if cfg["src"] == "F3":
+            with open(os.path.join(d, "0_Particle_Initial_Position.txt"), "w") as fh:
+                fh.write("".join("%r %r\n" % tuple(map(float, p)) for p in pos))
+        subprocess.run([exe], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+        with open(os.path.join(d, "ref_constants.txt")) as fh:
+            omega, nu, a_g, rho_s, rho_f, pi = (float(t) for t in fh.read().split())
+        with open(os.path.join(d, "ref_history.txt")) as fh:
+            toks = fh.read().split()
+        rows = np.array([float(t) for t in toks]).reshape(cfg["time_max"], 1 + 9 * n)
+        assert np.array_equal(rows[:, 0], np.arange(1, cfg["time_max"] + 1))
+        hist = rows[:, 1:].reshape(cfg["time_max"], n, 9)  # x y vx vy w fx fy tq angle
+        nx, ny = cfg["nx"], cfg["ny"]
+        nn = (nx + 1) * (ny + 1)
+
+        def grid(fname, nc):
+            with open(os.path.join(d, fname)) as fh:
+                a = np.array([float(t) for t in fh.read().split()])
+            assert a.size == nn * nc, fname
+            a = a.reshape(nx + 1, ny + 1, nc).transpose(1, 0, 2)
+            return np.ascontiguousarray(a if nc > 1 else a[..., 0])
+        out = dict(f=grid("0_F_output.txt", 9), ux=grid("0_UX_output.txt", 1),
+                   uy=grid("0_UY_output.txt", 1), rho=grid("0_RHO_output.txt", 1),
+                   g=grid("0_Gspm_output.txt", 2))
+    meta = dict(nx=cfg["nx"], ny=cfg["ny"], time_max=cfg["time_max"], src=cfg["src"],
+                omega=omega, nu=nu, a_g=a_g, rho_s=rho_s, rho_f=rho_f, pi=pi)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, meta=np.array(json.dumps(meta)), pos=pos, hist=hist, **out)
+    print("wrote", os.path.relpath(path, REPO), os.path.getsize(path), "bytes")
+
+
 def main(names):
     """names: the variants to write (default: every variant and the contact
     fixture)."""
@@ -171,6 +241,9 @@ def main(names):
         make_contact_models()
     if not names:
         make_contact()
+    for name in build_ref.NPARTICLE_BUILDS:
+        if not names or name in names:
+            make_nparticle(name)
     for name, cfg in build_ref.VARIANTS.items():
         if names and name not in names:
             continue

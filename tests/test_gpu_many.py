@@ -133,3 +133,45 @@ def test_hundred_particles_fp64_matches_oracle(orc):
     print("pairs: ref %d gpu %d" % (nref, len(pairs_g)), flush=True)
     assert nref > 0
     assert np.array_equal(pairs_g, pairs_ref)
+
+
+def set_stale_wall_distance(g, value):
+    """Set the contact force's stale wall distance (the D_PW the pair branch
+    reads, F3:760) through a snapshot: its last two doubles are that distance
+    and the module-global rate."""
+    snap = g.snapshot()
+    snap[-16:-8] = np.frombuffer(np.float64(value).tobytes(), dtype=np.uint8)
+    g.load_snapshot(snap)
+
+
+@pytest.mark.parametrize("name", ["f3_multi", "f2_dkt"])
+def test_fused_fp64_matches_multi_particle_reference_program(name):
+    """The fused FP64 path against whole multi-particle reference programs
+    (tests/golden/f3_multi.npz: F3, 12 disks, 300 steps; f2_dkt.npz: F2, two
+    disks in contact range, 400 steps), started like the compiled program
+    (stale wall distance 0): every disk's position, velocity and angle at
+    every step within 1e-9 relative, its force and torque within 1e-8, the
+    final populations and fields within the parity bounds."""
+    meta, gold = cases.load_golden(name)
+    cfg, part = cases.nparticle_setup(meta, gold["pos"])
+    n, steps = part.shape[1], meta["time_max"]
+    print("case: %s, %d disks, %d steps, fused FP64" % (name, n, steps), flush=True)
+    g = gpu_solver(cfg, max_particles=16)
+    g.initialize()
+    g.set_particles(part)
+    set_stale_wall_distance(g, 0.0)
+    hg = g.step(0, steps)
+    H = 12
+    # history columns in the fixture's order: x y vx vy w fx fy tq angle
+    cols = [0, 1, 3, 4, 5, 6, 7, 8, 2]
+    mine = np.stack([hg[:, [p * H + c for c in cols]] for p in range(n)], axis=1)
+    ref = gold["hist"]
+    for k, label in enumerate(("x", "y", "vx", "vy", "w", "fx", "fy", "tq", "angle")):
+        e = rel_err(mine[:, :, k], ref[:, :, k])
+        tol = 1e-8 if label in ("fx", "fy", "tq") else 1e-9
+        assert e <= tol, (label, e)
+    rho, ux, uy, gg, om = g.get_fields()
+    ref_f = dict(f=gold["f"], rho=gold["rho"], ux=gold["ux"], uy=gold["uy"], g=gold["g"],
+                 omega_bn=om)
+    check_field_errors(field_errors(g.f, rho, ux, uy, gg, om, ref_f), name)
+    g.close()

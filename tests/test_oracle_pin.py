@@ -230,3 +230,72 @@ def test_oracle_prescribed_oscillation_closed_form(orc):
     dv = np.diff(np.concatenate([[0.0], v]))
     assert np.allclose(fh, (1.0 / 1.5) * 29.45 * dv, rtol=1e-12, atol=1e-18)
     assert st.part[orc.PF_Y, 0] == 20.0 and st.part[orc.PF_VY, 0] == 0.0
+
+
+def nparticle_oracle(orc, name):
+    """The oracle set up as a whole multi-particle program build: its
+    parameters and disks, and the stale wall distance of the contact force at
+    0, the value the compiled program's saved D_PW holds before its first
+    call (static storage starts at zero; F3:760 reads it before F3:773-804
+    assign it)."""
+    meta, g = cases.load_golden(name)
+    cfg, part = cases.nparticle_setup(meta, g["pos"])
+    st = orc.State(orc.make_params(cfg), part.shape[1])
+    st.initialize()
+    st.set_particles(part)
+    st.c.dpw_stale = 0.0
+    return meta, g, cfg, part, st
+
+
+HCOLS = ("H_X", "H_Y", "H_VX", "H_VY", "H_W", "H_FX", "H_FY", "H_TQ", "H_ANG")
+
+
+def per_disk_history(orc, hist, n):
+    """Oracle history rows as [step, disk, (x y vx vy w fx fy tq angle)], the
+    layout of the fixtures' ref_history records."""
+    H = orc.H_PCOUNT
+    cols = [getattr(orc, c) for c in HCOLS]
+    return np.stack([hist[:, [p * H + c for c in cols]] for p in range(n)], axis=1)
+
+
+@pytest.mark.parametrize("name", ["f3_multi", "f2_dkt"])
+def test_oracle_multi_particle_step_vs_reference_program(orc, name):
+    """Whole multi-particle reference programs against the oracle, bit for
+    bit (oracle/build_ref.py patch_nparticle): F3, 12 disks in a 160 x 160
+    cavity, 300 steps; F2, two disks starting within contact range in a
+    120 x 240 channel, 400 steps.  Every disk's CENTER, VELOCITY, FORCE and
+    ANGLE at every step (the update loop over all disks, F3:312-328 without
+    and F2:311-332 with the angle; the multi-disk hydrodynamic sums,
+    F6:534-558; F2's assigned pair force, F2:715-744, which for two disks is
+    the N-body sum's result), and the final F, RHO, UX, UY and G (with
+    overlapping smoothed profiles, where the highest disk index sets G,
+    F6:487-494)."""
+    meta, g, cfg, part, st = nparticle_oracle(orc, name)
+    hist = st.run(0, meta["time_max"])
+    mine = per_disk_history(orc, hist, part.shape[1])
+    assert mine.shape == g["hist"].shape
+    assert np.array_equal(mine, g["hist"])
+    assert np.array_equal(st.f, g["f"])
+    assert np.array_equal(st.rho, g["rho"])
+    assert np.array_equal(st.ux, g["ux"])
+    assert np.array_equal(st.uy, g["uy"])
+    assert np.array_equal(st.g, g["g"])
+
+
+@pytest.mark.parametrize("name", ["f3_multi", "f2_dkt"])
+def test_multi_particle_fixtures_reach_the_rules(orc, name):
+    """The fixtures are ones where the rules in question decide results:
+    nodes in the smoothed profiles (phi > 0) of two disks at once, pairs
+    within contact range, every disk moving; F3's also has a disk within
+    wall range."""
+    meta, g, cfg, part, st = nparticle_oracle(orc, name)
+    x, y, d = part[orc.PF_X], part[orc.PF_Y], part[orc.PF_DIAM]
+    yy, xx = np.mgrid[0:meta["ny"] + 1, 0:meta["nx"] + 1]
+    reach = [np.hypot(xx - x[p], yy - y[p]) < 0.5 * d[p] + 0.5 for p in range(len(x))]
+    assert np.any(np.sum(reach, axis=0) >= 2)
+    pairs, npairs = st.contact(max_pairs=64)
+    assert npairs >= (4 if name == "f3_multi" else 1)
+    if name == "f3_multi":
+        assert np.any(part[orc.PF_Y] - 0.5 * part[orc.PF_DIAM] < 1.0)
+    h = g["hist"]
+    assert np.all(h[-1, :, 1] < h[0, :, 1])  # every disk falls
