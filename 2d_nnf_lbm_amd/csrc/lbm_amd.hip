@@ -29,9 +29,22 @@ struct lbm_handle {
     double *f = nullptr, *fp = nullptr, *rho = nullptr, *ux = nullptr, *uy = nullptr;
     double *g = nullptr, *obn = nullptr, *stress = nullptr;
     int *image = nullptr;
-    // reduced-format storage
+    // fused chain: storage populations (every format, FP64 included) and
+    // node rates (FP32 or FP64 as the storage), by chain parity
     void *sA = nullptr, *sB = nullptr;
-    float *rate32 = nullptr;
+    void *rate[2] = {nullptr, nullptr};
+    // STRESS and DOMAIN_IMAGE of the fused collisions, by iteration parity
+    double *astress[2] = {nullptr, nullptr};
+    int *aimage[2] = {nullptr, nullptr};
+    // contact pairs found by the fused launches, by chain parity
+    int *cpairs[2] = {nullptr, nullptr}, *cpaircnt[2] = {nullptr, nullptr};
+    int last_pairs = 0;  // which of them holds the last contact of the loop-end state
+    // reduced formats: the deviations of the canonical F in storage units as
+    // the FINAL launch computed them (FP32; w_k rho0 + h in FP64 rounds the
+    // smallest h), valid until the canonical F is written otherwise; a new
+    // chain's prologue starts from them, so a restart is exact
+    float *hexact = nullptr;
+    bool hexact_ok = false;
     void *erho[2] = {nullptr, nullptr};
     void *wtab = nullptr;  // wall velocities per edge class: 10 doubles then 10 floats
     unsigned long long *mpart[2] = {nullptr, nullptr};  // monitor partials by launch parity
@@ -71,11 +84,25 @@ struct lbm_handle {
     size_t fslot_bytes = 0;
     unsigned *pcount = nullptr;           // chunks done per particle (zero between launches)
     int kpart = 1;                        // helper blocks per particle
-    // launch tags never repeat on a handle (tile-list generations and
-    // tagged words); sr_base is the stochastic-rounding sequence position,
-    // restarted by lbm_initialize and carried by snapshots
-    unsigned tag_base = 1;
-    unsigned sr_base = 1;
+    // Word tags of the force partials never repeat on a handle (wtag_next);
+    // tile-list generations run along a chain (lg_next: beyond every one
+    // used so far); sr_next is the stochastic-rounding position of the
+    // collision of iteration res_kk + 1, restarted by lbm_initialize and
+    // carried by snapshots.
+    unsigned wtag_next = 1;
+    unsigned lg_next = 1;
+    unsigned sr_next = 1;
+    // Resident continuation of a fused run: the chain's last launch (index
+    // res_c, list generation res_lg, particle and wall-distance bases) ran
+    // loop iteration res_kk and collided iteration res_kk + 1; its inputs are
+    // intact, so the next fused call continues the chain, and the canonical
+    // FP64 arrays are materialised from them when something reads them
+    // (canon: they hold iteration res_kk).
+    bool res = false;
+    bool canon = true;
+    bool res_yield = false;  // the last launch stored yielded-count partials
+    int res_kk = 0, res_c = 0, res_pbase = 0, res_dbase = 0;
+    unsigned res_lg = 0;
 };
 
 namespace {
@@ -116,6 +143,28 @@ int check_launch(lbm_handle *h, const char *what) {
     if (e != hipSuccess) return fail(h, LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     return LBM_OK;
 }
+
+// The canonical FP64 arrays of a handle whose fused run is resident are
+// materialised (a FINAL launch) before anything reads them; a call that
+// changes them also ends the resident continuation, so the next fused call
+// starts a new chain from them (defined with the fused path below).
+int ensure_canonical(lbm_handle *h);
+inline void drop_resident(lbm_handle *h) {
+    h->res = false;
+    h->canon = true;
+}
+#define LBM_READS_STATE(h)                     \
+    do {                                       \
+        const int rc_ = ensure_canonical(h);   \
+        if (rc_) return rc_;                   \
+    } while (0)
+#define LBM_WRITES_STATE(h)                    \
+    do {                                       \
+        const int rc_ = ensure_canonical(h);   \
+        if (rc_) return rc_;                   \
+        drop_resident(h);                      \
+        (h)->hexact_ok = false;                \
+    } while (0)
 
 // ------------------------------------------------------------ static scratch
 // Device memory of lbm_step_hist_static: the monitor slots, the particle
@@ -331,10 +380,19 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
                 return rc;
             }
     }
-    if (p->storage != LBM_FP64) {
-        size_t sb = storage_bytes(p->storage);
+    {
+        const size_t sb = storage_bytes(p->storage);
+        const size_t rb = p->storage == LBM_FP64 ? 8 : 4;
         if ((rc = al(&h->sA, 9 * pl * sb)) || (rc = al(&h->sB, 9 * pl * sb)) ||
-            (rc = al((void **)&h->rate32, pl * 4))) {
+            (rc = al(&h->rate[0], pl * rb)) || (rc = al(&h->rate[1], pl * rb)) ||
+            (rc = al((void **)&h->astress[0], 3 * pl * 8)) ||
+            (rc = al((void **)&h->astress[1], 3 * pl * 8)) ||
+            (rc = al((void **)&h->aimage[0], pl * 4)) || (rc = al((void **)&h->aimage[1], pl * 4)) ||
+            (rc = al((void **)&h->cpairs[0], (size_t)P.maxp * MAXNB * 4)) ||
+            (rc = al((void **)&h->cpairs[1], (size_t)P.maxp * MAXNB * 4)) ||
+            (rc = al((void **)&h->cpaircnt[0], (size_t)P.maxp * 4)) ||
+            (rc = al((void **)&h->cpaircnt[1], (size_t)P.maxp * 4)) ||
+            (p->storage != LBM_FP64 && (rc = al((void **)&h->hexact, 9 * pl * 4)))) {
             *out = h;
             return rc;
         }
@@ -354,7 +412,9 @@ int lbm_destroy(lbm_handle *h) {
     if (!h) return LBM_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *bufs[] = {h->f, h->fp, h->rho, h->ux, h->uy, h->g, h->obn, h->stress, h->image,
-                    h->sA, h->sB, h->rate32, h->erho[0], h->erho[1], h->pbuf[0], h->pbuf[1],
+                    h->sA, h->sB, h->rate[0], h->rate[1], h->astress[0], h->astress[1],
+                    h->aimage[0], h->aimage[1], h->cpairs[0], h->cpairs[1], h->cpaircnt[0],
+                    h->cpaircnt[1], h->hexact, h->erho[0], h->erho[1], h->pbuf[0], h->pbuf[1],
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
@@ -385,9 +445,12 @@ int lbm_initialize(lbm_handle *h) {
     int rc = check_launch(h, "k_init");
     if (rc) return rc;
     // run state back to its start: the stochastic-rounding sequence restarts
-    // (launch tags go on: they only have to be unique), and the stale wall
-    // distance of the contact force is unset
-    h->sr_base = 1;
+    // (tags and list generations go on: they only have to be unique), no
+    // fused run is resident, and the stale wall distance of the contact
+    // force is unset
+    drop_resident(h);
+    h->hexact_ok = false;
+    h->sr_next = 1;
     for (int b = 0; b < 2; ++b)
         if (h->tl_hdr[b])
             HIPCHECK(h, hipMemsetAsync(h->tl_hdr[b], 0, (size_t)nblk(P.nn) * 8, h->stream));
@@ -427,24 +490,29 @@ static int download_planes(lbm_handle *h, const double *dev, int nc, double *hos
 
 int lbm_set_f(lbm_handle *h, const double *f) {
     if (!h || !f) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     return upload_planes(h, f, 9, h->f);
 }
 int lbm_get_f(lbm_handle *h, double *f) {
     if (!h || !f) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     return download_planes(h, h->f, 9, f);
 }
 int lbm_set_fpost(lbm_handle *h, const double *f) {
     if (!h || !f) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     return upload_planes(h, f, 9, h->fp);
 }
 int lbm_get_fpost(lbm_handle *h, double *f) {
     if (!h || !f) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     return download_planes(h, h->fp, 9, f);
 }
 
 int lbm_set_fields(lbm_handle *h, const double *rho, const double *ux, const double *uy,
                    const double *g, const double *omega_bn) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     int rc;
     if (rho && (rc = upload_planes(h, rho, 1, h->rho))) return rc;
     if (ux && (rc = upload_planes(h, ux, 1, h->ux))) return rc;
@@ -457,6 +525,7 @@ int lbm_set_fields(lbm_handle *h, const double *rho, const double *ux, const dou
 int lbm_get_fields(lbm_handle *h, double *rho, double *ux, double *uy, double *g,
                    double *omega_bn) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     int rc;
     if (rho && (rc = download_planes(h, h->rho, 1, rho))) return rc;
     if (ux && (rc = download_planes(h, h->ux, 1, ux))) return rc;
@@ -468,11 +537,13 @@ int lbm_get_fields(lbm_handle *h, double *rho, double *ux, double *uy, double *g
 
 int lbm_get_stress(lbm_handle *h, double *stress) {
     if (!h || !stress) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     return download_planes(h, h->stress, 3, stress);
 }
 
 int lbm_get_image(lbm_handle *h, int32_t *image) {
     if (!h || !image) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     HIPCHECK(h, hipMemcpyAsync(image, h->image, (size_t)h->P.nn * 4, hipMemcpyDeviceToHost,
                                h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -481,6 +552,7 @@ int lbm_get_image(lbm_handle *h, int32_t *image) {
 
 int lbm_set_particles(lbm_handle *h, int32_t n, const double *part) {
     if (!h || n < 0 || (n > 0 && !part)) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     if (n > h->prm.max_particles)
         return fail(h, LBM_ERR_CAPACITY, "more particles than max_particles");
     DevP &P = h->P;
@@ -526,6 +598,7 @@ int lbm_set_particles(lbm_handle *h, int32_t n, const double *part) {
 
 int lbm_get_particles(lbm_handle *h, double *part) {
     if (!h || !part) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     const DevP &P = h->P;
     std::vector<double> buf((size_t)PCOUNT * P.maxp);
     HIPCHECK(h, hipMemcpyAsync(buf.data(), h->pbuf[h->pcur], buf.size() * 8,
@@ -647,9 +720,11 @@ const char SNAP_MAGIC[8] = {'L', 'B', 'M', 'S', 'N', 'A', 'P', '1'};
 constexpr int SN_F = 0, SN_FP = 9, SN_RHO = 18, SN_UX = 19, SN_UY = 20, SN_G = 21, SN_OBN = 23,
               SN_STRESS = 24, SN_PLANES = 27;
 
-uint64_t snapshot_bytes(long long nn, int np) {
+// Reduced formats also hold the exact deviations of F (9 FP32 planes) after
+// DOMAIN_IMAGE, so a loaded snapshot continues bit for bit.
+uint64_t snapshot_bytes(long long nn, int np, int storage) {
     return sizeof(SnapHeader) + (uint64_t)nn * SN_PLANES * 8 + (uint64_t)nn * 4 +
-           (uint64_t)PCOUNT * np * 8 + 2 * 8;
+           (storage != LBM_FP64 ? (uint64_t)nn * 9 * 4 : 0) + (uint64_t)PCOUNT * np * 8 + 2 * 8;
 }
 
 }  // namespace
@@ -658,6 +733,7 @@ extern "C" {
 
 int lbm_restart_write(lbm_handle *h, const char *dir) {
     if (!h || !dir) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     const DevP &P = h->P;
     std::vector<double> f((size_t)P.nn * 9), rho(P.nn), ux(P.nn), uy(P.nn), g((size_t)P.nn * 2),
         obn(P.nn), part((size_t)PCOUNT * std::max(1, P.np));
@@ -714,6 +790,7 @@ int lbm_restart_write(lbm_handle *h, const char *dir) {
 
 int lbm_restart_read(lbm_handle *h, const char *dir) {
     if (!h || !dir) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     const size_t nn = (size_t)P.nn;
     std::vector<double> rec, f(nn * 9), rho(nn), ux(nn), uy(nn), g(nn * 2), obn(nn);
@@ -750,19 +827,20 @@ int lbm_restart_read(lbm_handle *h, const char *dir) {
 
 int lbm_snapshot_size(const lbm_handle *h, uint64_t *bytes) {
     if (!h || !bytes) return LBM_ERR_PARAM;
-    *bytes = snapshot_bytes(h->P.nn, h->P.np);
+    *bytes = snapshot_bytes(h->P.nn, h->P.np, h->prm.storage);
     return LBM_OK;
 }
 
 int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes) {
     if (!h || !buf) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     const DevP &P = h->P;
-    const uint64_t need = snapshot_bytes(P.nn, P.np);
+    const uint64_t need = snapshot_bytes(P.nn, P.np, h->prm.storage);
     if (bytes < need) return fail(h, LBM_ERR_CAPACITY, "snapshot buffer too small");
     SnapHeader hd;
     std::memset(&hd, 0, sizeof(hd));
     std::memcpy(hd.magic, SNAP_MAGIC, 8);
-    hd.version = 1; hd.gen_base = h->sr_base;
+    hd.version = 2; hd.gen_base = h->sr_next;
     hd.nx = P.nx; hd.ny = P.ny; hd.maxp = P.maxp; hd.np = P.np;
     hd.storage = h->prm.storage; hd.rheology = h->prm.rheology; hd.bytes = need;
     char *o = (char *)buf;
@@ -777,7 +855,23 @@ int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes) {
         return rc;
     int32_t *img = (int32_t *)(d + SN_PLANES * nn);
     if ((rc = lbm_get_image(h, img))) return rc;
-    double *tail = (double *)(img + nn);
+    char *next = (char *)(img + nn);
+    if (h->prm.storage != LBM_FP64) {
+        if (!h->hexact_ok) {
+            // canonical F set from outside: its deviations as a prologue derives them
+            const float S = h->prm.storage == LBM_FP16 ? StF16::S : 1.0f;
+            hipLaunchKernelGGL(k_hexact, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, S,
+                               h->hexact);
+            if ((rc = check_launch(h, "k_hexact"))) return rc;
+            h->hexact_ok = true;
+        }
+        float *hx = (float *)next;
+        for (int k = 0; k < 9; ++k)
+            HIPCHECK(h, hipMemcpyAsync(hx + k * nn, h->hexact + (size_t)k * P.plane, nn * 4,
+                                       hipMemcpyDeviceToHost, h->stream));
+        next += nn * 9 * 4;
+    }
+    double *tail = (double *)next;
     if (P.np > 0 && (rc = lbm_get_particles(h, tail))) return rc;
     tail += (size_t)PCOUNT * P.np;
     HIPCHECK(h, hipMemcpyAsync(tail, h->dpw[h->dcur], 8, hipMemcpyDeviceToHost, h->stream));
@@ -788,18 +882,21 @@ int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes) {
 
 int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes) {
     if (!h || !buf) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     SnapHeader hd;
     if (bytes < sizeof(hd)) return fail(h, LBM_ERR_PARAM, "snapshot too short");
     std::memcpy(&hd, buf, sizeof(hd));
-    if (std::memcmp(hd.magic, SNAP_MAGIC, 8) != 0 || hd.version != 1)
+    if (std::memcmp(hd.magic, SNAP_MAGIC, 8) != 0 || hd.version != 2)
         return fail(h, LBM_ERR_PARAM, "not an lbm snapshot");
+    if (hd.storage != h->prm.storage)
+        return fail(h, LBM_ERR_PARAM, "snapshot storage format differs from the handle's");
     if (hd.nx != P.nx || hd.ny != P.ny)
         return fail(h, LBM_ERR_PARAM, "snapshot grid differs from the handle's");
     if (hd.np > h->prm.max_particles)
         return fail(h, LBM_ERR_CAPACITY, "snapshot has more particles than max_particles");
     const size_t nn = (size_t)P.nn;
-    const uint64_t need = hd.np >= 0 ? snapshot_bytes(P.nn, hd.np) : 0;
+    const uint64_t need = hd.np >= 0 ? snapshot_bytes(P.nn, hd.np, hd.storage) : 0;
     if (hd.np < 0 || hd.bytes != need || bytes < need)
         return fail(h, LBM_ERR_PARAM, "snapshot size mismatch");
     const double *d = (const double *)((const char *)buf + sizeof(hd));
@@ -811,21 +908,34 @@ int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes) {
         return rc;
     const int32_t *img = (const int32_t *)(d + SN_PLANES * nn);
     HIPCHECK(h, hipMemcpyAsync(h->image, img, nn * 4, hipMemcpyHostToDevice, h->stream));
-    const double *tail = (const double *)(img + nn);
+    const char *next = (const char *)(img + nn);
+    const float *hx = nullptr;
+    if (h->prm.storage != LBM_FP64) {
+        hx = (const float *)next;
+        next += nn * 9 * 4;
+    }
+    const double *tail = (const double *)next;
     if ((rc = lbm_set_particles(h, hd.np, tail))) return rc;
+    if (hx) {
+        for (int k = 0; k < 9; ++k)
+            HIPCHECK(h, hipMemcpyAsync(h->hexact + (size_t)k * P.plane, hx + k * nn, nn * 4,
+                                       hipMemcpyHostToDevice, h->stream));
+    }
     tail += (size_t)PCOUNT * hd.np;
     HIPCHECK(h, hipMemcpyAsync(h->dpw[h->dcur], tail, 8, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(h, hipMemcpyAsync(h->lag, tail + 1, 8, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     // the stochastic-rounding sequence continues where the saved run was
-    // (launch tags go on)
-    h->sr_base = hd.gen_base;
+    // (launch tags go on), from the exact deviations
+    h->sr_next = hd.gen_base;
+    h->hexact_ok = hx != nullptr;
     return LBM_OK;
 }
 
 // --------------------------------------------------------------- drop-in
 int lbm_collide(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     if (h->prm.rheology != LBM_NEWTONIAN && h->prm.rate_mode == LBM_RATE_LAGGED) {
         hipLaunchKernelGGL(k_collide_lagged, dim3(1), dim3(64), 0, h->stream, P, h->f, h->rho,
@@ -839,6 +949,7 @@ int lbm_collide(lbm_handle *h) {
 
 int lbm_copy_post(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     HIPCHECK(h, hipMemcpyAsync(h->fp, h->f, (size_t)9 * h->P.plane * 8, hipMemcpyDeviceToDevice,
                                h->stream));
     return LBM_OK;
@@ -846,6 +957,7 @@ int lbm_copy_post(lbm_handle *h) {
 
 int lbm_stream(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     hipLaunchKernelGGL(k_stream, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->fp);
     return check_launch(h, "k_stream");
@@ -853,6 +965,7 @@ int lbm_stream(lbm_handle *h) {
 
 int lbm_boundary(lbm_handle *h, int32_t kk) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     (void)kk;
     const DevP &P = h->P;
     hipLaunchKernelGGL(k_boundary, dim3(nblk(P.nbnd)), dim3(NT), 0, h->stream, P, h->f, h->fp,
@@ -862,6 +975,7 @@ int lbm_boundary(lbm_handle *h, int32_t kk) {
 
 int lbm_macros(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     hipLaunchKernelGGL(k_macros, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->f, h->rho,
                        h->ux, h->uy);
@@ -870,6 +984,7 @@ int lbm_macros(lbm_handle *h) {
 
 int lbm_spm_phi(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     double *part = h->pbuf[h->pcur];
     if (P.np > 0) {
@@ -885,6 +1000,7 @@ int lbm_spm_phi(lbm_handle *h) {
 
 int lbm_spm_forcing(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     hipLaunchKernelGGL(k_forcing, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P,
                        h->pbuf[h->pcur], h->phi, h->porig, h->PS, h->ux, h->uy, h->g);
@@ -893,6 +1009,7 @@ int lbm_spm_forcing(lbm_handle *h) {
 
 int lbm_hydro_force(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     if (P.np == 0) return LBM_OK;
     hipLaunchKernelGGL(k_hydro_seq, dim3(nblk(P.np)), dim3(NT), 0, h->stream, P,
@@ -902,19 +1019,22 @@ int lbm_hydro_force(lbm_handle *h) {
 
 int lbm_correct(lbm_handle *h) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     hipLaunchKernelGGL(k_correct, dim3(nblk(P.nn)), dim3(NT), 0, h->stream, P, h->ux, h->uy,
                        h->g);
     return check_launch(h, "k_correct");
 }
 
-static int gather_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t *npairs) {
+// The pair list in (dcnt, dpairs): partner count and partners per particle.
+static int gather_pairs(lbm_handle *h, const int *dcnt, const int *dpairs, int32_t *pairs,
+                        int32_t max_pairs, int32_t *npairs) {
     const DevP &P = h->P;
     std::vector<int> cnt(P.np), nb((size_t)P.np * MAXNB);
     if (P.np > 0) {
-        HIPCHECK(h, hipMemcpyAsync(cnt.data(), h->paircnt, (size_t)P.np * 4,
+        HIPCHECK(h, hipMemcpyAsync(cnt.data(), dcnt, (size_t)P.np * 4,
                                    hipMemcpyDeviceToHost, h->stream));
-        HIPCHECK(h, hipMemcpyAsync(nb.data(), h->pairs, nb.size() * 4, hipMemcpyDeviceToHost,
+        HIPCHECK(h, hipMemcpyAsync(nb.data(), dpairs, nb.size() * 4, hipMemcpyDeviceToHost,
                                    h->stream));
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -939,6 +1059,7 @@ static int gather_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_
 
 int lbm_contact_force(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t *npairs) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     if (P.np > 0) {
         hipLaunchKernelGGL(k_contact, dim3(nblk(P.np)), dim3(NT), 0, h->stream, P,
@@ -948,17 +1069,20 @@ int lbm_contact_force(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t 
         if (rc) return rc;
         h->dcur ^= 1;
     }
-    if (pairs || npairs) return gather_pairs(h, pairs, max_pairs, npairs);
+    if (pairs || npairs) return gather_pairs(h, h->paircnt, h->pairs, pairs, max_pairs, npairs);
     return LBM_OK;
 }
 
 int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs, int32_t *npairs) {
     if (!h) return LBM_ERR_PARAM;
-    return gather_pairs(h, pairs, max_pairs, npairs);
+    LBM_READS_STATE(h);
+    return gather_pairs(h, h->cpaircnt[h->last_pairs], h->cpairs[h->last_pairs], pairs,
+                        max_pairs, npairs);
 }
 
 int lbm_particle_update(lbm_handle *h, int32_t kk) {
     if (!h) return LBM_ERR_PARAM;
+    LBM_WRITES_STATE(h);
     const DevP &P = h->P;
     if (P.np == 0) return LBM_OK;
     hipLaunchKernelGGL(k_update, dim3(nblk(P.np)), dim3(NT), 0, h->stream, P,
@@ -968,6 +1092,7 @@ int lbm_particle_update(lbm_handle *h, int32_t kk) {
 
 int lbm_total_mass(lbm_handle *h, double *mass) {
     if (!h || !mass) return LBM_ERR_PARAM;
+    LBM_READS_STATE(h);
     const DevP &P = h->P;
     unsigned nb = nblk(P.nn);
     hipLaunchKernelGGL(k_sum_partial, dim3(nb), dim3(NT), 0, h->stream, h->rho, P.nn, h->partial);
@@ -985,92 +1110,142 @@ int lbm_total_mass(lbm_handle *h, double *mass) {
 // ----------------------------------------------------------------- fused
 namespace {
 
-// Arguments of launch s of a fused call of nsteps steps on handle h (s = 0:
-// the prologue; fin: the last step), starting from the handle's current
-// buffers.  Buffers alternate with the parity of s; step_fields() sets the
-// per-step fields.
+// Where a call's launches sit in a chain of fused launches: chain index,
+// tag, list generation and rounding position of its launch s = 0, and the
+// particle and wall-distance buffers the chain started from.  A call that
+// starts a chain runs the prologue as s = 0; a call that continues one
+// starts at s = 1 (its s = 0 is the last launch of the call before).
+struct Chain {
+    int c0 = 0;
+    unsigned tag0 = 0, lg0 = 0, sr0 = 0;
+    int pbase = 0, dbase = 0;
+};
+
+// Arguments of launch s (chain launch c0 + s) of a fused call of nsteps
+// steps on handle h; fin: a FINAL launch with the inputs of that chain
+// launch.  step_fields() sets the per-step fields.
 template <bool DEV>
-StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fin) {
+StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fin,
+                   const Chain &ch) {
     const DevP &P = h->P;
-    const int np = P.np;
     StepArgs a;
     std::memset(&a, 0, sizeof(a));
     a.P = P;
     a.cf = h->f; a.cfp = h->fp; a.crho = h->rho; a.cux = h->ux; a.cuy = h->uy; a.cg = h->g;
-    a.cobn = h->obn; a.cstress = h->stress; a.cimage = h->image; a.rate32 = h->rate32;
-    a.pairs = h->pairs; a.paircnt = h->paircnt;
-    a.n_pblocks = np * h->kpart;
+    a.cobn = h->obn; a.cstress = h->stress; a.cimage = h->image;
+    for (int b = 0; b < 2; ++b) {
+        a.astress[b] = h->astress[b];
+        a.aimage[b] = h->aimage[b];
+        a.mpart[b] = h->mpart[b];
+    }
+    const int c = ch.c0 + s;
+    a.pairs = h->cpairs[c & 1]; a.paircnt = h->cpaircnt[c & 1];
+    a.n_pblocks = P.np * h->kpart;
     a.kpart = h->kpart;
     a.pcount = h->pcount;
     a.ntiles = (P.nn + NT - 1) / NT;
     a.err = h->err_flag;
     a.hist_base = record ? h->hist_part_use : nullptr;
     a.slots_base = record ? h->hist_slots_use : nullptr;
-    a.rowp = np * HPC;
+    a.rowp = P.np * HPC;
     a.kk0 = kk0;
-    a.tag0 = h->tag_base;
-    a.sr0 = h->sr_base;
+    a.tag0 = ch.tag0; a.lg0 = ch.lg0; a.sr0 = ch.sr0;
+    a.c0par = ch.c0 & 1;
+    a.fin = fin;
     a.tpb = 1;
-    a.mpart[0] = h->mpart[0]; a.mpart[1] = h->mpart[1];
     a.nred = (record && s > 0) ? 1 : 0;  // a reducer block when there is something to reduce
-    // tile lists: read by this launch's tiles (generation = its tag), written
-    // for the next launch
-    const unsigned g = h->tag_base + (unsigned)s;
+    // tile lists: read by this launch's tiles (its generation), written for
+    // the next launch
+    const unsigned g = ch.lg0 + (unsigned)s;
     a.tl_hdr_in = h->tl_hdr[g & 1]; a.tl_ids_in = h->tl_ids[g & 1];
     a.tl_hdr_out = h->tl_hdr[(g + 1) & 1]; a.tl_ids_out = h->tl_ids[(g + 1) & 1];
     a.fslot = h->fslot;
-    // storage buffers: FP64 runs ping-pong between F and F_POST
-    void *buf[2];
-    if (DEV) { buf[0] = h->sA; buf[1] = h->sB; }
-    else { buf[0] = h->f; buf[1] = h->fp; }
-    const int flip = np > 0 ? (s & 1) : 0;  // particle buffers advance once per launch
-    if (s == 0) {
-        // collide step kk0's state into f*_{kk0+1}; particles to kk0+1
-        a.src = DEV ? nullptr : buf[0];
-        a.dst = DEV ? buf[0] : buf[1];
+    void *buf[2] = {h->sA, h->sB};
+    if (fin) a.hx = h->hexact;
+    if (c == 0) {
+        a.hx_in = h->hexact_ok ? h->hexact : nullptr;
+        // prologue: collide step kk0's canonical state into f*_{kk0+1};
+        // particles to kk0+1
+        a.dst = buf[0];
         a.erho_dst = h->erho[0];
-        a.pin = h->pbuf[h->pcur]; a.pout = h->pbuf[h->pcur ^ 1];
-        a.spf_in = nullptr;
-        a.spf_out = DEV ? h->spf[h->pcur ^ 1] : nullptr;
-        a.pbox_in = h->pbox[h->pcur]; a.pbox_out = h->pbox[h->pcur ^ 1];
-        a.dpw_in = h->dpw[h->dcur]; a.dpw_out = h->dpw[h->dcur ^ 1];
+        a.rate_out = h->rate[0];
+        a.pin = h->pbuf[ch.pbase]; a.pout = h->pbuf[ch.pbase ^ 1];
+        a.spf_out = DEV ? h->spf[ch.pbase ^ 1] : nullptr;
+        a.pbox_in = h->pbox[ch.pbase]; a.pbox_out = h->pbox[ch.pbase ^ 1];
+        a.dpw_in = h->dpw[ch.dbase]; a.dpw_out = h->dpw[ch.dbase ^ 1];
         return a;
     }
-    const int cur = DEV ? ((s - 1) & 1) : (s & 1);  // buffer holding f*_k
-    a.src = buf[cur];
-    a.dst = buf[cur ^ 1];
-    const int e = (s - 1) & 1;
-    a.erho_src = h->erho[e];
-    a.erho_dst = h->erho[e ^ 1];
-    const int pc = h->pcur ^ flip;
+    a.src = buf[(c - 1) & 1];
+    a.dst = buf[c & 1];
+    a.erho_src = h->erho[(c - 1) & 1];
+    a.erho_dst = h->erho[c & 1];
+    a.rate_in = h->rate[(c - 1) & 1];
+    a.rate_out = h->rate[c & 1];
+    const int pc = ch.pbase ^ (c & 1);
     a.pin = h->pbuf[pc];
     a.pout = fin ? h->pbuf[pc] : h->pbuf[pc ^ 1];
     a.spf_in = DEV ? h->spf[pc] : nullptr;
     a.spf_out = (DEV && !fin) ? h->spf[pc ^ 1] : nullptr;
     a.pbox_in = h->pbox[pc];
     a.pbox_out = h->pbox[pc ^ 1];
-    const int dc = h->dcur ^ flip;
+    const int dc = ch.dbase ^ (c & 1);
     a.dpw_in = h->dpw[dc]; a.dpw_out = h->dpw[dc ^ 1];
     return a;
 }
 
-// Handle state after a fused call of nsteps steps: FP64 runs leave F in the
-// final step's destination and F_POST in its source; particle and wall-
-// distance buffers moved once per launch when there are particles.
-template <bool DEV>
-void finish_fused(lbm_handle *h, int32_t nsteps) {
-    if (!DEV) {
-        double *b0 = h->f, *b1 = h->fp;
-        const int cur = nsteps & 1;
-        h->f = cur ? b0 : b1;
-        h->fp = cur ? b1 : b0;
+// A chain that starts from the canonical state (prologue at s = 0).
+Chain new_chain(const lbm_handle *h) {
+    Chain ch;
+    ch.c0 = 0;
+    ch.tag0 = h->wtag_next;
+    ch.lg0 = h->lg_next;
+    ch.sr0 = h->sr_next;
+    ch.pbase = h->pcur;
+    ch.dbase = h->dcur;
+    return ch;
+}
+
+// The chain of the resident run, continued (s = 1 is the next launch).
+Chain resident_chain(const lbm_handle *h) {
+    Chain ch;
+    ch.c0 = h->res_c;
+    ch.tag0 = h->wtag_next - 1u;
+    ch.lg0 = h->res_lg;
+    ch.sr0 = h->sr_next;
+    ch.pbase = h->res_pbase;
+    ch.dbase = h->res_dbase;
+    return ch;
+}
+
+// Handle state after a call of nsteps launches on chain ch ended with
+// chain launch c0 + nsteps.  fin: that launch was a FINAL one (the canonical
+// arrays hold its iteration and nothing stays resident), else the run is
+// resident.  Either way the next collision (of iteration kk0 + nsteps + 1)
+// takes the rounding position of launch nsteps + 1 minus one, so calls
+// split anywhere give the results of one call.
+void finish_chain(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, bool fin,
+                  bool prologue, const Chain &ch) {
+    const int m = ch.c0 + nsteps;
+    h->wtag_next += (unsigned)nsteps + (prologue ? 1u : 0u);
+    h->sr_next = ch.sr0 + (unsigned)nsteps;
+    h->res_lg = ch.lg0 + (unsigned)nsteps;
+    h->lg_next = h->res_lg + 2u;
+    h->res_kk = kk0 + nsteps;
+    h->res_c = m;
+    h->res_pbase = ch.pbase;
+    h->res_dbase = ch.dbase;
+    h->res_yield = record;
+    if (fin) {
+        h->res = false;
+        h->canon = true;
+        h->hexact_ok = h->hexact != nullptr;
+        h->pcur = ch.pbase ^ (m & 1);
+        h->dcur = ch.dbase ^ (m & 1);
+        h->last_pairs = (m - 1) & 1;
+    } else {
+        h->res = true;
+        h->canon = false;
     }
-    if (h->P.np > 0) {
-        h->pcur ^= nsteps & 1;
-        h->dcur ^= nsteps & 1;
-    }
-    h->tag_base += (unsigned)nsteps + 1u;
-    h->sr_base += (unsigned)nsteps + 2u;
 }
 
 // Error bits reported by the kernels of the last call (cleared when read).
@@ -1102,44 +1277,52 @@ void launch_step(int boundary, dim3 grid, dim3 block, hipStream_t stream, const 
         hipLaunchKernelGGL((k_step<St, DEV, FINAL, LBM_BOUNCE_BACK>), grid, block, 0, stream, a);
 }
 
+// nsteps fused steps kk0+1 .. kk0+nsteps: the resident run continues when
+// it ended at kk0 (and, for a recording call, stored the yielded counts of
+// its last collision); otherwise a new chain starts from the canonical
+// state with the prologue.  The last launch is an ordinary one: the run
+// stays resident and the canonical arrays are materialised on demand.
 template <class St, bool DEV>
 int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed,
               double *ms_total, double *ms_main) {
     const DevP &P = h->P;
     const long long ntiles = (P.nn + NT - 1) / NT;
     const int n_pblocks = P.np * h->kpart;
-    // the step launches of a recording call carry one reducer block
+    const bool cont = h->res && h->res_kk == kk0 && (!record || h->res_yield);
+    int rc;
+    if (!cont) {
+        if ((rc = ensure_canonical(h))) return rc;
+        drop_resident(h);
+    }
+    const Chain ch = cont ? resident_chain(h) : new_chain(h);
     dim3 grid((unsigned)(n_pblocks + ntiles)), block(NT);
+    // the step launches of a recording call carry one reducer block
     dim3 sgrid((unsigned)(n_pblocks + (record ? 1 : 0) + ntiles));
 
     if (timed) (void)hipEventRecord(h->ev0, h->stream);
-    StepArgs a = make_args<DEV>(h, kk0, record, 0, nsteps == 0);
-    step_fields(a, 0, nsteps);
-    hipLaunchKernelGGL((k_prologue<St, DEV>), grid, block, 0, h->stream, a);
-    int rc = check_launch(h, "k_prologue");
-    if (rc) return rc;
-    for (int s = 1; s <= nsteps; ++s) {
-        const bool fin = s == nsteps;
-        a = make_args<DEV>(h, kk0, record, s, fin);
-        step_fields(a, s, nsteps);
-        // the non-final launches run back to back between evA and evB
-        if (timed && s == 1 && !fin) (void)hipEventRecord(h->evA, h->stream);
-        if (timed && fin && s > 1) (void)hipEventRecord(h->evB, h->stream);
-        if (fin)
-            launch_step<St, DEV, true>(P.boundary, sgrid, block, h->stream, a);
-        else
-            launch_step<St, DEV, false>(P.boundary, sgrid, block, h->stream, a);
-        rc = check_launch(h, "k_step");
-        if (rc) return rc;
+    StepArgs a;
+    if (!cont) {
+        a = make_args<DEV>(h, kk0, record, 0, false, ch);
+        step_fields(a, 0, nsteps);
+        hipLaunchKernelGGL((k_prologue<St, DEV>), grid, block, 0, h->stream, a);
+        if ((rc = check_launch(h, "k_prologue"))) return rc;
     }
+    // the step launches run back to back between evA and evB
+    if (timed) (void)hipEventRecord(h->evA, h->stream);
+    for (int s = 1; s <= nsteps; ++s) {
+        a = make_args<DEV>(h, kk0, record, s, false, ch);
+        step_fields(a, s, nsteps);
+        launch_step<St, DEV, false>(P.boundary, sgrid, block, h->stream, a);
+        if ((rc = check_launch(h, "k_step"))) return rc;
+    }
+    if (timed) (void)hipEventRecord(h->evB, h->stream);
     if (record) {
-        // the last launch's monitor partials (the prologue's when nsteps is 0)
+        // the mass partials of the last launch
         hipLaunchKernelGGL(k_monitor_final, dim3(1), block, 0, h->stream, a);
-        rc = check_launch(h, "k_monitor_final");
-        if (rc) return rc;
+        if ((rc = check_launch(h, "k_monitor_final"))) return rc;
     }
     if (timed) (void)hipEventRecord(h->ev1, h->stream);
-    finish_fused<DEV>(h, nsteps);
+    finish_chain(h, kk0, nsteps, record, false, !cont, ch);
     if ((rc = check_flags(h))) return rc;
     if (timed) {
         HIPCHECK(h, hipEventSynchronize(h->ev1));
@@ -1148,11 +1331,38 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
         *ms_total = t;
         if (ms_main) {
             float tm = 0.0f;
-            if (nsteps > 1) (void)hipEventElapsedTime(&tm, h->evA, h->evB);
+            (void)hipEventElapsedTime(&tm, h->evA, h->evB);
             *ms_main = tm;
         }
     }
     return LBM_OK;
+}
+
+// The FINAL launch that writes the loop-end state of iteration res_kk from
+// the inputs of the resident run's last launch.
+template <class St, bool DEV>
+int materialise(lbm_handle *h) {
+    const DevP &P = h->P;
+    const long long ntiles = (P.nn + NT - 1) / NT;
+    Chain ch;
+    ch.c0 = h->res_c - 1;
+    ch.tag0 = h->wtag_next - 1u;
+    ch.lg0 = h->res_lg - 1u;
+    ch.pbase = h->res_pbase;
+    ch.dbase = h->res_dbase;
+    StepArgs a = make_args<DEV>(h, h->res_kk - 1, false, 1, true, ch);
+    step_fields(a, 1, 1);
+    launch_step<St, DEV, true>(P.boundary, dim3((unsigned)(P.np * h->kpart + ntiles)), dim3(NT),
+                               h->stream, a);
+    int rc = check_launch(h, "k_step (final)");
+    if (rc) return rc;
+    h->wtag_next += 1u;
+    h->canon = true;
+    h->hexact_ok = h->hexact != nullptr;
+    h->pcur = h->res_pbase ^ (h->res_c & 1);
+    h->dcur = h->res_dbase ^ (h->res_c & 1);
+    h->last_pairs = (h->res_c - 1) & 1;
+    return check_flags(h);
 }
 
 int fused_dispatch(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed,
@@ -1162,6 +1372,16 @@ int fused_dispatch(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int 
         case LBM_FP32: return run_fused<StF32, true>(h, kk0, nsteps, record, timed, ms_total, ms_main);
         case LBM_FP16: return run_fused<StF16, true>(h, kk0, nsteps, record, timed, ms_total, ms_main);
         default: return run_fused<StBF16, true>(h, kk0, nsteps, record, timed, ms_total, ms_main);
+    }
+}
+
+int ensure_canonical(lbm_handle *h) {
+    if (!h->res || h->canon) return LBM_OK;
+    switch (h->prm.storage) {
+        case LBM_FP64: return materialise<StF64, false>(h);
+        case LBM_FP32: return materialise<StF32, true>(h);
+        case LBM_FP16: return materialise<StF16, true>(h);
+        default: return materialise<StBF16, true>(h);
     }
 }
 
@@ -1476,10 +1696,17 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
     const bool all = which.size() == b->sims.size();
     const bool record = hist != nullptr;
     const size_t n = b->sims.size();
+    std::vector<Chain> chains(n);
     for (int i : which) {
         lbm_handle *h = b->sims[i];
         int rc = check_fused_mode(h);
         if (rc) return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
+        // a batch call starts a chain from each simulation's canonical state
+        // and ends it with a FINAL launch
+        if ((rc = ensure_canonical(h)))
+            return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
+        drop_resident(h);
+        chains[i] = new_chain(h);
         if (record && (rc = prepare_history(h, nsteps)))
             return bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
         BHIPCHECK(b, hipStreamSynchronize(h->stream));
@@ -1491,8 +1718,8 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
         const lbm_handle *h = b->sims[i];
         const bool dev = b->storage != LBM_FP64;
         auto mk = [&](int s, bool fin) {
-            StepArgs a = dev ? make_args<true>(h, kk0, record, s, fin)
-                             : make_args<false>(h, kk0, record, s, fin);
+            StepArgs a = dev ? make_args<true>(h, kk0, record, s, fin, chains[i])
+                             : make_args<false>(h, kk0, record, s, fin, chains[i]);
             a.tpb = b->tpb[i];
             a.nred = 1;  // the block map always holds the reducer block
             return a;
@@ -1533,8 +1760,7 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
     int first = LBM_OK;
     for (int i : which) {
         lbm_handle *h = b->sims[i];
-        if (b->storage == LBM_FP64) finish_fused<false>(h, nsteps);
-        else finish_fused<true>(h, nsteps);
+        finish_chain(h, kk0, nsteps, record, true, true, chains[i]);
         if ((rc = check_flags(h)) && first == LBM_OK)
             first = bfail(b, rc, "simulation " + std::to_string(i) + ": " + h->err);
     }
@@ -1739,6 +1965,7 @@ struct Checkpoint {
 };
 
 int save_state(lbm_handle *h, double *dst, Checkpoint &c) {
+    LBM_READS_STATE(h);
     const size_t pl = (size_t)h->P.plane;
     const size_t pc = (size_t)PCOUNT * h->P.maxp;
     double *d = dst;
@@ -1750,14 +1977,16 @@ int save_state(lbm_handle *h, double *dst, Checkpoint &c) {
     HIPCHECK(h, hipMemcpyAsync(d, h->obn, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
     HIPCHECK(h, hipMemcpyAsync(d, h->pbuf[h->pcur], pc * 8, hipMemcpyDeviceToDevice, h->stream)); d += pc;
     HIPCHECK(h, hipMemcpyAsync(d, h->dpw[h->dcur], 8, hipMemcpyDeviceToDevice, h->stream));
-    c.pcur = h->pcur; c.dcur = h->dcur; c.gen = h->sr_base;
+    c.pcur = h->pcur; c.dcur = h->dcur; c.gen = h->sr_next;
     return LBM_OK;
 }
 
 int restore_state(lbm_handle *h, const double *src, const Checkpoint &c) {
     const size_t pl = (size_t)h->P.plane;
     const size_t pc = (size_t)PCOUNT * h->P.maxp;
-    h->pcur = c.pcur; h->dcur = c.dcur; h->sr_base = c.gen;
+    drop_resident(h);
+    h->hexact_ok = false;  // F is restored from the checkpoint
+    h->pcur = c.pcur; h->dcur = c.dcur; h->sr_next = c.gen;
     const double *d = src;
     HIPCHECK(h, hipMemcpyAsync(h->f, d, 9 * pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += 9 * pl;
     HIPCHECK(h, hipMemcpyAsync(h->rho, d, pl * 8, hipMemcpyDeviceToDevice, h->stream)); d += pl;
@@ -1886,6 +2115,7 @@ extern "C" int lbm_adjoint_velocity(lbm_handle *h, int32_t nsteps, int32_t windo
         if ((rc = restore_state(h, ck + cks[c].off, cks[c]))) return rc;
         for (int k = k0; k < k1; ++k) {
             if (k > k0 && (rc = lbm_step(h, k - 1, 1, nullptr))) return rc;
+            if ((rc = ensure_canonical(h))) return rc;
             double *sn = seg + (size_t)(k - k0) * snap_doubles;
             HIPCHECK(h, hipMemcpyAsync(sn, h->f, 9 * pl * 8, hipMemcpyDeviceToDevice, h->stream));
             HIPCHECK(h, hipMemcpyAsync(sn + 9 * pl, h->obn, pl * 8, hipMemcpyDeviceToDevice, h->stream));

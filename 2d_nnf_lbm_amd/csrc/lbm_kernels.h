@@ -375,6 +375,15 @@ __global__ void k_aos_to_soa(long long nn, int nc, const double *aos, double *so
     if (n >= nn) return;
     for (int c = 0; c < nc; ++c) soa[c * plane + n] = aos[n * nc + c];
 }
+// Deviations of the canonical F in storage units (scale S), as a prologue
+// derives them from F.
+__global__ void k_hexact(DevP P, const double *f, float S, float *hx) {
+    const long long n = (long long)blockIdx.x * NT + threadIdx.x;
+    if (n >= P.nn) return;
+    for (int k = 0; k < 9; ++k)
+        hx[k * P.plane + n] = (float)((f[k * P.plane + n] - (double)wdev(k) * P.rho0) * (double)S);
+}
+
 __global__ void k_soa_to_aos(long long nn, int nc, const double *soa, double *aos,
                              long long plane) {
     long long n = (long long)blockIdx.x * NT + threadIdx.x;
@@ -383,39 +392,62 @@ __global__ void k_soa_to_aos(long long nn, int nc, const double *soa, double *ao
 }
 
 // ================================================================= fused
-// One launch per time step.  Launch s (s = 1..n of a call of n steps) runs
-// step k = kk0 + s:
+// One launch per time step.  The launches of a fused run form a chain
+// c = 0, 1, 2, ...; chain launch c >= 1 runs loop iteration k (its call's
+// kk0 + s):
 //  * tile blocks pull-stream f*_k into f_k with the boundary rules, form
 //    rho and u, the SPM forcing G from particle state k (complete in memory
-//    since the launch before), correct u and collide into f*_{k+1} (or,
-//    FINAL, write the FP64 state); the tiles that hold nodes of a particle's
-//    reach box also write their share of its hydrodynamic force and torque
-//    (F6:542-556) as tagged words, and they are dispatched first;
-//  * particle blocks (one per particle, dispatched before the tiles)
-//    compute the contact force on state k (F3:739-804) at once, wait for
-//    their particle's partials, add them in a fixed order into FORCE_k, and
-//    run the Newton-Euler update to state k+1 (F6:299-313), which the next
-//    launch reads; the final launch only writes FORCE_k into the record.
-// The prologue (s = 0) collides the canonical FP64 state into the storage
-// format while its particle blocks advance state kk0 to kk0+1 with the
-// record's FORCE.  The particle state is double-buffered by launch parity.
+//    since the launch before), correct u and collide into f*_{k+1}, the
+//    collision of iteration k+1;
+//  * particle helper blocks (dispatched before the tiles) each gather one
+//    chunk of a particle's reach box: f_k, rho and u at its nodes exactly as
+//    the owning tile forms them, and the chunk's share of the hydrodynamic
+//    force and torque (F6:542-556) as tagged words; the block that completes
+//    a particle's count adds the chunks in a fixed order into FORCE_k,
+//    computes the contact force on state k (F3:739-804) and runs the
+//    Newton-Euler update to state k+1 (F6:299-313), which the next launch
+//    reads.
+// Chain launch 0 (the prologue) collides the canonical FP64 state into the
+// storage format while its particle blocks advance state kk0 to kk0+1 with
+// the record's FORCE.  A FINAL launch takes the inputs of a chain launch c
+// and, instead of colliding, writes the reference's loop-end state of
+// iteration k into the canonical FP64 arrays (F, F_POST, RHO, UX, UY, G,
+// OMEGA_BN, STRESS, DOMAIN_IMAGE and FORCE_k in the particle record).  The
+// storage populations, node rates, edge arrays and particle state are
+// double-buffered by chain parity, so a later call continues the chain
+// where the last launch left it, and the inputs of that last launch stay
+// intact for a FINAL launch that materialises its iteration on demand.
 struct StepArgs {
     DevP P;
     const void *src;              // post-collision populations f*_k (storage format)
-    void *dst;                    // f*_{k+1} (storage format) or, final step, f_k (FP64)
-    double *cf, *cfp;             // FP64 state arrays F, F_POST
+    void *dst;                    // f*_{k+1} (storage format)
+    double *cf, *cfp;             // canonical FP64 state arrays F, F_POST
     double *crho, *cux, *cuy, *cg, *cobn, *cstress;
     int *cimage;
-    float *rate32;                // node rates of the reduced formats
+    const void *rate_in;          // node rates of iteration k (FP32 or FP64 as the storage)
+    void *rate_out;               // node rates set by the collision of iteration k+1
+    // STRESS and DOMAIN_IMAGE of the collisions, by iteration parity: the
+    // last two collisions of every call write them (xstress, ximage), a
+    // FINAL launch copies iteration k's into the canonical arrays
+    double *astress[2];
+    int *aimage[2];
+    double *xstress;
+    int *ximage;
+    const double *astress_in;
+    const int *aimage_in;
     // edge arrays: previous-step density on the boundary nodes or, with
     // equilibrium edges, density and corrected velocity of each boundary
     // node's interior source (3 x nbnd)
     const void *erho_src;
     void *erho_dst;
     const double *pin;            // particle state k (SoA records)
-    double *pout;                 // particle state k+1 (final launch: same as pin)
+    double *pout;                 // particle state k+1 (FINAL: same as pin)
     const float *spf_in;          // FP32 profile records of state k (reduced formats)
     float *spf_out;               // those of state k+1
+    // reduced formats: deviations of the canonical F in storage units, exact
+    // (FINAL writes them beside F; a prologue starts from them when valid)
+    float *hx;
+    const float *hx_in;
     const int *pbox_in;           // reach boxes of state k [maxp][4]: i0, i1, j0, j1
     int *pbox_out;                // those of state k+1
     unsigned long long *fslot;    // tagged chunk force partials [maxp][kpart][SLOTW]
@@ -423,34 +455,37 @@ struct StepArgs {
     int kpart;                    // helper blocks (box chunks of NT nodes) per particle
     const double *dpw_in;
     double *dpw_out;
-    int *pairs, *paircnt;
+    int *pairs, *paircnt;         // contact pairs found by this launch
     double *hist_part;            // history row of step k, or null
     unsigned long long *slots_mass;   // NSLOT fixed-point mass sums of step k, or null
-    unsigned long long *slots_yield;  // NSLOT yielded-node counts of the collide's step
+    unsigned long long *slots_yield;  // NSLOT yielded-node counts of the collide's step, or null
+    int want_yield;               // the tiles store yielded-count partials
     int n_pblocks;
     int kk_upd;                   // loop iteration of the particle update in this launch
     long long ntiles;
-    int aux;                      // write STRESS / DOMAIN_IMAGE
+    int aux;                      // write STRESS / DOMAIN_IMAGE (FINAL: copy them)
+    int fin;                      // a FINAL launch
     // per-tile candidate lists (more than NP_LOOP particles): read for this
-    // launch (generation = its tag), written for the next one
+    // launch (generation list_gen), written for the next one (list_gen + 1)
     const unsigned long long *tl_hdr_in;
     const int *tl_ids_in;
     unsigned long long *tl_hdr_out;
     int *tl_ids_out;
-    unsigned tag;                 // this launch's tag (list generation, partial words)
-    unsigned sr;                  // stochastic-rounding sequence position
+    unsigned tag;                 // this launch's tag of the force partial words (unique)
+    unsigned list_gen;            // generation of the tile lists this launch reads
+    unsigned sr;                  // stochastic-rounding sequence position of its collide
     int *err;                     // LBM_EF_* bits
     // per-call bases from which step_fields() derives the per-step fields
     double *hist_base;            // history rows of the call, or null
     unsigned long long *slots_base;  // monitor slots of the call, or null
     int rowp;                     // history doubles per step (particles)
     int kk0;                      // step index before the call
-    unsigned tag0;                // tag of the call's prologue
-    unsigned sr0;                 // rounding sequence at the prologue
+    unsigned tag0, lg0, sr0;      // tag, list generation, rounding position at s = 0
+    int c0par;                    // chain parity of s = 0
     int tpb;                      // 256-node tiles per block (batched launches)
     // monitor partials: each tile block stores its fixed-point mass and its
     // yielded count (plain stores, [mass | yield] x ntiles, two buffers by
-    // launch parity); a reducer block of the next launch sums the previous
+    // chain parity); a reducer block of the next launch sums the previous
     // launch's partials into that launch's monitor slots
     unsigned long long *mpart[2];
     unsigned long long *mpart_out;        // this launch's partials, or null
@@ -467,33 +502,50 @@ enum {
     LBM_EF_SLOT = 16,     // a reach box holds more nodes than its helper blocks cover
 };
 
-// Per-step fields of launch s of a call of nsteps fused steps (s = 0: the
-// prologue; s = nsteps: the final step): the tag, the loop iteration of the
-// particle update, the history row and monitor slots, and whether the launch
-// writes the output-only fields.
+// Per-step fields of launch s of a call of nsteps steps (s = 0: the
+// prologue; a.fin: a FINAL launch, which materialises iteration kk0 + s):
+// the tag, list generation, rounding position, loop iteration of the
+// particle update, the history row and monitor slots, and whether the
+// launch writes the output-only fields.  History row r holds iteration
+// kk0 + r + 1: its mass comes from launch r + 1, its yielded count from the
+// collision in launch r (the yielded count of the last launch's collision
+// belongs to the next call's first row).
 __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nsteps) {
     const bool rec = a.slots_base != nullptr;
-    const bool fin = s == nsteps;
     a.tag = a.tag0 + (unsigned)s;
-    a.sr = a.sr0 + (unsigned)s + 1u;
+    a.list_gen = a.lg0 + (unsigned)s;
+    a.sr = a.sr0 + (unsigned)s;
     a.kk_upd = a.kk0 + s + 1;
-    if (s == 0) {
+    const size_t row2 = 2 * NSLOT;
+    if (a.fin) {
+        a.hist_part = (rec && s > 0) ? a.hist_base + (size_t)(s - 1) * (size_t)a.rowp : nullptr;
+        a.slots_mass = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * row2 : nullptr;
+        a.slots_yield = nullptr;
+        a.want_yield = 0;
+        a.aux = 1;
+    } else if (s == 0) {
         a.hist_part = nullptr;
         a.slots_mass = nullptr;
         a.slots_yield = rec ? a.slots_base + NSLOT : nullptr;
-        a.aux = nsteps == 1;
+        a.want_yield = rec;
+        a.aux = nsteps <= 1;
     } else {
         a.hist_part = a.hist_base ? a.hist_base + (size_t)(s - 1) * (size_t)a.rowp : nullptr;
-        a.slots_mass = rec ? a.slots_base + (size_t)(s - 1) * 2 * NSLOT : nullptr;
-        a.slots_yield = (rec && !fin) ? a.slots_base + (size_t)s * 2 * NSLOT + NSLOT : nullptr;
-        a.aux = (s == nsteps - 1) || fin;
+        a.slots_mass = rec ? a.slots_base + (size_t)(s - 1) * row2 : nullptr;
+        a.slots_yield = (rec && s < nsteps) ? a.slots_base + (size_t)s * row2 + NSLOT : nullptr;
+        a.want_yield = rec;
+        a.aux = s >= nsteps - 1;
     }
-    // the previous launch (s - 1) wrote mass partials from its second launch
-    // on and yield partials unless it was the last (it never is here)
-    a.mpart_out = rec ? a.mpart[s & 1] : nullptr;
-    a.mpart_in = (rec && s > 0) ? a.mpart[(s - 1) & 1] : nullptr;
-    a.red_mass = (rec && s > 1) ? a.slots_base + (size_t)(s - 2) * 2 * NSLOT : nullptr;
-    a.red_yield = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * 2 * NSLOT + NSLOT : nullptr;
+    const int ax = (a.kk0 + s + 1) & 1;  // parity of the iteration this launch collides
+    a.xstress = a.astress[ax];
+    a.ximage = a.aimage[ax];
+    a.astress_in = a.astress[ax ^ 1];
+    a.aimage_in = a.aimage[ax ^ 1];
+    const int par = (a.c0par + s) & 1;
+    a.mpart_out = rec ? a.mpart[par] : nullptr;
+    a.mpart_in = (rec && s > 0) ? a.mpart[par ^ 1] : nullptr;
+    a.red_mass = (rec && s > 1) ? a.slots_base + (size_t)(s - 2) * row2 : nullptr;
+    a.red_yield = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * row2 + NSLOT : nullptr;
 }
 
 __device__ __forceinline__ void set_err(const StepArgs &a, int bits) { atomicOr(a.err, bits); }
@@ -577,7 +629,8 @@ __device__ void particle_advance(const StepArgs &a, int p) {
         for (int f = 0; f < PCOUNT; ++f) bad = bad || !__builtin_isfinite(snew[f]);
         if (bad) set_err(a, LBM_EF_NAN);
     }
-    if (P.np > NP_LOOP) register_tiles(P, nb, p, a.tag + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
+    if (P.np > NP_LOOP)
+        register_tiles(P, nb, p, a.list_gen + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
 // Block totals of the monitors: FP64 mass as a fixed-point integer (so the
@@ -690,24 +743,25 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 double fk = a.cf[k * P.plane + n];
-                H[k] = (float)((fk - (double)wdev(k) * P.rho0) * (double)S);
+                H[k] = a.hx_in ? a.hx_in[k * P.plane + n]
+                               : (float)((fk - (double)wdev(k) * P.rho0) * (double)S);
             }
             const DevMom mo = dev_moments(H);
             float om = (float)(P.rheology != 0 ? a.cobn[n] : P.omega), om2, str[3];
             const float rhof = (float)r;
             const float rinvS = frcp(rhof) * St::invS;
-            if (a.aux || a.slots_yield)
+            if (a.aux || a.want_yield)
                 collide_dev<true, true>(P, H, mo, S, rhof, rinvS, (float)ux, (float)uy, (float)gx,
                                         (float)gy, om, om2, str, yielded);
             else
                 collide_dev<true, false>(P, H, mo, S, rhof, rinvS, (float)ux, (float)uy,
                                          (float)gx, (float)gy, om, om2, str, yielded);
             store_dev<St>(P, a.dst, (unsigned)n, H, node_hash((unsigned)n, a.sr));
-            if (P.rheology != 0) a.rate32[n] = om2;
+            if (P.rheology != 0) ((float *)a.rate_out)[n] = om2;
             if (a.aux) {
-                a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
-                a.cstress[2 * P.plane + n] = str[2];
-                if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
+                a.xstress[n] = str[0]; a.xstress[P.plane + n] = str[1];
+                a.xstress[2 * P.plane + n] = str[2];
+                if (P.rheology != 0) a.ximage[n] = yielded ? 0 : 5;
             }
             if (P.boundary == 2) {
                 if (b < 0) eq_put(a, P, i, j, (float)(r - P.rho0), (float)ux, (float)uy);
@@ -724,12 +778,12 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             collide64(P, f, r, ux, uy, gx, gy, om, o);
 #pragma unroll
             for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
-            if (P.rheology != 0) a.cobn[n] = o.omega_bn;
+            if (P.rheology != 0) ((double *)a.rate_out)[n] = o.omega_bn;
             yielded = o.yielded;
             if (a.aux) {
-                a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
-                a.cstress[2 * P.plane + n] = o.stress[2];
-                if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
+                a.xstress[n] = o.stress[0]; a.xstress[P.plane + n] = o.stress[1];
+                a.xstress[2 * P.plane + n] = o.stress[2];
+                if (P.rheology != 0) a.ximage[n] = o.yielded ? 0 : 5;
             }
             if (P.boundary == 2) {
                 if (b < 0) eq_put(a, P, i, j, r, ux, uy);
@@ -738,7 +792,7 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             }
         }
     }
-    if (a.slots_yield) tile_monitors<DEV>(a, tile, 0.0, yielded, false, true);
+    if (a.want_yield) tile_monitors<DEV>(a, tile, 0.0, yielded, false, true);
 }
 
 template <class St, bool DEV>
@@ -812,7 +866,7 @@ __device__ __forceinline__ int cand_resolve(const StepArgs &a, long long t, cons
         return __builtin_popcount(mask);
     }
     const unsigned long long h = uniform_u64(c.hdr);
-    if ((unsigned)(h >> 32) != a.tag) return 0;
+    if ((unsigned)(h >> 32) != a.list_gen) return 0;
     ids = a.tl_ids_in + t * MAXC;
     return min((int)(h & 0xffffffffu), MAXC);
 }
@@ -927,6 +981,16 @@ __device__ __forceinline__ void dev_velocity(const DevP &P, const DevMom &mo, fl
     uy = mo.jy * rinvS;
 }
 
+// FINAL launch, node n: STRESS and the yield-flag DOMAIN_IMAGE of
+// iteration k from the copies its collision wrote (by iteration parity), or,
+// Newtonian, DOMAIN_IMAGE from the particles (F6:438, F6:460-467).
+__device__ __forceinline__ void final_aux(const StepArgs &a, long long n, int i, int j) {
+    const DevP &P = a.P;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.cstress[c * P.plane + n] = a.astress_in[c * P.plane + n];
+    a.cimage[n] = P.rheology != 0 ? a.aimage_in[n] : image_node(P, a.pin, i, j);
+}
+
 // Outputs of a tile's node, reduced-precision storage: (FINAL = false) the
 // collision of step k+1 stored as f*_{k+1}, the rate, edge arrays and
 // auxiliary fields, or (FINAL = true) the FP64 state of step k; then the
@@ -947,26 +1011,27 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, 
                 double w0 = (double)wdev(k) * P.rho0;
                 a.cf[k * P.plane + n] = w0 + (double)H[k] * (double)invS;
                 a.cfp[k * P.plane + n] = w0 + (double)St::ld(s0 + k * P.plane + n) * (double)invS;
+                if (a.hx) a.hx[k * P.plane + n] = H[k];
             }
             a.crho[n] = (double)P.rho0 + (double)mo.rd * (double)invS;
             a.cux[n] = ucx; a.cuy[n] = ucy;
             a.cg[n] = gx; a.cg[P.plane + n] = gy;
-            if (P.rheology != 0) a.cobn[n] = a.rate32[n];
-            if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
+            if (P.rheology != 0) a.cobn[n] = ((const float *)a.rate_in)[n];
+            final_aux(a, n, i, j);
         } else {
             float om2, str[3];
-            if (a.aux || a.slots_yield)
+            if (a.aux || a.want_yield)
                 collide_dev<FORCED, true>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2, str,
                                           yielded);
             else
                 collide_dev<FORCED, false>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2,
                                            str, yielded);
             store_dev<St>(P, a.dst, (unsigned)n, H, node_hash((unsigned)n, a.sr));
-            if (P.rheology != 0) a.rate32[n] = om2;
+            if (P.rheology != 0) ((float *)a.rate_out)[n] = om2;
             if (a.aux) {
-                a.cstress[n] = str[0]; a.cstress[P.plane + n] = str[1];
-                a.cstress[2 * P.plane + n] = str[2];
-                if (P.rheology != 0) a.cimage[n] = yielded ? 0 : 5;
+                a.xstress[n] = str[0]; a.xstress[P.plane + n] = str[1];
+                a.xstress[2 * P.plane + n] = str[2];
+                if (P.rheology != 0) a.ximage[n] = yielded ? 0 : 5;
             }
             if (!lean) {
                 if ((BND >= 0 ? BND : P.boundary) == 2) {
@@ -978,10 +1043,10 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, 
             }
         }
     }
-    if (a.slots_mass || (!FINAL && a.slots_yield)) {
+    if (a.slots_mass || (!FINAL && a.want_yield)) {
         const double mass = valid ? (double)P.rho0 + (double)mo.rd * (double)invS : 0.0;
         tile_monitors<true>(a, tile, mass, yielded, a.slots_mass != nullptr,
-                            !FINAL && a.slots_yield != nullptr);
+                            !FINAL && a.want_yield);
     }
 }
 
@@ -999,7 +1064,9 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
     const typename St::T *src = (const typename St::T *)a.src;
     // the node rate and the candidates' inputs are requested before the
     // populations (loads complete in order)
-    const float om = (!FINAL && P.rheology != 0) ? a.rate32[valid ? n : P.nn - 1] : P.fomega;
+    const float om = (!FINAL && P.rheology != 0)
+                         ? ((const float *)a.rate_in)[valid ? n : P.nn - 1]
+                         : P.fomega;
     const CandPre cp = cand_prefetch(a, tile);
     int jw;
     const bool lean = wave_interior<BND>(P, tile, jw);
@@ -1044,7 +1111,7 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
     const typename St::T *src = (const typename St::T *)a.src;
     // the node rate and the candidates' inputs are requested before the
     // populations (loads complete in order)
-    const double om = (!FINAL && P.rheology != 0) ? a.cobn[nc] : P.omega;
+    const double om = (!FINAL && P.rheology != 0) ? ((const double *)a.rate_in)[nc] : P.omega;
     const CandPre cp = cand_prefetch(a, tile);
     double f[9], rho, ux, uy;
     load_fk64<St, BND>(P, src, (const double *)a.erho_src, nc, i, j, f);
@@ -1059,24 +1126,27 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
     int yielded = 0;
     if (valid) {
         if (FINAL) {
-            double *cfk = (double *)a.dst;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) cfk[k * P.plane + n] = f[k];
+            for (int k = 0; k < 9; ++k) {
+                a.cf[k * P.plane + n] = f[k];
+                a.cfp[k * P.plane + n] = St::ld(src + k * P.plane + n);
+            }
             a.crho[n] = rho; a.cux[n] = ucx; a.cuy[n] = ucy;
             a.cg[n] = gx; a.cg[P.plane + n] = gy;
-            if (a.aux && P.rheology == 0) a.cimage[n] = image_node(P, a.pin, i, j);
+            if (P.rheology != 0) a.cobn[n] = ((const double *)a.rate_in)[n];
+            final_aux(a, n, i, j);
         } else {
             typename St::T *dst = (typename St::T *)a.dst;
             CollideOut o;
             collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
 #pragma unroll
             for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
-            if (P.rheology != 0) a.cobn[n] = o.omega_bn;
+            if (P.rheology != 0) ((double *)a.rate_out)[n] = o.omega_bn;
             yielded = o.yielded;
             if (a.aux) {
-                a.cstress[n] = o.stress[0]; a.cstress[P.plane + n] = o.stress[1];
-                a.cstress[2 * P.plane + n] = o.stress[2];
-                if (P.rheology != 0) a.cimage[n] = o.yielded ? 0 : 5;
+                a.xstress[n] = o.stress[0]; a.xstress[P.plane + n] = o.stress[1];
+                a.xstress[2 * P.plane + n] = o.stress[2];
+                if (P.rheology != 0) a.ximage[n] = o.yielded ? 0 : 5;
             }
             const int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
             if ((BND >= 0 ? BND : P.boundary) == 2) {
@@ -1086,9 +1156,9 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
             }
         }
     }
-    if (a.slots_mass || (!FINAL && a.slots_yield))
+    if (a.slots_mass || (!FINAL && a.want_yield))
         tile_monitors<false>(a, tile, valid ? rho : 0.0, yielded, a.slots_mass != nullptr,
-                             !FINAL && a.slots_yield != nullptr);
+                             !FINAL && a.want_yield);
 }
 
 template <class St, bool DEV, bool FINAL, int BND>
@@ -1305,7 +1375,8 @@ __device__ void particle_helper(const StepArgs &a, int p, int c) {
         for (int f = 0; f < PCOUNT; ++f) bad = bad || !__builtin_isfinite(snew[f]);
         if (bad) set_err(a, LBM_EF_NAN);
     }
-    if (P.np > NP_LOOP) register_tiles(P, nb, p, a.tag + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
+    if (P.np > NP_LOOP)
+        register_tiles(P, nb, p, a.list_gen + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
 // Block layout of a step launch: the particle helper blocks (particle-major,

@@ -264,7 +264,7 @@ class D2Q9:
 
     def step_timed(self, kk0, nsteps):
         """Fused steps timed with HIP events: (ms over all launches of the call,
-        ms over the nsteps-1 back-to-back non-final fused step kernels)."""
+        ms over its nsteps back-to-back step launches)."""
         t, m = ctypes.c_double(0), ctypes.c_double(0)
         self._call("lbm_step_timed", int(kk0), int(nsteps), ctypes.byref(t), ctypes.byref(m))
         return t.value, m.value

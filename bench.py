@@ -64,7 +64,7 @@ def measure(pkg, configs, storage, steps, warmup, workload=None):
     wall = time.perf_counter() - t0
     n = configs.nodes(cfg)
     bpn = configs.bytes_per_node(storage, cfg["rheology"] == 1)
-    per_launch = ms_main / (steps - 1)  # ms per launch of the fused step kernel
+    per_launch = ms_main / steps  # ms per launch of the fused step kernel
     # the same steps with the per-step history (particle rows, mass and
     # yielded count: what the reference writes every step), by wall clock
     rec_steps = min(steps, 500)

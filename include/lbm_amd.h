@@ -160,11 +160,14 @@ int lbm_restart_read(lbm_handle *h, const char *dir);
 /* Binary snapshot of the whole run state in host memory: F, F_POST, RHO,
  * UX, UY, G, OMEGA_BN, STRESS, DOMAIN_IMAGE, the particle records, the stale
  * wall distance of the contact force, the module-global rate and the
- * position of the stochastic-rounding sequence.  Loading it into a handle
+ * position of the stochastic-rounding sequence; with reduced-precision
+ * storage also the deviations h = f - w_k rho0 of F exactly (FP32: the
+ * FP64 value w_k rho0 + h rounds the smallest h).  Loading it into a handle
  * created with the same parameters continues the run exactly: the next
  * lbm_step gives the same results bit for bit as it would have given on the
- * saved handle (a handle with another grid is refused).  No counterpart in the reference (its restart
- * files are the text ones above); for callers that checkpoint in memory. */
+ * saved handle (a handle with another grid or storage format is refused).
+ * No counterpart in the reference (its restart files are the text ones
+ * above); for callers that checkpoint in memory. */
 int lbm_snapshot_size(const lbm_handle *h, uint64_t *bytes);
 int lbm_snapshot_save(lbm_handle *h, void *buf, uint64_t bytes);
 int lbm_snapshot_load(lbm_handle *h, const void *buf, uint64_t bytes);
@@ -196,16 +199,25 @@ int lbm_total_mass(lbm_handle *h, double *mass);  /* SUM(RHO), F6:348 */
 /* Fused time steps kk0+1 .. kk0+nsteps (the loop body F6:293-349), all on the
  * device, in the storage precision of the handle.  hist (may be NULL):
  * nsteps rows of (n_particles*LBM_H_PCOUNT + 2) doubles -- the particle
- * records, then SUM(RHO) and the number of yielded nodes. */
+ * records, then SUM(RHO) and the number of yielded nodes.
+ * The state stays resident on the device between calls: a call that
+ * continues the previous one (kk0 equal to its last step, and nothing set
+ * or read through the per-subroutine entries in between) runs exactly
+ * nsteps step launches; the reference's loop-end arrays (F, F_POST, RHO,
+ * UX, UY, G, OMEGA_BN, STRESS, DOMAIN_IMAGE, the particle records) are
+ * written when a get/set call, a per-subroutine entry or a snapshot needs
+ * them.  Results do not depend on how the steps are split into calls. */
 int lbm_step(lbm_handle *h, int32_t kk0, int32_t nsteps, double *hist);
-/* Pairs found by the last contact evaluation of lbm_step. */
+/* Pairs found by the contact evaluation of the last step of lbm_step
+ * (the contact force of its loop-end state). */
 int lbm_last_pairs(lbm_handle *h, int32_t *pairs, int32_t max_pairs,
                    int32_t *npairs);
 /* Like lbm_step (no history), timed on the handle's stream with HIP events:
- * ms_total covers every launch of the call; ms_main covers the nsteps-1
- * non-final fused step kernels, which run back to back with no other work
- * between them (0 when nsteps == 1), so ms_main/(nsteps-1) is the average
- * duration of one launch of the main kernel. */
+ * ms_total covers every launch of the call; ms_main covers its nsteps step
+ * launches, which run back to back with no other work between them (a call
+ * that starts from the loop-end arrays first runs one prologue launch,
+ * outside ms_main), so ms_main/nsteps is the average duration of one launch
+ * of the main kernel. */
 int lbm_step_timed(lbm_handle *h, int32_t kk0, int32_t nsteps,
                    double *ms_total, double *ms_main);
 /* lbm_step with the history, allocating nothing: no counterpart in the

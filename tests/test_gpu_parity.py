@@ -372,3 +372,37 @@ def pkg_solver(cfg):
     import importlib
     c = dict(cfg, storage="fp64", max_particles=8)
     return importlib.import_module("2d_nnf_lbm_amd").D2Q9(c)
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp16", "bf16"])
+def test_split_calls_reads_and_sets_equal_one_call(storage):
+    """The state stays resident between lbm_step calls and the loop-end
+    arrays are written when read: 37 steps in one call equal 20 + 10 + 7
+    steps with the populations read after the first call and the whole state
+    saved and loaded back (a new chain from the loop-end arrays) after the
+    second -- history rows, populations, fields, stress, image and particles
+    bit for bit in every storage format.  In FP64 and FP16 the populations
+    written back unchanged through lbm_set_f restart exactly as well."""
+    cfg, part = f6_case(40, 120, True)
+    print("case: split calls with reads and sets,", storage, flush=True)
+    g1 = gpu_solver(cfg, storage=storage)
+    g1.initialize(); g1.set_particles(part)
+    h1 = g1.step(0, 37)
+    g2 = gpu_solver(cfg, storage=storage)
+    g2.initialize(); g2.set_particles(part)
+    ha = g2.step(0, 20)
+    f_mid = g2.f                       # materialises iteration 20; the run stays resident
+    hb = g2.step(20, 10)
+    g2.load_snapshot(g2.snapshot())    # ends the resident run: a new chain follows
+    if storage != "bf16":
+        g2.f = g2.f                    # exact through the FP64 populations
+    hc = g2.step(30, 7)
+    assert f_mid.shape == g1.f.shape
+    assert np.array_equal(h1, np.concatenate([ha, hb, hc]))
+    assert np.array_equal(g1.f, g2.f)
+    assert np.array_equal(g1.fpost, g2.fpost)
+    for x, y in zip(g1.get_fields(), g2.get_fields()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(g1.stress, g2.stress)
+    assert np.array_equal(g1.image, g2.image)
+    assert np.array_equal(g1.get_particles(), g2.get_particles())

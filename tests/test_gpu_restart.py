@@ -6,8 +6,8 @@ F6:377-405, READ_INPUT F6:174-198) and the binary run-state snapshot.
 - Writing and reading back restores the state bit for bit, and a run
   continued from the files equals the uninterrupted run.
 - A snapshot loaded into a fresh handle continues the run bit for bit, in
-  FP64 and in the reduced formats (whose stochastic-rounding sequence the
-  snapshot carries)."""
+  FP64 and in the reduced formats (the snapshot carries the stochastic-
+  rounding sequence and, reduced formats, the exact deviations of F)."""
 import importlib
 import os
 import subprocess
