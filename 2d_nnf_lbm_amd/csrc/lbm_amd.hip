@@ -100,6 +100,9 @@ struct lbm_handle {
     // (canon: they hold iteration res_kk).
     bool res = false;
     bool canon = true;
+    // the fused path addresses each 9-plane population buffer with 32-bit
+    // buffer offsets: below 4 GiB (FP64: about 59.6 million nodes)
+    bool fused_ok = true;
     bool res_yield = false;  // the last launch stored yielded-count partials
     int res_kk = 0, res_c = 0, res_pbase = 0, res_dbase = 0;
     unsigned res_lg = 0;
@@ -294,6 +297,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
             return bad_create("reduced-format population buffer of 4 GiB or more");
         }
         P.buf_bytes = (unsigned)std::min<unsigned long long>(bb, 0xFFFFFFFFULL);
+        h->fused_ok = bb < (1ULL << 32);
     }
     // magic numbers for row = node / W (Granlund-Montgomery round-up method)
     if (P.nn >= (1LL << 31)) { delete h; return bad_create("2^31 or more nodes"); }
@@ -380,7 +384,7 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
                 return rc;
             }
     }
-    {
+    if (h->fused_ok) {
         const size_t sb = storage_bytes(p->storage);
         const size_t rb = p->storage == LBM_FP64 ? 8 : 4;
         if ((rc = al(&h->sA, 9 * pl * sb)) || (rc = al(&h->sB, 9 * pl * sb)) ||
@@ -1386,6 +1390,10 @@ int ensure_canonical(lbm_handle *h) {
 }
 
 int check_fused_mode(lbm_handle *h) {
+    if (!h->fused_ok)
+        return fail(h, LBM_ERR_UNSUPPORTED,
+                    "the fused path addresses each population buffer below 4 GiB (FP64: about "
+                    "59.6 million nodes); use the per-subroutine entries for this grid");
     if (h->prm.rheology != LBM_NEWTONIAN && h->prm.rate_mode == LBM_RATE_LAGGED)
         return fail(h, LBM_ERR_UNSUPPORTED,
                     "the lagged module-global rate is a sequential recurrence; the GPU path "

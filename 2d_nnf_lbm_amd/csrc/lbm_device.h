@@ -141,6 +141,13 @@ struct StF64 {
     using T = double;
     __device__ static double ld(const T *p) { return *p; }
     __device__ static void st(T *p, double v, uint32_t) { *p = v; }
+    __device__ static double bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t r, double v, unsigned vo, unsigned so) {
+        using W = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(W, v), r, (int)vo, (int)so, 0);
+    }
 };
 struct StF32 {
     using T = float;
@@ -454,6 +461,20 @@ __device__ __forceinline__ void load_fk64(const DevP &P, const typename St::T *s
     }
 }
 
+// Raw buffer loads of the nine pulled FP64 populations of an interior node
+// (every source in the domain): one vector offset for the node, a uniform
+// offset per population.
+__device__ __forceinline__ void load_interior64(const DevP &P, const double *src, long long node,
+                                                double v[9]) {
+    const auto rs = make_rsrc(src, P.buf_bytes);
+    const unsigned vo = (unsigned)node * 8u;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const unsigned so = (unsigned)((k * P.plane - cxk(k) - (long long)cyk(k) * P.W) * 8LL);
+        v[k] = StF64::bld(rs, vo, so);
+    }
+}
+
 // MACROSCOPICVARIABLES, F6:1405-1408.
 __device__ __forceinline__ void macros64(const double v[9], double &rho, double &ux,
                                          double &uy) {
@@ -463,7 +484,30 @@ __device__ __forceinline__ void macros64(const double v[9], double &rho, double 
     uy = rinv * (v[5] + v[2] + v[6] - (v[7] + v[4] + v[8]));
 }
 
-// Solid fraction of one disk at node (i, j), F6:451-459.
+// sin x for |x| <= pi/2 from its Taylor series through x^21 (the first term
+// left out, x^23/23!, is below 1e-18 |x| there), evaluated by Horner's rule
+// in x^2 with fused multiply-adds: within about an ulp of sin, with a small
+// fraction of the registers of the general-argument sin.
+__device__ __forceinline__ double sin_halfpi(double x) {
+    const double x2 = x * x;
+    // sin x = x + x^3 (c1 + c2 x^2 + ... + c10 x^18), c_k = (-1)^k / (2k+1)!
+    double p = 1.0 / 51090942171709440000.0;                 // c10 = 1/21!
+    p = __builtin_fma(p, x2, -1.0 / 121645100408832000.0);  // c9 = -1/19!
+    p = __builtin_fma(p, x2, 1.0 / 355687428096000.0);      // c8 = 1/17!
+    p = __builtin_fma(p, x2, -1.0 / 1307674368000.0);       // c7 = -1/15!
+    p = __builtin_fma(p, x2, 1.0 / 6227020800.0);           // c6 = 1/13!
+    p = __builtin_fma(p, x2, -1.0 / 39916800.0);            // c5 = -1/11!
+    p = __builtin_fma(p, x2, 1.0 / 362880.0);               // c4 = 1/9!
+    p = __builtin_fma(p, x2, -1.0 / 5040.0);                // c3 = -1/7!
+    p = __builtin_fma(p, x2, 1.0 / 120.0);                  // c2 = 1/5!
+    p = __builtin_fma(p, x2, -1.0 / 6.0);                   // c1 = -1/3!
+    return __builtin_fma(x * x2, p, x);
+}
+
+// Solid fraction of one disk at node (i, j), F6:451-459.  POLY: the interface
+// sine by sin_halfpi (its argument is within pi/2 there), as the fused FP64
+// step evaluates it; otherwise the library sine.
+template <bool POLY = false>
 __device__ __forceinline__ double phi_at(double xc, double yc, double diam, double zeta,
                                          int i, int j) {
     double rp = 0.5 * diam;
@@ -473,7 +517,7 @@ __device__ __forceinline__ double phi_at(double xc, double yc, double diam, doub
     if (di <= -(0.5 * zeta))
         ph = 0.0;
     else if (fabs(di) < 0.5 * zeta)
-        ph = 0.5 * (sin(PI_F * di / zeta) + 1.0);
+        ph = 0.5 * ((POLY ? sin_halfpi(PI_F * di / zeta) : sin(PI_F * di / zeta)) + 1.0);
     else if (di >= 0.5 * zeta)
         ph = 1.0;
     return ph;
@@ -693,8 +737,20 @@ struct CollideOut {
     int yielded;         // tau_mag > tau_y
 };
 
+__device__ __forceinline__ void rheology64(const DevP &P, double rho, double om, double d11,
+                                           double d22, double d12, CollideOut &o);
+
 // Collision_MRT at one node, F6:878-958, with the node's own rate om
 // (Newtonian runs pass OMEGA).  f is overwritten by the post-collision values.
+// REF: the Chai strain rate in the reference's operation order (F6:903-916:
+// nine strain moments f1, then their sums); otherwise from the three column
+// sums those sums reduce to -- with e_k = s_k (m_k - meq_k),
+//   f1_1+f1_5+f1_6+f1_3+f1_7+f1_8 = 24 e0 + 6 e1 + 18 e7,
+//   f1_5+f1_2+f1_6+f1_7+f1_4+f1_8 = 24 e0 + 6 e1 - 18 e7,
+//   f1_5-f1_6+f1_7-f1_8 = 36 e8 --
+// which keeps four non-equilibrium moments live instead of eighteen values
+// (the fused step's register budget) and rounds differently in the last bit.
+template <bool REF = true>
 __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho, double ux,
                                           double uy, double gx, double gy, double om,
                                           CollideOut &o) {
@@ -726,8 +782,16 @@ __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho
     meq[5] = rho * (uy); meq[6] = rho * (-uy);
     meq[7] = rho * (ux * ux - uy * uy);
     meq[8] = rho * (ux * uy);
+    double e0 = 0.0, e1 = 0.0, e7 = 0.0, e8 = 0.0;
+    if constexpr (REF) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) mne[k] = m[k] - meq[k];
+        for (int k = 0; k < 9; ++k) mne[k] = m[k] - meq[k];
+    } else {
+        e0 = s[0] * (m[0] - meq[0]);
+        e1 = s[1] * (m[1] - meq[1]);
+        e7 = s[7] * (m[7] - meq[7]);
+        e8 = s[8] * (m[8] - meq[8]);
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k)
         m[k] = s[k] * meq[k] + (1.0 - s[k]) * m[k] + (1.0 - 0.5 * s[k]) * mf[k];
@@ -742,6 +806,8 @@ __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho
     f[7] = d * (4.0 * m[0] + 2.0 * m[1] + m[2] - 6.0 * m[3] - 3.0 * m[4] - 6.0 * m[5] - 3.0 * m[6] + 9.0 * m[8]);
     f[8] = d * (4.0 * m[0] + 2.0 * m[1] + m[2] + 6.0 * m[3] + 3.0 * m[4] - 6.0 * m[5] - 3.0 * m[6] - 9.0 * m[8]);
     // Chai strain rate, F6:903-916
+    double c11, c22, c12;
+    if constexpr (REF) {
     f1[1] = 4.0 * s[0] * mne[0] - 1.0 * s[1] * mne[1] - 2.0 * s[2] * mne[2] + 6.0 * s[3] * mne[3] - 6.0 * s[4] * mne[4] + 9.0 * s[7] * mne[7];
     f1[2] = 4.0 * s[0] * mne[0] - 1.0 * s[1] * mne[1] - 2.0 * s[2] * mne[2] + 6.0 * s[5] * mne[5] - 6.0 * s[6] * mne[6] - 9.0 * s[7] * mne[7];
     f1[3] = 4.0 * s[0] * mne[0] - 1.0 * s[1] * mne[1] - 2.0 * s[2] * mne[2] - 6.0 * s[3] * mne[3] + 6.0 * s[4] * mne[4] + 9.0 * s[7] * mne[7];
@@ -750,23 +816,38 @@ __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho
     f1[6] = 4.0 * s[0] * mne[0] + 2.0 * s[1] * mne[1] + 1.0 * s[2] * mne[2] - 6.0 * s[3] * mne[3] - 3.0 * s[4] * mne[4] + 6.0 * s[5] * mne[5] + 3.0 * s[6] * mne[6] - 9.0 * s[8] * mne[8];
     f1[7] = 4.0 * s[0] * mne[0] + 2.0 * s[1] * mne[1] + 1.0 * s[2] * mne[2] - 6.0 * s[3] * mne[3] - 3.0 * s[4] * mne[4] - 6.0 * s[5] * mne[5] - 3.0 * s[6] * mne[6] + 9.0 * s[8] * mne[8];
     f1[8] = 4.0 * s[0] * mne[0] + 2.0 * s[1] * mne[1] + 1.0 * s[2] * mne[2] + 6.0 * s[3] * mne[3] + 3.0 * s[4] * mne[4] - 6.0 * s[5] * mne[5] - 3.0 * s[6] * mne[6] - 9.0 * s[8] * mne[8];
+    c11 = f1[1] + f1[5] + f1[6] + f1[3] + f1[7] + f1[8];
+    c22 = f1[5] + f1[2] + f1[6] + f1[7] + f1[4] + f1[8];
+    c12 = f1[5] - f1[6] + f1[7] - f1[8];
+    } else {
+        const double base = 24.0 * e0 + 6.0 * e1;
+        c11 = base + 18.0 * e7;
+        c22 = base - 18.0 * e7;
+        c12 = 36.0 * e8;
+    }
     double zarib = -1.5 * d;
-    double d11 = zarib * (f1[1] + f1[5] + f1[6] + f1[3] + f1[7] + f1[8])
+    double d11 = zarib * c11
                + 0.75 * ((s[7] - s[1]) * (gx * ux + gy * uy) - s[7] * (2.0 * gx * ux));
-    double d22 = zarib * (f1[5] + f1[2] + f1[6] + f1[7] + f1[4] + f1[8])
+    double d22 = zarib * c22
                + 0.75 * ((s[7] - s[1]) * (gx * ux + gy * uy) - s[7] * (2.0 * gy * uy));
-    double d12 = zarib * (f1[5] - f1[6] + f1[7] - f1[8])
+    double d12 = zarib * c12
                + 0.75 * (-(s[7] * (gx * uy + gy * ux)));
+    rheology64(P, rho, om, d11, d22, d12, o);
+}
+
+// Stress, yield flag and the node's new rate from the strain rate d11, d22,
+// d12 (before the 1/rho of F6:921): Newtonian stress 2 nu d (F6:918), or the
+// Papanastasiou-Bingham viscosity (F6:921-957) or, rheology 2, the
+// Papanastasiou-Casson lines F6 carries commented out (F6:941, F6:944; the
+// model of F3:986-989, whose zero-shear limit is written without the square).
+__device__ __forceinline__ void rheology64(const DevP &P, double rho, double om, double d11,
+                                           double d22, double d12, CollideOut &o) {
     o.stress[0] = P.nu * 2.0 * d11;
     o.stress[1] = P.nu * 2.0 * d22;
     o.stress[2] = P.nu * 2.0 * d12;
     o.omega_bn = om;
     o.yielded = 0;
     if (P.rheology != 0) {
-        // Papanastasiou-Bingham viscosity, F6:921-957, or (rheology 2) the
-        // Papanastasiou-Casson lines F6 carries commented out (F6:941,
-        // F6:944; the model of F3:986-989, whose zero-shear limit is written
-        // without the square)
         double inv_rho = 1.0 / rho;
         d11 = d11 * inv_rho; d22 = d22 * inv_rho; d12 = d12 * inv_rho;
         double gam = sqrt(2.0 * fabs(d11 * d11 + d22 * d22 + 2.0 * (d12 * d12)));
@@ -789,6 +870,73 @@ __device__ __forceinline__ void collide64(const DevP &P, double f[9], double rho
         nu_a = nu_a * inv_rho;
         o.omega_bn = 1.0 / (3.0 * nu_a + 0.5);
     }
+}
+
+// The same collision for the fused FP64 step, written as a change of the
+// populations so that only f and the non-equilibrium moments are live (the
+// moment vectors of the reference form keep four times as many values in
+// registers).  With n_k = m_k - meq_k and the relaxation change
+// D_k = m*_k - m_k = -s_k n_k + (1 - s_k/2) mf_k (F6:889-890), the
+// reference's back transform f* = d A m* (A = 36 M^-1, the integer rows of
+// F6:892-900, d = fl32(1/36)) equals (36 d) f + d A D: the float-literal
+// factor 36 d of the reference is kept, the rounding differs in the last
+// bits.  The strain rate comes from the column sums of collide64<false>.
+__device__ __forceinline__ void collide64_fused(const DevP &P, double f[9], double rho,
+                                                double ux, double uy, double gx, double gy,
+                                                double om, CollideOut &o) {
+    double s[9];
+    rates64(P, om, s);
+    // moments (F6:878-886) as non-equilibrium parts (F6:964-989)
+    const double u2 = ux * ux + uy * uy;
+    const double n0 = (f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8]) - rho;
+    const double n1 = (-(4.0 * f[0]) - f[1] - f[2] - f[3] - f[4] + 2.0 * (f[5] + f[6] + f[7] + f[8]))
+                    - rho * (-2.0 + 3.0 * u2);
+    const double n2 = (4.0 * f[0] - 2.0 * (f[1] + f[2] + f[3] + f[4]) + (f[5] + f[6] + f[7] + f[8]))
+                    - rho * (1.0 - 3.0 * u2);
+    const double j3 = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+    const double j5 = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+    const double n3 = j3 - rho * ux, n5 = j5 - rho * uy;
+    const double n4 = (-(2.0 * (f[1] - f[3])) + f[5] - f[6] - f[7] + f[8]) + rho * ux;
+    const double n6 = (-(2.0 * (f[2] - f[4])) + f[5] + f[6] - f[7] - f[8]) + rho * uy;
+    const double n7 = (f[1] - f[2] + f[3] - f[4]) - rho * (ux * ux - uy * uy);
+    const double n8 = (f[5] - f[6] + f[7] - f[8]) - rho * (ux * uy);
+    // forcing moments mf (F6:964-989)
+    const double fx = rho * (gx + 0.0), fy = rho * (gy + 0.0);
+    const double fu = 6.0 * (fx * ux + fy * uy);
+    // D_k = -s_k n_k + (1 - s_k/2) mf_k
+    const double D0 = -s[0] * n0;
+    const double D1 = -s[1] * n1 + (1.0 - 0.5 * s[1]) * fu;
+    const double D2 = -s[2] * n2 - (1.0 - 0.5 * s[2]) * fu;
+    const double D3 = -s[3] * n3 + (1.0 - 0.5 * s[3]) * fx;
+    const double D4 = -s[4] * n4 - (1.0 - 0.5 * s[4]) * fx;
+    const double D5 = -s[5] * n5 + (1.0 - 0.5 * s[5]) * fy;
+    const double D6 = -s[6] * n6 - (1.0 - 0.5 * s[6]) * fy;
+    const double D7 = -s[7] * n7 + (1.0 - 0.5 * s[7]) * (2.0 * (fx * ux - fy * uy));
+    const double D8 = -s[8] * n8 + (1.0 - 0.5 * s[8]) * (fy * ux + fx * uy);
+    // f* = (36 d) f + d A D
+    const double d = D_MRT, c = 36.0 * D_MRT;
+    const double a0 = 4.0 * D0, b1 = a0 - D1 - 2.0 * D2, b5 = a0 + 2.0 * D1 + D2;
+    const double x3 = 6.0 * D3 - 6.0 * D4, x5 = 6.0 * D5 - 6.0 * D6, x7 = 9.0 * D7;
+    const double y3 = 6.0 * D3 + 3.0 * D4, y5 = 6.0 * D5 + 3.0 * D6, y8 = 9.0 * D8;
+    f[0] = c * f[0] + d * (4.0 * (D0 - D1 + D2));
+    f[1] = c * f[1] + d * (b1 + x3 + x7);
+    f[2] = c * f[2] + d * (b1 + x5 - x7);
+    f[3] = c * f[3] + d * (b1 - x3 + x7);
+    f[4] = c * f[4] + d * (b1 - x5 - x7);
+    f[5] = c * f[5] + d * (b5 + y3 + y5 + y8);
+    f[6] = c * f[6] + d * (b5 - y3 + y5 - y8);
+    f[7] = c * f[7] + d * (b5 - y3 - y5 + y8);
+    f[8] = c * f[8] + d * (b5 + y3 - y5 - y8);
+    // Chai strain rate (F6:903-916) from the column sums
+    const double e0 = s[0] * n0, e1 = s[1] * n1, e7 = s[7] * n7, e8 = s[8] * n8;
+    const double base = 24.0 * e0 + 6.0 * e1;
+    const double zarib = -1.5 * d;
+    double d11 = zarib * (base + 18.0 * e7)
+               + 0.75 * ((s[7] - s[1]) * (gx * ux + gy * uy) - s[7] * (2.0 * gx * ux));
+    double d22 = zarib * (base - 18.0 * e7)
+               + 0.75 * ((s[7] - s[1]) * (gx * ux + gy * uy) - s[7] * (2.0 * gy * uy));
+    double d12 = zarib * (36.0 * e8) + 0.75 * (-(s[7] * (gx * uy + gy * ux)));
+    rheology64(P, rho, om, d11, d22, d12, o);
 }
 
 // ------------------------------------------------ FP32 deviation pipeline

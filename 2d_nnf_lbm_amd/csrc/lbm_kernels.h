@@ -756,8 +756,11 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             else
                 collide_dev<true, false>(P, H, mo, S, rhof, rinvS, (float)ux, (float)uy,
                                          (float)gx, (float)gy, om, om2, str, yielded);
-            store_dev<St>(P, a.dst, (unsigned)n, H, node_hash((unsigned)n, a.sr));
-            if (P.rheology != 0) ((float *)a.rate_out)[n] = om2;
+            store_dev<St>(P, a.dst, n, H, node_hash(n, a.sr));
+            if (P.rheology != 0)
+                __builtin_amdgcn_raw_buffer_store_b32(
+                    __builtin_bit_cast(unsigned, om2),
+                    make_rsrc(a.rate_out, (unsigned)(P.plane * 4)), (int)(n * 4u), 0, 0);
             if (a.aux) {
                 a.xstress[n] = str[0]; a.xstress[P.plane + n] = str[1];
                 a.xstress[2 * P.plane + n] = str[2];
@@ -775,7 +778,7 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             for (int k = 0; k < 9; ++k) f[k] = a.cf[k * P.plane + n];
             CollideOut o;
             double om = P.rheology != 0 ? a.cobn[n] : P.omega;
-            collide64(P, f, r, ux, uy, gx, gy, om, o);
+            collide64_fused(P, f, r, ux, uy, gx, gy, om, o);
 #pragma unroll
             for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
             if (P.rheology != 0) ((double *)a.rate_out)[n] = o.omega_bn;
@@ -943,7 +946,7 @@ __device__ __forceinline__ void tile_forcing(const StepArgs &a, int cnt, unsigne
                 best = p;
             }
         } else {
-            const double ph = valid ? phi_at(cd.xc, cd.yc, cd.dm, cd.zt, i, j) : 0.0;
+            const double ph = valid ? phi_at<true>(cd.xc, cd.yc, cd.dm, cd.zt, i, j) : 0.0;
             if (ph > 0.0 && p > best) {
                 gx = ph * ((cd.vx - cd.w * ((double)j - cd.yc)) - u);
                 gy = ph * ((cd.vy + cd.w * ((double)i - cd.xc)) - v);
@@ -984,7 +987,7 @@ __device__ __forceinline__ void dev_velocity(const DevP &P, const DevMom &mo, fl
 // FINAL launch, node n: STRESS and the yield-flag DOMAIN_IMAGE of
 // iteration k from the copies its collision wrote (by iteration parity), or,
 // Newtonian, DOMAIN_IMAGE from the particles (F6:438, F6:460-467).
-__device__ __forceinline__ void final_aux(const StepArgs &a, long long n, int i, int j) {
+__device__ __forceinline__ void final_aux(const StepArgs &a, unsigned n, int i, int j) {
     const DevP &P = a.P;
 #pragma unroll
     for (int c = 0; c < 3; ++c) a.cstress[c * P.plane + n] = a.astress_in[c * P.plane + n];
@@ -996,7 +999,7 @@ __device__ __forceinline__ void final_aux(const StepArgs &a, long long n, int i,
 // auxiliary fields, or (FINAL = true) the FP64 state of step k; then the
 // monitor partials.  FORCED: G may be non-zero.
 template <class St, bool FINAL, int BND, bool FORCED>
-__device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, long long n,
+__device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, unsigned n,
                                              bool valid, bool lean, int i, int j, float (&H)[9],
                                              const DevMom &mo, float rho, float rinvS, float ucx,
                                              float ucy, float gx, float gy, float om) {
@@ -1026,8 +1029,11 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, 
             else
                 collide_dev<FORCED, false>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2,
                                            str, yielded);
-            store_dev<St>(P, a.dst, (unsigned)n, H, node_hash((unsigned)n, a.sr));
-            if (P.rheology != 0) ((float *)a.rate_out)[n] = om2;
+            store_dev<St>(P, a.dst, n, H, node_hash(n, a.sr));
+            if (P.rheology != 0)
+                __builtin_amdgcn_raw_buffer_store_b32(
+                    __builtin_bit_cast(unsigned, om2),
+                    make_rsrc(a.rate_out, (unsigned)(P.plane * 4)), (int)(n * 4u), 0, 0);
             if (a.aux) {
                 a.xstress[n] = str[0]; a.xstress[P.plane + n] = str[1];
                 a.xstress[2 * P.plane + n] = str[2];
@@ -1059,13 +1065,15 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, 
 template <class St, bool FINAL, int BND>
 __device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
     const DevP &P = a.P;
-    const long long n = tile * NT + threadIdx.x;
-    const bool valid = n < P.nn;
+    const unsigned n = (unsigned)tile * NT + threadIdx.x;
+    const bool valid = n < (unsigned)P.nn;
     const typename St::T *src = (const typename St::T *)a.src;
     // the node rate and the candidates' inputs are requested before the
-    // populations (loads complete in order)
+    // populations (loads complete in order); lanes past the last node read
+    // the zero padding of the rate plane
     const float om = (!FINAL && P.rheology != 0)
-                         ? ((const float *)a.rate_in)[valid ? n : P.nn - 1]
+                         ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                               make_rsrc(a.rate_in, (unsigned)(P.plane * 4)), (int)(n * 4u), 0, 0))
                          : P.fomega;
     const CandPre cp = cand_prefetch(a, tile);
     int jw;
@@ -1074,12 +1082,12 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
     float H[9];
     if (lean) {
         j = jw;
-        i = (int)(n - (long long)jw * P.W);
+        i = (int)(n - (unsigned)jw * (unsigned)P.W);
         load_interior_dev<St>(P, src, n, H);
     } else {
-        const long long nc = valid ? n : P.nn - 1;
-        j = row_of(P, (unsigned)nc);
-        i = (int)(nc - (long long)j * P.W);
+        const unsigned nc = valid ? n : (unsigned)P.nn - 1u;
+        j = row_of(P, nc);
+        i = (int)(nc - (unsigned)j * (unsigned)P.W);
         load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, nc, i, j, H);
     }
     unsigned mask;
@@ -1104,17 +1112,30 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
 template <class St, bool FINAL, int BND>
 __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
     const DevP &P = a.P;
-    const long long n = tile * NT + threadIdx.x;
-    const bool valid = n < P.nn;
-    const long long nc = valid ? n : P.nn - 1;
-    const int j = row_of(P, (unsigned)nc), i = (int)(nc - (long long)j * P.W);
+    const unsigned n = (unsigned)tile * NT + threadIdx.x;
+    const bool valid = n < (unsigned)P.nn;
+    const unsigned nc = valid ? n : (unsigned)P.nn - 1u;
     const typename St::T *src = (const typename St::T *)a.src;
     // the node rate and the candidates' inputs are requested before the
-    // populations (loads complete in order)
-    const double om = (!FINAL && P.rheology != 0) ? ((const double *)a.rate_in)[nc] : P.omega;
+    // populations (loads complete in order); lanes past the last node read
+    // the zero padding of the rate plane
+    const double om = (!FINAL && P.rheology != 0)
+                          ? StF64::bld(make_rsrc(a.rate_in, (unsigned)(P.plane * 8)), n * 8u, 0u)
+                          : P.omega;
     const CandPre cp = cand_prefetch(a, tile);
+    int jw;
+    const bool lean = wave_interior<BND>(P, tile, jw);
+    int i, j;
     double f[9], rho, ux, uy;
-    load_fk64<St, BND>(P, src, (const double *)a.erho_src, nc, i, j, f);
+    if (lean) {
+        j = jw;
+        i = (int)(n - (unsigned)jw * (unsigned)P.W);
+        load_interior64(P, src, n, f);
+    } else {
+        j = row_of(P, nc);
+        i = (int)(nc - (unsigned)j * (unsigned)P.W);
+        load_fk64<St, BND>(P, src, (const double *)a.erho_src, nc, i, j, f);
+    }
     unsigned mask;
     const int *ids;
     const int cnt = cand_resolve(a, tile, cp, mask, ids);
@@ -1136,12 +1157,14 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
             if (P.rheology != 0) a.cobn[n] = ((const double *)a.rate_in)[n];
             final_aux(a, n, i, j);
         } else {
-            typename St::T *dst = (typename St::T *)a.dst;
             CollideOut o;
-            collide64(P, f, rho, ucx, ucy, gx, gy, om, o);
+            collide64_fused(P, f, rho, ucx, ucy, gx, gy, om, o);
+            const auto drs = make_rsrc(a.dst, P.buf_bytes);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) St::st(dst + k * P.plane + n, f[k], 0u);
-            if (P.rheology != 0) ((double *)a.rate_out)[n] = o.omega_bn;
+            for (int k = 0; k < 9; ++k)
+                StF64::bst(drs, f[k], n * 8u, (unsigned)(k * P.plane * 8LL));
+            if (P.rheology != 0)
+                StF64::bst(make_rsrc(a.rate_out, (unsigned)(P.plane * 8)), o.omega_bn, n * 8u, 0u);
             yielded = o.yielded;
             if (a.aux) {
                 a.xstress[n] = o.stress[0]; a.xstress[P.plane + n] = o.stress[1];
@@ -1208,7 +1231,7 @@ __device__ __forceinline__ void chunk_force(const StepArgs &a, int p, int c, dou
             acc[2] = rp * ((double)dx * dv - (double)dy * du);
         }
     } else {
-        const double ph = valid ? phi_at(cd.xc, cd.yc, cd.dm, cd.zt, i, j) : 0.0;
+        const double ph = valid ? phi_at<true>(cd.xc, cd.yc, cd.dm, cd.zt, i, j) : 0.0;
         if (ph > 0.0) {
             double f[9], r, u, v;
             load_fk64<St, BND>(P, src, (const double *)a.erho_src, n, i, j, f);
@@ -1379,11 +1402,18 @@ __device__ void particle_helper(const StepArgs &a, int p, int c) {
         register_tiles(P, nb, p, a.list_gen + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
+// Resident blocks per CU the step kernels are compiled for (the register
+// budget: 4 wavefronts per block, 512 VGPRs per SIMD lane): the FP64 step 5
+// without spills (6, with one spilled value per node, measured slower); the
+// reduced formats are left to the compiler (about 7 wavefronts per SIMD:
+// forcing 8 made the two-disk FP16 step 20 % slower).
+#define STEP_BLOCKS_PER_CU(DEV, FINAL) ((FINAL) ? 1 : (DEV) ? 1 : 5)
+
 // Block layout of a step launch: the particle helper blocks (particle-major,
 // K per particle), the reducer slot, then the tiles in one contiguous range
 // per XCD.
 template <class St, bool DEV, bool FINAL, int BND>
-__global__ __launch_bounds__(NT) void k_step(StepArgs a) {
+__global__ __launch_bounds__(NT, STEP_BLOCKS_PER_CU(DEV, FINAL)) void k_step(StepArgs a) {
     const long long b = blockIdx.x;
     if (b < a.n_pblocks) {
         particle_helper<St, DEV, FINAL, BND>(a, (int)(b / a.kpart), (int)(b % a.kpart));
