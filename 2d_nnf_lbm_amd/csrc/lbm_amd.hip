@@ -1617,7 +1617,7 @@ int build_map(lbm_batch *b, const std::vector<int> &which) {
     const size_t base = map.size();
     map.resize(base + tiles.size());
     for (long long g = 0; g < T; ++g)
-        map[base + g] = tiles[xcd_rank((long long)base + g, (long long)base, T)];
+        map[base + g] = tiles[xcd_rank((int)(base + g), (int)base, (int)T)];
     if (map.size() > b->bmap_cap) {
         if (b->bmap) (void)hipFree(b->bmap);
         b->bmap = nullptr;

@@ -15,17 +15,33 @@ namespace lbm {
 // m blocks base .. base + m - 1 of a launch: the position of block g in the
 // order that gives XCD 0 the first n_0 positions, XCD 1 the next n_1, ...
 // (n_x: how many of those blocks XCD x runs), each XCD's blocks in dispatch
-// order.  A bijection of [0, m).
-__host__ __device__ __forceinline__ long long xcd_rank(long long g, long long base, long long m) {
-    const int x = (int)(g & 7);
-    const int b8 = (int)(base & 7);
-    long long start = 0;
-    for (int y = 0; y < x; ++y) {
-        const int r = (y - b8) & 7;  // first offset from base that XCD y runs
-        start += r < m ? (m - r + 7) >> 3 : 0;
-    }
-    const long long j = g - base;
-    return start + ((j - ((x - b8) & 7)) >> 3);
+// order.  A bijection of [0, m).  With m = 8 q + rem, the offsets from base
+// that XCD y runs are those congruent to (y - base) mod 8, q + 1 of them when
+// that residue is below rem and q otherwise; the XCDs with the extra block
+// are base mod 8 .. base mod 8 + rem - 1 (cyclically), bits of the mask M,
+// so XCD x starts at x q + popcount(M & ((1 << x) - 1)).
+__host__ __device__ __forceinline__ int xcd_rank(int g, int base, int m) {
+    const unsigned x = (unsigned)g & 7u, b8 = (unsigned)base & 7u;
+    const unsigned low = (1u << ((unsigned)m & 7u)) - 1u;
+    const unsigned M = ((low << b8) | (low >> (8u - b8))) & 0xffu;
+    const int start = (int)x * (m >> 3) + __builtin_popcount(M & ((1u << x) - 1u));
+    return start + ((g - base - (int)((x - b8) & 7u)) >> 3);
+}
+
+// The arguments of a launch, read in place from the kernel argument segment
+// through a pointer the compiler cannot see through.  A kernel that names
+// its by-value argument struct directly gets all the fields it uses loaded
+// at entry (they are invariant), which for the fused step (1 KiB of
+// arguments, two block roles) keeps more scalar registers live than there
+// are and spills them to vector registers; read through this pointer, each
+// field is loaded where it is used.  The struct must be the kernel's only
+// explicit argument (it then starts the segment).
+template <class T>
+__device__ __forceinline__ const T &launch_args() {
+    typedef const __attribute__((address_space(4))) T CT;
+    CT *p = (CT *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const T *)p;
 }
 
 // =============================================================== drop-in
@@ -760,7 +776,8 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
             if (P.rheology != 0)
                 __builtin_amdgcn_raw_buffer_store_b32(
                     __builtin_bit_cast(unsigned, om2),
-                    make_rsrc(a.rate_out, (unsigned)(P.plane * 4)), (int)(n * 4u), 0, 0);
+                    make_rsrc(a.rate_out, (unsigned)(P.plane * 4)), (int)(n * 4u), 0,
+                    ST_THROUGH);
             if (a.aux) {
                 a.xstress[n] = str[0]; a.xstress[P.plane + n] = str[1];
                 a.xstress[2 * P.plane + n] = str[2];
@@ -799,14 +816,16 @@ __device__ __forceinline__ void prologue_tile(const StepArgs &a, long long tile)
 }
 
 template <class St, bool DEV>
-__global__ __launch_bounds__(NT) void k_prologue(StepArgs a) {
+__global__ __launch_bounds__(NT) void k_prologue(StepArgs args) {
+    const StepArgs &a = launch_args<StepArgs>();
+    (void)args;
     if ((int)blockIdx.x < a.n_pblocks) {
         // the first helper slot of each particle advances it
         if ((int)blockIdx.x % a.kpart == 0) particle_advance<DEV>(a, (int)blockIdx.x / a.kpart);
         return;
     }
     if ((int)blockIdx.x < a.n_pblocks + a.nred) return;  // nothing before the prologue
-    prologue_tile<St, DEV>(a, xcd_rank(blockIdx.x, a.n_pblocks + a.nred, a.ntiles));
+    prologue_tile<St, DEV>(a, xcd_rank((int)blockIdx.x, (int)(a.n_pblocks + a.nred), (int)a.ntiles));
 }
 
 // DOMAIN_IMAGE from the particles alone (F6:438, F6:460-467).
@@ -831,51 +850,34 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
 }
 
 // ---------------------------------------------------------------- tiles
-// Which particles can reach tile t in this launch?  In two parts, so that
-// the loads they need are in flight together with the tile's population
-// loads: cand_prefetch issues them (up to NP_LOOP particles: lane l loads
-// particle l's reach box of state k; beyond: the tile's candidate list
-// header of this launch), cand_resolve turns them into the candidates (a bit
-// per particle whose box meets the tile, or the list).  Uniform over the
-// block.
-struct CandPre {
-    int4 box;
-    unsigned long long hdr;
-};
-// (The loads are unconditional, from valid addresses, so that no branch
-// makes the wavefront wait for them before the population loads.)
-__device__ __forceinline__ CandPre cand_prefetch(const StepArgs &a, long long t) {
-    CandPre c;
-    const int np = a.P.np, lane = threadIdx.x & 63;
-    c.box = *(const int4 *)(a.pbox_in + 4 * min(lane, max(np - 1, 0)));
-    // a vector load (a scalar one would be waited for with the kernel
-    // argument loads, before the population loads)
-    c.hdr = np > NP_LOOP ? __hip_atomic_load(a.tl_hdr_in + t, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP)
-                         : 0ull;
-    return c;
-}
-__device__ __forceinline__ int cand_resolve(const StepArgs &a, long long t, const CandPre &c,
-                                            unsigned &mask, const int *&ids) {
+// Which particles can reach tile t in this launch?  Up to NP_LOOP particles:
+// a bit per particle whose reach box of state k meets the tile; beyond: the
+// tile's candidate list of this launch (header: generation << 32 | count).
+// Scalar loads and scalar arithmetic, uniform over the block; a tile calls
+// it after issuing its population loads, which the scalar loads do not wait
+// for.  (These buffers were written by the previous launch, not this one.)
+__device__ __forceinline__ int tile_candidates(const StepArgs &a, int t, unsigned &mask,
+                                               const int *&ids) {
     const DevP &P = a.P;
     mask = 0u;
     ids = nullptr;
-    if (P.np == 0) return 0;
-    if (P.np <= NP_LOOP) {
-        const bool hit = (int)(threadIdx.x & 63) < P.np && c.box.x <= c.box.y &&
-                         c.box.z <= c.box.w &&
-                         tile_meets_box(P, t, c.box.x, c.box.y, c.box.z, c.box.w);
-        mask = (unsigned)__ballot(hit);
-        return __builtin_popcount(mask);
+    const int np = P.np;
+    if (np <= 0) return 0;
+    if (np <= NP_LOOP) {
+        const TileSpan sp = tile_span(P, t);
+        unsigned m = 0u;
+        for (int p = 0; p < np; ++p) {
+            const int *w = a.pbox_in + 4 * p;
+            if (tile_meets_box(P, sp, cload(w), cload(w + 1), cload(w + 2), cload(w + 3)))
+                m |= 1u << p;
+        }
+        mask = m;
+        return __builtin_popcount(m);
     }
-    const unsigned long long h = uniform_u64(c.hdr);
+    const unsigned long long h = cload(a.tl_hdr_in + t);
     if ((unsigned)(h >> 32) != a.list_gen) return 0;
-    ids = a.tl_ids_in + t * MAXC;
+    ids = a.tl_ids_in + (size_t)t * MAXC;
     return min((int)(h & 0xffffffffu), MAXC);
-}
-__device__ __forceinline__ int tile_candidates(const StepArgs &a, long long t, unsigned &mask,
-                                               const int *&ids) {
-    return cand_resolve(a, t, cand_prefetch(a, t), mask, ids);
 }
 
 // A value every lane holds alike, as a wavefront-uniform one.
@@ -960,15 +962,17 @@ __device__ __forceinline__ void tile_forcing(const StepArgs &a, int cnt, unsigne
 // edge rules (for equilibrium edges, also away from the nodes that feed the
 // edge arrays)?  Uniform over the wavefront.
 template <int BND>
-__device__ __forceinline__ bool wave_interior(const DevP &P, long long tile, int &jw) {
-    const long long w0 = tile * NT + (long long)__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
-    const long long w1 = w0 + 63;
+__device__ __forceinline__ bool wave_interior(const DevP &P, int tile, int &jw) {
+    const unsigned w0 = (unsigned)tile * NT +
+                        (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+    const unsigned w1 = w0 + 63u;
     jw = 0;
-    if (w1 >= P.nn) return false;
+    if (w1 >= (unsigned)P.nn) return false;
     const int bnd = BND >= 0 ? BND : P.boundary;
     const int e = bnd == 2 ? 2 : 1;
-    const int j0 = row_of(P, (unsigned)w0), j1 = row_of(P, (unsigned)w1);
-    const int i0 = (int)(w0 - (long long)j0 * P.W), i1 = (int)(w1 - (long long)j1 * P.W);
+    const int j0 = row_of(P, w0), j1 = row_of(P, w1);
+    const int i0 = (int)(w0 - (unsigned)j0 * (unsigned)P.W);
+    const int i1 = (int)(w1 - (unsigned)j1 * (unsigned)P.W);
     jw = j0;
     return j0 == j1 && j0 >= e && j0 <= P.ny - e && i0 >= e && i1 <= P.nx - e;
 }
@@ -999,7 +1003,7 @@ __device__ __forceinline__ void final_aux(const StepArgs &a, unsigned n, int i, 
 // auxiliary fields, or (FINAL = true) the FP64 state of step k; then the
 // monitor partials.  FORCED: G may be non-zero.
 template <class St, bool FINAL, int BND, bool FORCED>
-__device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, unsigned n,
+__device__ __forceinline__ void tile_dev_out(const StepArgs &a, int tile, unsigned n,
                                              bool valid, bool lean, int i, int j, float (&H)[9],
                                              const DevMom &mo, float rho, float rinvS, float ucx,
                                              float ucy, float gx, float gy, float om) {
@@ -1033,7 +1037,8 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, 
             if (P.rheology != 0)
                 __builtin_amdgcn_raw_buffer_store_b32(
                     __builtin_bit_cast(unsigned, om2),
-                    make_rsrc(a.rate_out, (unsigned)(P.plane * 4)), (int)(n * 4u), 0, 0);
+                    make_rsrc(a.rate_out, (unsigned)(P.plane * 4)), (int)(n * 4u), 0,
+                    ST_THROUGH);
             if (a.aux) {
                 a.xstress[n] = str[0]; a.xstress[P.plane + n] = str[1];
                 a.xstress[2 * P.plane + n] = str[2];
@@ -1059,23 +1064,20 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, long long tile, 
 // One fused time step k over a tile, reduced-precision storage (FP32
 // collide): pull-stream + boundary rules (f_k), density and velocity, SPM
 // forcing G_k from the particles that can reach the tile, velocity
-// correction, then the outputs (tile_dev_out).  The candidate particles are
-// looked up after the population loads are issued, so that no load waits
-// for another before the tile's stream of populations starts.
+// correction, then the outputs (tile_dev_out).  The node rate and the
+// populations are requested first; a wavefront of interior nodes pulls with
+// one vector offset, any other with pull_lane; the candidate particles are
+// looked up (scalar) while those loads are in flight.
 template <class St, bool FINAL, int BND>
-__device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
+__device__ __forceinline__ void tile_dev(const StepArgs &a, int tile) {
     const DevP &P = a.P;
     const unsigned n = (unsigned)tile * NT + threadIdx.x;
     const bool valid = n < (unsigned)P.nn;
-    const typename St::T *src = (const typename St::T *)a.src;
-    // the node rate and the candidates' inputs are requested before the
-    // populations (loads complete in order); lanes past the last node read
-    // the zero padding of the rate plane
+    // lanes past the last node read the zero padding of the rate plane
     const float om = (!FINAL && P.rheology != 0)
                          ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                make_rsrc(a.rate_in, (unsigned)(P.plane * 4)), (int)(n * 4u), 0, 0))
                          : P.fomega;
-    const CandPre cp = cand_prefetch(a, tile);
     int jw;
     const bool lean = wave_interior<BND>(P, tile, jw);
     int i, j;
@@ -1083,16 +1085,16 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
     if (lean) {
         j = jw;
         i = (int)(n - (unsigned)jw * (unsigned)P.W);
-        load_interior_dev<St>(P, src, n, H);
+        load_interior_dev<St>(P, (const typename St::T *)a.src, n, H);
     } else {
         const unsigned nc = valid ? n : (unsigned)P.nn - 1u;
         j = row_of(P, nc);
         i = (int)(nc - (unsigned)j * (unsigned)P.W);
-        load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, nc, i, j, H);
+        load_fk_lane_dev<St, BND>(P, a.src, (const float *)a.erho_src, nc, i, j, H);
     }
     unsigned mask;
     const int *ids;
-    const int cnt = cand_resolve(a, tile, cp, mask, ids);
+    const int cnt = tile_candidates(a, tile, mask, ids);
     const DevMom mo = dev_moments(H);
     float rho, rinvS, ux, uy;
     dev_velocity(P, mo, St::invS, rho, rinvS, ux, uy);
@@ -1110,19 +1112,16 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, long long tile) {
 
 // The same step with FP64 storage, in the reference's operation order.
 template <class St, bool FINAL, int BND>
-__device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
+__device__ __forceinline__ void tile_f64(const StepArgs &a, int tile) {
     const DevP &P = a.P;
     const unsigned n = (unsigned)tile * NT + threadIdx.x;
     const bool valid = n < (unsigned)P.nn;
     const unsigned nc = valid ? n : (unsigned)P.nn - 1u;
     const typename St::T *src = (const typename St::T *)a.src;
-    // the node rate and the candidates' inputs are requested before the
-    // populations (loads complete in order); lanes past the last node read
-    // the zero padding of the rate plane
+    // lanes past the last node read the zero padding of the rate plane
     const double om = (!FINAL && P.rheology != 0)
                           ? StF64::bld(make_rsrc(a.rate_in, (unsigned)(P.plane * 8)), n * 8u, 0u)
                           : P.omega;
-    const CandPre cp = cand_prefetch(a, tile);
     int jw;
     const bool lean = wave_interior<BND>(P, tile, jw);
     int i, j;
@@ -1134,11 +1133,11 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
     } else {
         j = row_of(P, nc);
         i = (int)(nc - (unsigned)j * (unsigned)P.W);
-        load_fk64<St, BND>(P, src, (const double *)a.erho_src, nc, i, j, f);
+        load_fk_lane64<BND>(P, a.src, (const double *)a.erho_src, nc, i, j, f);
     }
     unsigned mask;
     const int *ids;
-    const int cnt = cand_resolve(a, tile, cp, mask, ids);
+    const int cnt = tile_candidates(a, tile, mask, ids);
     macros64(f, rho, ux, uy);
     if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
     double gx = 0.0, gy = 0.0;
@@ -1185,7 +1184,7 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, long long tile) {
 }
 
 template <class St, bool DEV, bool FINAL, int BND>
-__device__ __forceinline__ void step_tile(const StepArgs &a, long long tile) {
+__device__ __forceinline__ void step_tile(const StepArgs &a, int tile) {
     if constexpr (DEV) tile_dev<St, FINAL, BND>(a, tile);
     else tile_f64<St, FINAL, BND>(a, tile);
 }
@@ -1219,7 +1218,7 @@ __device__ __forceinline__ void chunk_force(const StepArgs &a, int p, int c, dou
         const float ph = valid ? phi_prof(cd.s, i, j, dx, dy) : 0.0f;
         if (ph > 0.0f) {
             float H[9];
-            load_fk_dev<St, BND>(P, src, (const float *)a.erho_src, n, i, j, H);
+            load_fk_lane_dev<St, BND>(P, a.src, (const float *)a.erho_src, (unsigned)n, i, j, H);
             const DevMom mo = dev_moments(H);
             float rho, rinvS, ux, uy;
             dev_velocity(P, mo, St::invS, rho, rinvS, ux, uy);
@@ -1234,7 +1233,7 @@ __device__ __forceinline__ void chunk_force(const StepArgs &a, int p, int c, dou
         const double ph = valid ? phi_at<true>(cd.xc, cd.yc, cd.dm, cd.zt, i, j) : 0.0;
         if (ph > 0.0) {
             double f[9], r, u, v;
-            load_fk64<St, BND>(P, src, (const double *)a.erho_src, n, i, j, f);
+            load_fk_lane64<BND>(P, a.src, (const double *)a.erho_src, (unsigned)n, i, j, f);
             macros64(f, r, u, v);
             const double uxp = cd.vx - cd.w * ((double)j - cd.yc);
             const double uyp = cd.vy + cd.w * ((double)i - cd.xc);
@@ -1310,7 +1309,7 @@ __device__ __forceinline__ bool sum_force_slots(const StepArgs &a, int p, double
 // requested before the chunk's populations, so the finishing block has them
 // at hand.
 template <class St, bool DEV, bool FINAL, int BND>
-__device__ void particle_helper(const StepArgs &a, int p, int c) {
+__device__ __forceinline__ void particle_helper(const StepArgs &a, int p, int c) {
     const DevP &P = a.P;
     const int m = P.maxp;
     static_assert(NP_LDS <= NT, "one staged particle per thread");
@@ -1413,17 +1412,19 @@ __device__ void particle_helper(const StepArgs &a, int p, int c) {
 // K per particle), the reducer slot, then the tiles in one contiguous range
 // per XCD.
 template <class St, bool DEV, bool FINAL, int BND>
-__global__ __launch_bounds__(NT, STEP_BLOCKS_PER_CU(DEV, FINAL)) void k_step(StepArgs a) {
-    const long long b = blockIdx.x;
+__global__ __launch_bounds__(NT, STEP_BLOCKS_PER_CU(DEV, FINAL)) void k_step(StepArgs args) {
+    const StepArgs &a = launch_args<StepArgs>();
+    (void)args;
+    const int b = (int)blockIdx.x;
     if (b < a.n_pblocks) {
-        particle_helper<St, DEV, FINAL, BND>(a, (int)(b / a.kpart), (int)(b % a.kpart));
+        particle_helper<St, DEV, FINAL, BND>(a, b / a.kpart, b % a.kpart);
         return;
     }
     if (b < a.n_pblocks + a.nred) {
         monitor_reduce(a.mpart_in, a.ntiles, a.red_mass, a.red_yield);
         return;
     }
-    step_tile<St, DEV, FINAL, BND>(a, xcd_rank(b, a.n_pblocks + a.nred, a.ntiles));
+    step_tile<St, DEV, FINAL, BND>(a, xcd_rank(b, a.n_pblocks + a.nred, (int)a.ntiles));
 }
 
 // ================================================================= batch
