@@ -1308,6 +1308,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
     if (!cont) {
         a = make_args<DEV>(h, kk0, record, 0, false, ch);
         step_fields(a, 0, nsteps);
+        finish_head(a, (unsigned)storage_bytes(h->prm.storage));
         hipLaunchKernelGGL((k_prologue<St, DEV>), grid, block, 0, h->stream, a);
         if ((rc = check_launch(h, "k_prologue"))) return rc;
     }
@@ -1316,6 +1317,7 @@ int run_fused(lbm_handle *h, int32_t kk0, int32_t nsteps, bool record, int timed
     for (int s = 1; s <= nsteps; ++s) {
         a = make_args<DEV>(h, kk0, record, s, false, ch);
         step_fields(a, s, nsteps);
+        finish_head(a, (unsigned)storage_bytes(h->prm.storage));
         launch_step<St, DEV, false>(P.boundary, sgrid, block, h->stream, a);
         if ((rc = check_launch(h, "k_step"))) return rc;
     }
@@ -1356,6 +1358,7 @@ int materialise(lbm_handle *h) {
     ch.dbase = h->res_dbase;
     StepArgs a = make_args<DEV>(h, h->res_kk - 1, false, 1, true, ch);
     step_fields(a, 1, 1);
+    finish_head(a, (unsigned)storage_bytes(h->prm.storage));
     launch_step<St, DEV, true>(P.boundary, dim3((unsigned)(P.np * h->kpart + ntiles)), dim3(NT),
                                h->stream, a);
     int rc = check_launch(h, "k_step (final)");
@@ -1730,6 +1733,7 @@ int batch_run(lbm_batch *b, int32_t kk0, int32_t nsteps, double *hist, bool time
                              : make_args<false>(h, kk0, record, s, fin, chains[i]);
             a.tpb = b->tpb[i];
             a.nred = 1;  // the block map always holds the reducer block
+            finish_head(a, (unsigned)storage_bytes(h->prm.storage));
             return a;
         };
         args[i] = mk(0, false);

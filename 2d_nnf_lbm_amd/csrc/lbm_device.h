@@ -81,6 +81,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsig
                                              0x00020000);
 }
 
+// A load through the constant address space: data that no block writes
+// during the launch, at a wavefront-uniform address, is read with scalar
+// loads that need no wait for the vector memory pipe.
+template <class T>
+__device__ __forceinline__ T cload(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
 // Cache policy of the step's population and rate stores: SC1 (coherent at
 // device scope), which writes each line through the XCD's L2 as it is
 // stored.  With the default write-back policy the lines a launch writes
@@ -255,6 +263,22 @@ template <> __device__ __forceinline__ const float *wall_table<float>(const DevP
 }
 enum { WT_CORNER = 0, WT_TOP, WT_BOTTOM, WT_RIGHT, WT_LEFT };
 
+// Wall velocity (u, v) of edge class wc for a lane: the whole table is read
+// with scalar loads (it is uniform and tiny) and the lane's pair selected,
+// so that no lane waits on a load whose address depends on its node.
+template <class T>
+__device__ __forceinline__ void wall_velocity(const DevP &P, int wc, T &u, T &v) {
+    const T *t = wall_table<T>(P);
+    u = cload(t + 2 * WT_CORNER);
+    v = cload(t + 2 * WT_CORNER + 1);
+#pragma unroll
+    for (int q = WT_TOP; q <= WT_LEFT; ++q) {
+        const T tu = cload(t + 2 * q), tv = cload(t + 2 * q + 1);
+        u = wc == q ? tu : u;
+        v = wc == q ? tv : v;
+    }
+}
+
 // Zou-He edge rule (F1:1246-1258 top and bottom, F1:1319-1341 left and
 // right) at edge node (i, j), corners excluded: the three unknown
 // populations from the node's own streamed ones and the wall velocity
@@ -285,7 +309,8 @@ __device__ __forceinline__ void zou_he_edge(const DevP &P, int i, int j, T rho_b
     const bool rgt = !top && !bot && !lft;
     const bool vert = top || bot;  // top or bottom edge: normal along y
     const int wc = top ? WT_TOP : bot ? WT_BOTTOM : lft ? WT_LEFT : WT_RIGHT;
-    const T ux = wall_table<T>(P)[2 * wc], uy = wall_table<T>(P)[2 * wc + 1];
+    T ux, uy;
+    wall_velocity<T>(P, wc, ux, uy);
     const T a1 = vert ? v[1] : v[2], a2 = vert ? v[3] : v[4];
     const T b1 = top ? v[2] : bot ? v[4] : lft ? v[3] : v[1];
     const T b2 = top ? v[6] : bot ? v[7] : lft ? v[6] : v[5];
@@ -416,7 +441,8 @@ __device__ __forceinline__ void bounce_back_node(const DevP &P, int i, int j, T 
     const bool l = i == 0, r = i == P.nx, b = j == 0, t = j == P.ny;
     const bool cor = (l || r) && (b || t);
     const int wc = cor ? WT_CORNER : t ? WT_TOP : b ? WT_BOTTOM : r ? WT_RIGHT : WT_LEFT;
-    const T uw = wall_table<T>(P)[2 * wc], vw = wall_table<T>(P)[2 * wc + 1];
+    T uw, vw;
+    wall_velocity<T>(P, wc, uw, vw);
     v[1] = l ? v[1] - a46 * (-uw) : v[1];
     v[2] = b ? v[2] - a46 * (-vw) : v[2];
     v[3] = r ? v[3] - a46 * (uw) : v[3];
@@ -616,22 +642,14 @@ struct TileSpan {
     unsigned n0, n1;
     int jl, jh;
 };
-__device__ __forceinline__ TileSpan tile_span(const DevP &P, int t) {
-    TileSpan s;
-    s.n0 = (unsigned)t * NT;
-    s.n1 = min(s.n0 + (unsigned)(NT - 1), (unsigned)P.nn - 1u);
-    s.jl = row_of(P, s.n0);
-    s.jh = row_of(P, s.n1);
-    return s;
-}
-__device__ __forceinline__ bool tile_meets_box(const DevP &P, const TileSpan &t, int i0, int i1,
-                                               int j0, int j1) {
+__device__ __forceinline__ bool tile_meets_box(int W, const TileSpan &t, int i0, int i1, int j0,
+                                               int j1) {
     const int ja = max(t.jl, j0), jb = min(t.jh, j1);
     if (ja > jb || i0 > i1) return false;
     if (jb - ja >= 2) return true;
     auto seg = [&](int j) {
-        const int lo = j == t.jl ? (int)(t.n0 - (unsigned)j * (unsigned)P.W) : 0;
-        const int hi = j == t.jh ? (int)(t.n1 - (unsigned)j * (unsigned)P.W) : P.W - 1;
+        const int lo = j == t.jl ? (int)(t.n0 - (unsigned)j * (unsigned)W) : 0;
+        const int hi = j == t.jh ? (int)(t.n1 - (unsigned)j * (unsigned)W) : W - 1;
         return max(lo, i0) <= min(hi, i1);
     };
     return seg(ja) || seg(jb);
@@ -668,13 +686,6 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
     return x;
 }
 
-// A load through the constant address space: data that no block writes
-// during the launch, at a wavefront-uniform address, is read with scalar
-// loads that need no wait for the vector memory pipe.
-template <class T>
-__device__ __forceinline__ T cload(const T *p) {
-    return *(const __attribute__((address_space(4))) T *)p;
-}
 
 __device__ __forceinline__ void put_tagged_f64(unsigned long long *w, unsigned tag, double v) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
@@ -1000,8 +1011,11 @@ __device__ __forceinline__ void pull_lane(const DevP &P, const void *src, unsign
         const int si = i - cxk(k), sj = j - cyk(k);
         const bool in = (unsigned)si <= (unsigned)P.nx && (unsigned)sj <= (unsigned)P.ny;
         const unsigned s = in ? node - (unsigned)(cxk(k) + cyk(k) * P.W) : node;
-        const int kk = (!in && bb_replaced(k, el, er, eb, et)) ? opp_k(k) : k;
-        v[k] = St::bld(rs, (s + (unsigned)kk * (unsigned)P.plane) * sz, 0u);
+        // plane offsets of k and opp(k): uniform, no per-lane multiply
+        const unsigned pk = (unsigned)k * (unsigned)P.plane;
+        const unsigned po = (unsigned)opp_k(k) * (unsigned)P.plane;
+        const unsigned e = (!in && bb_replaced(k, el, er, eb, et)) ? po : pk;
+        v[k] = St::bld(rs, (s + e) * sz, 0u);
     }
 }
 

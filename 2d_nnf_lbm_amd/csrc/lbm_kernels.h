@@ -5,6 +5,8 @@
 // pass over the grid.  Layout in HBM: structure of arrays, one plane per
 // population (plane stride P.plane elements), node index j*(nx+1)+i.
 #pragma once
+#include <cstring>
+
 #include "lbm_device.h"
 
 namespace lbm {
@@ -433,7 +435,27 @@ __global__ void k_soa_to_aos(long long nn, int nc, const double *soa, double *ao
 // double-buffered by chain parity, so a later call continues the chain
 // where the last launch left it, and the inputs of that last launch stay
 // intact for a FINAL launch that materialises its iteration on demand.
+// The fields a tile block needs before its first population load, copied
+// into the first 128 bytes of the launch arguments (finish_head) so that a
+// wavefront reads them with two scalar loads at entry instead of a chain of
+// dependent ones: the block layout (helper blocks, first tile block, tiles),
+// the grid geometry of the row division and the interior test, the node-rate
+// and population inputs with their byte sizes, the stochastic-rounding
+// position, and the byte offsets of the nine pulled populations of an
+// interior node relative to the node's own.
+struct StepHead {
+    int n_pblocks, tile0, ntiles, np;
+    unsigned nn, W, w_magic, w_shift;
+    int nx, ny, margin, rheology;   // margin: 1, or 2 with equilibrium edges
+    unsigned sr, buf_bytes, rate_bytes, plane_bytes;
+    const void *src, *rate_in;
+    unsigned pull[9];
+    unsigned spare[3];
+};
+static_assert(sizeof(StepHead) == 128, "two 16-dword scalar loads");
+
 struct StepArgs {
+    StepHead hd;                  // copies of fields below (finish_head)
     DevP P;
     const void *src;              // post-collision populations f*_k (storage format)
     void *dst;                    // f*_{k+1} (storage format)
@@ -510,6 +532,23 @@ struct StepArgs {
     int nred;                     // reducer block slots in the block layout (0 or 1)
 };
 
+// The header of a launch's arguments, read with two 16-dword scalar loads.
+__device__ __forceinline__ StepHead load_head(const StepArgs &a) {
+    typedef unsigned V16 __attribute__((ext_vector_type(16)));
+    const __attribute__((address_space(4))) V16 *p =
+        (const __attribute__((address_space(4))) V16 *)&a.hd;
+    struct {
+        V16 lo, hi;
+    } v = {p[0], p[1]};
+    return __builtin_bit_cast(StepHead, v);
+}
+
+// j = n / W from the header (row_of).
+__device__ __forceinline__ int row_of(const StepHead &h, unsigned n) {
+    unsigned t = __umulhi(h.w_magic, n);
+    return (int)((t + ((n - t) >> 1)) >> (h.w_shift - 1u));
+}
+
 // error bits of the fused path
 enum {
     LBM_EF_LIST = 1,      // a tile's candidate list overflowed
@@ -562,6 +601,37 @@ __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nst
     a.mpart_in = (rec && s > 0) ? a.mpart[par ^ 1] : nullptr;
     a.red_mass = (rec && s > 1) ? a.slots_base + (size_t)(s - 2) * row2 : nullptr;
     a.red_yield = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * row2 + NSLOT : nullptr;
+    a.hd.sr = a.sr;
+}
+
+// Fill the header from the other fields (host, after the last change to
+// them); elem: bytes per stored population.
+inline void finish_head(StepArgs &a, unsigned elem) {
+    const DevP &P = a.P;
+    StepHead &h = a.hd;
+    std::memset(&h, 0, sizeof h);
+    h.n_pblocks = a.n_pblocks;
+    h.tile0 = a.n_pblocks + a.nred;
+    h.ntiles = (int)a.ntiles;
+    h.np = P.np;
+    h.nn = (unsigned)P.nn;
+    h.W = (unsigned)P.W;
+    h.w_magic = P.w_magic;
+    h.w_shift = (unsigned)P.w_shift;
+    h.nx = P.nx;
+    h.ny = P.ny;
+    h.margin = P.boundary == 2 ? 2 : 1;
+    h.rheology = P.rheology;
+    h.sr = a.sr;
+    h.buf_bytes = P.buf_bytes;
+    h.rate_bytes = (unsigned)(P.plane * (elem == 8 ? 8 : 4));
+    h.plane_bytes = (unsigned)(P.plane * elem);
+    h.src = a.src;
+    h.rate_in = a.rate_in;
+    const int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    const int cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    for (int k = 0; k < 9; ++k)
+        h.pull[k] = (unsigned)((k * P.plane - cx[k] - (long long)cy[k] * P.W) * (long long)elem);
 }
 
 __device__ __forceinline__ void set_err(const StepArgs &a, int bits) { atomicOr(a.err, bits); }
@@ -856,28 +926,31 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
 // Scalar loads and scalar arithmetic, uniform over the block; a tile calls
 // it after issuing its population loads, which the scalar loads do not wait
 // for.  (These buffers were written by the previous launch, not this one.)
-__device__ __forceinline__ int tile_candidates(const StepArgs &a, int t, unsigned &mask,
-                                               const int *&ids) {
-    const DevP &P = a.P;
+__device__ __forceinline__ int tile_candidates(const StepArgs &a, const StepHead &h, int t,
+                                               unsigned &mask, const int *&ids) {
     mask = 0u;
     ids = nullptr;
-    const int np = P.np;
+    const int np = h.np;
     if (np <= 0) return 0;
     if (np <= NP_LOOP) {
-        const TileSpan sp = tile_span(P, t);
+        TileSpan sp;
+        sp.n0 = (unsigned)t * NT;
+        sp.n1 = min(sp.n0 + (unsigned)(NT - 1), h.nn - 1u);
+        sp.jl = row_of(h, sp.n0);
+        sp.jh = row_of(h, sp.n1);
         unsigned m = 0u;
         for (int p = 0; p < np; ++p) {
             const int *w = a.pbox_in + 4 * p;
-            if (tile_meets_box(P, sp, cload(w), cload(w + 1), cload(w + 2), cload(w + 3)))
+            if (tile_meets_box((int)h.W, sp, cload(w), cload(w + 1), cload(w + 2), cload(w + 3)))
                 m |= 1u << p;
         }
         mask = m;
         return __builtin_popcount(m);
     }
-    const unsigned long long h = cload(a.tl_hdr_in + t);
-    if ((unsigned)(h >> 32) != a.list_gen) return 0;
+    const unsigned long long hdr = cload(a.tl_hdr_in + t);
+    if ((unsigned)(hdr >> 32) != a.list_gen) return 0;
     ids = a.tl_ids_in + (size_t)t * MAXC;
-    return min((int)(h & 0xffffffffu), MAXC);
+    return min((int)(hdr & 0xffffffffu), MAXC);
 }
 
 // A value every lane holds alike, as a wavefront-uniform one.
@@ -962,19 +1035,28 @@ __device__ __forceinline__ void tile_forcing(const StepArgs &a, int cnt, unsigne
 // edge rules (for equilibrium edges, also away from the nodes that feed the
 // edge arrays)?  Uniform over the wavefront.
 template <int BND>
-__device__ __forceinline__ bool wave_interior(const DevP &P, int tile, int &jw) {
+__device__ __forceinline__ bool wave_interior(const StepHead &h, int tile, int &jw) {
     const unsigned w0 = (unsigned)tile * NT +
                         (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     const unsigned w1 = w0 + 63u;
     jw = 0;
-    if (w1 >= (unsigned)P.nn) return false;
-    const int bnd = BND >= 0 ? BND : P.boundary;
-    const int e = bnd == 2 ? 2 : 1;
-    const int j0 = row_of(P, w0), j1 = row_of(P, w1);
-    const int i0 = (int)(w0 - (unsigned)j0 * (unsigned)P.W);
-    const int i1 = (int)(w1 - (unsigned)j1 * (unsigned)P.W);
+    if (w1 >= h.nn) return false;
+    const int e = BND == 2 ? 2 : (BND >= 0 ? 1 : h.margin);
+    const int j0 = row_of(h, w0), j1 = row_of(h, w1);
+    const int i0 = (int)(w0 - (unsigned)j0 * h.W);
+    const int i1 = (int)(w1 - (unsigned)j1 * h.W);
     jw = j0;
-    return j0 == j1 && j0 >= e && j0 <= P.ny - e && i0 >= e && i1 <= P.nx - e;
+    return j0 == j1 && j0 >= e && j0 <= h.ny - e && i0 >= e && i1 <= h.nx - e;
+}
+
+// The nine pulled populations of an interior node from the header's
+// offsets: one vector offset for the node, a uniform offset per population.
+template <class St, class V>
+__device__ __forceinline__ void load_interior_h(const StepHead &h, unsigned n, V v[9]) {
+    const auto rs = make_rsrc(h.src, h.buf_bytes);
+    const unsigned vo = n * (unsigned)sizeof(typename St::T);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = St::bld(rs, vo, h.pull[k]);
 }
 
 // Density and uncorrected velocity of a node from its moments in storage
@@ -1002,9 +1084,10 @@ __device__ __forceinline__ void final_aux(const StepArgs &a, unsigned n, int i, 
 // collision of step k+1 stored as f*_{k+1}, the rate, edge arrays and
 // auxiliary fields, or (FINAL = true) the FP64 state of step k; then the
 // monitor partials.  FORCED: G may be non-zero.
-template <class St, bool FINAL, int BND, bool FORCED>
-__device__ __forceinline__ void tile_dev_out(const StepArgs &a, int tile, unsigned n,
-                                             bool valid, bool lean, int i, int j, float (&H)[9],
+template <class St, bool FINAL, int BND, bool FORCED, bool LEAN>
+__device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &h, int tile,
+                                             unsigned n,
+                                             bool valid, int i, int j, float (&H)[9],
                                              const DevMom &mo, float rho, float rinvS, float ucx,
                                              float ucy, float gx, float gy, float om) {
     const DevP &P = a.P;
@@ -1033,7 +1116,7 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, int tile, unsign
             else
                 collide_dev<FORCED, false>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2,
                                            str, yielded);
-            store_dev<St>(P, a.dst, n, H, node_hash(n, a.sr));
+            store_dev<St>(P, a.dst, n, H, node_hash(n, h.sr));
             if (P.rheology != 0)
                 __builtin_amdgcn_raw_buffer_store_b32(
                     __builtin_bit_cast(unsigned, om2),
@@ -1044,7 +1127,7 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, int tile, unsign
                 a.xstress[2 * P.plane + n] = str[2];
                 if (P.rheology != 0) a.ximage[n] = yielded ? 0 : 5;
             }
-            if (!lean) {
+            if constexpr (!LEAN) {
                 if ((BND >= 0 ? BND : P.boundary) == 2) {
                     if (!(i == 0 || j == 0 || i == P.nx || j == P.ny))
                         eq_put(a, P, i, j, mo.rd * invS, ucx, ucy);
@@ -1064,37 +1147,33 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, int tile, unsign
 // One fused time step k over a tile, reduced-precision storage (FP32
 // collide): pull-stream + boundary rules (f_k), density and velocity, SPM
 // forcing G_k from the particles that can reach the tile, velocity
-// correction, then the outputs (tile_dev_out).  The node rate and the
-// populations are requested first; a wavefront of interior nodes pulls with
-// one vector offset, any other with pull_lane; the candidate particles are
-// looked up (scalar) while those loads are in flight.
-template <class St, bool FINAL, int BND>
-__device__ __forceinline__ void tile_dev(const StepArgs &a, int tile) {
+// correction, then the outputs (tile_dev_out).  LEAN: the wavefront holds
+// interior nodes only and pulls with one vector offset; otherwise pull_lane.
+// The two are separate instantiations of the whole body: with a run-time
+// branch between the loads the compiler joins the paths before the collide
+// and the interior path (most wavefronts) loses its straight-line schedule
+// (measured: 20 % slower).  The candidate particles are looked up (scalar)
+// while the population loads are in flight.
+template <class St, bool FINAL, int BND, bool LEAN>
+__device__ __forceinline__ void tile_dev_body(const StepArgs &a, const StepHead &h, int tile,
+                                              unsigned n, float om, int jw) {
     const DevP &P = a.P;
-    const unsigned n = (unsigned)tile * NT + threadIdx.x;
-    const bool valid = n < (unsigned)P.nn;
-    // lanes past the last node read the zero padding of the rate plane
-    const float om = (!FINAL && P.rheology != 0)
-                         ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                               make_rsrc(a.rate_in, (unsigned)(P.plane * 4)), (int)(n * 4u), 0, 0))
-                         : P.fomega;
-    int jw;
-    const bool lean = wave_interior<BND>(P, tile, jw);
+    const bool valid = n < h.nn;
     int i, j;
     float H[9];
-    if (lean) {
+    if constexpr (LEAN) {
         j = jw;
-        i = (int)(n - (unsigned)jw * (unsigned)P.W);
-        load_interior_dev<St>(P, (const typename St::T *)a.src, n, H);
+        i = (int)(n - (unsigned)jw * h.W);
+        load_interior_h<St>(h, n, H);
     } else {
-        const unsigned nc = valid ? n : (unsigned)P.nn - 1u;
-        j = row_of(P, nc);
-        i = (int)(nc - (unsigned)j * (unsigned)P.W);
-        load_fk_lane_dev<St, BND>(P, a.src, (const float *)a.erho_src, nc, i, j, H);
+        const unsigned nc = valid ? n : h.nn - 1u;
+        j = row_of(h, nc);
+        i = (int)(nc - (unsigned)j * h.W);
+        load_fk_lane_dev<St, BND>(P, h.src, (const float *)a.erho_src, nc, i, j, H);
     }
     unsigned mask;
     const int *ids;
-    const int cnt = tile_candidates(a, tile, mask, ids);
+    const int cnt = tile_candidates(a, h, tile, mask, ids);
     const DevMom mo = dev_moments(H);
     float rho, rinvS, ux, uy;
     dev_velocity(P, mo, St::invS, rho, rinvS, ux, uy);
@@ -1102,42 +1181,52 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, int tile) {
     if (cnt > 0) {
         float gx, gy;
         tile_forcing<true, float>(a, cnt, mask, ids, i, j, valid, ux, uy, gx, gy);
-        tile_dev_out<St, FINAL, BND, true>(a, tile, n, valid, lean, i, j, H, mo, rho, rinvS,
-                                           ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
+        tile_dev_out<St, FINAL, BND, true, LEAN>(a, h, tile, n, valid, i, j, H, mo, rho, rinvS,
+                                                 ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
     } else {
-        tile_dev_out<St, FINAL, BND, false>(a, tile, n, valid, lean, i, j, H, mo, rho, rinvS, ux,
-                                            uy, 0.0f, 0.0f, om);
+        tile_dev_out<St, FINAL, BND, false, LEAN>(a, h, tile, n, valid, i, j, H, mo, rho, rinvS,
+                                                  ux, uy, 0.0f, 0.0f, om);
     }
 }
-
-// The same step with FP64 storage, in the reference's operation order.
 template <class St, bool FINAL, int BND>
-__device__ __forceinline__ void tile_f64(const StepArgs &a, int tile) {
-    const DevP &P = a.P;
+__device__ __forceinline__ void tile_dev(const StepArgs &a, const StepHead &h, int tile) {
     const unsigned n = (unsigned)tile * NT + threadIdx.x;
-    const bool valid = n < (unsigned)P.nn;
-    const unsigned nc = valid ? n : (unsigned)P.nn - 1u;
-    const typename St::T *src = (const typename St::T *)a.src;
-    // lanes past the last node read the zero padding of the rate plane
-    const double om = (!FINAL && P.rheology != 0)
-                          ? StF64::bld(make_rsrc(a.rate_in, (unsigned)(P.plane * 8)), n * 8u, 0u)
-                          : P.omega;
+    // the node rate first; lanes past the last node read the zero padding of
+    // the rate plane
+    const float om = (!FINAL && h.rheology != 0)
+                         ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                               make_rsrc(h.rate_in, h.rate_bytes), (int)(n * 4u), 0, 0))
+                         : a.P.fomega;
     int jw;
-    const bool lean = wave_interior<BND>(P, tile, jw);
+    if (wave_interior<BND>(h, tile, jw))
+        tile_dev_body<St, FINAL, BND, true>(a, h, tile, n, om, jw);
+    else
+        tile_dev_body<St, FINAL, BND, false>(a, h, tile, n, om, 0);
+}
+
+// The same step with FP64 storage, in the reference's operation order
+// (interior and edge wavefronts as separate instantiations, as tile_dev).
+template <class St, bool FINAL, int BND, bool LEAN>
+__device__ __forceinline__ void tile_f64_body(const StepArgs &a, const StepHead &h, int tile,
+                                              unsigned n, double om, int jw) {
+    const DevP &P = a.P;
+    const bool valid = n < h.nn;
+    const typename St::T *src = (const typename St::T *)h.src;
     int i, j;
     double f[9], rho, ux, uy;
-    if (lean) {
+    if constexpr (LEAN) {
         j = jw;
-        i = (int)(n - (unsigned)jw * (unsigned)P.W);
-        load_interior64(P, src, n, f);
+        i = (int)(n - (unsigned)jw * h.W);
+        load_interior_h<StF64>(h, n, f);
     } else {
-        j = row_of(P, nc);
-        i = (int)(nc - (unsigned)j * (unsigned)P.W);
-        load_fk_lane64<BND>(P, a.src, (const double *)a.erho_src, nc, i, j, f);
+        const unsigned nc = valid ? n : h.nn - 1u;
+        j = row_of(h, nc);
+        i = (int)(nc - (unsigned)j * h.W);
+        load_fk_lane64<BND>(P, h.src, (const double *)a.erho_src, nc, i, j, f);
     }
     unsigned mask;
     const int *ids;
-    const int cnt = tile_candidates(a, tile, mask, ids);
+    const int cnt = tile_candidates(a, h, tile, mask, ids);
     macros64(f, rho, ux, uy);
     if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
     double gx = 0.0, gy = 0.0;
@@ -1170,11 +1259,14 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, int tile) {
                 a.xstress[2 * P.plane + n] = o.stress[2];
                 if (P.rheology != 0) a.ximage[n] = o.yielded ? 0 : 5;
             }
-            const int b = (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
-            if ((BND >= 0 ? BND : P.boundary) == 2) {
-                if (b < 0) eq_put(a, P, i, j, rho, ucx, ucy);
-            } else if (b >= 0) {
-                ((double *)a.erho_dst)[b] = rho;
+            if constexpr (!LEAN) {
+                const int b =
+                    (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
+                if ((BND >= 0 ? BND : P.boundary) == 2) {
+                    if (b < 0) eq_put(a, P, i, j, rho, ucx, ucy);
+                } else if (b >= 0) {
+                    ((double *)a.erho_dst)[b] = rho;
+                }
             }
         }
     }
@@ -1183,10 +1275,25 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, int tile) {
                              !FINAL && a.want_yield);
 }
 
+template <class St, bool FINAL, int BND>
+__device__ __forceinline__ void tile_f64(const StepArgs &a, const StepHead &h, int tile) {
+    const unsigned n = (unsigned)tile * NT + threadIdx.x;
+    // the node rate first; lanes past the last node read the zero padding of
+    // the rate plane
+    const double om = (!FINAL && h.rheology != 0)
+                          ? StF64::bld(make_rsrc(h.rate_in, h.rate_bytes), n * 8u, 0u)
+                          : a.P.omega;
+    int jw;
+    if (wave_interior<BND>(h, tile, jw))
+        tile_f64_body<St, FINAL, BND, true>(a, h, tile, n, om, jw);
+    else
+        tile_f64_body<St, FINAL, BND, false>(a, h, tile, n, om, 0);
+}
+
 template <class St, bool DEV, bool FINAL, int BND>
-__device__ __forceinline__ void step_tile(const StepArgs &a, int tile) {
-    if constexpr (DEV) tile_dev<St, FINAL, BND>(a, tile);
-    else tile_f64<St, FINAL, BND>(a, tile);
+__device__ __forceinline__ void step_tile(const StepArgs &a, const StepHead &h, int tile) {
+    if constexpr (DEV) tile_dev<St, FINAL, BND>(a, h, tile);
+    else tile_f64<St, FINAL, BND>(a, h, tile);
 }
 
 // ---------------------------------------------------------------- particles
@@ -1415,16 +1522,17 @@ template <class St, bool DEV, bool FINAL, int BND>
 __global__ __launch_bounds__(NT, STEP_BLOCKS_PER_CU(DEV, FINAL)) void k_step(StepArgs args) {
     const StepArgs &a = launch_args<StepArgs>();
     (void)args;
+    const StepHead h = load_head(a);
     const int b = (int)blockIdx.x;
-    if (b < a.n_pblocks) {
+    if (b < h.n_pblocks) {
         particle_helper<St, DEV, FINAL, BND>(a, b / a.kpart, b % a.kpart);
         return;
     }
-    if (b < a.n_pblocks + a.nred) {
+    if (b < h.tile0) {
         monitor_reduce(a.mpart_in, a.ntiles, a.red_mass, a.red_yield);
         return;
     }
-    step_tile<St, DEV, FINAL, BND>(a, xcd_rank(b, a.n_pblocks + a.nred, (int)a.ntiles));
+    step_tile<St, DEV, FINAL, BND>(a, h, xcd_rank(b, h.tile0, h.ntiles));
 }
 
 // ================================================================= batch
@@ -1463,6 +1571,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ 
                                                    const int2 *__restrict__ bmap) {
     const int2 m = bmap[blockIdx.x];
     const StepArgs &a = args[m.x];
+    const StepHead h = load_head(a);
     if (m.y < a.n_pblocks) {
         particle_helper<St, DEV, FINAL, BND>(a, m.y / a.kpart, m.y % a.kpart);
         return;
@@ -1474,7 +1583,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ 
     const long long tb = m.y - a.n_pblocks - a.nred;
     for (int r = 0; r < a.tpb; ++r) {
         const long long t = tb * a.tpb + r;
-        if (t < a.ntiles) step_tile<St, DEV, FINAL, BND>(a, t);
+        if (t < a.ntiles) step_tile<St, DEV, FINAL, BND>(a, h, (int)t);
     }
 }
 
