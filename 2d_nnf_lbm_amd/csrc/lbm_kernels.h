@@ -1031,22 +1031,90 @@ __device__ __forceinline__ void tile_forcing(const StepArgs &a, int cnt, unsigne
     }
 }
 
-// Is every node of this thread's wavefront an interior node away from the
-// edge rules (for equilibrium edges, also away from the nodes that feed the
-// edge arrays)?  Uniform over the wavefront.
+// Left and right edge nodes of a PULL_SIDES wavefront (rows 1 .. ny-1, so
+// no corner), after the interior pull of every lane: with Zou-He edges the
+// rule replaces exactly the populations whose sources lie outside and reads
+// only pulled ones, so it applies to the loaded values as they are; with
+// bounce-back the replaced populations (left: 1, 5, 8; right: 3, 6, 7) are
+// the node's own opposite ones (3, 7, 6; 1, 8, 5) less the wall terms of
+// bounce_back_node.  The same values as load_fk_lane_dev / load_fk_lane64;
+// only the boundary lanes execute the loads.
+template <class St, int BND, class V>
+__device__ __forceinline__ bool side_fetch(const DevP &P, const StepHead &h, unsigned n, int i,
+                                           V v[9]) {
+    const bool l = i == 0, r = i == P.nx;
+    if (!(l || r)) return false;
+    if ((BND >= 0 ? BND : P.boundary) == 1) return true;
+    // three loads with the plane chosen per lane, then selects at fixed
+    // indices (a store per branch at different indices would put v in
+    // scratch memory)
+    const auto rs = make_rsrc(h.src, h.buf_bytes);
+    const unsigned vo = n * (unsigned)sizeof(typename St::T), pb = h.plane_bytes;
+    const V x = St::bld(rs, vo + (l ? 3u : 1u) * pb, 0u);  // left: f*_3 -> 1; right: f*_1 -> 3
+    const V y = St::bld(rs, vo + (l ? 7u : 8u) * pb, 0u);  // left: f*_7 -> 5; right: f*_8 -> 6
+    const V z = St::bld(rs, vo + (l ? 6u : 5u) * pb, 0u);  // left: f*_6 -> 8; right: f*_5 -> 7
+    v[1] = l ? x : v[1];
+    v[3] = r ? x : v[3];
+    v[5] = l ? y : v[5];
+    v[6] = r ? y : v[6];
+    v[8] = l ? z : v[8];
+    v[7] = r ? z : v[7];
+    return true;
+}
+template <class St, int BND>
+__device__ __forceinline__ void side_edges_dev(const DevP &P, const StepHead &h,
+                                               const float *erho, unsigned n, int i, int j,
+                                               float H[9]) {
+    if (!side_fetch<St, BND>(P, h, n, i, H)) return;
+    const float S = St::S;
+    if ((BND >= 0 ? BND : P.boundary) == 1) {
+        zou_he_edge<float>(P, i, j, S * P.frho0, H);
+        return;
+    }
+    const float rw = erho[bnd_index(P, i, j)];
+    const float a = (float)INV6 * S, b = (float)(4.0 * INV6) * S;
+    bounce_back_node<float>(P, i, j, a * rw, b * rw, H);
+}
 template <int BND>
-__device__ __forceinline__ bool wave_interior(const StepHead &h, int tile, int &jw) {
+__device__ __forceinline__ void side_edges64(const DevP &P, const StepHead &h,
+                                             const double *erho, unsigned n, int i, int j,
+                                             double v[9]) {
+    if (!side_fetch<StF64, BND>(P, h, n, i, v)) return;
+    if ((BND >= 0 ? BND : P.boundary) == 1) {
+        zou_he_edge<double>(P, i, j, 0.0, v);
+        return;
+    }
+    const double clb2 = 1.0;
+    const double rw = erho[bnd_index(P, i, j)];
+    bounce_back_node<double>(P, i, j, INV6 * clb2 * rw, 4.0 * INV6 * clb2 * rw, v);
+}
+
+// How a wavefront pulls its populations (uniform over the wavefront):
+// PULL_INTERIOR: every node is an interior node away from the edge rules
+// (for equilibrium edges, also away from the nodes that feed the edge
+// arrays); one vector offset per node.  PULL_SIDES: every row the wavefront
+// touches lies strictly inside the domain, so its only boundary nodes are on
+// the left and right edges (bounce-back or Zou-He edges): the interior pull
+// for every lane, then those lanes alone replace the populations whose
+// sources lie outside (side_edges).  PULL_GENERAL: anything else (the
+// bottom and top rows, equilibrium edges, the grid's last wavefront): the
+// per-lane pull of load_fk_lane_dev.  jw: the wavefront's first row.
+enum { PULL_INTERIOR = 0, PULL_SIDES = 1, PULL_GENERAL = 2 };
+template <int BND>
+__device__ __forceinline__ int wave_class(const StepHead &h, int tile, int &jw) {
     const unsigned w0 = (unsigned)tile * NT +
                         (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     const unsigned w1 = w0 + 63u;
     jw = 0;
-    if (w1 >= h.nn) return false;
+    if (w1 >= h.nn) return PULL_GENERAL;
     const int e = BND == 2 ? 2 : (BND >= 0 ? 1 : h.margin);
     const int j0 = row_of(h, w0), j1 = row_of(h, w1);
     const int i0 = (int)(w0 - (unsigned)j0 * h.W);
     const int i1 = (int)(w1 - (unsigned)j1 * h.W);
     jw = j0;
-    return j0 == j1 && j0 >= e && j0 <= h.ny - e && i0 >= e && i1 <= h.nx - e;
+    if (j0 == j1 && j0 >= e && j0 <= h.ny - e && i0 >= e && i1 <= h.nx - e) return PULL_INTERIOR;
+    if (e == 1 && j0 >= 1 && j1 <= h.ny - 1) return PULL_SIDES;
+    return PULL_GENERAL;
 }
 
 // The nine pulled populations of an interior node from the header's
@@ -1084,10 +1152,10 @@ __device__ __forceinline__ void final_aux(const StepArgs &a, unsigned n, int i, 
 // collision of step k+1 stored as f*_{k+1}, the rate, edge arrays and
 // auxiliary fields, or (FINAL = true) the FP64 state of step k; then the
 // monitor partials.  FORCED: G may be non-zero.
-template <class St, bool FINAL, int BND, bool FORCED, bool LEAN>
+template <class St, bool FINAL, int BND, bool FORCED>
 __device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &h, int tile,
-                                             unsigned n,
-                                             bool valid, int i, int j, float (&H)[9],
+                                             unsigned n, bool valid, bool edges, int i, int j,
+                                             float (&H)[9],
                                              const DevMom &mo, float rho, float rinvS, float ucx,
                                              float ucy, float gx, float gy, float om) {
     const DevP &P = a.P;
@@ -1127,7 +1195,7 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &
                 a.xstress[2 * P.plane + n] = str[2];
                 if (P.rheology != 0) a.ximage[n] = yielded ? 0 : 5;
             }
-            if constexpr (!LEAN) {
+            if (edges) {
                 if ((BND >= 0 ? BND : P.boundary) == 2) {
                     if (!(i == 0 || j == 0 || i == P.nx || j == P.ny))
                         eq_put(a, P, i, j, mo.rd * invS, ucx, ucy);
@@ -1145,32 +1213,41 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &
 }
 
 // One fused time step k over a tile, reduced-precision storage (FP32
-// collide): pull-stream + boundary rules (f_k), density and velocity, SPM
-// forcing G_k from the particles that can reach the tile, velocity
-// correction, then the outputs (tile_dev_out).  LEAN: the wavefront holds
-// interior nodes only and pulls with one vector offset; otherwise pull_lane.
-// The two are separate instantiations of the whole body: with a run-time
-// branch between the loads the compiler joins the paths before the collide
-// and the interior path (most wavefronts) loses its straight-line schedule
-// (measured: 20 % slower).  The candidate particles are looked up (scalar)
-// while the population loads are in flight.
-template <class St, bool FINAL, int BND, bool LEAN>
-__device__ __forceinline__ void tile_dev_body(const StepArgs &a, const StepHead &h, int tile,
-                                              unsigned n, float om, int jw) {
+// collide): pull-stream + boundary rules (f_k, by the wavefront's pull
+// class), density and velocity, SPM forcing G_k from the particles that can
+// reach the tile, velocity correction, then the outputs (tile_dev_out).  The
+// node rate is requested first; the candidate particles are looked up
+// (scalar) while the population loads are in flight.
+template <class St, bool FINAL, int BND>
+__device__ __forceinline__ void tile_dev(const StepArgs &a, const StepHead &h, int tile) {
     const DevP &P = a.P;
+    const unsigned n = (unsigned)tile * NT + threadIdx.x;
     const bool valid = n < h.nn;
+    // lanes past the last node read the zero padding of the rate plane
+    const float om = (!FINAL && h.rheology != 0)
+                         ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                               make_rsrc(h.rate_in, h.rate_bytes), (int)(n * 4u), 0, 0))
+                         : P.fomega;
+    int jw;
+    const int kind = wave_class<BND>(h, tile, jw);
     int i, j;
     float H[9];
-    if constexpr (LEAN) {
+    if (kind == PULL_INTERIOR) {
         j = jw;
         i = (int)(n - (unsigned)jw * h.W);
         load_interior_h<St>(h, n, H);
+    } else if (kind == PULL_SIDES) {
+        j = row_of(h, n);
+        i = (int)(n - (unsigned)j * h.W);
+        load_interior_h<St>(h, n, H);
+        side_edges_dev<St, BND>(P, h, (const float *)a.erho_src, n, i, j, H);
     } else {
         const unsigned nc = valid ? n : h.nn - 1u;
         j = row_of(h, nc);
         i = (int)(nc - (unsigned)j * h.W);
         load_fk_lane_dev<St, BND>(P, h.src, (const float *)a.erho_src, nc, i, j, H);
     }
+    const bool edges = kind != PULL_INTERIOR;
     unsigned mask;
     const int *ids;
     const int cnt = tile_candidates(a, h, tile, mask, ids);
@@ -1181,49 +1258,45 @@ __device__ __forceinline__ void tile_dev_body(const StepArgs &a, const StepHead 
     if (cnt > 0) {
         float gx, gy;
         tile_forcing<true, float>(a, cnt, mask, ids, i, j, valid, ux, uy, gx, gy);
-        tile_dev_out<St, FINAL, BND, true, LEAN>(a, h, tile, n, valid, i, j, H, mo, rho, rinvS,
-                                                 ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
+        tile_dev_out<St, FINAL, BND, true>(a, h, tile, n, valid, edges, i, j, H, mo, rho, rinvS,
+                                           ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
     } else {
-        tile_dev_out<St, FINAL, BND, false, LEAN>(a, h, tile, n, valid, i, j, H, mo, rho, rinvS,
-                                                  ux, uy, 0.0f, 0.0f, om);
+        tile_dev_out<St, FINAL, BND, false>(a, h, tile, n, valid, edges, i, j, H, mo, rho,
+                                            rinvS, ux, uy, 0.0f, 0.0f, om);
     }
 }
-template <class St, bool FINAL, int BND>
-__device__ __forceinline__ void tile_dev(const StepArgs &a, const StepHead &h, int tile) {
-    const unsigned n = (unsigned)tile * NT + threadIdx.x;
-    // the node rate first; lanes past the last node read the zero padding of
-    // the rate plane
-    const float om = (!FINAL && h.rheology != 0)
-                         ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                               make_rsrc(h.rate_in, h.rate_bytes), (int)(n * 4u), 0, 0))
-                         : a.P.fomega;
-    int jw;
-    if (wave_interior<BND>(h, tile, jw))
-        tile_dev_body<St, FINAL, BND, true>(a, h, tile, n, om, jw);
-    else
-        tile_dev_body<St, FINAL, BND, false>(a, h, tile, n, om, 0);
-}
 
-// The same step with FP64 storage, in the reference's operation order
-// (interior and edge wavefronts as separate instantiations, as tile_dev).
-template <class St, bool FINAL, int BND, bool LEAN>
-__device__ __forceinline__ void tile_f64_body(const StepArgs &a, const StepHead &h, int tile,
-                                              unsigned n, double om, int jw) {
+// The same step with FP64 storage, in the reference's operation order.
+template <class St, bool FINAL, int BND>
+__device__ __forceinline__ void tile_f64(const StepArgs &a, const StepHead &h, int tile) {
     const DevP &P = a.P;
+    const unsigned n = (unsigned)tile * NT + threadIdx.x;
     const bool valid = n < h.nn;
     const typename St::T *src = (const typename St::T *)h.src;
+    // lanes past the last node read the zero padding of the rate plane
+    const double om = (!FINAL && h.rheology != 0)
+                          ? StF64::bld(make_rsrc(h.rate_in, h.rate_bytes), n * 8u, 0u)
+                          : P.omega;
+    int jw;
+    const int kind = wave_class<BND>(h, tile, jw);
     int i, j;
     double f[9], rho, ux, uy;
-    if constexpr (LEAN) {
+    if (kind == PULL_INTERIOR) {
         j = jw;
         i = (int)(n - (unsigned)jw * h.W);
         load_interior_h<StF64>(h, n, f);
+    } else if (kind == PULL_SIDES) {
+        j = row_of(h, n);
+        i = (int)(n - (unsigned)j * h.W);
+        load_interior_h<StF64>(h, n, f);
+        side_edges64<BND>(P, h, (const double *)a.erho_src, n, i, j, f);
     } else {
         const unsigned nc = valid ? n : h.nn - 1u;
         j = row_of(h, nc);
         i = (int)(nc - (unsigned)j * h.W);
         load_fk_lane64<BND>(P, h.src, (const double *)a.erho_src, nc, i, j, f);
     }
+    const bool edges = kind != PULL_INTERIOR;
     unsigned mask;
     const int *ids;
     const int cnt = tile_candidates(a, h, tile, mask, ids);
@@ -1259,7 +1332,7 @@ __device__ __forceinline__ void tile_f64_body(const StepArgs &a, const StepHead 
                 a.xstress[2 * P.plane + n] = o.stress[2];
                 if (P.rheology != 0) a.ximage[n] = o.yielded ? 0 : 5;
             }
-            if constexpr (!LEAN) {
+            if (edges) {
                 const int b =
                     (i == 0 || j == 0 || i == P.nx || j == P.ny) ? bnd_index(P, i, j) : -1;
                 if ((BND >= 0 ? BND : P.boundary) == 2) {
@@ -1273,21 +1346,6 @@ __device__ __forceinline__ void tile_f64_body(const StepArgs &a, const StepHead 
     if (a.slots_mass || (!FINAL && a.want_yield))
         tile_monitors<false>(a, tile, valid ? rho : 0.0, yielded, a.slots_mass != nullptr,
                              !FINAL && a.want_yield);
-}
-
-template <class St, bool FINAL, int BND>
-__device__ __forceinline__ void tile_f64(const StepArgs &a, const StepHead &h, int tile) {
-    const unsigned n = (unsigned)tile * NT + threadIdx.x;
-    // the node rate first; lanes past the last node read the zero padding of
-    // the rate plane
-    const double om = (!FINAL && h.rheology != 0)
-                          ? StF64::bld(make_rsrc(h.rate_in, h.rate_bytes), n * 8u, 0u)
-                          : a.P.omega;
-    int jw;
-    if (wave_interior<BND>(h, tile, jw))
-        tile_f64_body<St, FINAL, BND, true>(a, h, tile, n, om, jw);
-    else
-        tile_f64_body<St, FINAL, BND, false>(a, h, tile, n, om, 0);
 }
 
 template <class St, bool DEV, bool FINAL, int BND>
