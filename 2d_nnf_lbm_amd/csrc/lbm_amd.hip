@@ -82,7 +82,6 @@ struct lbm_handle {
     float *spf[2] = {nullptr, nullptr};
     unsigned long long *fslot = nullptr;  // chunk force partials [maxp][kpart][SLOTW]
     size_t fslot_bytes = 0;
-    unsigned *pcount = nullptr;           // chunks done per particle (zero between launches)
     int kpart = 1;                        // helper blocks per particle
     // Word tags of the force partials never repeat on a handle (wtag_next);
     // tile-list generations run along a chain (lg_next: beyond every one
@@ -347,7 +346,6 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         (rc = al((void **)&h->scalar, 8)) ||
         (rc = al((void **)&h->err_flag, 4)) || (rc = al((void **)&h->lag, 8)) ||
         (rc = al((void **)&h->pbox[0], (size_t)P.maxp * 4 * 4)) ||
-        (rc = al((void **)&h->pcount, (size_t)P.maxp * 4)) ||
         (rc = al((void **)&h->pbox[1], (size_t)P.maxp * 4 * 4)) ||
         (rc = al((void **)&h->spf[0], (size_t)P.maxp * SPF * 4)) ||
         (rc = al((void **)&h->spf[1], (size_t)P.maxp * SPF * 4)) ||
@@ -422,7 +420,7 @@ int lbm_destroy(lbm_handle *h) {
                     h->dpw[0], h->dpw[1], h->pairs, h->paircnt, h->phi, h->porig,
                     h->staging, h->partial, h->scalar, h->hist_part, h->hist_slots,
                     h->tl_hdr[0], h->tl_hdr[1], h->tl_ids[0], h->tl_ids[1], h->err_flag,
-                    h->pbox[0], h->pbox[1], h->spf[0], h->spf[1], h->fslot, h->pcount, h->lag, h->wtab,
+                    h->pbox[0], h->pbox[1], h->spf[0], h->spf[1], h->fslot, h->lag, h->wtab,
                     h->mpart[0], h->mpart[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1146,7 +1144,6 @@ StepArgs make_args(const lbm_handle *h, int32_t kk0, bool record, int s, bool fi
     a.pairs = h->cpairs[c & 1]; a.paircnt = h->cpaircnt[c & 1];
     a.n_pblocks = P.np * h->kpart;
     a.kpart = h->kpart;
-    a.pcount = h->pcount;
     a.ntiles = (P.nn + NT - 1) / NT;
     a.err = h->err_flag;
     a.hist_base = record ? h->hist_part_use : nullptr;

@@ -489,7 +489,6 @@ struct StepArgs {
     const int *pbox_in;           // reach boxes of state k [maxp][4]: i0, i1, j0, j1
     int *pbox_out;                // those of state k+1
     unsigned long long *fslot;    // tagged chunk force partials [maxp][kpart][SLOTW]
-    unsigned *pcount;             // per particle: chunks done in this launch (wraps to 0)
     int kpart;                    // helper blocks (box chunks of NT nodes) per particle
     const double *dpw_in;
     double *dpw_out;
@@ -1424,9 +1423,10 @@ __device__ __forceinline__ void chunk_force(const StepArgs &a, int p, int c, dou
 // FORCE and TORQUE on particle p from the tagged chunk partials 0 .. K-1
 // (K = a.kpart), by one wavefront: lane l takes chunks l, l + 64, ... in
 // order, waiting (bounded) until their words carry this launch's tag; a
-// fixed shuffle tree then adds the lanes.  Every chunk's block wrote its
-// words before it counted itself done, so the wait is short.  Valid in every
-// lane; false when the wait ran out.
+// fixed shuffle tree then adds the lanes.  The other chunks' blocks run at
+// the same time as the caller (see particle_helper), so the wait is about
+// as long as the slowest chunk's gather.  Valid in every lane; false when
+// the wait ran out.
 __device__ __forceinline__ bool sum_force_slots(const StepArgs &a, int p, double F[3]) {
     const int lane = threadIdx.x & 63;
     const unsigned long long *q = a.fslot + (size_t)p * a.kpart * SLOTW;
@@ -1462,17 +1462,21 @@ __device__ __forceinline__ bool sum_force_slots(const StepArgs &a, int p, double
 }
 
 // Helper block c of particle p in launch s >= 1 (all NT threads): the chunk
-// partial of the force (chunk_force), stored as tagged words in slot (p, c);
-// then the block counts itself done on p's counter (a wrapping increment, so
-// the counter is back at zero after the launch).  The block that completes
-// the count finishes the particle: the contact force on state k (F3:739-804),
-// FORCE_k from the K partials in chunk order, the history row of step k, and
-// the Newton-Euler update to state k+1 (F6:299-313) with its reach box,
-// profile record and, for many particles, tile registration; the final
-// launch only writes FORCE_k into the record.  No block waits for another
-// that has not yet run.  The particle's record and the contact inputs are
-// requested before the chunk's populations, so the finishing block has them
-// at hand.
+// partial of the force (chunk_force), stored as tagged words in slot (p, c).
+// Chunk 0 then finishes the particle: FORCE_k from the K partials in chunk
+// order (waiting, bounded, for the other chunks' words), the contact force
+// on state k (F3:739-804), the history row of step k, and the Newton-Euler
+// update to state k+1 (F6:299-313) with its reach box, profile record and,
+// for many particles, tile registration; the final launch only writes
+// FORCE_k into the record.  The wait is safe: the helper blocks are the
+// lowest block indices of the launch, dispatched in order before any tile
+// block, so the K - 1 blocks chunk 0 waits for have been given CU slots (a
+// launch has far fewer helper blocks than the GPU has slots).  Waiting on
+// the words themselves costs one memory round trip after the last chunk
+// stores them; a done-counter whose last incrementer finishes would cost
+// two (the counter's device-scope atomic, then the words).  The particle's
+// record and the contact inputs are requested before the chunk's
+// populations, so the finishing block has them at hand.
 template <class St, bool DEV, bool FINAL, int BND>
 __device__ __forceinline__ void particle_helper(const StepArgs &a, int p, int c) {
     const DevP &P = a.P;
@@ -1489,17 +1493,13 @@ __device__ __forceinline__ void particle_helper(const StepArgs &a, int p, int c)
     }
     double Fc[3];
     chunk_force<St, DEV, BND>(a, p, c, Fc);
-    __shared__ int slast;
     if (t == 0) {
         unsigned long long *o = a.fslot + ((size_t)p * a.kpart + c) * SLOTW;
 #pragma unroll
         for (int k = 0; k < 3; ++k) put_tagged_f64(o + 2 * k, a.tag, Fc[k]);
-        const unsigned k1 = (unsigned)a.kpart - 1u;
-        slast = atomicInc(a.pcount + p, k1) == k1;
     }
-    __syncthreads();
-    if (!slast) return;
-    // ---- the particle's last chunk: finish it
+    if (c != 0) return;
+    // ---- chunk 0: finish the particle
     __shared__ double srec[PCOUNT];
     __shared__ double scon[3 * NP_LDS];
     __shared__ double sdpw;
