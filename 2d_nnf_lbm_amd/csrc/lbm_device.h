@@ -170,7 +170,12 @@ struct StF64 {
 };
 struct StF32 {
     using T = float;
+    using R = unsigned;  // a stored value's bits (braw), to cvt later
     static constexpr float S = 1.0f, invS = 1.0f;
+    __device__ static R braw(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0);
+    }
+    __device__ static float cvt(R b) { return __builtin_bit_cast(float, b); }
     __device__ static float ld(const T *p) { return *p; }
     __device__ static void st(T *p, float v, uint32_t) { *p = v; }
     __device__ static float bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
@@ -184,6 +189,15 @@ struct StF32 {
 };
 struct StF16 {
     using T = uint16_t;
+    // one 32-bit register per value (two 16-bit values packed into one
+    // register would be waited for where they are packed)
+    using R = unsigned;
+    __device__ static R braw(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, 0);
+    }
+    __device__ static float cvt(R b) {
+        return (float)__builtin_bit_cast(_Float16, (unsigned short)b);
+    }
     // h is stored as 2^10 h, clear of the FP16 subnormals
     static constexpr float S = 1024.0f, invS = 1.0f / 1024.0f;
     __device__ static float ld(const T *p) { return (float)__builtin_bit_cast(_Float16, *p); }
@@ -212,6 +226,11 @@ struct StF16 {
 // the stored value unbiased.
 struct StBF16 {
     using T = uint16_t;
+    using R = unsigned;
+    __device__ static R braw(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, 0);
+    }
+    __device__ static float cvt(R b) { return __builtin_bit_cast(float, b << 16); }
     static constexpr float S = 1.0f, invS = 1.0f;
     __device__ static float ld(const T *p) {
         return __builtin_bit_cast(float, (uint32_t)(*p) << 16);

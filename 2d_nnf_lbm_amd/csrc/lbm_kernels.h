@@ -1125,6 +1125,16 @@ __device__ __forceinline__ void load_interior_h(const StepHead &h, unsigned n, V
 #pragma unroll
     for (int k = 0; k < 9; ++k) v[k] = St::bld(rs, vo, h.pull[k]);
 }
+// The same as the stored bits (reduced formats), converted later with
+// St::cvt: the loads are then not waited for until the first use of a value.
+template <class St>
+__device__ __forceinline__ void load_interior_raw(const StepHead &h, unsigned n,
+                                                  typename St::R v[9]) {
+    const auto rs = make_rsrc(h.src, h.buf_bytes);
+    const unsigned vo = n * (unsigned)sizeof(typename St::T);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = St::braw(rs, vo, h.pull[k]);
+}
 
 // Density and uncorrected velocity of a node from its moments in storage
 // units (reduced formats); the tiles and the particle helper blocks form
@@ -1231,10 +1241,13 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, const StepHead &h, i
     const int kind = wave_class<BND>(h, tile, jw);
     int i, j;
     float H[9];
+    typename St::R R[9];
     if (kind == PULL_INTERIOR) {
+        // the stored bits, converted after the candidate lookup (whose scalar
+        // loads then wait while these are in flight)
         j = jw;
         i = (int)(n - (unsigned)jw * h.W);
-        load_interior_h<St>(h, n, H);
+        load_interior_raw<St>(h, n, R);
     } else if (kind == PULL_SIDES) {
         j = row_of(h, n);
         i = (int)(n - (unsigned)j * h.W);
@@ -1250,6 +1263,10 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, const StepHead &h, i
     unsigned mask;
     const int *ids;
     const int cnt = tile_candidates(a, h, tile, mask, ids);
+    if (kind == PULL_INTERIOR) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) H[k] = St::cvt(R[k]);
+    }
     const DevMom mo = dev_moments(H);
     float rho, rinvS, ux, uy;
     dev_velocity(P, mo, St::invS, rho, rinvS, ux, uy);
