@@ -249,7 +249,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dense-steps", type=int, default=200,
                     help="timed steps of the 504-disk BF16 cavity (0 skips it)")
-    ap.add_argument("--ensemble-steps", type=int, default=40,
+    ap.add_argument("--ensemble-steps", type=int, default=400,
                     help="timed steps of the 256-cavity FP16 batch (0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=4096,
                     help="steps of the settling-velocity adjoint run (0 skips it)")
