@@ -240,6 +240,105 @@ def cpu_baseline():
                        "%d steps, wall %.2f s" % (steps, wall))
 
 
+def run_reference(name, files=None, timeout=900):
+    """Runs the reference timing program oracle/_ref/<name> once in a fresh
+    temporary directory (with `files`, name -> text, written there first).
+    Returns (wall seconds, the program's RUNTIME, the per-iteration CPU clock
+    rows (KK, seconds since the program's START) it wrote to fort.98), or None
+    when the build is absent or the run fails."""
+    exe = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory(prefix="refrun_") as d:
+        for fname, text in (files or {}).items():
+            with open(os.path.join(d, fname), "w") as fh:
+                fh.write(text)
+        t0 = time.perf_counter()
+        r = subprocess.run([exe], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                           text=True, timeout=timeout)
+        wall = time.perf_counter() - t0
+        if r.returncode:
+            log("bench: %s exited with %d" % (name, r.returncode))
+            return None
+        m = re.search(r"RUNTIME=\s*([0-9.Ee+-]+)", r.stdout)
+        clock = []
+        path = os.path.join(d, "fort.98")
+        if os.path.exists(path):
+            with open(path) as fh:
+                clock = [(int(a), float(b)) for a, b in (ln.split() for ln in fh if ln.strip())]
+    return wall, (float(m.group(1)) if m else None), clock
+
+
+def per_step_seconds(clock):
+    """Seconds per time-loop iteration from the clock rows, leaving out the
+    first iteration and everything before it (set-up, first touches)."""
+    (k0, t0), (k1, t1) = clock[0], clock[-1]
+    return (t1 - t0) / (k1 - k0)
+
+
+def cpu_baseline_newtonian():
+    """The reference Newtonian single-disk program (F6 with the one-disk
+    workload's 200 x 600 cell grid and MRT rate set, 600 steps) on one host
+    core."""
+    res = run_reference("f6_newt_200x600")
+    if res is None:
+        return None
+    wall, runtime, _ = res
+    steps, nodes = 600, 201 * 601
+    return dict(value=nodes * steps / wall / 1e6, unit="MLUPS", cores=1, kind="reference",
+                sample="reference free-fall program (F6, Newtonian, NX=200 NY=600, alternative "
+                       "MRT rates), %d steps, wall %.2f s (set-up included), CPU_TIME %.2f s"
+                       % (steps, wall, runtime or float("nan")))
+
+
+def cpu_baseline_ensemble(configs):
+    """The reference oscillating-cylinder program (F4, Papanastasiou fluid)
+    on square cavities of sides 128, 512 and 1024, one host core each; the
+    per-node cost of one step, interpolated linearly in the side, summed over
+    the 256 cavities of the batch gives the CPU time of one batch step."""
+    sides, spn = [], []
+    rows = {}
+    for name, side in (("f4_cav_128", 128), ("f4_cav_512", 512), ("f4_cav_1024", 1024)):
+        res = run_reference(name)
+        if res is None or len(res[2]) < 2:
+            return None
+        sec = per_step_seconds(res[2])
+        n = (side + 1) ** 2
+        sides.append(side)
+        spn.append(sec / n)
+        rows[str(side)] = dict(steps_timed=res[2][-1][0] - res[2][0][0],
+                               ms_per_step=sec * 1e3, mlups=n / sec / 1e6,
+                               wall_s=round(res[0], 2))
+    ens = configs.cavity_ensemble()
+    nodes = [configs.nodes(c) for c, _ in ens]
+    step_s = sum(n * float(np.interp(c["nx"], sides, spn)) for n, (c, _) in zip(nodes, ens))
+    return dict(value=sum(nodes) / step_s / 1e6, unit="MLUPS", cores=1, kind="reference",
+                sample="reference oscillating-cylinder program (F4, Papanastasiou fluid, "
+                       "output off) on cavities of side 128, 512 and 1024, per-step CPU clock "
+                       "after the first step; the batch's 256 sides priced by linear "
+                       "interpolation of the per-node cost (extrapolated, one core)",
+                seconds_per_batch_step=step_s, measured=rows)
+
+
+def cpu_baseline_dense(configs):
+    """The reference multi-particle program (F3) with the dense workload's
+    504 disks on the 2048 x 2048 cell cavity, three steps on one host core;
+    per-step cost from its CPU clock after the first step.  About 2.5 minutes
+    and 17 GB of host memory (per-particle full-grid solid fractions)."""
+    pos = np.loadtxt(configs.DENSE_FILE)
+    text = "".join("%r %r\n" % (float(x), float(y)) for x, y in pos)
+    res = run_reference("f3_dense_2048", {"0_Particle_Initial_Position.txt": text}, timeout=1200)
+    if res is None or len(res[2]) < 2:
+        return None
+    sec = per_step_seconds(res[2])
+    n = 2049 * 2049
+    return dict(value=n / sec / 1e6, unit="MLUPS", cores=1, kind="reference",
+                sample="reference multi-particle program (F3, 504 disks from the dense "
+                       "workload's positions, NX=NY=2048), %d steps, per-step CPU clock after "
+                       "the first step, wall %.1f s" % (res[2][-1][0], res[0]),
+                seconds_per_step=sec)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +346,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--accuracy-steps", type=int, default=50000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-dense", action="store_true",
+                    help="also time the reference multi-particle program on the 504-disk "
+                         "cavity (about 2.5 minutes and 17 GB of host memory)")
+    ap.add_argument("--newtonian-steps", type=int, default=2000,
+                    help="timed steps of the one-disk Newtonian FP64 run (0 skips it)")
     ap.add_argument("--dense-steps", type=int, default=200,
                     help="timed steps of the 504-disk BF16 cavity (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=400,
@@ -275,6 +379,20 @@ def main():
     r64 = measure(pkg, configs, "fp64", args.steps, args.warmup)
     log("bench: fp16 leg")
     r16 = measure(pkg, configs, "fp16", args.steps, args.warmup)
+    newt = None
+    if args.newtonian_steps > 0:
+        log("bench: one-disk newtonian, fp64")
+        wl = configs.one_disk_newtonian("fp64")
+        rn = measure(pkg, configs, "fp64", args.newtonian_steps, 100, workload=wl)
+        newt = dict(workload="one disk (D=25, density ratio 1.5) falling from rest in a "
+                             "Newtonian fluid (tau=0.6), 201x601 nodes, bounce-back walls, "
+                             "plain FP64",
+                    nodes=configs.nodes(wl[0]), steps=args.newtonian_steps,
+                    mlups=rn["mlups"], ms_per_step=rn["ms_per_step"],
+                    kernel_us_per_launch=rn["ms_main_kernel"] * 1e3,
+                    achieved_gbs=rn["achieved_gbs"],
+                    roofline_frac=rn["achieved_gbs"] / HBM_PEAK_GBS,
+                    algorithmic_bytes_per_node=rn["bytes_per_node"])
     dense = None
     if args.dense_steps > 0:
         log("bench: 504-disk cavity, bf16")
@@ -303,6 +421,15 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         cpu = cpu_baseline()
+        if newt is not None:
+            log("bench: cpu baseline, one-disk newtonian")
+            newt["cpu_baseline"] = cpu_baseline_newtonian()
+        if ensemble is not None:
+            log("bench: cpu baseline, cavity programs")
+            ensemble["cpu_baseline"] = cpu_baseline_ensemble(configs)
+        if dense is not None and args.cpu_dense:
+            log("bench: cpu baseline, 504-disk program")
+            dense["cpu_baseline"] = cpu_baseline_dense(configs)
     cfg, _ = configs.two_disk_bingham("fp16")
     n = configs.nodes(cfg)
     line = {
@@ -354,6 +481,7 @@ def main():
                            "note": "the same FP16 steps with the per-step history "
                                    "(particle rows, mass, yielded count), wall clock"},
         "fp16_over_fp64": r16["mlups"] / r64["mlups"],
+        "one_disk_newtonian_fp64": newt,
         "dense_sedimentation_bf16": dense,
         "cavity_ensemble_fp16": ensemble,
         "adjoint_fp32": adjoint,

@@ -104,12 +104,76 @@ VARIANTS = {
     "f6_eq_bing_local": dict(nx=40, ny=120, non_newtonian=True, time_max=1000,
                              local_rate=True, boundary="equilibrium"),
 }
-# CPU-baseline build: the reference's Bingham settling disk on the 401x1601
-# node grid (NX=400, NY=1600), timed for a bounded number of steps.
+# CPU-baseline builds: the reference's Bingham settling disk on the 401x1601
+# node grid (NX=400, NY=1600), timed for a bounded number of steps; its
+# Newtonian single disk on the one-disk workload's 201x601 grid with the
+# alternative rate set.
 TIMING_VARIANTS = {
     "f6_bing_400x1600": dict(nx=400, ny=1600, non_newtonian=True, time_max=40,
                              write_output=False),
+    "f6_newt_200x600": dict(nx=200, ny=600, non_newtonian=False, time_max=600,
+                            write_output=False, rates="mohammad"),
 }
+# The oscillating-cylinder program (F4) on square cavities of three sides
+# (the ensemble's size range), Papanastasiou fluid, output files off, the
+# time loop bounded; its characteristic length L_C (cylinder diameter, and
+# the inner wall box of 10 x 15 L_C, F4:179-180) set to side/30 so the box
+# lies inside the cavity.  name -> (side, steps)
+F4_TIMING = {"f4_cav_128": (128, 2000), "f4_cav_512": (512, 100), "f4_cav_1024": (1024, 25)}
+# The multi-particle program (F3) with the dense workload's 504 disks on the
+# 2048 x 2048 cell cavity, three steps (its per-particle full-grid solid
+# fractions take about 17 GB).
+F3_TIMING = {"f3_dense_2048": dict(src="F3", nx=2048, ny=2048, n_particle=504, time_max=3,
+                                   clock=True, write_output=False)}
+
+
+def patch_f4_timing(text, side, steps):
+    """F4 as shipped with NX = NY = side, NON_NEWTONIAN and WRITE_OUTPUT
+    switched (F4:4, F4:6), L_C = side/30 (F4:27), the time loop bounded to
+    `steps` iterations (F4:216) and IF(0)/IF(1) written as logicals."""
+    reps = [
+        (r"(INTEGER,PARAMETER :: NX=)\d+", r"\g<1>%d" % side),
+        (r"(INTEGER,PARAMETER :: NY=)\d+", r"\g<1>%d" % side),
+        (r"(LOGICAL,PARAMETER :: NON_NEWTONIAN=)\s*\.FALSE\.", r"\g<1> .TRUE."),
+        (r"(LOGICAL,PARAMETER :: WRITE_OUTPUT=)\s*\.TRUE\.", r"\g<1> .FALSE."),
+        (r"(DOUBLE PRECISION,PARAMETER :: L_C=)50\.0", r"\g<1>%.1f" % (side / 30.0)),
+        (r"DO WHILE\(KK<INT\(6\.0\*T_C\)\)", "DO WHILE(KK<%d)" % steps),
+    ]
+    for pat, rep in reps:
+        text, n = re.subn(pat, rep, text, count=1)
+        if n != 1:
+            raise RuntimeError("F4 timing patch did not apply: " + pat)
+    # integer IF conditions as logicals (flang rejects the integer form)
+    text = re.sub(r"\bIF\(0\)", "IF(.FALSE.)", text)
+    text = re.sub(r"\bIF\(1\)", "IF(.TRUE.)", text)
+    # the program's CPU clock (START is set at F4:135) written after every
+    # loop iteration, so a timing can leave the set-up out
+    text, n = re.subn(r"(\n(\s*)WRITE\(8,\*\) KK,AvgDen[^\n]*)",
+                      r"\1\n\2CALL CPU_TIME(FINISH);WRITE(98,*) KK,FINISH-START;FLUSH(98)",
+                      text, count=1)
+    if n != 1:
+        raise RuntimeError("F4 timing patch: the loop's density line was not found")
+    return text
+
+
+def build_f4_timing(name, side, steps, flags):
+    with open(F4, "rb") as fh:
+        text = fh.read().decode("ascii", errors="replace").replace("\r", "")
+    src = patch_f4_timing(text, side, steps)
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmpdir = tempfile.mkdtemp(prefix="refbuild_", dir="/tmp")
+    try:
+        path = os.path.join(tmpdir, name + ".f90")
+        with open(path, "w") as fh:
+            fh.write(src)
+        exe = os.path.join(OUT_DIR, name)
+        r = subprocess.run([FLANG] + flags + [path, "-o", exe], cwd=tmpdir,
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode:
+            raise RuntimeError("%s build failed:\n%s" % (name, r.stdout[-3000:]))
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    return exe
 
 
 def patch_source(text, nx, ny, non_newtonian, time_max, local_rate=False,
@@ -325,12 +389,14 @@ NPARTICLE_BUILDS = {
 }
 
 
-def patch_nparticle(text, nx, ny, time_max, n_particle=None, centers=None):
+def patch_nparticle(text, nx, ny, time_max, n_particle=None, centers=None, clock=False,
+                    write_output=True):
     """F3 or F2 made runnable and observable, without touching the
     algorithm:
       * IF(0)/IF(1) to logicals and PAUSE removed, as for F6;
       * PARAMETERs NX, NY (F3:8-9, F2:7-8), TIME_MAX (F3:37, F2:35) and
-        WRITE_OUTPUT (F3:6, F2:5: the final state dump, F3:425-447); F3's
+        WRITE_OUTPUT (F3:6, F2:5: the final state dump, F3:425-447; on
+        unless `write_output` is false); F3's
         initial N_PARTICLE (F3:68);
       * F3: the STOP after the initial domain image (F3:202), which ends the
         program before its time loop, commented out;
@@ -339,7 +405,9 @@ def patch_nparticle(text, nx, ny, time_max, n_particle=None, centers=None):
         written to ref_constants.txt after OMEGA is set (F3:99, F2:97), and,
         at the end of every loop iteration (after the mean-density line,
         F3:378, F2:371), KK and each disk's CENTER, VELOCITY, FORCE and
-        ANGLE written to ref_history.txt."""
+        ANGLE written to ref_history.txt; with `clock`, also KK and the
+        program's CPU clock since START (F3:136, F2:138) to fort.98, so a
+        timing can leave the set-up out."""
     lines = text.replace("\r", "").split("\n")
     out = []
     in_loop = False
@@ -353,7 +421,8 @@ def patch_nparticle(text, nx, ny, time_max, n_particle=None, centers=None):
         s = re.sub(r"(INTEGER,PARAMETER :: NX=)\d+", r"\g<1>%d" % nx, s)
         s = re.sub(r"(INTEGER,PARAMETER :: NY=)\d+", r"\g<1>%d" % ny, s)
         s = re.sub(r"(INTEGER,PARAMETER :: TIME_MAX=).*$", r"\g<1>%d" % time_max, s)
-        s = re.sub(r"(LOGICAL,PARAMETER :: WRITE_OUTPUT=)\s*\.\w+\.", r"\g<1> .TRUE.", s)
+        s = re.sub(r"(LOGICAL,PARAMETER :: WRITE_OUTPUT=)\s*\.\w+\.",
+                   r"\g<1> .%s." % ("TRUE" if write_output else "FALSE"), s)
         if n_particle is not None:
             s = re.sub(r"(INTEGER :: N_PARTICLE=)\d+", r"\g<1>%d" % n_particle, s)
         if centers is not None:
@@ -376,6 +445,8 @@ def patch_nparticle(text, nx, ny, time_max, n_particle=None, centers=None):
         if in_loop and re.match(r"\s*WRITE\(8,\*\) KK,AvgDen", s):
             out.append("\t\tWRITE(97,*) KK,(DISK(IP)%CENTER,DISK(IP)%VELOCITY,"
                        "DISK(IP)%FORCE,DISK(IP)%ANGLE,IP=1,N_PARTICLE)")
+            if clock:
+                out.append("\t\tCALL CPU_TIME(FINISH);WRITE(98,*) KK,FINISH-START;FLUSH(98)")
             in_loop = False
     assert n_particle is None or stop_done, "F3's STOP before the time loop not found"
     return "\n".join(out) + "\n"
@@ -480,6 +551,13 @@ def main(argv):
     if "--timing" in argv or "--all" in argv:
         for name, cfg in TIMING_VARIANTS.items():
             built.append(build(name, cfg, timing_flags))
+        if os.path.exists(F4):
+            for name, (side, steps) in F4_TIMING.items():
+                built.append(build_f4_timing(name, side, steps, timing_flags))
+        if os.path.exists(F3):
+            # its static arrays exceed 2 GB: the medium code model
+            for name, cfg in F3_TIMING.items():
+                built.append(build_nparticle(name, cfg, timing_flags + ["-mcmodel=medium"]))
     for b in built:
         print("built", os.path.relpath(b))
     return 0
