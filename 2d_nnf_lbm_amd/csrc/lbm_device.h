@@ -673,6 +673,25 @@ __device__ __forceinline__ bool tile_meets_box(int W, const TileSpan &t, int i0,
     };
     return seg(ja) || seg(jb);
 }
+// The same test with a box that differs between lanes (one particle per
+// lane), branch-free: each clause is an integer that is <= 0 when it holds
+// (rows share a node, the box has a column, a full row is shared or one of
+// the two partial rows meets the box's columns), so the test is a
+// min/max/select chain ending in one compare.  `live` false: no node.
+__device__ __forceinline__ bool span_meets_box_lane(int W, const TileSpan &t, int i0, int i1,
+                                                    int j0, int j1, bool live) {
+    const int ja = max(t.jl, j0), jb = min(t.jh, j1);
+    const int d = jb - ja;
+    const int iF = (int)(t.n0 - (unsigned)t.jl * (unsigned)W);
+    const int iL = (int)(t.n1 - (unsigned)t.jh * (unsigned)W);
+    const int lo_a = ja == t.jl ? iF : 0, hi_a = ja == t.jh ? iL : W - 1;
+    const int lo_b = jb == t.jl ? iF : 0, hi_b = jb == t.jh ? iL : W - 1;
+    const int mx_a = max(lo_a, i0) - min(hi_a, i1);
+    const int mx_b = max(lo_b, i0) - min(hi_b, i1);
+    const int any = min(min(2 - d, mx_a), mx_b);
+    const int v = max(max(-d, i0 - i1), any);
+    return (live ? v : 1) <= 0;
+}
 
 // Force partials cross between blocks of one launch as generation-tagged
 // 64-bit words: (tag << 32) | 32-bit payload, written and read with
@@ -1155,8 +1174,8 @@ __device__ __forceinline__ float fexp(float x) {
 // (1 - s_k/2) / N_k come precomputed in DevP (cD*).  The momentum S rho u_c
 // of the corrected velocity is jx + FX/2 with FX = S rho gx, so n3 = -FX/2
 // and n5 = -FY/2 (zero without forcing) are formed without cancellation.
-template <bool FORCED, bool OUT>
-__device__ __forceinline__ void collide_dev(const DevP &P, float H[9], const DevMom &mo,
+template <bool FORCED, bool OUT, class CP = DevP>
+__device__ __forceinline__ void collide_dev(const CP &P, float H[9], const DevMom &mo,
                                             float S, float rho, float rinvS, float ucx,
                                             float ucy, float gx, float gy, float om,
                                             float &om_out, float stress[3], int &yielded) {

@@ -454,8 +454,25 @@ struct StepHead {
 };
 static_assert(sizeof(StepHead) == 128, "two 16-dword scalar loads");
 
+// What a reduced-precision tile's collision and stores read besides the
+// header: the FP32 collision parameters of DevP under the same names (so
+// collide_dev reads either), the output flags (bit 0: the collision forms
+// stress and yield flags, aux or want_yield; bit 1: aux) and the two
+// destination buffers.  A single-simulation tile reads it with two scalar
+// loads before its collision, so the collision waits on no other scalar
+// load.
+struct CollideHead {
+    int q_mode, rheology;
+    float cD1a, cD1b, cD2a, cD2b, cC3, cC5, cDq1, cDq2, cS1, cS2;
+    float fnu, fgamma_min2, fmu0, ftau_y, fm_pap, fsqrt_mu0, fcas_lim;
+    unsigned out;
+    void *dst, *rate_out;
+};
+static_assert(sizeof(CollideHead) == 96, "a 16-dword and an 8-dword scalar load");
+
 struct StepArgs {
     StepHead hd;                  // copies of fields below (finish_head)
+    CollideHead ch;               // likewise
     DevP P;
     const void *src;              // post-collision populations f*_k (storage format)
     void *dst;                    // f*_{k+1} (storage format)
@@ -542,6 +559,21 @@ __device__ __forceinline__ StepHead load_head(const StepArgs &a) {
     return __builtin_bit_cast(StepHead, v);
 }
 
+__device__ __forceinline__ CollideHead load_collide(const StepArgs &a) {
+    typedef unsigned V16 __attribute__((ext_vector_type(16)));
+    typedef unsigned V8 __attribute__((ext_vector_type(8)));
+    typedef const __attribute__((address_space(4))) char CC;
+    CC *p = (CC *)&a.ch;
+    const V16 lo = *(const __attribute__((address_space(4))) V16 *)p;
+    const V8 hi = *(const __attribute__((address_space(4))) V8 *)(p + 64);
+    unsigned w[24];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) w[k] = lo[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[16 + k] = hi[k];
+    return __builtin_bit_cast(CollideHead, w);
+}
+
 // j = n / W from the header (row_of).
 __device__ __forceinline__ int row_of(const StepHead &h, unsigned n) {
     unsigned t = __umulhi(h.w_magic, n);
@@ -601,6 +633,7 @@ __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nst
     a.red_mass = (rec && s > 1) ? a.slots_base + (size_t)(s - 2) * row2 : nullptr;
     a.red_yield = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * row2 + NSLOT : nullptr;
     a.hd.sr = a.sr;
+    a.ch.out = ((a.aux || a.want_yield) ? 1u : 0u) | (a.aux ? 2u : 0u);
 }
 
 // Fill the header from the other fields (host, after the last change to
@@ -631,6 +664,18 @@ inline void finish_head(StepArgs &a, unsigned elem) {
     const int cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
     for (int k = 0; k < 9; ++k)
         h.pull[k] = (unsigned)((k * P.plane - cx[k] - (long long)cy[k] * P.W) * (long long)elem);
+    CollideHead &c = a.ch;
+    std::memset(&c, 0, sizeof c);
+    c.q_mode = P.q_mode;
+    c.rheology = P.rheology;
+    c.cD1a = P.cD1a; c.cD1b = P.cD1b; c.cD2a = P.cD2a; c.cD2b = P.cD2b;
+    c.cC3 = P.cC3; c.cC5 = P.cC5; c.cDq1 = P.cDq1; c.cDq2 = P.cDq2;
+    c.cS1 = P.cS1; c.cS2 = P.cS2;
+    c.fnu = P.fnu; c.fgamma_min2 = P.fgamma_min2; c.fmu0 = P.fmu0; c.ftau_y = P.ftau_y;
+    c.fm_pap = P.fm_pap; c.fsqrt_mu0 = P.fsqrt_mu0; c.fcas_lim = P.fcas_lim;
+    c.out = ((a.aux || a.want_yield) ? 1u : 0u) | (a.aux ? 2u : 0u);
+    c.dst = a.dst;
+    c.rate_out = a.rate_out;
 }
 
 __device__ __forceinline__ void set_err(const StepArgs &a, int bits) { atomicOr(a.err, bits); }
@@ -810,6 +855,17 @@ __device__ __forceinline__ void store_dev(const DevP &P, void *dst, unsigned n, 
                 (unsigned)(k * P.plane * (long long)sizeof(typename St::T)), sr_word(nh, k));
 }
 
+// The same from the header's byte sizes (a tile's stores).
+template <class St>
+__device__ __forceinline__ void store_dev_h(const StepHead &h, void *dst, unsigned n,
+                                            const float H[9], uint32_t nh) {
+    const auto drs = make_rsrc(dst, h.buf_bytes);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        St::bst(drs, H[k], n * (unsigned)sizeof(typename St::T), (unsigned)k * h.plane_bytes,
+                sr_word(nh, k));
+}
+
 // Prologue of a fused run: collide the FP64 state of step k (F, RHO, corrected
 // UX/UY, G, OMEGA_BN) into f*_{k+1} in the storage format.
 template <class St, bool DEV>
@@ -919,32 +975,76 @@ __device__ __forceinline__ int image_node(const DevP &P, const double *part, int
 }
 
 // ---------------------------------------------------------------- tiles
-// Which particles can reach tile t in this launch?  Up to NP_LOOP particles:
-// a bit per particle whose reach box of state k meets the tile; beyond: the
-// tile's candidate list of this launch (header: generation << 32 | count).
-// Scalar loads and scalar arithmetic, uniform over the block; a tile calls
-// it after issuing its population loads, which the scalar loads do not wait
-// for.  (These buffers were written by the previous launch, not this one.)
+// The nodes of the calling thread's wavefront in tile t: its first node, its
+// last node (clamped to the grid's last node) and their rows.  Uniform.  A
+// wavefront wholly past the last node gets a span whose first row lies
+// below its last (it meets no box).
+__device__ __forceinline__ TileSpan wave_span(const StepHead &h, int tile) {
+    TileSpan s;
+    s.n0 = (unsigned)tile * NT +
+           (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+    s.n1 = min(s.n0 + 63u, h.nn - 1u);
+    s.jl = row_of(h, s.n0);
+    s.jh = row_of(h, s.n1);
+    return s;
+}
+
+// Reach boxes of state k for the per-wavefront candidate test (1 ..
+// NP_LOOP particles): lane p holds particle p's box (i0, i1, j0, j1); lanes
+// past the last particle read zeros (the buffer's bound).  Issued before the
+// population loads, so waiting for it does not wait for them.  (Written by
+// the previous launch.)
+struct BoxLane {
+    int i0, i1, j0, j1;
+};
+__device__ __forceinline__ BoxLane load_box_lane(const StepArgs &a, const StepHead &h) {
+    BoxLane b{1, 0, 1, 0};
+    if (h.np > 0 && h.np <= NP_LOOP) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+            make_rsrc(a.pbox_in, (unsigned)h.np * 16u), (int)((threadIdx.x & 63u) * 16u), 0, 0);
+        b = BoxLane{(int)v[0], (int)v[1], (int)v[2], (int)v[3]};
+    }
+    return b;
+}
+
+// Which particles can reach the calling wavefront's nodes in this launch?
+// Up to NP_LOOP particles: a bit per particle whose reach box of state k
+// meets the nodes, either per wavefront with one lane per particle (LANES:
+// vector work and one compare mask, boxes from load_box_lane) or per tile
+// with scalar loads and scalar arithmetic; beyond NP_LOOP: the tile's
+// candidate list of this launch (header: generation << 32 | count).  A
+// candidate whose box misses a node contributes nothing there, so the
+// wavefront's set and its tile's give the same forcing.  (These buffers
+// were written by the previous launch, not this one.)
+template <bool LANES>
 __device__ __forceinline__ int tile_candidates(const StepArgs &a, const StepHead &h, int t,
+                                               const TileSpan &sp, const BoxLane &b,
                                                unsigned &mask, const int *&ids) {
     mask = 0u;
     ids = nullptr;
     const int np = h.np;
     if (np <= 0) return 0;
     if (np <= NP_LOOP) {
-        TileSpan sp;
-        sp.n0 = (unsigned)t * NT;
-        sp.n1 = min(sp.n0 + (unsigned)(NT - 1), h.nn - 1u);
-        sp.jl = row_of(h, sp.n0);
-        sp.jh = row_of(h, sp.n1);
-        unsigned m = 0u;
-        for (int p = 0; p < np; ++p) {
-            const int *w = a.pbox_in + 4 * p;
-            if (tile_meets_box((int)h.W, sp, cload(w), cload(w + 1), cload(w + 2), cload(w + 3)))
-                m |= 1u << p;
+        if (LANES) {
+            const bool hit = span_meets_box_lane((int)h.W, sp, b.i0, b.i1, b.j0, b.j1,
+                                                 (int)(threadIdx.x & 63u) < np);
+            mask = (unsigned)__builtin_amdgcn_ballot_w64(hit);
+        } else {
+            TileSpan ts;
+            ts.n0 = (unsigned)t * NT;
+            ts.n1 = min(ts.n0 + (unsigned)(NT - 1), h.nn - 1u);
+            ts.jl = row_of(h, ts.n0);
+            ts.jh = row_of(h, ts.n1);
+            unsigned m = 0u;
+            for (int p = 0; p < np; ++p) {
+                const int *w = a.pbox_in + 4 * p;
+                if (tile_meets_box((int)h.W, ts, cload(w), cload(w + 1), cload(w + 2),
+                                   cload(w + 3)))
+                    m |= 1u << p;
+            }
+            mask = m;
         }
-        mask = m;
-        return __builtin_popcount(m);
+        return __builtin_popcount(mask);
     }
     const unsigned long long hdr = cload(a.tl_hdr_in + t);
     if ((unsigned)(hdr >> 32) != a.list_gen) return 0;
@@ -1100,16 +1200,13 @@ __device__ __forceinline__ void side_edges64(const DevP &P, const StepHead &h,
 // per-lane pull of load_fk_lane_dev.  jw: the wavefront's first row.
 enum { PULL_INTERIOR = 0, PULL_SIDES = 1, PULL_GENERAL = 2 };
 template <int BND>
-__device__ __forceinline__ int wave_class(const StepHead &h, int tile, int &jw) {
-    const unsigned w0 = (unsigned)tile * NT +
-                        (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
-    const unsigned w1 = w0 + 63u;
+__device__ __forceinline__ int wave_class(const StepHead &h, const TileSpan &sp, int &jw) {
     jw = 0;
-    if (w1 >= h.nn) return PULL_GENERAL;
+    if (sp.n0 + 63u >= h.nn) return PULL_GENERAL;
     const int e = BND == 2 ? 2 : (BND >= 0 ? 1 : h.margin);
-    const int j0 = row_of(h, w0), j1 = row_of(h, w1);
-    const int i0 = (int)(w0 - (unsigned)j0 * h.W);
-    const int i1 = (int)(w1 - (unsigned)j1 * h.W);
+    const int j0 = sp.jl, j1 = sp.jh;
+    const int i0 = (int)(sp.n0 - (unsigned)j0 * h.W);
+    const int i1 = (int)(sp.n1 - (unsigned)j1 * h.W);
     jw = j0;
     if (j0 == j1 && j0 >= e && j0 <= h.ny - e && i0 >= e && i1 <= h.nx - e) return PULL_INTERIOR;
     if (e == 1 && j0 >= 1 && j1 <= h.ny - 1) return PULL_SIDES;
@@ -1161,8 +1258,10 @@ __device__ __forceinline__ void final_aux(const StepArgs &a, unsigned n, int i, 
 // collision of step k+1 stored as f*_{k+1}, the rate, edge arrays and
 // auxiliary fields, or (FINAL = true) the FP64 state of step k; then the
 // monitor partials.  FORCED: G may be non-zero.
-template <class St, bool FINAL, int BND, bool FORCED>
-__device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &h, int tile,
+template <class St, bool FINAL, int BND, bool FORCED, class CP>
+__device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &h,
+                                             const CP &cp, unsigned out, void *dst,
+                                             void *rate_out, int tile,
                                              unsigned n, bool valid, bool edges, int i, int j,
                                              float (&H)[9],
                                              const DevMom &mo, float rho, float rinvS, float ucx,
@@ -1187,19 +1286,18 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &
             final_aux(a, n, i, j);
         } else {
             float om2, str[3];
-            if (a.aux || a.want_yield)
-                collide_dev<FORCED, true>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2, str,
+            if (out & 1u)
+                collide_dev<FORCED, true>(cp, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2, str,
                                           yielded);
             else
-                collide_dev<FORCED, false>(P, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2,
+                collide_dev<FORCED, false>(cp, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2,
                                            str, yielded);
-            store_dev<St>(P, a.dst, n, H, node_hash(n, h.sr));
-            if (P.rheology != 0)
+            store_dev_h<St>(h, dst, n, H, node_hash(n, h.sr));
+            if (cp.rheology != 0)
                 __builtin_amdgcn_raw_buffer_store_b32(
-                    __builtin_bit_cast(unsigned, om2),
-                    make_rsrc(a.rate_out, (unsigned)(P.plane * 4)), (int)(n * 4u), 0,
-                    ST_THROUGH);
-            if (a.aux) {
+                    __builtin_bit_cast(unsigned, om2), make_rsrc(rate_out, h.rate_bytes),
+                    (int)(n * 4u), 0, ST_THROUGH);
+            if (out & 2u) {
                 a.xstress[n] = str[0]; a.xstress[P.plane + n] = str[1];
                 a.xstress[2 * P.plane + n] = str[2];
                 if (P.rheology != 0) a.ximage[n] = yielded ? 0 : 5;
@@ -1224,10 +1322,14 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &
 // One fused time step k over a tile, reduced-precision storage (FP32
 // collide): pull-stream + boundary rules (f_k, by the wavefront's pull
 // class), density and velocity, SPM forcing G_k from the particles that can
-// reach the tile, velocity correction, then the outputs (tile_dev_out).  The
-// node rate is requested first; the candidate particles are looked up
-// (scalar) while the population loads are in flight.
-template <class St, bool FINAL, int BND>
+// reach the wavefront, velocity correction, then the outputs
+// (tile_dev_out).  The node rate (and, SINGLE, the particles' reach boxes)
+// are requested first; the candidate test runs while the population loads
+// are in flight.  SINGLE: a single-simulation launch (the batch kernel, which
+// keeps more of each simulation's arguments in scalar registers, tests the
+// candidates per tile with scalar work and reads the collision parameters
+// from DevP).
+template <class St, bool FINAL, int BND, bool SINGLE>
 __device__ __forceinline__ void tile_dev(const StepArgs &a, const StepHead &h, int tile) {
     const DevP &P = a.P;
     const unsigned n = (unsigned)tile * NT + threadIdx.x;
@@ -1237,48 +1339,75 @@ __device__ __forceinline__ void tile_dev(const StepArgs &a, const StepHead &h, i
                          ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                make_rsrc(h.rate_in, h.rate_bytes), (int)(n * 4u), 0, 0))
                          : P.fomega;
+    const TileSpan sp = wave_span(h, tile);
+    const BoxLane bx = SINGLE ? load_box_lane(a, h) : BoxLane{1, 0, 1, 0};
+    CollideHead ch;
+    if constexpr (SINGLE) ch = load_collide(a);
     int jw;
-    const int kind = wave_class<BND>(h, tile, jw);
+    const int kind = wave_class<BND>(h, sp, jw);
     int i, j;
     float H[9];
-    typename St::R R[9];
+    unsigned mask;
+    const int *ids;
+    int cnt = 0;
+    // SINGLE: each branch tests the candidates after issuing its own loads,
+    // so the test waits for the reach boxes alone; the batch looks them up
+    // (scalar) after the branches, while the loads are in flight
     if (kind == PULL_INTERIOR) {
-        // the stored bits, converted after the candidate lookup (whose scalar
-        // loads then wait while these are in flight)
+        // the stored bits, converted after the candidate test
+        typename St::R R[9];
         j = jw;
         i = (int)(n - (unsigned)jw * h.W);
         load_interior_raw<St>(h, n, R);
+        cnt = SINGLE ? tile_candidates<true>(a, h, tile, sp, bx, mask, ids)
+                     : tile_candidates<false>(a, h, tile, sp, bx, mask, ids);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) H[k] = St::cvt(R[k]);
     } else if (kind == PULL_SIDES) {
         j = row_of(h, n);
         i = (int)(n - (unsigned)j * h.W);
         load_interior_h<St>(h, n, H);
+        if constexpr (SINGLE) cnt = tile_candidates<true>(a, h, tile, sp, bx, mask, ids);
         side_edges_dev<St, BND>(P, h, (const float *)a.erho_src, n, i, j, H);
     } else {
         const unsigned nc = valid ? n : h.nn - 1u;
         j = row_of(h, nc);
         i = (int)(nc - (unsigned)j * h.W);
         load_fk_lane_dev<St, BND>(P, h.src, (const float *)a.erho_src, nc, i, j, H);
+        if constexpr (SINGLE) cnt = tile_candidates<true>(a, h, tile, sp, bx, mask, ids);
+    }
+    if constexpr (!SINGLE) {
+        if (kind != PULL_INTERIOR) cnt = tile_candidates<false>(a, h, tile, sp, bx, mask, ids);
     }
     const bool edges = kind != PULL_INTERIOR;
-    unsigned mask;
-    const int *ids;
-    const int cnt = tile_candidates(a, h, tile, mask, ids);
-    if (kind == PULL_INTERIOR) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) H[k] = St::cvt(R[k]);
-    }
     const DevMom mo = dev_moments(H);
     float rho, rinvS, ux, uy;
     dev_velocity(P, mo, St::invS, rho, rinvS, ux, uy);
     if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
-    if (cnt > 0) {
-        float gx, gy;
-        tile_forcing<true, float>(a, cnt, mask, ids, i, j, valid, ux, uy, gx, gy);
-        tile_dev_out<St, FINAL, BND, true>(a, h, tile, n, valid, edges, i, j, H, mo, rho, rinvS,
-                                           ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
+    if constexpr (SINGLE) {
+        if (cnt > 0) {
+            float gx, gy;
+            tile_forcing<true, float>(a, cnt, mask, ids, i, j, valid, ux, uy, gx, gy);
+            tile_dev_out<St, FINAL, BND, true>(a, h, ch, ch.out, ch.dst, ch.rate_out, tile, n,
+                                               valid, edges, i, j, H, mo, rho, rinvS,
+                                               ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
+        } else {
+            tile_dev_out<St, FINAL, BND, false>(a, h, ch, ch.out, ch.dst, ch.rate_out, tile, n,
+                                                valid, edges, i, j, H, mo, rho, rinvS, ux, uy,
+                                                0.0f, 0.0f, om);
+        }
     } else {
-        tile_dev_out<St, FINAL, BND, false>(a, h, tile, n, valid, edges, i, j, H, mo, rho,
-                                            rinvS, ux, uy, 0.0f, 0.0f, om);
+        if (cnt > 0) {
+            float gx, gy;
+            tile_forcing<true, float>(a, cnt, mask, ids, i, j, valid, ux, uy, gx, gy);
+            tile_dev_out<St, FINAL, BND, true>(a, h, P, a.ch.out, a.dst, a.rate_out, tile, n,
+                                               valid, edges, i, j, H, mo, rho, rinvS,
+                                               ux + 0.5f * gx, uy + 0.5f * gy, gx, gy, om);
+        } else {
+            tile_dev_out<St, FINAL, BND, false>(a, h, P, a.ch.out, a.dst, a.rate_out, tile, n,
+                                                valid, edges, i, j, H, mo, rho, rinvS, ux, uy,
+                                                0.0f, 0.0f, om);
+        }
     }
 }
 
@@ -1293,8 +1422,9 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, const StepHead &h, i
     const double om = (!FINAL && h.rheology != 0)
                           ? StF64::bld(make_rsrc(h.rate_in, h.rate_bytes), n * 8u, 0u)
                           : P.omega;
+    const TileSpan sp = wave_span(h, tile);
     int jw;
-    const int kind = wave_class<BND>(h, tile, jw);
+    const int kind = wave_class<BND>(h, sp, jw);
     int i, j;
     double f[9], rho, ux, uy;
     if (kind == PULL_INTERIOR) {
@@ -1315,7 +1445,7 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, const StepHead &h, i
     const bool edges = kind != PULL_INTERIOR;
     unsigned mask;
     const int *ids;
-    const int cnt = tile_candidates(a, h, tile, mask, ids);
+    const int cnt = tile_candidates<false>(a, h, tile, sp, BoxLane{1, 0, 1, 0}, mask, ids);
     macros64(f, rho, ux, uy);
     if (valid && !__builtin_isfinite(rho)) set_err(a, LBM_EF_NAN);
     double gx = 0.0, gy = 0.0;
@@ -1364,9 +1494,9 @@ __device__ __forceinline__ void tile_f64(const StepArgs &a, const StepHead &h, i
                              !FINAL && a.want_yield);
 }
 
-template <class St, bool DEV, bool FINAL, int BND>
+template <class St, bool DEV, bool FINAL, int BND, bool SINGLE = true>
 __device__ __forceinline__ void step_tile(const StepArgs &a, const StepHead &h, int tile) {
-    if constexpr (DEV) tile_dev<St, FINAL, BND>(a, h, tile);
+    if constexpr (DEV) tile_dev<St, FINAL, BND, SINGLE>(a, h, tile);
     else tile_f64<St, FINAL, BND>(a, h, tile);
 }
 
@@ -1583,12 +1713,15 @@ __device__ __forceinline__ void particle_helper(const StepArgs &a, int p, int c)
         register_tiles(P, nb, p, a.list_gen + 1u, a.tl_hdr_out, a.tl_ids_out, a.err);
 }
 
-// Resident blocks per CU the step kernels are compiled for (the register
-// budget: 4 wavefronts per block, 512 VGPRs per SIMD lane): the FP64 step 5
+// Resident blocks per CU the step kernels are compiled for, which with
+// 4-wavefront blocks is wavefronts per SIMD (the register budget: 512 VGPRs
+// per SIMD lane, 800 SGPRs per SIMD): the FP64 step 5
 // without spills (6, with one spilled value per node, measured slower); the
-// reduced formats are left to the compiler (about 7 wavefronts per SIMD:
-// forcing 8 made the two-disk FP16 step 20 % slower).
-#define STEP_BLOCKS_PER_CU(DEV, FINAL) ((FINAL) ? 1 : (DEV) ? 1 : 5)
+// reduced formats 8 (60 VGPRs, the collide header's scalar registers kept
+// within the 8-wavefront budget; left to the compiler they took 104 scalar
+// registers and 7 wavefronts).  The batch kernel is left to the compiler
+// (bounded at 8 it spills 126 scalar registers and runs 40 % slower).
+#define STEP_BLOCKS_PER_CU(DEV, FINAL) ((FINAL) ? 1 : (DEV) ? 8 : 5)
 
 // Block layout of a step launch: the particle helper blocks (particle-major,
 // K per particle), the reducer slot, then the tiles in one contiguous range
@@ -1658,7 +1791,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(const StepArgs *__restrict__ 
     const long long tb = m.y - a.n_pblocks - a.nred;
     for (int r = 0; r < a.tpb; ++r) {
         const long long t = tb * a.tpb + r;
-        if (t < a.ntiles) step_tile<St, DEV, FINAL, BND>(a, h, (int)t);
+        if (t < a.ntiles) step_tile<St, DEV, FINAL, BND, false>(a, h, (int)t);
     }
 }
 
