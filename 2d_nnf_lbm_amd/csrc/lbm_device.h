@@ -128,7 +128,7 @@ __device__ __forceinline__ int bnd_index(const DevP &P, int i, int j) {
 // formats store h = f - w_i*rho0 (times 2^10 for FP16 to stay clear of
 // subnormals), so load/store are exact inverses up to the format's rounding.
 // 32-bit integer hash (xorshift-multiply finaliser) for stochastic rounding.
-__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352du;
     x ^= x >> 15; x *= 0x846ca68bu;
     x ^= x >> 16;

@@ -447,7 +447,7 @@ struct StepHead {
     int n_pblocks, tile0, ntiles, np;
     unsigned nn, W, w_magic, w_shift;
     int nx, ny, margin, rheology;   // margin: 1, or 2 with equilibrium edges
-    unsigned sr, buf_bytes, rate_bytes, plane_bytes;
+    unsigned srh, buf_bytes, rate_bytes, plane_bytes;  // srh: mix32(sr), node_hash_h
     const void *src, *rate_in;
     unsigned pull[9];
     unsigned spare[3];
@@ -632,7 +632,7 @@ __host__ __device__ __forceinline__ void step_fields(StepArgs &a, int s, int nst
     a.mpart_in = (rec && s > 0) ? a.mpart[par ^ 1] : nullptr;
     a.red_mass = (rec && s > 1) ? a.slots_base + (size_t)(s - 2) * row2 : nullptr;
     a.red_yield = (rec && s > 0) ? a.slots_base + (size_t)(s - 1) * row2 + NSLOT : nullptr;
-    a.hd.sr = a.sr;
+    a.hd.srh = mix32(a.sr);
     a.ch.out = ((a.aux || a.want_yield) ? 1u : 0u) | (a.aux ? 2u : 0u);
 }
 
@@ -654,7 +654,7 @@ inline void finish_head(StepArgs &a, unsigned elem) {
     h.ny = P.ny;
     h.margin = P.boundary == 2 ? 2 : 1;
     h.rheology = P.rheology;
-    h.sr = a.sr;
+    h.srh = mix32(a.sr);
     h.buf_bytes = P.buf_bytes;
     h.rate_bytes = (unsigned)(P.plane * (elem == 8 ? 8 : 4));
     h.plane_bytes = (unsigned)(P.plane * elem);
@@ -842,6 +842,10 @@ __device__ __forceinline__ void eq_put(const StepArgs &a, const DevP &P, int i, 
 // (24 x 24-bit products issue at full rate).
 __device__ __forceinline__ uint32_t node_hash(unsigned n, unsigned sr) {
     return mix32(n ^ mix32(sr));
+}
+// The same from the header's mix32(sr).
+__device__ __forceinline__ uint32_t node_hash_h(unsigned n, const StepHead &h) {
+    return mix32(n ^ h.srh);
 }
 
 // Store of the nine post-collision values of node n in storage units.
@@ -1292,7 +1296,7 @@ __device__ __forceinline__ void tile_dev_out(const StepArgs &a, const StepHead &
             else
                 collide_dev<FORCED, false>(cp, H, mo, S, rho, rinvS, ucx, ucy, gx, gy, om, om2,
                                            str, yielded);
-            store_dev_h<St>(h, dst, n, H, node_hash(n, h.sr));
+            store_dev_h<St>(h, dst, n, H, node_hash_h(n, h));
             if (cp.rheology != 0)
                 __builtin_amdgcn_raw_buffer_store_b32(
                     __builtin_bit_cast(unsigned, om2), make_rsrc(rate_out, h.rate_bytes),
