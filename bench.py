@@ -7,9 +7,14 @@ the whole time step (streaming, bounce-back, smoothed-profile coupling, force
 and torque, particle update, MRT/Papanastasiou collision) with the state
 resident in HBM.  The headline value is the MLUPS of the FP32-collide /
 FP16-stored path; the plain FP64 path is measured beside it, and the
-reference program is timed on one host core as the CPU baseline.
+reference program is timed on one host core as the CPU baseline.  Further
+legs: the one-disk Newtonian FP64 case, the 504-disk BF16 cavity, the
+256-cavity FP16 batch, the settling-velocity adjoint and the FP16-vs-FP64
+accuracy run, each with the reference program of the same kind timed beside
+it where one exists (the 504-disk program only with --cpu-dense).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--accuracy-steps A]
+                       [--cpu-dense] [--no-cpu-baseline]
 """
 import argparse
 import importlib
