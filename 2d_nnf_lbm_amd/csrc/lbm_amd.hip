@@ -71,7 +71,7 @@ struct lbm_handle {
     int dev = 0;  // device the handle's memory lives on
     // timing events: whole sequence (ev0..ev1), back-to-back main launches (evA..evB)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
-    // per-tile candidate lists of many-particle runs
+    // per-wavefront candidate lists of many-particle runs
     unsigned long long *tl_hdr[2] = {nullptr, nullptr};
     int *tl_ids[2] = {nullptr, nullptr};
     int *err_flag = nullptr;
@@ -374,7 +374,8 @@ int lbm_create(const lbm_params *p, lbm_handle **out) {
         h->P.wtabf = (const float *)((const char *)h->wtab + 80);
     }
     if (p->max_particles > NP_LOOP) {
-        size_t nt = (size_t)nblk(P.nn);
+        // one list per wavefront of every tile block
+        size_t nt = (size_t)nblk(P.nn) * (NT / LIST_UNIT);
         for (int b = 0; b < 2; ++b)
             if ((rc = al((void **)&h->tl_hdr[b], nt * 8)) ||
                 (rc = al((void **)&h->tl_ids[b], nt * MAXC * 4))) {
@@ -455,7 +456,8 @@ int lbm_initialize(lbm_handle *h) {
     h->sr_next = 1;
     for (int b = 0; b < 2; ++b)
         if (h->tl_hdr[b])
-            HIPCHECK(h, hipMemsetAsync(h->tl_hdr[b], 0, (size_t)nblk(P.nn) * 8, h->stream));
+            HIPCHECK(h, hipMemsetAsync(h->tl_hdr[b], 0,
+                                       (size_t)nblk(P.nn) * (NT / LIST_UNIT) * 8, h->stream));
     double inf = INFINITY;
     HIPCHECK(h, hipMemcpyAsync(h->dpw[0], &inf, 8, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(h, hipMemcpyAsync(h->dpw[1], &inf, 8, hipMemcpyHostToDevice, h->stream));

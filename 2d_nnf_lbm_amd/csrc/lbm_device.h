@@ -765,25 +765,30 @@ __device__ __forceinline__ float phi_prof(const Prof &s, int i, int j, float &dx
 }
 
 constexpr int NP_LOOP = 16;    // above this many particles, tiles use candidate lists
-constexpr int MAXC = 64;       // candidate particles per tile
+constexpr int MAXC = 64;       // candidate particles per list unit
+constexpr int LIST_UNIT = 64;  // nodes per candidate list: one wavefront's nodes
 constexpr int NP_LDS = 128;    // up to this many particles, contact inputs are staged in LDS
 
-// Registers particle p in the candidate list of every grid tile that its
-// reach box b touches, for generation gen (the tag of the launch whose
-// tiles read the lists).  Each tile header packs
-// (generation << 32 | count); the first registration of a new generation
-// resets the count (lock-free CAS loop, one contender per touching particle).
-// Called by all NT threads of a block: thread r takes box rows j0+r, j0+r+NT,
-// ...; a tile that row j shares with row j-1 is left to row j-1, so each
-// tile is registered once per particle.
+// Registers particle p in the candidate list of every list unit (the
+// LIST_UNIT consecutive nodes of one wavefront) that its reach box b
+// touches, for generation gen (the tag of the launch whose tiles read the
+// lists).  Each unit header packs (generation << 32 | count); the first
+// registration of a new generation resets the count (lock-free CAS loop,
+// one contender per touching particle).  Called by all NT threads of a
+// block: thread r takes box rows j0+r, j0+r+NT, ...; a unit that row j
+// shares with row j-1 is left to row j-1, so each unit is registered once
+// per particle.  (A box narrower than a unit touches one or two units per
+// row, as many as it would 256-node tiles, and a wavefront then meets only
+// the particles that reach its own nodes.)
 __device__ __forceinline__ void register_tiles(const DevP &P, Box b, int p, unsigned gen,
                                                unsigned long long *hdr, int *ids, int *err) {
     if (b.i0 > b.i1 || b.j0 > b.j1) return;
     const int i0 = b.i0, i1 = b.i1, j0 = b.j0, j1 = b.j1;
     for (int j = j0 + (int)threadIdx.x; j <= j1; j += NT) {
-        long long tlo = ((long long)j * P.W + i0) / NT, thi = ((long long)j * P.W + i1) / NT;
+        long long tlo = ((long long)j * P.W + i0) / LIST_UNIT;
+        const long long thi = ((long long)j * P.W + i1) / LIST_UNIT;
         if (j > j0) {
-            long long prev = ((long long)(j - 1) * P.W + i1) / NT;
+            long long prev = ((long long)(j - 1) * P.W + i1) / LIST_UNIT;
             if (prev >= tlo) tlo = prev + 1;
         }
         for (long long t = tlo; t <= thi; ++t) {

@@ -519,7 +519,7 @@ struct StepArgs {
     long long ntiles;
     int aux;                      // write STRESS / DOMAIN_IMAGE (FINAL: copy them)
     int fin;                      // a FINAL launch
-    // per-tile candidate lists (more than NP_LOOP particles): read for this
+    // per-wavefront candidate lists (more than NP_LOOP particles): read for this
     // launch (generation list_gen), written for the next one (list_gen + 1)
     const unsigned long long *tl_hdr_in;
     const int *tl_ids_in;
@@ -582,7 +582,7 @@ __device__ __forceinline__ int row_of(const StepHead &h, unsigned n) {
 
 // error bits of the fused path
 enum {
-    LBM_EF_LIST = 1,      // a tile's candidate list overflowed
+    LBM_EF_LIST = 1,      // a wavefront candidate list overflowed
     LBM_EF_NAN = 2,       // a non-finite density or particle state
     LBM_EF_WAIT = 4,      // force partials did not appear within the wait bound
     LBM_EF_SLOT = 16,     // a reach box holds more nodes than its helper blocks cover
@@ -1015,8 +1015,9 @@ __device__ __forceinline__ BoxLane load_box_lane(const StepArgs &a, const StepHe
 // Up to NP_LOOP particles: a bit per particle whose reach box of state k
 // meets the nodes, either per wavefront with one lane per particle (LANES:
 // vector work and one compare mask, boxes from load_box_lane) or per tile
-// with scalar loads and scalar arithmetic; beyond NP_LOOP: the tile's
-// candidate list of this launch (header: generation << 32 | count).  A
+// with scalar loads and scalar arithmetic; beyond NP_LOOP: the
+// wavefront's candidate list of this launch (header: generation << 32 |
+// count).  A
 // candidate whose box misses a node contributes nothing there, so the
 // wavefront's set and its tile's give the same forcing.  (These buffers
 // were written by the previous launch, not this one.)
@@ -1050,9 +1051,10 @@ __device__ __forceinline__ int tile_candidates(const StepArgs &a, const StepHead
         }
         return __builtin_popcount(mask);
     }
-    const unsigned long long hdr = cload(a.tl_hdr_in + t);
+    const unsigned u = sp.n0 / (unsigned)LIST_UNIT;  // the wavefront's list unit
+    const unsigned long long hdr = cload(a.tl_hdr_in + u);
     if ((unsigned)(hdr >> 32) != a.list_gen) return 0;
-    ids = a.tl_ids_in + (size_t)t * MAXC;
+    ids = a.tl_ids_in + (size_t)u * MAXC;
     return min((int)(hdr & 0xffffffffu), MAXC);
 }
 

@@ -356,6 +356,9 @@ def main():
                          "cavity (about 2.5 minutes and 17 GB of host memory)")
     ap.add_argument("--newtonian-steps", type=int, default=2000,
                     help="timed steps of the one-disk Newtonian FP64 run (0 skips it)")
+    ap.add_argument("--free-steps", type=int, default=200,
+                    help="timed steps of the particle-free 2049x2049 FP16 Bingham cavity, the "
+                         "tile path alone at a large grid (0 skips it)")
     ap.add_argument("--dense-steps", type=int, default=200,
                     help="timed steps of the 504-disk BF16 cavity (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=400,
@@ -412,6 +415,21 @@ def main():
                      achieved_gbs=rd["achieved_gbs"],
                      roofline_frac=rd["achieved_gbs"] / HBM_PEAK_GBS,
                      algorithmic_bytes_per_node=rd["bytes_per_node"])
+    free = None
+    if args.free_steps > 0:
+        log("bench: particle-free 2049x2049 cavity, fp16")
+        cfg, part = configs.dense_sedimentation(0.1, "fp16")
+        rf = measure(pkg, configs, "fp16", args.free_steps, 20, workload=(cfg, part[:, :0]))
+        free = dict(workload="the 504-disk cavity's fluid (Papanastasiou-Bingham, Bn=0.1 "
+                             "parameters) with no particles, at rest (every node takes the "
+                             "small-rate branch), 2049x2049 nodes, FP32 collide / FP16 "
+                             "deviation storage",
+                    nodes=configs.nodes(cfg), steps=args.free_steps,
+                    mlups=rf["mlups"], ms_per_step=rf["ms_per_step"],
+                    kernel_us_per_launch=rf["ms_main_kernel"] * 1e3,
+                    achieved_gbs=rf["achieved_gbs"],
+                    roofline_frac=rf["achieved_gbs"] / HBM_PEAK_GBS,
+                    algorithmic_bytes_per_node=rf["bytes_per_node"])
     ensemble = None
     if args.ensemble_steps > 0:
         log("bench: 256-cavity batch, fp16")
@@ -488,6 +506,7 @@ def main():
         "fp16_over_fp64": r16["mlups"] / r64["mlups"],
         "one_disk_newtonian_fp64": newt,
         "dense_sedimentation_bf16": dense,
+        "particle_free_cavity_fp16": free,
         "cavity_ensemble_fp16": ensemble,
         "adjoint_fp32": adjoint,
         "accuracy_fp16_vs_fp64": acc,
