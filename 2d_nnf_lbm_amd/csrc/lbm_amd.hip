@@ -1906,9 +1906,10 @@ int lbm_batch_get_tiles(const lbm_batch *b, int32_t *tiles_per_block) {
 
 int lbm_batch_tune(lbm_batch *b, int32_t steps, int32_t *tiles_per_block) {
     if (!b || steps < 2) return LBM_ERR_PARAM;
+    const size_t n = b->sims.size();
     // size buckets: floor(log2(nodes)) of each grid
-    std::vector<int> bucket(b->sims.size());
-    for (size_t i = 0; i < b->sims.size(); ++i) {
+    std::vector<int> bucket(n);
+    for (size_t i = 0; i < n; ++i) {
         long long nn = b->sims[i]->P.nn;
         int k = 0;
         while ((1LL << (k + 1)) <= nn) ++k;
@@ -1917,30 +1918,53 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps, int32_t *tiles_per_block) {
     std::vector<int> keys(bucket);
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const std::vector<int> all;  // every simulation, in one launch per step
+    const int ns = std::max(steps, 8);
+    // time of one step of the whole batch under the current tiling: a call
+    // of 2 + ns steps less a call of 2 steps, so the call's prologue and final
+    // launch cancel; the runs only time the tilings, so a state that diverges
+    // under them (the tuner may run on an uninitialised batch) is not an
+    // error here
+    auto per_step = [&](double &ms) {
+        double t_short = 0.0, t_long = 0.0;
+        int rc = batch_run(b, 0, 2, nullptr, false, nullptr, all);  // settle
+        if (!rc || rc == LBM_ERR_NAN) rc = batch_run(b, 0, 2, nullptr, true, &t_short, all);
+        if (!rc || rc == LBM_ERR_NAN) rc = batch_run(b, 0, 2 + ns, nullptr, true, &t_long, all);
+        if (rc && rc != LBM_ERR_NAN) return rc;
+        ms = (t_long - t_short) / ns;
+        return (int)LBM_OK;
+    };
+    // the best uniform tiling, then one pass over the size buckets trying
+    // the other tilings for that bucket alone (kept when 1 % faster)
     const int cand[] = {1, 2, 4, 8};
+    double best = 1e300;
+    int best_t = 1;
+    for (int t : cand) {
+        for (size_t i = 0; i < n; ++i) b->tpb[i] = t;
+        double ms = 0.0;
+        int rc = per_step(ms);
+        if (rc) return rc;
+        if (ms < best) { best = ms; best_t = t; }
+    }
+    for (size_t i = 0; i < n; ++i) b->tpb[i] = best_t;
     for (int key : keys) {
-        std::vector<int> which;
-        for (size_t i = 0; i < b->sims.size(); ++i)
-            if (bucket[i] == key) which.push_back((int)i);
-        double best = 1e300;
-        int best_t = 1;
+        int keep = best_t;
         for (int t : cand) {
-            for (int i : which) b->tpb[i] = t;
+            if (t == best_t) continue;
+            for (size_t i = 0; i < n; ++i)
+                if (bucket[i] == key) b->tpb[i] = t;
             double ms = 0.0;
-            // one untimed call to settle, then the timed one; the runs only
-            // time the tilings, so a state that diverges under them (the
-            // tuner may run on an uninitialised batch) is not an error here
-            int rc = batch_run(b, 0, 2, nullptr, false, nullptr, which);
-            if (!rc || rc == LBM_ERR_NAN) rc = batch_run(b, 0, steps, nullptr, true, &ms, which);
-            if (rc && rc != LBM_ERR_NAN) return rc;
-            if (ms < best) { best = ms; best_t = t; }
+            int rc = per_step(ms);
+            if (rc) return rc;
+            if (ms < 0.99 * best) { best = ms; keep = t; }
         }
-        for (int i : which) b->tpb[i] = best_t;
+        for (size_t i = 0; i < n; ++i)
+            if (bucket[i] == key) b->tpb[i] = keep;
     }
     b->map_dirty = true;
     drop_graph(b);
     if (tiles_per_block)
-        for (size_t i = 0; i < b->sims.size(); ++i) tiles_per_block[i] = b->tpb[i];
+        for (size_t i = 0; i < n; ++i) tiles_per_block[i] = b->tpb[i];
     return LBM_OK;
 }
 

@@ -276,10 +276,13 @@ int lbm_batch_set_graph(lbm_batch *b, int32_t chunk);
 /* 256-node tiles handled by one 256-thread block, per simulation (1..64). */
 int lbm_batch_set_tiles(lbm_batch *b, const int32_t *tiles_per_block);
 int lbm_batch_get_tiles(const lbm_batch *b, int32_t *tiles_per_block);
-/* Autotuner: for each size bucket (floor(log2(nodes))) times `steps` steps of
- * that bucket's simulations with 1, 2, 4 and 8 tiles per block and keeps the
- * fastest.  The timed steps advance those simulations: initialise and set the
- * particles again afterwards.  tiles_per_block (may be NULL) gets the choice. */
+/* Autotuner: times whole-batch steps (max(steps, 8) of them, less a 2-step
+ * call, so each call's start and end launches cancel) with 1, 2, 4 and 8
+ * tiles per block for every simulation, keeps the fastest, then tries the
+ * other three for each size bucket (floor(log2(nodes))) alone, keeping a
+ * change that is 1 % faster.  The timed steps advance the simulations:
+ * initialise and set the particles again afterwards.  tiles_per_block (may
+ * be NULL) gets the choice. */
 int lbm_batch_tune(lbm_batch *b, int32_t steps, int32_t *tiles_per_block);
 
 #ifdef __cplusplus
