@@ -781,22 +781,49 @@ __device__ __forceinline__ void tile_monitors(const StepArgs &a, long long tile,
     }
 }
 
-// Sum of n partials by one block: wrap-around integer addition, so any order
-// gives the same total.
-__device__ __forceinline__ unsigned long long block_sum_u64(const unsigned long long *x,
-                                                            long long n) {
-    __shared__ unsigned long long red[NT];
-    unsigned long long acc = 0;
-    for (long long t = threadIdx.x; t < n; t += NT) acc += x[t];
-    red[threadIdx.x] = acc;
+// Sums of n partials of x and of y (y may be null) by one block:
+// wrap-around integer addition, so any order gives the same totals.  Each
+// thread issues eight loads of each array before it adds any: the partials
+// come from the L2, and a loop that waited for each load in turn made this
+// one block the last to finish a recording launch over a large grid
+// (16,401 partials: 64 round trips per thread, about 13 us).
+__device__ __forceinline__ void block_sum2_u64(const unsigned long long *x,
+                                               const unsigned long long *y, long long n,
+                                               unsigned long long &sx, unsigned long long &sy) {
+    __shared__ unsigned long long red[2][NT];
+    constexpr int U = 8;
+    unsigned long long ax = 0, ay = 0;
+    long long t = threadIdx.x;
+    for (; t + (long long)(U - 1) * NT < n; t += (long long)U * NT) {
+        unsigned long long vx[U], vy[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            vx[u] = x[t + (long long)u * NT];
+            vy[u] = y ? y[t + (long long)u * NT] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ax += vx[u];
+            ay += vy[u];
+        }
+    }
+    for (; t < n; t += NT) {
+        ax += x[t];
+        if (y) ay += y[t];
+    }
+    red[0][threadIdx.x] = ax;
+    red[1][threadIdx.x] = ay;
     __syncthreads();
     for (int w = NT / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
         __syncthreads();
     }
-    const unsigned long long r = red[0];
+    sx = red[0][0];
+    sy = red[1][0];
     __syncthreads();
-    return r;
 }
 
 // Reducer block: the previous launch's monitor partials into its slots (the
@@ -805,13 +832,11 @@ __device__ __forceinline__ void monitor_reduce(const unsigned long long *part, l
                                                unsigned long long *mass_slot,
                                                unsigned long long *yield_slot) {
     if (!part) return;
-    if (mass_slot) {
-        const unsigned long long m = block_sum_u64(part, ntiles);
-        if (threadIdx.x == 0) *mass_slot = m;
-    }
-    if (yield_slot) {
-        const unsigned long long y = block_sum_u64(part + ntiles, ntiles);
-        if (threadIdx.x == 0) *yield_slot = y;
+    unsigned long long m, y;
+    block_sum2_u64(part, yield_slot ? part + ntiles : nullptr, ntiles, m, y);
+    if (threadIdx.x == 0) {
+        if (mass_slot) *mass_slot = m;
+        if (yield_slot) *yield_slot = y;
     }
 }
 
